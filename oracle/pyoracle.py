@@ -1,0 +1,259 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "boccadorata-public_amd"))
+from bcdor_amd.formats import Propagator  # noqa: E402
+
+_LIB = None
+
+I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class Params(C.Structure):
+    _fields_ = [("mode", C.c_int), ("itr_max", C.c_int), ("lanczos", C.c_int), ("lanc_kind", C.c_int),
+                ("lanc_n", C.c_int), ("conv_kind", C.c_int), ("conv", C.c_double), ("t_fac", C.c_double),
+                ("ext_u1", C.c_int), ("n_pivot_sets", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("last_diff", C.c_double), ("e_kolt", C.c_double),
+                ("u1_tot", C.c_double), ("a_tot", C.c_double), ("n_fw_poles", C.c_long),
+                ("n_bk_poles", C.c_long), ("lost_poles", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.orc_sys_create.restype = vp
+        L.orc_sys_create.argtypes = [C.c_int] + [I32P] * 7 + [F64P, C.c_long] + [I32P] * 5 + [F64P]
+        L.orc_sys_free.argtypes = [vp]
+        L.orc_sys_n_me.restype = C.c_long; L.orc_sys_n_me.argtypes = [vp]
+        L.orc_vpp_get.restype = C.c_double
+        L.orc_vpp_get.argtypes = [vp] + [C.c_int] * 5
+        L.orc_prop_import.restype = vp
+        L.orc_prop_import.argtypes = [vp, I32P, I32P, F64P, F64P, F64P, F64P]
+        L.orc_prop_bare.restype = vp; L.orc_prop_bare.argtypes = [vp]
+        L.orc_prop_free.argtypes = [vp]
+        L.orc_prop_create.restype = vp; L.orc_prop_create.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_prop_counts.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_prop_export.argtypes = [vp, I32P, I32P, F64P, F64P, F64P, F64P, F64P, F64P]
+        L.orc_count_poles.argtypes = [vp, C.c_int, vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_build_sigma2.restype = C.c_int
+        L.orc_build_sigma2.argtypes = [vp, C.c_int, vp, F64P, C.c_int, F64P, C.c_int,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_hf_pot.argtypes = [vp, C.c_int, vp, F64P, C.c_int]
+        L.orc_kin_ho.argtypes = [vp, C.c_int, C.c_double, F64P, C.c_int]
+        L.orc_sort_u.argtypes = [C.c_int, C.c_int, C.c_int, F64P]
+        L.orc_sort_d.argtypes = [C.c_int, C.c_int, C.c_int, F64P]
+        L.orc_reduce_mtx_lncz.restype = C.c_int
+        L.orc_reduce_mtx_lncz.argtypes = [C.c_int, C.c_int, C.c_int, F64P, I32P, C.c_int, C.c_int, C.c_int,
+                                          F64P, C.c_int, F64P, I32P]
+        L.orc_reduce_selfen_lncz.restype = C.c_int
+        L.orc_reduce_selfen_lncz.argtypes = [C.c_int, C.c_int, F64P, C.c_int, F64P, C.c_int, F64P, I32P, I32P,
+                                             F64P, I32P, C.POINTER(C.c_int), F64P, I32P, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)]
+        L.orc_eigh.restype = C.c_int; L.orc_eigh.argtypes = [C.c_int, F64P, F64P]
+        L.orc_seek_efermi.restype = C.c_double; L.orc_seek_efermi.argtypes = [vp, vp, C.c_int]
+        L.orc_dyson.restype = C.c_int
+        L.orc_dyson.argtypes = [C.c_int, C.c_int, F64P, F64P,
+                                C.c_int, C.c_int, F64P, I32P, C.c_int,
+                                C.c_int, C.c_int, F64P, I32P, C.c_int,
+                                C.c_double, F64P, C.POINTER(C.c_int), F64P, C.POINTER(C.c_int), F64P]
+        L.orc_run.restype = C.c_int
+        L.orc_run.argtypes = [vp, vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, C.POINTER(Stats)]
+        _LIB = L
+    return _LIB
+
+
+class System:
+    """Model space + interaction held by the oracle (ModSpace_t + VppInt_t)."""
+
+    def __init__(self, ms, vpp):
+        L = lib()
+        self.ms = ms
+        i = lambda x: np.ascontiguousarray(x, np.int32)
+        self.h = L.orc_sys_create(ms.nsubsh, i(ms.l), i(ms.j2), i(ms.ip), i(ms.ch), i(ms.no), i(ms.orb_n),
+                                  i(ms.orb_F), np.ascontiguousarray(ms.orb_e, np.float64), vpp.n,
+                                  i(vpp.a), i(vpp.b), i(vpp.c), i(vpp.d), i(vpp.J),
+                                  np.ascontiguousarray(vpp.v, np.float64))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sys_free(self.h)
+            self.h = None
+
+    def n_me(self) -> int:
+        return lib().orc_sys_n_me(self.h)
+
+    def get(self, a, b, c, d, J) -> float:
+        return lib().orc_vpp_get(self.h, a, b, c, d, J)
+
+
+class Prop:
+    def __init__(self, handle):
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_prop_free(self.h)
+            self.h = None
+
+    @classmethod
+    def from_propagator(cls, sysm: System, g: Propagator) -> "Prop":
+        nspb = sysm.ms.nspb
+        fqp = np.zeros((len(g.ep), nspb)); fqp[:, :g.fqp.shape[1]] = g.fqp[:, :nspb]
+        fqh = np.zeros((len(g.eh), nspb)); fqh[:, :g.fqh.shape[1]] = g.fqh[:, :nspb]
+        return cls(lib().orc_prop_import(sysm.h, np.ascontiguousarray(g.np_, np.int32),
+                                         np.ascontiguousarray(g.nh, np.int32),
+                                         np.ascontiguousarray(g.ep, np.float64), np.ascontiguousarray(fqp),
+                                         np.ascontiguousarray(g.eh, np.float64), np.ascontiguousarray(fqh)))
+
+    @classmethod
+    def bare(cls, sysm: System) -> "Prop":
+        return cls(lib().orc_prop_bare(sysm.h))
+
+    def to_propagator(self, nsubsh: int) -> Propagator:
+        L = lib()
+        nq, nh, ns = C.c_int(), C.c_int(), C.c_int()
+        L.orc_prop_counts(self.h, C.byref(nq), C.byref(nh), C.byref(ns))
+        nq, nh, ns = nq.value, nh.value, ns.value
+        np_ = np.zeros(nsubsh, np.int32); nh_ = np.zeros(nsubsh, np.int32)
+        ep = np.zeros(max(nq, 1)); zp = np.zeros(max(nq, 1)); fqp = np.zeros(max(nq, 1) * ns)
+        eh = np.zeros(max(nh, 1)); zh = np.zeros(max(nh, 1)); fqh = np.zeros(max(nh, 1) * ns)
+        L.orc_prop_export(self.h, np_, nh_, ep, zp, fqp, eh, zh, fqh)
+        return Propagator(np_, nh_, ep[:nq], zp[:nq], fqp[:nq * ns].reshape(nq, ns), eh[:nh], zh[:nh],
+                          fqh[:nh * ns].reshape(nh, ns))
+
+
+def count_poles(sysm: System, ish: int, g: Prop):
+    a, b = C.c_int(), C.c_int()
+    lib().orc_count_poles(sysm.h, ish, g.h, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def build_sigma2(sysm: System, ish: int, g: Prop):
+    nfw, nbk = count_poles(sysm, ish, g)
+    w = 2 + int(sysm.ms.no[ish])
+    fw = np.zeros(max(nfw, 1) * w); bk = np.zeros(max(nbk, 1) * w)
+    a, b = C.c_int(), C.c_int()
+    rc = lib().orc_build_sigma2(sysm.h, ish, g.h, fw, nfw, bk, nbk, C.byref(a), C.byref(b))
+    if rc:
+        raise RuntimeError("orc_build_sigma2 failed")
+    return fw[:a.value * w].reshape(a.value, w), bk[:b.value * w].reshape(b.value, w)
+
+
+def hf_pot(sysm: System, ish: int, g: Prop):
+    n = int(sysm.ms.no[ish])
+    u = np.zeros(n * n)
+    lib().orc_hf_pot(sysm.h, ish, g.h, u, n)
+    return u.reshape(n, n)
+
+
+def sort_rows(rows: np.ndarray, descending: bool, col: int = 0) -> np.ndarray:
+    r = np.ascontiguousarray(rows, np.float64).copy()
+    f = lib().orc_sort_d if descending else lib().orc_sort_u
+    if r.shape[0]:
+        f(r.shape[0], r.shape[1], col, r.reshape(-1))
+    return r
+
+
+def reduce_selfen_lncz(n_orb, rows_fw, rows_bk, pivots, itr_fw, itr_bk):
+    """Returns (out_fw, ipiv_fw, out_bk, ipiv_bk, ntot_den, itr_fw', itr_bk')."""
+    fw = np.ascontiguousarray(rows_fw, np.float64).copy(); bk = np.ascontiguousarray(rows_bk, np.float64).copy()
+    npv = len(itr_fw)
+    piv = np.ascontiguousarray(pivots, np.float64).reshape(-1).copy()
+    ifw = np.ascontiguousarray(itr_fw, np.int32).copy(); ibk = np.ascontiguousarray(itr_bk, np.int32).copy()
+    den = max(npv + 1, 2); dim = den + n_orb
+    cfw = max(min(int(ifw[ifw > 0].sum()), fw.shape[0]), 1)
+    cbk = max(min(int(ibk[ibk > 0].sum()), bk.shape[0]), 1)
+    ofw = np.zeros(cfw * dim); obk = np.zeros(cbk * dim)
+    pfw = np.zeros(dim, np.int32); pbk = np.zeros(dim, np.int32)
+    a, b, d = C.c_int(), C.c_int(), C.c_int()
+    rc = lib().orc_reduce_selfen_lncz(n_orb, fw.shape[0], fw.reshape(-1) if fw.size else np.zeros(1),
+                                      bk.shape[0], bk.reshape(-1) if bk.size else np.zeros(1),
+                                      npv, piv, ifw, ibk, ofw, pfw, C.byref(a), obk, pbk, C.byref(b), C.byref(d))
+    if rc < 0:
+        raise RuntimeError(f"orc_reduce_selfen_lncz failed ({rc})")
+    return (ofw[:a.value * dim].reshape(a.value, dim), pfw, obk[:b.value * dim].reshape(b.value, dim), pbk,
+            d.value, ifw, ibk)
+
+
+def eigh(a: np.ndarray):
+    n = a.shape[0]
+    m = np.ascontiguousarray(a, np.float64).copy()
+    w = np.zeros(max(n, 1))
+    if lib().orc_eigh(n, m.reshape(-1), w):
+        raise RuntimeError("orc_eigh did not converge")
+    return w[:n], m
+
+
+def dyson(n_orb, j2, h, u1, fw, ipiv_fw, npiv_fw, bk, ipiv_bk, npiv_bk, efermi, den_fw=2, den_bk=2):
+    fw = np.ascontiguousarray(fw, np.float64).copy(); bk = np.ascontiguousarray(bk, np.float64).copy()
+    n = n_orb + fw.shape[0] + bk.shape[0]
+    sq = np.zeros(n * (n_orb + 2)); sh = np.zeros(n * (n_orb + 2)); k3 = np.zeros(3)
+    a, b = C.c_int(), C.c_int()
+    ipf = np.ascontiguousarray(ipiv_fw if ipiv_fw is not None else np.zeros(1), np.int32)
+    ipb = np.ascontiguousarray(ipiv_bk if ipiv_bk is not None else np.zeros(1), np.int32)
+    rc = lib().orc_dyson(n_orb, j2, np.ascontiguousarray(h, np.float64).reshape(-1),
+                         np.ascontiguousarray(u1, np.float64).reshape(-1),
+                         den_fw, fw.shape[0], fw.reshape(-1) if fw.size else np.zeros(1), ipf, npiv_fw,
+                         den_bk, bk.shape[0], bk.reshape(-1) if bk.size else np.zeros(1), ipb, npiv_bk,
+                         efermi, sq, C.byref(a), sh, C.byref(b), k3)
+    if rc:
+        raise RuntimeError(f"orc_dyson failed ({rc})")
+    w = n_orb + 2
+    return sq[:a.value * w].reshape(a.value, w), sh[:b.value * w].reshape(b.value, w), k3
+
+
+def seek_efermi(sysm: System, g: Prop, ish: int) -> float:
+    return lib().orc_seek_efermi(sysm.h, g.h, ish)
+
+
+MODES = {"hf": 0, "2nd": 1, "sc0": 2}
+
+
+def run(sysm: System, g: Prop, mode="hf", itr_max=-1, lanczos=False, lanc_kind=1, lanc_n=50,
+        conv_kind=1, conv=1e-8, t_fac=0.0, ext_u1=1, pivots=None):
+    """Run_itrs restated; returns (Prop, Stats).  pivots: dict ish -> (vec, itr_fw, itr_bk)."""
+    L = lib()
+    par = Params(MODES[mode], itr_max, int(lanczos), lanc_kind, lanc_n, conv_kind, conv, t_fac, ext_u1, 0)
+    st = Stats()
+    out = Prop(L.orc_prop_create(sysm.h, 1, 1))
+    args = [None] * 6
+    keep = []
+    if pivots is not None:
+        ns = sysm.ms.nsubsh
+        pn = np.zeros(ns, np.int32); po = np.zeros(ns, np.int32); pio = np.zeros(ns, np.int32)
+        vec, ifw, ibk = [], [], []
+        off = ioff = 0
+        for i in range(ns):
+            v, f, b = pivots[i]
+            pn[i] = len(f); po[i] = off; pio[i] = ioff
+            vec.append(np.asarray(v, np.float64).reshape(-1)); ifw.append(np.asarray(f, np.int32))
+            ibk.append(np.asarray(b, np.int32))
+            off += np.asarray(v).size; ioff += len(f)
+        vec = np.concatenate(vec) if vec else np.zeros(1)
+        ifw = np.concatenate(ifw).astype(np.int32); ibk = np.concatenate(ibk).astype(np.int32)
+        keep = [pn, po, pio, vec, ifw, ibk]
+        args = [a.ctypes.data_as(C.c_void_p) for a in keep]
+    rc = L.orc_run(sysm.h, g.h, C.byref(par), *args, out.h, C.byref(st))
+    if rc:
+        raise RuntimeError(f"orc_run failed ({rc})")
+    return out, st

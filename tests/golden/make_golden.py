@@ -1,0 +1,166 @@
+"""Write golden vectors of the compiled reference (oracle/_ref/ref_harness).
+
+Run in the build container (it needs /root/reference to build the harness):
+    make -C oracle ref && python tests/golden/make_golden.py
+Each case writes tests/golden/<case>.npz with the inputs (model space,
+propagator, pivots; the interaction is regenerated from its seed and checked
+against the stored digest) and the reference outputs per subshell: pole
+counts, Sigma^inf, Sigma(2) rows in build order and Lanczos-reduced rows.
+The input propagators come from the oracle (HF) or are random; they are
+written to the reference's propagator format with %.17g, so the reference
+reads exactly the same numbers.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "boccadorata-public_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+from bcdor_amd import formats, synth  # noqa: E402
+import pyoracle as po  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def v_digest(vpp: synth.Interaction) -> str:
+    h = hashlib.sha256()
+    for a in (vpp.a, vpp.b, vpp.c, vpp.d, vpp.J):
+        h.update(np.ascontiguousarray(a, np.int32).tobytes())
+    h.update(np.ascontiguousarray(vpp.v, np.float64).tobytes())
+    return h.hexdigest()
+
+
+def parse(path: str) -> dict:
+    out = {"count": {}, "hf": {}, "fw": {}, "bk": {}, "itr": {}, "lfw": {}, "lbk": {}, "ipiv": {}}
+    with open(path) as f:
+        lines = f.read().split("\n")
+    p = 0
+    while p < len(lines):
+        t = lines[p].split()
+        if not t:
+            p += 1
+            continue
+        tag = t[0]
+        if tag == "nme":
+            out["nme"] = int(t[1])
+        elif tag == "count":
+            out["count"][int(t[1])] = (int(t[2]), int(t[3]))
+        elif tag in ("hf", "fw", "bk", "lfw", "lbk"):
+            ish, n, w = int(t[1]), int(t[2]), int(t[3])
+            rows = np.array([[float(x) for x in lines[p + 1 + k].split()] for k in range(n)]).reshape(n, w)
+            out[tag][ish] = rows
+            p += n
+        elif tag == "itr":
+            ish, npv = int(t[1]), int(t[2])
+            vals = [int(x) for x in t[3:]]
+            out["itr"][ish] = (np.array(vals[0::2], np.int32), np.array(vals[1::2], np.int32))
+        elif tag == "ipiv":
+            ish = int(t[1])
+            s = lines[p].split(None, 4)[4] if len(t) > 4 else "|"
+            left, right = s.split("|")
+            out["ipiv"][ish] = (np.array([int(x) for x in left.split()], np.int32),
+                                np.array([int(x) for x in right.split()], np.int32))
+        p += 1
+    return out
+
+
+def hf_pivots(ms, g: formats.Propagator, kind: int, n: int) -> dict:
+    """Pivots as DysPivots writes them from an HF propagator (quasiholes then
+    quasiparticles by ascending energy); iteration counts: kind 0 gives every
+    pivot n per side, kind 1 splits n Krylov vectors per side over the pivots."""
+    piv = {}
+    for i in range(ms.nsubsh):
+        ep, zp, fqp, eh, zh, fqh = g.subshell(i)
+        vec = [fqh[j, :ms.no[i]] for j in np.argsort(eh, kind="stable")] + \
+              [fqp[j, :ms.no[i]] for j in np.argsort(ep, kind="stable")]
+        m = len(vec)
+        if kind == 0:
+            itr = np.full(m, n, np.int32)
+        else:
+            itr = np.array([n // m + (1 if k < n % m else 0) for k in range(m)], np.int32)
+        piv[i] = (np.array(vec).reshape(m, ms.no[i]), itr.copy(), itr.copy())
+    return piv
+
+
+def random_propagator(ms, rng, nfrag=2) -> formats.Propagator:
+    """A 'dressed' propagator: nfrag quasiparticles and nfrag quasiholes per
+    subshell with random energies and overlaps, to exercise the fragment
+    loops of the Sigma(2) build beyond the HF case."""
+    nspb = ms.nspb
+    np_, nh, ep, eh, fp, fh = [], [], [], [], [], []
+    for i in range(ms.nsubsh):
+        cp = rng.integers(1, nfrag + 1); chh = rng.integers(1, nfrag + 1)
+        for _ in range(cp):
+            x = np.zeros(nspb); x[:ms.no[i]] = rng.standard_normal(ms.no[i]) * 0.5
+            ep.append(rng.uniform(20, 120)); fp.append(x)
+        for _ in range(chh):
+            x = np.zeros(nspb); x[:ms.no[i]] = rng.standard_normal(ms.no[i]) * 0.5
+            eh.append(rng.uniform(-60, 0)); fh.append(x)
+        np_.append(cp); nh.append(chh)
+    fp = np.array(fp); fh = np.array(fh)
+    return formats.Propagator(np.array(np_, np.int32), np.array(nh, np.int32), np.array(ep),
+                              (fp ** 2).sum(1), fp, np.array(eh), (fh ** 2).sum(1), fh)
+
+
+def run_case(name, nmax, seed, prop_kind, piv_kind, piv_n, keep=None, store_v=True):
+    ms, vpp = synth.make_system(nmax, seed)
+    S = po.System(ms, vpp)
+    if prop_kind == "hf":
+        gh, _ = po.run(S, po.Prop.bare(S), "hf", itr_max=-1, conv_kind=0, conv=1e-7)
+        g = gh.to_propagator(ms.nsubsh)
+        piv = hf_pivots(ms, g, piv_kind, piv_n)
+    else:
+        rng = np.random.default_rng(1000 + seed)
+        g = random_propagator(ms, rng)
+        piv = {}
+        for i in range(ms.nsubsh):
+            m = int(rng.integers(1, 4))
+            piv[i] = (rng.standard_normal((m, ms.no[i])), rng.integers(0, 6, m).astype(np.int32),
+                      rng.integers(0, 6, m).astype(np.int32))
+    with tempfile.TemporaryDirectory() as td:
+        f_ms, f_v, f_g, f_p, f_o = (os.path.join(td, x) for x in ("msp", "v.bcd", "prop", "piv", "out"))
+        formats.write_model_space(f_ms, ms)
+        formats.write_vpp_bcd(f_v, ms, vpp)
+        formats.write_propagator(f_g, ms, g)
+        formats.write_pivots(f_p, ms, piv)
+        subprocess.run([HARNESS, f_ms, f_v, f_g, f_p, f_o], check=True, cwd=td)
+        res = parse(f_o)
+    keep = range(ms.nsubsh) if keep is None else keep
+    data = dict(nmax=nmax, seed=seed, v_digest=v_digest(vpp), nme_ref=res["nme"],
+                prop_np=g.np_, prop_nh=g.nh, prop_ep=g.ep, prop_fqp=g.fqp, prop_eh=g.eh, prop_fqh=g.fqh,
+                channels=np.array(list(keep), np.int32))
+    if store_v:
+        data.update(v_a=vpp.a, v_b=vpp.b, v_c=vpp.c, v_d=vpp.d, v_J=vpp.J, v_v=vpp.v)
+    for i in keep:
+        vec, ifw, ibk = piv[i]
+        data[f"piv_vec_{i}"] = vec; data[f"piv_fw_{i}"] = ifw; data[f"piv_bk_{i}"] = ibk
+        data[f"count_{i}"] = np.array(res["count"][i], np.int32)
+        data[f"hf_{i}"] = res["hf"][i]
+        data[f"fw_{i}"] = res["fw"][i]; data[f"bk_{i}"] = res["bk"][i]
+        if i in res["itr"]:
+            data[f"itr_{i}"] = np.stack(res["itr"][i])
+            data[f"lfw_{i}"] = res["lfw"][i]; data[f"lbk_{i}"] = res["lbk"][i]
+            data[f"ipfw_{i}"] = res["ipiv"][i][0]; data[f"ipbk_{i}"] = res["ipiv"][i][1]
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **data)
+    print(f"{name}: n_me={res['nme']} channels={len(list(keep))}")
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build the harness first: make -C oracle ref")
+    run_case("c1_hf_split50", 3, 0, "hf", 1, 50)
+    run_case("c1_hf_piv10", 3, 0, "hf", 0, 10)
+    run_case("c1_dressed", 3, 5, "random", 0, 0)
+    run_case("c2_hf_split50", 7, 0, "hf", 1, 50, keep=[0, 2, 7, 14, 15, 22, 29], store_v=False)
+
+
+if __name__ == "__main__":
+    main()
