@@ -1,0 +1,225 @@
+"""Pins of the oracle's Dyson stage, eigensolver and driver that do not need
+the compiled reference (its Dyson routine links LAPACK, an external library).
+
+* Reference-held fixture: the Lanczos-reduced Sigma(2) of examples/bcdwk with
+  the HF propagator examples/sp_prop_O16_HF reproduces
+  examples/sp_prop_O16_2nd_Lanc50 (the reference's own 1-iteration output)
+  to its print precision, and its Koltun energy -134.847 (examples/results.txt).
+* LAPACK (numpy.linalg.eigh) on random, degenerate and graded matrices.
+* Closed forms: one-orbit channels satisfy the secular equation
+  w - h - sum_p m_p^2/(w - E_p) = 0; a Lanczos run to saturation reproduces the
+  unreduced Dyson spectrum (similarity transform) for every solution with
+  nonzero spectroscopic factor; sum of Z over all solutions = n_orb.
+* HF fixed point: a converged HF propagator diagonalises h = U1 + Sigma^inf.
+"""
+import glob
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from bcdor_amd import formats, synth
+from helpers import GOLDEN
+
+EX = os.path.join(GOLDEN, "ref_examples")
+
+
+def fixture_channels():
+    ms = formats.read_model_space(os.path.join(EX, "input_msp_N7"))
+    ghf = formats.read_propagator(os.path.join(EX, "sp_prop_O16_HF"), ms)
+    with gzip.open(os.path.join(EX, "sp_prop_O16_2nd_Lanc50.gz"), "rt") as f:
+        txt = f.read()
+    tmp = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"lanc50_{os.getpid()}")
+    with open(tmp, "w") as f:
+        f.write(txt)
+    g2 = formats.read_propagator(tmp, ms)
+    os.unlink(tmp)
+    sig = {}
+    for path in glob.glob(os.path.join(EX, "bcdwk", "*.bin")):
+        s = formats.read_bcdwk_old(path)
+        sig[s.ish] = s
+    return ms, ghf, g2, sig
+
+
+def hf_matrix(ms, g, i):
+    """h = U diag(e) U^T from a converged HF propagator (exact at the HF fixed point)."""
+    ep, zp, fqp, eh, zh, fqh = g.subshell(i)
+    n = ms.no[i]
+    U = np.concatenate([fqp[:, :n], fqh[:, :n]], 0).T
+    e = np.concatenate([ep, eh])
+    return U @ np.diag(e) @ U.T
+
+
+def efermi(ms, g, i):
+    """Seek_EFermi (DysonMatrix.cpp:619-635) on the propagator arrays."""
+    sh_p = np.repeat(np.arange(ms.nsubsh), g.np_); sh_h = np.repeat(np.arange(ms.nsubsh), g.nh)
+    x1 = g.ep[0] + 100.0; x2 = g.eh[0] - 100.0
+    m = ms.ch[sh_p] == ms.ch[i]
+    if m.any():
+        x1 = min(x1, g.ep[m].min())
+    m = ms.ch[sh_h] == ms.ch[i]
+    if m.any():
+        x2 = max(x2, g.eh[m].max())
+    return 0.5 * (x1 + x2)
+
+
+def kinetic(ms, i, hw, A=16):
+    n = ms.no[i]
+    t = np.zeros((n, n))
+    l = ms.l[i]
+    for r in range(n):
+        for c in range(n):
+            na, nb = ms.orb_n[ms.orb0[i] + r], ms.orb_n[ms.orb0[i] + c]
+            if na == nb:
+                t[r, c] = (na + nb + l + 1.5) / 2.0
+            elif abs(na - nb) == 1:
+                t[r, c] = np.sqrt((na + nb + 1) * (na + nb + 2 * l + 2) / 4.0) / 2.0
+    return t * hw * (A - 1) / A
+
+
+def test_eigh_matches_lapack():
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 3, 7, 31, 64, 130):
+        a = rng.standard_normal((n, n)) * 10
+        a = a + a.T
+        w, v = po.eigh(a)
+        wl = np.linalg.eigh(a)[0]
+        assert np.max(np.abs(w - wl)) < 1e-12 * max(1.0, np.abs(wl).max())
+        assert np.max(np.abs(v.T @ v - np.eye(n))) < 1e-13
+        assert np.max(np.abs(a @ v - v * w)) < 1e-12 * max(1.0, np.abs(wl).max())
+
+
+def test_eigh_degenerate_and_graded():
+    rng = np.random.default_rng(1)
+    q, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    d = np.repeat([-3.0, 0.0, 5.0, 5.0 + 1e-9], 10)
+    a = (q * d) @ q.T
+    a = 0.5 * (a + a.T)
+    w, v = po.eigh(a)
+    assert np.max(np.abs(np.sort(w) - np.sort(d))) < 1e-12
+    assert np.max(np.abs(a @ v - v * w)) < 1e-12
+    g = np.diag(10.0 ** np.linspace(-6, 6, 25)) + np.diag(np.ones(24), 1) + np.diag(np.ones(24), -1)
+    w, v = po.eigh(g)
+    assert np.max(np.abs(w - np.linalg.eigh(g)[0])) < 1e-9
+
+
+def test_dyson_reference_fixture():
+    """bcdwk + sp_prop_O16_HF -> sp_prop_O16_2nd_Lanc50 (ExtSE run of the reference)."""
+    ms, ghf, g2, sig = fixture_channels()
+    e_tot = 0.0
+    worst_e = worst_z = worst_x = 0.0
+    for i in range(ms.nsubsh):
+        s = sig[i]
+        h_tot = hf_matrix(ms, ghf, i)
+        u1 = kinetic(ms, i, 14.0)
+        ef = efermi(ms, ghf, i)
+        sq, sh, k3 = po.dyson(int(ms.no[i]), int(ms.j2[i]), h_tot - u1, u1, s.fw, s.i_piv_fw, int(s.i_piv_fw[0]),
+                              s.bk, s.i_piv_bk, int(s.i_piv_bk[0]), ef, den_fw=s.ntot_den, den_bk=s.ntot_den)
+        ep, zp, fqp, eh, zh, fqh = g2.subshell(i)
+        assert sq.shape[0] == len(ep) and sh.shape[0] == len(eh), i
+        n = ms.no[i]
+        for sol, e, z, x in ((sq, ep, zp, fqp), (sh, eh, zh, fqh)):
+            worst_e = max(worst_e, np.max(np.abs(sol[:, 0] - e), initial=0))
+            zz = (sol[:, 2:] ** 2).sum(1)
+            worst_z = max(worst_z, np.max(np.abs(zz - z), initial=0))
+            # overlap signs are arbitrary: compare up to a sign per solution
+            xs = sol[:, 2:2 + n]
+            sgn = np.sign(np.sum(xs * x[:, :n], 1)); sgn[sgn == 0] = 1
+            worst_x = max(worst_x, np.max(np.abs(xs * sgn[:, None] - x[:, :n]), initial=0))
+        e_tot += (k3[0] + k3[1]) / 2.0
+    # print precision of the fixture: e %15.6f, Z %12.3f (percent), X %18.6e; h is
+    # rebuilt from a 7-digit HF propagator, which dominates the residue
+    assert worst_e < 1e-5, worst_e
+    assert worst_z < 1e-5, worst_z
+    assert worst_x < 1e-5, worst_x
+    assert abs(e_tot - (-134.847)) < 2e-3, e_tot
+
+
+def test_koltun_fixtures():
+    """Koltun sum rule (MGKoltun_sumrule, Koltun_MBPT.cpp:14-91) of the reference's
+    propagators: E_HF = -62.119 and E_sc0 = -122.072 (examples/results.txt)."""
+    ms = formats.read_model_space(os.path.join(EX, "input_msp_N7"))
+    ghf = formats.read_propagator(os.path.join(EX, "sp_prop_O16_HF"), ms)
+    with gzip.open(os.path.join(EX, "sp_prop_O16_2nd_sc0.gz"), "rt") as f:
+        txt = f.read()
+    tmp = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"sc0_{os.getpid()}")
+    with open(tmp, "w") as f:
+        f.write(txt)
+    gsc = formats.read_propagator(tmp, ms)
+    os.unlink(tmp)
+    for g, ref in ((ghf, -62.119), (gsc, -122.072)):
+        e = 0.0
+        for i in range(ms.nsubsh):
+            _, _, _, eh, zh, fqh = g.subshell(i)
+            t = kinetic(ms, i, 14.0)
+            n = ms.no[i]
+            for k in range(len(eh)):
+                x = fqh[k, :n]
+                e += (ms.j2[i] + 1) * ((x @ x) * eh[k] + x @ t @ x) / 2.0
+        assert abs(e - ref) < 1.5e-3, (e, ref)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    ms, vpp = synth.make_system(3, 0)
+    S = po.System(ms, vpp)
+    ghf, st = po.run(S, po.Prop.bare(S), "hf", itr_max=-1, conv_kind=0, conv=1e-12)
+    return ms, vpp, S, ghf
+
+
+def test_hf_fixed_point(cfg1):
+    ms, vpp, S, ghf = cfg1
+    g = ghf.to_propagator(ms.nsubsh)
+    for i in range(ms.nsubsh):
+        h = po.hf_pot(S, i, ghf) + np.diag(ms.orb_e[ms.orb0[i]:ms.orb0[i] + ms.no[i]])
+        assert np.max(np.abs(h - hf_matrix(ms, g, i))) < 1e-9
+
+
+def test_secular_equation_one_orbit(cfg1):
+    ms, vpp, S, ghf = cfg1
+    g = ghf.to_propagator(ms.nsubsh)
+    checked = 0
+    for i in range(ms.nsubsh):
+        if ms.no[i] != 1:
+            continue
+        fw, bk = po.build_sigma2(S, i, ghf)
+        h = po.hf_pot(S, i, ghf)[0, 0] + ms.orb_e[ms.orb0[i]]
+        sq, sh, _ = po.dyson(1, int(ms.j2[i]), np.array([[h]]), np.zeros((1, 1)), fw, None, 0, bk, None, 0,
+                             po.seek_efermi(S, ghf, i))
+        poles = np.concatenate([fw, bk])
+        for rec in np.concatenate([sq, sh]):
+            w, z = rec[0], rec[2] ** 2
+            if z < 1e-8:
+                continue
+            d = w - poles[:, 0]
+            sig = np.sum(poles[:, 2] ** 2 / d)
+            dsig = -np.sum(poles[:, 2] ** 2 / d ** 2)
+            assert abs(w - h - sig) < 1e-9 * max(1.0, abs(w)) * (1 + abs(dsig)), (i, w)
+            # spectroscopic factor of a secular root: 1/(1 - dSigma/dw)
+            assert abs(z - 1.0 / (1.0 - dsig)) < 1e-9
+            checked += 1
+    assert checked > 0
+
+
+def test_lanczos_saturation_and_sum_rule(cfg1):
+    ms, vpp, S, ghf = cfg1
+    g = ghf.to_propagator(ms.nsubsh)
+    for i in (0, 2, 5, 9):
+        n = int(ms.no[i])
+        fw, bk = po.build_sigma2(S, i, ghf)
+        h = po.hf_pot(S, i, ghf) + np.diag(ms.orb_e[ms.orb0[i]:ms.orb0[i] + n])
+        ef = po.seek_efermi(S, ghf, i)
+        sq0, sh0, _ = po.dyson(n, int(ms.j2[i]), h, np.zeros((n, n)), fw, None, 0, bk, None, 0, ef)
+        full = np.concatenate([sq0, sh0])
+        assert abs(np.sum(full[:, 1]) - n) < 1e-10           # completeness: sum Z = n_orb
+        piv = np.eye(n)
+        ofw, pfw, obk, pbk, den, _, _ = po.reduce_selfen_lncz(n, fw, bk, piv, np.full(n, fw.shape[0], np.int32),
+                                                              np.full(n, bk.shape[0], np.int32))
+        sq1, sh1, _ = po.dyson(n, int(ms.j2[i]), h, np.zeros((n, n)), ofw, pfw, int(pfw[0]), obk, pbk,
+                               int(pbk[0]), ef, den_fw=den, den_bk=den)
+        red = np.concatenate([sq1, sh1])
+        big0 = np.sort(full[full[:, 1] > 1e-6, 0]); big1 = np.sort(red[red[:, 1] > 1e-6, 0])
+        assert big0.shape == big1.shape
+        assert np.max(np.abs(big0 - big1) / np.maximum(1, np.abs(big0))) < 1e-9
