@@ -1,0 +1,576 @@
+// bcd_batch.hip -- device layout and orchestration of the batched SCGF solve.
+//
+// The driver mirrors Run_itrs (Main.cpp:105-479) for the chain HF=-1 ->
+// MakeSelfEn 2nd [Lanczos] -> ExtSE=-1: every stage is a short sequence of
+// kernel launches over all channels of all instances on one stream; the host
+// only checks the per-instance convergence flags between sweeps.
+#include "bcd_batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace bcd {
+
+static long pad8(long x) { return (x + 7) & ~7L; }
+
+int DevArena::grow(size_t bytes) {
+  if (bytes <= cap) return BCD_OK;
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  if (hipMalloc(&ptr, bytes) != hipSuccess) { ptr = nullptr; return BCD_ERR_NOMEM; }
+  cap = bytes;
+  return BCD_OK;
+}
+
+void DevArena::release() {
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+Batch::~Batch() {
+  for (auto *a : arenas()) a->release();
+  for (auto &e : ev) { if (e) hipEventDestroy(e); }
+  for (auto &kv : kev) { if (kv.first) hipEventDestroy(kv.first); if (kv.second) hipEventDestroy(kv.second); }
+  if (stream) hipStreamDestroy(stream);
+}
+
+std::vector<DevArena *> Batch::arenas() {
+  return {&a_tab_i, &a_tab_l, &a_tab_d, &a_inst, &a_chan, &a_blocks, &a_work, &a_prop, &a_rows,
+          &a_lws, &a_mat, &a_jobs};
+}
+
+#define HTRY(x)                                   \
+  do {                                            \
+    hipError_t e_ = (x);                          \
+    if (e_ != hipSuccess) return BCD_ERR_DEVICE;  \
+  } while (0)
+
+#define BTRY(x)                 \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != BCD_OK) return r_; \
+  } while (0)
+
+int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const bcd_solve_params &p,
+                 const std::vector<HostProp> &g0, bool sigma_only_) {
+  device = dev;
+  sigma_only = sigma_only_;
+  par = p;
+  sys = systems;
+  n_inst = (int)sys.size();
+  HTRY(hipSetDevice(device));
+  HTRY(hipStreamCreate(&stream));
+  for (auto &e : ev) HTRY(hipEventCreate(&e));
+
+  // ---- static tables, concatenated per element type
+  std::vector<int> ti;
+  std::vector<long> tl;
+  std::vector<double> td;
+  struct Off { long l, j2, ip, ch, no, orb0, orb_sh, qoff, pidx, npair, voff, V; };
+  std::vector<Off> off(n_inst);
+  auto put_i = [&](const std::vector<int> &v) { long o = (long)ti.size(); ti.insert(ti.end(), v.begin(), v.end()); ti.resize(pad8(ti.size())); return o; };
+  long rho_tot = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    const HostSystem &S = *sys[i];
+    Off &o = off[i];
+    o.l = put_i(S.l); o.j2 = put_i(S.j2); o.ip = put_i(S.ip); o.ch = put_i(S.ch); o.no = put_i(S.no);
+    o.orb0 = put_i(S.orb0); o.orb_sh = put_i(S.orb_sh); o.qoff = put_i(S.qoff); o.pidx = put_i(S.pidx);
+    o.npair = put_i(S.npair);
+    o.voff = (long)tl.size(); tl.insert(tl.end(), S.voff.begin(), S.voff.end()); tl.resize(pad8(tl.size()));
+    o.V = (long)td.size(); td.insert(td.end(), S.V.begin(), S.V.end()); td.resize(pad8(td.size()));
+  }
+  BTRY(a_tab_i.grow(sizeof(int) * std::max<size_t>(ti.size(), 1)));
+  BTRY(a_tab_l.grow(sizeof(long) * std::max<size_t>(tl.size(), 1)));
+  BTRY(a_tab_d.grow(sizeof(double) * std::max<size_t>(td.size(), 1)));
+  HTRY(hipMemcpy(a_tab_i.ptr, ti.data(), sizeof(int) * ti.size(), hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(a_tab_l.ptr, tl.data(), sizeof(long) * tl.size(), hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(a_tab_d.ptr, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice));
+  const int *Ti = (const int *)a_tab_i.ptr;
+  const long *Tl = (const long *)a_tab_l.ptr;
+  const double *Td = (const double *)a_tab_d.ptr;
+
+  // ---- instances and channels
+  h_insts.resize(n_inst);
+  h_chans.clear();
+  h_blocks.clear();
+  nImax = 1;
+  const bool dyn = par.mode >= 1;
+  long o_u1 = 0, o_h = 0, o_w = 0, o_tri = 0, o_mat = 0, o_piv = 0, o_lr = 0, o_e = 0, o_x = 0, o_blk = 0;
+  std::vector<long> e_off, x_off;
+  std::vector<int> cap;
+  for (int i = 0; i < n_inst; ++i) {
+    const HostSystem &S = *sys[i];
+    const Off &o = off[i];
+    InstDev &I = h_insts[i];
+    I.nsubsh = S.nsubsh; I.norb = S.norb; I.nspb = S.nspb; I.jmax = S.jmax; I.nchan2b = S.nchan2b;
+    I.gc0 = (int)h_chans.size(); I.nq = S.nq; I._pad = 0;
+    I.l = Ti + o.l; I.j2 = Ti + o.j2; I.ip = Ti + o.ip; I.ch = Ti + o.ch; I.no = Ti + o.no;
+    I.orb0 = Ti + o.orb0; I.orb_sh = Ti + o.orb_sh; I.qoff = Ti + o.qoff; I.pidx = Ti + o.pidx;
+    I.voff = Tl + o.voff; I.npair = Ti + o.npair; I.V = Td + o.V;
+    I.rho_off = rho_tot;
+    rho_tot += pad8(S.nq);
+    for (int s = 0; s < S.nsubsh; ++s) {
+      ChanDev C;
+      std::memset(&C, 0, sizeof C);
+      const int nI = S.no[s];
+      nImax = std::max(nImax, nI);
+      C.inst = i; C.ish = s; C.nI = nI; C.j2 = S.j2[s]; C.ch = S.ch[s]; C.orb0 = S.orb0[s];
+      // Lanczos vectors per side and the resulting Dyson dimension bound
+      long lside = par.lanc_kind == 0 ? (long)par.lanc_n * std::min(nI, kMaxPiv) : par.lanc_n;
+      long lfw = std::min<long>(lside, S.mbound_fw[s]), lbk = std::min<long>(lside, S.mbound_bk[s]);
+      long cp = nI;
+      if (dyn && !sigma_only) cp += par.lanczos ? lfw + lbk : S.mbound_fw[s] + S.mbound_bk[s];
+      if (!g0.empty()) cp = std::max<long>(cp, g0[i].np[s] + g0[i].nh[s]);
+      C.cap = (int)cp;
+      C.blk_off = o_blk; C.blk_cnt = (int)S.blocks[s].size();
+      for (BlockDev B : S.blocks[s]) { B.gc = I.gc0 + s; h_blocks.push_back(B); }
+      o_blk += C.blk_cnt;
+      C.u1_off = o_u1; o_u1 += pad8((long)nI * nI);
+      C.h_off = o_h; o_h += pad8((long)nI * nI);
+      C.w_off = o_w; o_w += pad8((long)nI * (nI + 1) / 2 * S.nq);
+      C.tri_off = o_tri; o_tri += pad8(cp * (2 + nI));
+      C.mat_off = o_mat; o_mat += pad8(cp * cp);
+      C.piv_off = o_piv; o_piv += pad8((long)kMaxPiv * nI);
+      const int width = std::max(2, std::min(nI, kMaxPiv) + 1) + nI;
+      C.lfw_cap = par.lanczos ? (int)lfw : 0; C.lbk_cap = par.lanczos ? (int)lbk : 0;
+      C.lfw_off = o_lr; o_lr += pad8((long)C.lfw_cap * width);
+      C.lbk_off = o_lr; o_lr += pad8((long)C.lbk_cap * width);
+      e_off.push_back(o_e); o_e += pad8(cp);
+      x_off.push_back(o_x); o_x += pad8(cp * nI);
+      cap.push_back((int)cp);
+      h_chans.push_back(C);
+      capmax = std::max<long>(capmax, cp);
+    }
+  }
+  n_gc = (int)h_chans.size();
+  n_blk = (long)h_blocks.size();
+  mat_len = o_mat;
+
+  // ---- one-body parts
+  std::vector<double> u1(std::max<long>(o_u1, 1), 0.0);
+  for (const ChanDev &C : h_chans) sys[C.inst]->u1(C.ish, par.t_fac, par.ext_u1, u1.data() + C.u1_off);
+
+  // ---- device buffers (fixed size)
+  BTRY(a_inst.grow(pad8(sizeof(InstDev) * n_inst) + pad8(sizeof(InstState) * n_inst) + 64));
+  d_insts = (InstDev *)a_inst.ptr;
+  d_is = (InstState *)((char *)a_inst.ptr + pad8(sizeof(InstDev) * n_inst));
+  HTRY(hipMemcpy(d_insts, h_insts.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice));
+
+  size_t chan_bytes = pad8(sizeof(ChanDev) * n_gc) + pad8(sizeof(ChanState) * n_gc) + 2 * pad8(sizeof(long) * n_gc) +
+                      2 * pad8(sizeof(long) * n_gc) + pad8(sizeof(int) * n_gc) + 64;
+  BTRY(a_chan.grow(chan_bytes));
+  char *cp8 = (char *)a_chan.ptr;
+  d_chans = (ChanDev *)cp8; cp8 += pad8(sizeof(ChanDev) * n_gc);
+  d_cs = (ChanState *)cp8; cp8 += pad8(sizeof(ChanState) * n_gc);
+  d_row_fw_base = (long *)cp8; cp8 += pad8(sizeof(long) * n_gc);
+  d_row_bk_base = (long *)cp8; cp8 += pad8(sizeof(long) * n_gc);
+  d_e_off = (long *)cp8; cp8 += pad8(sizeof(long) * n_gc);
+  d_x_off = (long *)cp8; cp8 += pad8(sizeof(long) * n_gc);
+  d_cap = (int *)cp8;
+  HTRY(hipMemcpy(d_chans, h_chans.data(), sizeof(ChanDev) * n_gc, hipMemcpyHostToDevice));
+  HTRY(hipMemset(d_cs, 0, sizeof(ChanState) * n_gc));
+  HTRY(hipMemset(d_row_fw_base, 0, sizeof(long) * n_gc * 2));
+  HTRY(hipMemcpy(d_e_off, e_off.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(d_x_off, x_off.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(d_cap, cap.data(), sizeof(int) * n_gc, hipMemcpyHostToDevice));
+
+  BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 3 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) + 64));
+  cp8 = (char *)a_blocks.ptr;
+  d_blocks = (BlockDev *)cp8; cp8 += pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1));
+  d_blk_fw_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
+  d_blk_bk_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
+  d_work_list = (int *)cp8;
+  if (n_blk) HTRY(hipMemcpy(d_blocks, h_blocks.data(), sizeof(BlockDev) * n_blk, hipMemcpyHostToDevice));
+
+  // work doubles: u1, h, W, rho, tri, piv, lrows, kolt ; ints: status, work_count
+  l_u1 = 0; l_h = l_u1 + pad8(o_u1); l_w = l_h + pad8(o_h); l_rho = l_w + pad8(o_w);
+  l_tri = l_rho + pad8(rho_tot); l_piv = l_tri + pad8(o_tri); l_lr = l_piv + pad8(o_piv);
+  l_kolt = l_lr + pad8(o_lr); l_end = l_kolt + pad8(3L * n_gc) + 16;
+  BTRY(a_work.grow(sizeof(double) * l_end));
+  double *W0 = (double *)a_work.ptr;
+  d_u1 = W0 + l_u1; d_h = W0 + l_h; d_w = W0 + l_w; d_rho = W0 + l_rho; d_tri = W0 + l_tri;
+  d_piv = W0 + l_piv; d_lrows = W0 + l_lr; d_kolt = W0 + l_kolt;
+  d_status = (int *)(W0 + l_end - 16);
+  d_work_count = d_status + 2;
+  HTRY(hipMemset(a_work.ptr, 0, sizeof(double) * l_end));
+  HTRY(hipMemcpy(d_u1, u1.data(), sizeof(double) * o_u1, hipMemcpyHostToDevice));
+
+  // propagators: 3 buffers (initial, ping, pong), counts
+  const long pe = pad8(o_e), px = pad8(o_x), pn = pad8(n_gc);
+  BTRY(a_prop.grow(sizeof(double) * 3 * (pe + px) + sizeof(int) * 6 * pn + 64));
+  double *P0 = (double *)a_prop.ptr;
+  for (int b = 0; b < 3; ++b) { pe_[b] = P0 + b * (pe + px); px_[b] = pe_[b] + pe; }
+  int *C0 = (int *)(P0 + 3 * (pe + px));
+  for (int b = 0; b < 3; ++b) { pnqp_[b] = C0 + 2 * b * pn; pnqh_[b] = pnqp_[b] + pn; }
+  prop_e_len = pe; prop_x_len = px; prop_n_len = pn;
+  HTRY(hipMemset(a_prop.ptr, 0, a_prop.cap));
+
+  // initial propagator
+  std::vector<double> he(pe, 0.0), hx(px, 0.0);
+  std::vector<int> hqp(pn, 0), hqh(pn, 0);
+  for (int i = 0; i < n_inst; ++i) {
+    const HostSystem &S = *sys[i];
+    const HostProp &G = g0[i];
+    long kp = 0, kh = 0;
+    for (int s = 0; s < S.nsubsh; ++s) {
+      const int gc = h_insts[i].gc0 + s, nI = S.no[s];
+      hqp[gc] = G.np[s]; hqh[gc] = G.nh[s];
+      for (int k = 0; k < G.np[s]; ++k, ++kp) {
+        he[e_off[gc] + k] = G.ep[kp];
+        for (int r = 0; r < nI; ++r) hx[x_off[gc] + (long)k * nI + r] = G.fqp[kp * G.nspb + r];
+      }
+      for (int k = 0; k < G.nh[s]; ++k, ++kh) {
+        he[e_off[gc] + G.np[s] + k] = G.eh[kh];
+        for (int r = 0; r < nI; ++r) hx[x_off[gc] + (long)(G.np[s] + k) * nI + r] = G.fqh[kh * G.nspb + r];
+      }
+    }
+  }
+  HTRY(hipMemcpy(pe_[2], he.data(), sizeof(double) * pe, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(px_[2], hx.data(), sizeof(double) * px, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(pnqp_[2], hqp.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(pnqh_[2], hqh.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
+
+  // jobs
+  BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) + 64));
+  d_djobs = (DysonJob *)a_jobs.ptr;
+  d_ljobs = (LancJob *)((char *)a_jobs.ptr + pad8(sizeof(DysonJob) * n_gc));
+  HTRY(hipMemset(a_jobs.ptr, 0, a_jobs.cap));
+
+  // dense Dyson matrices in HBM when the largest one does not fit in LDS
+  if (tridiag_lds((int)capmax, true) > (size_t)kLdsBytes) BTRY(a_mat.grow(sizeof(double) * (mat_len + 16)));
+
+  // kernel attributes for large dynamic LDS
+  HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  h_is.assign(n_inst, InstState());
+  return BCD_OK;
+}
+
+PropView Batch::view(bool with_state) const {
+  PropView P;
+  for (int b = 0; b < 2; ++b) { P.e[b] = pe_[b]; P.x[b] = px_[b]; P.nqp[b] = pnqp_[b]; P.nqh[b] = pnqh_[b]; }
+  P.e_off = d_e_off; P.x_off = d_x_off; P.cap = d_cap;
+  P.is = with_state ? d_is : nullptr;
+  return P;
+}
+
+size_t Batch::tridiag_lds(int n, bool in_lds) const {
+  size_t d = (size_t)(2 + nImax) * n + nImax + 16;
+  if (in_lds) d += (size_t)n * n;
+  return d * sizeof(double);
+}
+
+void Batch::kt_begin(int k) {
+  if (!profile) return;
+  auto &p = kev[k];
+  if (!p.first) { hipEventCreate(&p.first); hipEventCreate(&p.second); }
+  hipEventRecord(p.first, stream);
+}
+
+void Batch::kt_end(int k) {
+  if (!profile) return;
+  hipEventRecord(kev[k].second, stream);
+  hipEventSynchronize(kev[k].second);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, kev[k].first, kev[k].second);
+  ktime[k] += ms;
+  kcount[k] += 1;
+}
+
+int Batch::sweep(int src, int &n_active) {
+  PropView P = view(true);
+  const int gb = (n_gc + 127) / 128;
+  kt_begin(KT_DENSITY);
+  hipLaunchKernelGGL(k_density, dim3(n_gc), dim3(64), 0, stream, d_insts, d_chans, 0, n_gc, P, d_rho);
+  kt_end(KT_DENSITY);
+  kt_begin(KT_SIGINF);
+  hipLaunchKernelGGL(k_sigma_inf, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_is, d_w, d_rho, d_h);
+  kt_end(KT_SIGINF);
+  hipLaunchKernelGGL(k_dyson_prep, dim3(gb), dim3(128), 0, stream, d_chans, 0, n_gc, d_cs, P, src, d_h, d_u1,
+                     d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_lrows, d_tri,
+                     (double *)a_mat.ptr, d_kolt, d_djobs);
+  const int nmax = src == 0 ? nImax : (int)capmax;
+  const bool in_lds = tridiag_lds(nmax, true) <= (size_t)kLdsBytes;
+  kt_begin(KT_TRIDIAG);
+  if (in_lds)
+    hipLaunchKernelGGL(k_tridiag<true>, dim3(n_gc), dim3(256), tridiag_lds(nmax, true), stream, d_djobs, n_gc);
+  else
+    hipLaunchKernelGGL(k_tridiag<false>, dim3(n_gc), dim3(256), tridiag_lds(nmax, false), stream, d_djobs, n_gc);
+  kt_end(KT_TRIDIAG);
+  kt_begin(KT_QL);
+  hipLaunchKernelGGL(k_ql, dim3(n_gc), dim3(64), (size_t)(2 + nImax) * nmax * sizeof(double), stream, d_djobs,
+                     n_gc, d_status);
+  kt_end(KT_QL);
+  kt_begin(KT_FINAL);
+  hipLaunchKernelGGL(k_finalize, dim3(n_inst), dim3(256), 0, stream, d_insts, d_chans, P, d_cs, d_kolt,
+                     par.conv_kind, 0);
+  kt_end(KT_FINAL);
+  HTRY(hipGetLastError());
+  HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
+  n_active = 0;
+  for (const InstState &s : h_is) {
+    if (s.err) return BCD_ERR_CAPACITY;
+    n_active += s.active;
+  }
+  int st = 0;
+  HTRY(hipMemcpy(&st, d_status, sizeof(int), hipMemcpyDeviceToHost));
+  if (st) return BCD_ERR_EIGEN;
+  return BCD_OK;
+}
+
+int Batch::stage(int src, int itr_max, double conv, bool hf) {
+  // stage start: keep the current buffer, reset the sweep controls
+  HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
+  for (InstState &s : h_is) { s.active = 1; s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0; }
+  HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_finalize, dim3(n_inst), dim3(256), 0, stream, d_insts, d_chans, view(true), d_cs, d_kolt,
+                     par.conv_kind, 1);
+  HTRY(hipGetLastError());
+  const int hard_cap = itr_max >= 0 ? itr_max + 1 : kMaxSweeps;
+  int n_active = n_inst;
+  for (int it = 0; it < hard_cap && n_active > 0; ++it) BTRY(sweep(src, n_active));
+  for (int i = 0; i < n_inst; ++i) (hf ? hf_its : sc_its)[i] = h_is[i].its;
+  return BCD_OK;
+}
+
+int Batch::build_sigma2(int only_ish) {
+  PropView P = view(true);
+  HTRY(hipMemsetAsync(d_work_count, 0, sizeof(int), stream));
+  kt_begin(KT_PLAN);
+  hipLaunchKernelGGL(k_sigma2_plan, dim3(n_gc), dim3(64), 0, stream, d_chans, d_blocks, 0, n_gc, P, d_insts, d_cs,
+                     d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, only_ish);
+  kt_end(KT_PLAN);
+  HTRY(hipGetLastError());
+  h_cs.resize(n_gc);
+  HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
+  // row buffers and Lanczos workspace sized by the actual pole counts
+  std::vector<long> bfw(n_gc), bbk(n_gc);
+  long tfw = 0, tbk = 0, tws = 0;
+  bool relayout = false;
+  for (int gc = 0; gc < n_gc; ++gc) {
+    ChanDev &C = h_chans[gc];
+    const long w = 2 + C.nI;
+    bfw[gc] = tfw; tfw += pad8((long)h_cs[gc].mfw * w);
+    bbk[gc] = tbk; tbk += pad8((long)h_cs[gc].mbk * w);
+    if (par.lanczos) {
+      const long lws_fw = tws, lws_bk = tws + pad8((long)C.lfw_cap * h_cs[gc].mfw);
+      tws = lws_bk + pad8((long)C.lbk_cap * h_cs[gc].mbk);
+      if (C.lws_fw_off != lws_fw || C.lws_bk_off != lws_bk) relayout = true;
+      C.lws_fw_off = lws_fw; C.lws_bk_off = lws_bk;
+    }
+    if (!sigma_only && !par.lanczos && h_cs[gc].mfw + h_cs[gc].mbk + C.nI > C.cap) return BCD_ERR_CAPACITY;
+  }
+  n_fw_tot.assign(n_inst, 0); n_bk_tot.assign(n_inst, 0);
+  for (int gc = 0; gc < n_gc; ++gc) { n_fw_tot[h_chans[gc].inst] += h_cs[gc].mfw; n_bk_tot[h_chans[gc].inst] += h_cs[gc].mbk; }
+  BTRY(a_rows.grow(sizeof(double) * (tfw + tbk + 16)));
+  d_rows_fw = (double *)a_rows.ptr;
+  d_rows_bk = d_rows_fw + tfw;
+  if (par.lanczos) BTRY(a_lws.grow(sizeof(double) * (tws + 16)));
+  HTRY(hipMemcpyAsync(d_row_fw_base, bfw.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice, stream));
+  HTRY(hipMemcpyAsync(d_row_bk_base, bbk.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice, stream));
+  if (relayout) HTRY(hipMemcpyAsync(d_chans, h_chans.data(), sizeof(ChanDev) * n_gc, hipMemcpyHostToDevice, stream));
+  if (coupling_flops < 0.0) {
+    // 2 * poles * K * n per block, K = (alpha,beta) pairs * gamma, from the
+    // fragment counts the build used (reference formulation, Op_c.cpp:454-468)
+    std::vector<int> cq(pnqp_[0] ? n_gc : 0), chh(n_gc), cq1(n_gc), ch1(n_gc);
+    HTRY(hipMemcpy(cq.data(), pnqp_[0], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    HTRY(hipMemcpy(chh.data(), pnqh_[0], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    HTRY(hipMemcpy(cq1.data(), pnqp_[1], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    HTRY(hipMemcpy(ch1.data(), pnqh_[1], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    double f = 0.0;
+    for (const BlockDev &B : h_blocks) {
+      const ChanDev &C = h_chans[B.gc];
+      const HostSystem &S = *sys[C.inst];
+      const int g0 = h_insts[C.inst].gc0, cur = h_is[C.inst].cur;
+      const int *NQ = cur ? cq1.data() : cq.data(), *NH = cur ? ch1.data() : chh.data();
+      if (only_ish >= 0 && C.ish != only_ish) continue;
+      const int na = S.no[B.a], nb = S.no[B.b], ng = S.no[B.g];
+      long npair = 0;
+      for (int al = 0; al < na; ++al) npair += nb - ((B.a == B.b) ? al + B.nJab : 0);
+      const double K = (double)npair * ng;
+      for (int side = 0; side < 2; ++side) {
+        const int cA = side ? NH[g0 + B.a] : NQ[g0 + B.a], cB = side ? NH[g0 + B.b] : NQ[g0 + B.b];
+        const int cG = side ? NQ[g0 + B.g] : NH[g0 + B.g];
+        long poles;
+        if (B.a == B.b) { int nb2 = cA + 1; if (B.nJab == 1) nb2 -= 2; poles = (long)(cA * nb2 / 2) * cG; }
+        else poles = (long)cA * cB * cG;
+        f += 2.0 * poles * K * C.nI;
+      }
+    }
+    coupling_flops = f;
+  }
+  int nwork = 0;
+  HTRY(hipMemcpyAsync(&nwork, d_work_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
+  if (nwork > 0) {
+    kt_begin(KT_SIGMA2);
+    hipLaunchKernelGGL(k_sigma2, dim3(std::min(nwork, 16384)), dim3(64), 0, stream, d_insts, d_chans, d_blocks, P,
+                       d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base, d_row_bk_base,
+                       d_rows_fw, d_rows_bk);
+    kt_end(KT_SIGMA2);
+    HTRY(hipGetLastError());
+  }
+  return BCD_OK;
+}
+
+int Batch::lanczos() {
+  PropView P = view(true);
+  hipLaunchKernelGGL(k_lanczos_prep, dim3((n_gc + 63) / 64), dim3(64), 0, stream, d_chans, 0, n_gc, P, d_cs, d_piv,
+                     par.lanc_kind, par.lanc_n, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_lrows,
+                     (double *)a_lws.ptr, d_ljobs);
+  HTRY(hipGetLastError());
+  int mmax = 0;
+  for (int gc = 0; gc < n_gc; ++gc) mmax = std::max(mmax, std::max(h_cs[gc].mfw, h_cs[gc].mbk));
+  kt_begin(KT_LANCZOS);
+  if (mmax <= 256 * 8)
+    hipLaunchKernelGGL((k_lanczos<256, 8>), dim3(2 * n_gc), dim3(256), 0, stream, d_ljobs, 2 * n_gc);
+  else if (mmax <= 256 * 24)
+    hipLaunchKernelGGL((k_lanczos<256, 24>), dim3(2 * n_gc), dim3(256), 0, stream, d_ljobs, 2 * n_gc);
+  else if (mmax <= 512 * 24)
+    hipLaunchKernelGGL((k_lanczos<512, 24>), dim3(2 * n_gc), dim3(512), 0, stream, d_ljobs, 2 * n_gc);
+  else if (mmax <= 1024 * 24)
+    hipLaunchKernelGGL((k_lanczos<1024, 24>), dim3(2 * n_gc), dim3(1024), 0, stream, d_ljobs, 2 * n_gc);
+  else
+    return BCD_ERR_CAPACITY;
+  kt_end(KT_LANCZOS);
+  HTRY(hipGetLastError());
+  HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
+  for (int gc = 0; gc < n_gc; ++gc)
+    if (h_cs[gc].err || h_cs[gc].lfw < 0 || h_cs[gc].lbk < 0) return BCD_ERR_LANCZOS;
+  return BCD_OK;
+}
+
+// Algorithmic work of the last solve (see bcd_batch_work in bcdor_amd.h).
+int Batch::work(double *w) {
+  for (int k = 0; k < 5; ++k) w[k] = 0.0;
+  w[0] = coupling_flops;
+  for (int gc = 0; gc < (int)h_cs.size(); ++gc) {
+    const ChanDev &C = h_chans[gc];
+    const ChanState &Q = h_cs[gc];
+    for (int side = 0; side < 2 && par.lanczos; ++side) {
+      const double M = side ? Q.mbk : Q.mfw, L = side ? Q.lbk : Q.lfw;
+      if (L > 0) w[1] += 8.0 * M * (L * (L - 1.0) + L * (5.0 + C.nI + Q.npiv_tot));
+    }
+  }
+  for (int gc = 0; gc < n_gc; ++gc) {
+    const ChanDev &C = h_chans[gc];
+    const int i = C.inst;
+    const double n0 = C.nI;
+    double n1 = C.nI;
+    if (par.mode >= 1 && gc < (int)h_cs.size())
+      n1 += par.lanczos ? std::max(h_cs[gc].lfw, 0) + std::max(h_cs[gc].lbk, 0) : h_cs[gc].mfw + h_cs[gc].mbk;
+    const double hi = hf_its.empty() ? 0 : hf_its[i], si = sc_its.empty() ? 0 : sc_its[i];
+    w[3] += hi + si;
+    w[4] += hi * n0 * n0 * n0 + si * n1 * n1 * n1;
+  }
+  w[2] = 9.0 * w[4];
+  return BCD_OK;
+}
+
+int Batch::prepare() {
+  HTRY(hipSetDevice(device));
+  HTRY(hipMemcpyAsync(pe_[0], pe_[2], sizeof(double) * (prop_e_len + prop_x_len), hipMemcpyDeviceToDevice, stream));
+  HTRY(hipMemcpyAsync(pnqp_[0], pnqp_[2], sizeof(int) * 2 * prop_n_len, hipMemcpyDeviceToDevice, stream));
+  for (InstState &s : h_is) { std::memset(&s, 0, sizeof s); s.cur = 0; s.active = 1; }
+  HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
+  HTRY(hipStreamSynchronize(stream));
+  return BCD_OK;
+}
+
+int Batch::solve() {
+  HTRY(hipSetDevice(device));
+  for (int k = 0; k < KT_N; ++k) { ktime[k] = 0.0; kcount[k] = 0; }
+  hf_its.assign(n_inst, 0);
+  sc_its.assign(n_inst, 0);
+  HTRY(hipMemsetAsync(d_status, 0, sizeof(int), stream));
+  HTRY(hipEventRecord(ev[0], stream));
+  // start from the initial propagator in buffer 0
+  HTRY(hipMemcpyAsync(pe_[0], pe_[2], sizeof(double) * (prop_e_len + prop_x_len), hipMemcpyDeviceToDevice, stream));
+  HTRY(hipMemcpyAsync(pnqp_[0], pnqp_[2], sizeof(int) * 2 * prop_n_len, hipMemcpyDeviceToDevice, stream));
+  for (InstState &s : h_is) { std::memset(&s, 0, sizeof s); s.cur = 0; s.active = 1; }
+  HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
+  kt_begin(KT_WBUILD);
+  hipLaunchKernelGGL(k_wbuild, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_w);
+  kt_end(KT_WBUILD);
+  HTRY(hipGetLastError());
+  HTRY(hipEventRecord(ev[1], stream));
+  BTRY(stage(0, par.hf_itr_max, par.hf_conv, true));
+  HTRY(hipEventRecord(ev[2], stream));
+  if (par.mode >= 1) {
+    BTRY(build_sigma2(-1));
+    HTRY(hipEventRecord(ev[3], stream));
+    if (par.lanczos) BTRY(lanczos());
+    HTRY(hipEventRecord(ev[4], stream));
+    BTRY(stage(par.lanczos ? 2 : 1, par.mode == 1 ? 0 : par.sc_itr_max, par.sc_conv, false));
+  } else {
+    HTRY(hipEventRecord(ev[3], stream));
+    HTRY(hipEventRecord(ev[4], stream));
+  }
+  HTRY(hipEventRecord(ev[5], stream));
+  HTRY(hipEventSynchronize(ev[5]));
+  for (int k = 0; k < 5; ++k) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+    stage_ms[k] = ms;
+  }
+  float tot = 0.f;
+  hipEventElapsedTime(&tot, ev[0], ev[5]);
+  stage_ms[5] = tot;
+  HTRY(hipMemcpy(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost));
+  return BCD_OK;
+}
+
+int Batch::result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep, double *zp, double *fqp,
+                  double *eh, double *zh, double *fqh) {
+  if (inst < 0 || inst >= n_inst) return BCD_ERR_ARG;
+  const InstDev &I = h_insts[inst];
+  const HostSystem &S = *sys[inst];
+  const int cur = h_is[inst].cur;
+  std::vector<int> cq(S.nsubsh), ch_(S.nsubsh);
+  HTRY(hipMemcpy(cq.data(), pnqp_[cur] + I.gc0, sizeof(int) * S.nsubsh, hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(ch_.data(), pnqh_[cur] + I.gc0, sizeof(int) * S.nsubsh, hipMemcpyDeviceToHost));
+  int tq = 0, th = 0;
+  for (int s = 0; s < S.nsubsh; ++s) { tq += cq[s]; th += ch_[s]; }
+  if (n_qp) *n_qp = tq;
+  if (n_qh) *n_qh = th;
+  if (!ep) return BCD_OK;
+  if (nspb < S.nspb) return BCD_ERR_ARG;
+  std::vector<long> eo(n_gc), xo(n_gc);
+  HTRY(hipMemcpy(eo.data(), d_e_off, sizeof(long) * n_gc, hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(xo.data(), d_x_off, sizeof(long) * n_gc, hipMemcpyDeviceToHost));
+  int kp = 0, kh = 0;
+  for (int s = 0; s < S.nsubsh; ++s) {
+    const int gc = I.gc0 + s, nI = S.no[s], tot = cq[s] + ch_[s];
+    std::vector<double> e(std::max(tot, 1)), x(std::max(tot * nI, 1));
+    if (tot) {
+      HTRY(hipMemcpy(e.data(), pe_[cur] + eo[gc], sizeof(double) * tot, hipMemcpyDeviceToHost));
+      HTRY(hipMemcpy(x.data(), px_[cur] + xo[gc], sizeof(double) * tot * nI, hipMemcpyDeviceToHost));
+    }
+    if (np) np[s] = cq[s];
+    if (nh) nh[s] = ch_[s];
+    for (int k = 0; k < tot; ++k) {
+      const bool qp = k < cq[s];
+      double *E = qp ? ep : eh, *Z = qp ? zp : zh, *F = qp ? fqp : fqh;
+      const int j = qp ? kp++ : kh++;
+      E[j] = e[k];
+      double z = 0.0;
+      for (int r = 0; r < nspb; ++r) {
+        const double v = r < nI ? x[(long)k * nI + r] : 0.0;
+        F[(long)j * nspb + r] = v;
+        z += v * v;
+      }
+      Z[j] = z;
+    }
+  }
+  return BCD_OK;
+}
+
+}  // namespace bcd

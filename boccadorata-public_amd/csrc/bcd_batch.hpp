@@ -1,0 +1,96 @@
+// bcd_batch.hpp -- a device-resident batch of SCGF instances.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+#include "bcd_system.hpp"
+
+namespace bcd {
+
+constexpr int kLdsBytes = 160 * 1024 - 1024;   // dynamic LDS ceiling used by the Dyson kernels
+constexpr int kMaxSweeps = 2000;
+
+// Starting propagator of one instance, fragments as SpProp_t stores them.
+struct HostProp {
+  int nspb = 0;
+  std::vector<int> np, nh;
+  std::vector<double> ep, fqp, eh, fqh;
+};
+
+struct DevArena {
+  void *ptr = nullptr;
+  size_t cap = 0;
+  int grow(size_t bytes);
+  void release();
+};
+
+enum KernelTimer { KT_SIGMA2, KT_LANCZOS, KT_TRIDIAG, KT_QL, KT_DENSITY, KT_SIGINF, KT_FINAL, KT_WBUILD, KT_PLAN, KT_N };
+
+struct Batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bcd_solve_params par{};
+  std::vector<const HostSystem *> sys;
+  int n_inst = 0, n_gc = 0, nImax = 1;
+  long n_blk = 0, capmax = 0, mat_len = 0;
+  std::vector<InstDev> h_insts;
+  std::vector<ChanDev> h_chans;
+  std::vector<BlockDev> h_blocks;
+  std::vector<ChanState> h_cs;
+  std::vector<InstState> h_is;
+  std::vector<int> hf_its, sc_its;
+  std::vector<long> n_fw_tot, n_bk_tot;
+
+  DevArena a_tab_i, a_tab_l, a_tab_d, a_inst, a_chan, a_blocks, a_work, a_prop, a_rows, a_lws, a_mat, a_jobs;
+  InstDev *d_insts = nullptr;
+  InstState *d_is = nullptr;
+  ChanDev *d_chans = nullptr;
+  ChanState *d_cs = nullptr;
+  long *d_row_fw_base = nullptr, *d_row_bk_base = nullptr, *d_e_off = nullptr, *d_x_off = nullptr;
+  int *d_cap = nullptr;
+  BlockDev *d_blocks = nullptr;
+  int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_list = nullptr, *d_work_count = nullptr;
+  int *d_status = nullptr;
+  double *d_u1 = nullptr, *d_h = nullptr, *d_w = nullptr, *d_rho = nullptr, *d_tri = nullptr, *d_piv = nullptr;
+  double *d_lrows = nullptr, *d_kolt = nullptr, *d_rows_fw = nullptr, *d_rows_bk = nullptr;
+  long l_u1 = 0, l_h = 0, l_w = 0, l_rho = 0, l_tri = 0, l_piv = 0, l_lr = 0, l_kolt = 0, l_end = 0;
+  double *pe_[3] = {nullptr, nullptr, nullptr}, *px_[3] = {nullptr, nullptr, nullptr};
+  int *pnqp_[3] = {nullptr, nullptr, nullptr}, *pnqh_[3] = {nullptr, nullptr, nullptr};
+  long prop_e_len = 0, prop_x_len = 0, prop_n_len = 0;
+  DysonJob *d_djobs = nullptr;
+  LancJob *d_ljobs = nullptr;
+
+  hipEvent_t ev[6] = {};
+  double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+  bool profile = false;
+  std::pair<hipEvent_t, hipEvent_t> kev[KT_N] = {};
+  double ktime[KT_N] = {};
+  long kcount[KT_N] = {};
+
+  ~Batch();
+  std::vector<DevArena *> arenas();
+  // sigma_only: the batch serves stage calls (Sigma(2), Sigma^inf) and never
+  // runs a Dyson sweep, so no Dyson storage is sized for the poles.
+  bool sigma_only = false;
+  int setup(int dev, const std::vector<const HostSystem *> &systems, const bcd_solve_params &p,
+            const std::vector<HostProp> &g0, bool sigma_only_ = false);
+  PropView view(bool with_state) const;
+  size_t tridiag_lds(int n, bool in_lds) const;
+  void kt_begin(int k);
+  void kt_end(int k);
+  int sweep(int src, int &n_active);
+  int stage(int src, int itr_max, double conv, bool hf);
+  int build_sigma2(int only_ish);
+  int prepare();
+  int work(double *w);
+  double coupling_flops = -1.0;
+  int lanczos();
+  int solve();
+  int result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep, double *zp, double *fqp,
+             double *eh, double *zh, double *fqh);
+};
+
+}  // namespace bcd

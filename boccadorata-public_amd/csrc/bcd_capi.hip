@@ -1,0 +1,580 @@
+// bcd_capi.hip -- the extern "C" boundary (include/bcdor_amd.h).
+//
+// Every entry point validates its arguments on the host, then runs the work
+// on the GPU through the kernels of bcd_kernels.hip.  Without a device the
+// entry points return BCD_ERR_DEVICE; nothing falls back to the CPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "bcd_batch.hpp"
+#include "bcdor_amd.h"
+
+using namespace bcd;
+
+struct bcd_ctx {
+  int device = 0;
+  HostSystem sys;
+  hipStream_t stream = nullptr;
+  DevArena scratch;
+};
+
+struct bcd_batch {
+  std::vector<std::unique_ptr<HostSystem>> systems;
+  Batch b;
+};
+
+namespace {
+
+#define HTRY(x)                                  \
+  do {                                           \
+    hipError_t e_ = (x);                         \
+    if (e_ != hipSuccess) return BCD_ERR_DEVICE; \
+  } while (0)
+
+#define BTRY(x)                  \
+  do {                           \
+    int r_ = (x);                \
+    if (r_ != BCD_OK) return r_; \
+  } while (0)
+
+int device_ok(int dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || dev < 0 || dev >= n) return BCD_ERR_DEVICE;
+  return BCD_OK;
+}
+
+HostProp to_host_prop(const HostSystem &S, const bcd_prop *g) {
+  HostProp H;
+  H.nspb = g->nspb;
+  H.np.assign(g->np, g->np + S.nsubsh);
+  H.nh.assign(g->nh, g->nh + S.nsubsh);
+  long tp = 0, th = 0;
+  for (int s = 0; s < S.nsubsh; ++s) { tp += g->np[s]; th += g->nh[s]; }
+  H.ep.assign(g->ep, g->ep + tp);
+  H.eh.assign(g->eh, g->eh + th);
+  H.fqp.assign(g->fqp, g->fqp + tp * g->nspb);
+  H.fqh.assign(g->fqh, g->fqh + th * g->nspb);
+  return H;
+}
+
+// SpProp_t(ModSpace_t*, i_xe=1) (Sp_Propagator.cpp:292-378): holes at spe-10,
+// particles at spe, unit overlaps, orbit order.
+HostProp bare_prop(const HostSystem &S) {
+  HostProp H;
+  H.nspb = S.nspb;
+  H.np.assign(S.nsubsh, 0);
+  H.nh.assign(S.nsubsh, 0);
+  for (int s = 0; s < S.nsubsh; ++s)
+    for (int k = 0; k < S.no[s]; ++k) {
+      const int o = S.orb0[s] + k;
+      std::vector<double> x(S.nspb, 0.0);
+      if (S.orb_n[o] < S.no[s]) x[S.orb_n[o]] = 1.0;
+      if (S.orb_F[o] == 1) {
+        H.eh.push_back(S.orb_e[o] - 10.0);
+        H.fqh.insert(H.fqh.end(), x.begin(), x.end());
+        ++H.nh[s];
+      } else {
+        H.ep.push_back(S.orb_e[o]);
+        H.fqp.insert(H.fqp.end(), x.begin(), x.end());
+        ++H.np[s];
+      }
+    }
+  return H;
+}
+
+bool prop_ok(const HostSystem &S, const bcd_prop *g) {
+  if (!g || !g->np || !g->nh || g->nspb < S.nspb) return false;
+  for (int s = 0; s < S.nsubsh; ++s)
+    if (g->np[s] < 0 || g->nh[s] < 0) return false;
+  return true;
+}
+
+// A one-instance batch over the context's system, started from g.
+int stage_batch(bcd_ctx *ctx, const bcd_prop *g, int mode, Batch &b) {
+  std::vector<const HostSystem *> one{&ctx->sys};
+  bcd_solve_params p;
+  std::memset(&p, 0, sizeof p);
+  p.mode = mode;
+  p.hf_itr_max = 0;
+  p.sc_itr_max = 0;
+  p.conv_kind = 0;
+  p.hf_conv = p.sc_conv = 1e-7;
+  std::vector<HostProp> g0{to_host_prop(ctx->sys, g)};
+  BTRY(b.setup(ctx->device, one, p, g0, true));
+  return b.prepare();
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) hipFree(p); }
+  int alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 8) == hipSuccess ? BCD_OK : BCD_ERR_NOMEM; }
+};
+
+// One Lanczos job on device data.  Host-side checks follow Reduce_Mtx_Lncz
+// (Lanczos.cpp:50-82) and return its error codes instead of exiting.
+int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout, int *i_piv_out, int NA_in,
+                int NB_in, int NPLS_in, const double *d_Min, int nt_pvts, double *pivots, int *n_itr) {
+  if (NPLS_in < 1 || NPLS_out < 1) return -1;
+  if (NB_out != NB_in) return -100;
+  if (NB_in > kMaxNB || NB_in < 1 || nt_pvts > kMaxPiv || nt_pvts < 1) return BCD_ERR_ARG;
+  int nLanc = 0, nact = 0;
+  for (int i = 0; i < nt_pvts; ++i)
+    if (n_itr[i] > 0) { nLanc += n_itr[i]; ++nact; }
+  if (nLanc < 1) return -2;
+  if (nLanc > NPLS_in) nLanc = NPLS_in;
+  if (nLanc > NPLS_out) return -101;
+  if (nact + 1 > NA_out && nLanc < NPLS_in) return -102;
+  const int Tout = NA_out + NB_out;
+  DevBuf dpiv, ditr, dout, dipv, dn, dV, djob;
+  BTRY(dpiv.alloc(sizeof(double) * nt_pvts * NB_in));
+  BTRY(ditr.alloc(sizeof(int) * nt_pvts));
+  BTRY(dout.alloc(sizeof(double) * (size_t)Tout * nLanc));
+  BTRY(dipv.alloc(sizeof(int) * (nt_pvts + 2)));
+  BTRY(dn.alloc(sizeof(int)));
+  BTRY(dV.alloc(sizeof(double) * (size_t)nLanc * NPLS_in));
+  BTRY(djob.alloc(sizeof(LancJob)));
+  HTRY(hipMemcpy(dpiv.p, pivots, sizeof(double) * nt_pvts * NB_in, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(ditr.p, n_itr, sizeof(int) * nt_pvts, hipMemcpyHostToDevice));
+  LancJob J;
+  std::memset(&J, 0, sizeof J);
+  J.Min = d_Min; J.Tin = NA_in + NB_in; J.NA_in = NA_in; J.NB = NB_in; J.M = NPLS_in;
+  J.piv = (const double *)dpiv.p; J.nt_pvts = nt_pvts; J.nLanc = nLanc; J.itr = (const int *)ditr.p;
+  J.Mout = (double *)dout.p; J.Tout = Tout; J.NA_out = NA_out; J.i_piv_out = (int *)dipv.p;
+  J.n_out = (int *)dn.p; J.V = (double *)dV.p; J.inst = -1;
+  HTRY(hipMemcpy(djob.p, &J, sizeof J, hipMemcpyHostToDevice));
+  const LancJob *jp = (const LancJob *)djob.p;
+  if (NPLS_in <= 256 * 8)
+    hipLaunchKernelGGL((k_lanczos<256, 8>), dim3(1), dim3(256), 0, ctx->stream, jp, 1);
+  else if (NPLS_in <= 256 * 24)
+    hipLaunchKernelGGL((k_lanczos<256, 24>), dim3(1), dim3(256), 0, ctx->stream, jp, 1);
+  else if (NPLS_in <= 512 * 24)
+    hipLaunchKernelGGL((k_lanczos<512, 24>), dim3(1), dim3(512), 0, ctx->stream, jp, 1);
+  else if (NPLS_in <= 1024 * 24)
+    hipLaunchKernelGGL((k_lanczos<1024, 24>), dim3(1), dim3(1024), 0, ctx->stream, jp, 1);
+  else
+    return BCD_ERR_CAPACITY;
+  HTRY(hipGetLastError());
+  HTRY(hipStreamSynchronize(ctx->stream));
+  int nout = 0;
+  HTRY(hipMemcpy(&nout, dn.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (nout < 0) return nout;
+  HTRY(hipMemcpy(Mout, dout.p, sizeof(double) * (size_t)Tout * nLanc, hipMemcpyDeviceToHost));
+  int npv = 0;
+  HTRY(hipMemcpy(&npv, dipv.p, sizeof(int), hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(i_piv_out, dipv.p, sizeof(int) * (npv + 2), hipMemcpyDeviceToHost));
+  // the reference normalises every pivot it starts from in place
+  int ip = -1;
+  for (int k = 0; k < npv; ++k) {
+    ++ip;
+    while (ip < nt_pvts && n_itr[ip] < 1) ++ip;
+    double x1 = 0.0;
+    for (int l = 0; l < NB_in; ++l) x1 += pivots[ip * NB_in + l] * pivots[ip * NB_in + l];
+    x1 = std::sqrt(x1);
+    for (int l = 0; l < NB_in; ++l) pivots[ip * NB_in + l] /= x1;
+  }
+  return nout;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcd_version(void) { return 1; }
+
+const char *bcd_error_string(int code) {
+  switch (code) {
+    case BCD_OK: return "ok";
+    case BCD_ERR_ARG: return "invalid argument";
+    case BCD_ERR_DEVICE: return "no usable GPU or a HIP call failed";
+    case BCD_ERR_CAPACITY: return "output capacity or a kernel limit exceeded";
+    case BCD_ERR_LANCZOS: return "Lanczos reduction failed";
+    case BCD_ERR_EIGEN: return "eigensolver did not converge";
+    case BCD_ERR_NOMEM: return "device allocation failed";
+    default: return code < 0 ? "Reduce_Mtx_Lncz error code" : "unknown";
+  }
+}
+
+int bcd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int bcd_ctx_create(int device, const bcd_modspace *ms, const bcd_vpp *vpp, bcd_ctx **out) {
+  if (!out || !ms) return BCD_ERR_ARG;
+  *out = nullptr;
+  BTRY(device_ok(device));
+  std::unique_ptr<bcd_ctx> c(new (std::nothrow) bcd_ctx);
+  if (!c) return BCD_ERR_NOMEM;
+  c->device = device;
+  BTRY(c->sys.build(ms, vpp));
+  HTRY(hipSetDevice(device));
+  HTRY(hipStreamCreate(&c->stream));
+  *out = c.release();
+  return BCD_OK;
+}
+
+void bcd_ctx_destroy(bcd_ctx *ctx) {
+  if (!ctx) return;
+  ctx->scratch.release();
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int bcd_count_poles(bcd_ctx *ctx, int ish, const bcd_prop *g, int *nfw, int *nbk) {
+  if (!ctx || !nfw || !nbk || ish < 0 || ish >= ctx->sys.nsubsh || !prop_ok(ctx->sys, g)) return BCD_ERR_ARG;
+  Batch b;
+  BTRY(stage_batch(ctx, g, 1, b));
+  PropView P = b.view(true);
+  HTRY(hipMemsetAsync(b.d_work_count, 0, sizeof(int), b.stream));
+  hipLaunchKernelGGL(k_sigma2_plan, dim3(b.n_gc), dim3(64), 0, b.stream, b.d_chans, b.d_blocks, 0, b.n_gc, P,
+                     b.d_insts, b.d_cs, b.d_blk_fw_off, b.d_blk_bk_off, b.d_work_list, b.d_work_count, ish);
+  HTRY(hipGetLastError());
+  ChanState cs;
+  HTRY(hipMemcpyAsync(&cs, b.d_cs + ish, sizeof cs, hipMemcpyDeviceToHost, b.stream));
+  HTRY(hipStreamSynchronize(b.stream));
+  *nfw = cs.mfw;
+  *nbk = cs.mbk;
+  return BCD_OK;
+}
+
+int bcd_sigma2_build(bcd_ctx *ctx, int ish, const bcd_prop *g, double *rows_fw, int cap_fw, double *rows_bk,
+                     int cap_bk, int *nfw, int *nbk) {
+  if (!ctx || !nfw || !nbk || ish < 0 || ish >= ctx->sys.nsubsh || !prop_ok(ctx->sys, g)) return BCD_ERR_ARG;
+  Batch b;
+  BTRY(stage_batch(ctx, g, 1, b));
+  BTRY(b.build_sigma2(ish));
+  HTRY(hipStreamSynchronize(b.stream));
+  const ChanState &cs = b.h_cs[ish];
+  *nfw = cs.mfw;
+  *nbk = cs.mbk;
+  if (cs.mfw > cap_fw || cs.mbk > cap_bk) return BCD_ERR_CAPACITY;
+  const int w = 2 + ctx->sys.no[ish];
+  long bf = 0, bb = 0;
+  HTRY(hipMemcpy(&bf, b.d_row_fw_base + ish, sizeof(long), hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(&bb, b.d_row_bk_base + ish, sizeof(long), hipMemcpyDeviceToHost));
+  if (cs.mfw && rows_fw) HTRY(hipMemcpy(rows_fw, b.d_rows_fw + bf, sizeof(double) * cs.mfw * w, hipMemcpyDeviceToHost));
+  if (cs.mbk && rows_bk) HTRY(hipMemcpy(rows_bk, b.d_rows_bk + bb, sizeof(double) * cs.mbk * w, hipMemcpyDeviceToHost));
+  return BCD_OK;
+}
+
+int bcd_hf_pot(bcd_ctx *ctx, int ish, const bcd_prop *g, double *u, int ndim) {
+  if (!ctx || !u || ish < 0 || ish >= ctx->sys.nsubsh || ndim < 1 || !prop_ok(ctx->sys, g)) return BCD_ERR_ARG;
+  Batch b;
+  BTRY(stage_batch(ctx, g, 0, b));
+  PropView P = b.view(true);
+  hipLaunchKernelGGL(k_wbuild, dim3(1), dim3(256), 0, b.stream, b.d_insts, b.d_chans, ish, ish + 1, b.d_w);
+  hipLaunchKernelGGL(k_density, dim3(b.n_gc), dim3(64), 0, b.stream, b.d_insts, b.d_chans, 0, b.n_gc, P, b.d_rho);
+  hipLaunchKernelGGL(k_sigma_inf, dim3(1), dim3(256), 0, b.stream, b.d_insts, b.d_chans, ish, ish + 1, b.d_is,
+                     b.d_w, b.d_rho, b.d_h);
+  HTRY(hipGetLastError());
+  const int n = ctx->sys.no[ish];
+  std::vector<double> h(n * n);
+  HTRY(hipMemcpyAsync(h.data(), b.d_h + b.h_chans[ish].h_off, sizeof(double) * n * n, hipMemcpyDeviceToHost,
+                      b.stream));
+  HTRY(hipStreamSynchronize(b.stream));
+  for (int r = 0; r < n && r < ndim; ++r)
+    for (int c = 0; c < n && c < ndim; ++c) u[r * ndim + c] = h[r * n + c];
+  return n > ndim ? n - ndim : BCD_OK;
+}
+
+int bcd_reduce_mtx_lncz(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout, int *i_piv_out,
+                        int NA_in, int NB_in, int NPLS_in, const double *Min, int nt_pvts, double *pivots,
+                        int *n_itr) {
+  if (!ctx || !Mout || !i_piv_out || !Min || !pivots || !n_itr) return BCD_ERR_ARG;
+  if (NPLS_in < 1 || NPLS_out < 1) return -1;
+  HTRY(hipSetDevice(ctx->device));
+  DevBuf dmin;
+  BTRY(dmin.alloc(sizeof(double) * (size_t)NPLS_in * (NA_in + NB_in)));
+  HTRY(hipMemcpy(dmin.p, Min, sizeof(double) * (size_t)NPLS_in * (NA_in + NB_in), hipMemcpyHostToDevice));
+  return run_lanczos(ctx, NA_out, NB_out, NPLS_out, Mout, i_piv_out, NA_in, NB_in, NPLS_in,
+                     (const double *)dmin.p, nt_pvts, pivots, n_itr);
+}
+
+int bcd_reduce_selfen_lncz(bcd_ctx *ctx, int n_orb, int nfw, double *rows_fw, int nbk, double *rows_bk,
+                           int nt_pvts, double *pivots, int *n_itr_fw, int *n_itr_bk, double *out_fw,
+                           int *i_piv_fw, int *nfw_out, double *out_bk, int *i_piv_bk, int *nbk_out,
+                           int *ntot_den) {
+  if (!ctx || n_orb < 1 || nfw < 0 || nbk < 0 || !pivots || !n_itr_fw || !n_itr_bk || !nfw_out || !nbk_out ||
+      !ntot_den || !i_piv_fw || !i_piv_bk)
+    return BCD_ERR_ARG;
+  HTRY(hipSetDevice(ctx->device));
+  const int T = 2 + n_orb, nden = std::max(2, nt_pvts + 1), dim = nden + n_orb;
+  for (int k = 0; k < dim; ++k) { i_piv_fw[k] = -100; i_piv_bk[k] = -100; }
+  *ntot_den = nden;
+  *nfw_out = *nbk_out = 0;
+  // sort each side on the device (fw ascending, bk descending, as the
+  // reference does before the reduction, Self_energy_ext_Op_e.cpp:119-130)
+  DevBuf dfw_in, dfw, dbk_in, dbk;
+  BTRY(dfw_in.alloc(sizeof(double) * (size_t)std::max(nfw, 1) * T));
+  BTRY(dfw.alloc(sizeof(double) * (size_t)std::max(nfw, 1) * T));
+  BTRY(dbk_in.alloc(sizeof(double) * (size_t)std::max(nbk, 1) * T));
+  BTRY(dbk.alloc(sizeof(double) * (size_t)std::max(nbk, 1) * T));
+  if (nfw) {
+    HTRY(hipMemcpy(dfw_in.p, rows_fw, sizeof(double) * (size_t)nfw * T, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_rank_sort, dim3((nfw + 255) / 256), dim3(256), 0, ctx->stream, (const double *)dfw_in.p,
+                       nfw, T, 0, 0, (double *)dfw.p);
+  }
+  if (nbk) {
+    HTRY(hipMemcpy(dbk_in.p, rows_bk, sizeof(double) * (size_t)nbk * T, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_rank_sort, dim3((nbk + 255) / 256), dim3(256), 0, ctx->stream, (const double *)dbk_in.p,
+                       nbk, T, 0, 1, (double *)dbk.p);
+  }
+  HTRY(hipGetLastError());
+  HTRY(hipStreamSynchronize(ctx->stream));
+  if (nfw) HTRY(hipMemcpy(rows_fw, dfw.p, sizeof(double) * (size_t)nfw * T, hipMemcpyDeviceToHost));
+  if (nbk) HTRY(hipMemcpy(rows_bk, dbk.p, sizeof(double) * (size_t)nbk * T, hipMemcpyDeviceToHost));
+  int tot[2] = {0, 0};
+  for (int k = 0; k < nt_pvts; ++k) {
+    if (n_itr_fw[k] > 0) tot[0] += n_itr_fw[k];
+    if (n_itr_bk[k] > 0) tot[1] += n_itr_bk[k];
+  }
+  for (int side = 0; side < 2; ++side) {
+    const int M = side ? nbk : nfw;
+    int *itr = side ? n_itr_bk : n_itr_fw;
+    const int nl = std::min(tot[side], M);
+    if (M < 1 || nl < 1) continue;
+    if (tot[side] > M) {
+      const double x1 = (double)M / (double)tot[side];
+      for (int k = 0; k < nt_pvts; ++k) itr[k] = (int)(x1 * itr[k] + 1.0);
+    }
+    const int r = run_lanczos(ctx, nden, n_orb, std::max(1, nl), side ? out_bk : out_fw, side ? i_piv_bk : i_piv_fw,
+                              2, n_orb, M, (const double *)(side ? dbk.p : dfw.p), nt_pvts, pivots, itr);
+    if (r < 0) return r;
+    (side ? *nbk_out : *nfw_out) = r;
+  }
+  return BCD_OK;
+}
+
+int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1, int ntot_den_fw, int nfw,
+              const double *rows_fw, const int *i_piv_fw, int npiv_fw, int ntot_den_bk, int nbk,
+              const double *rows_bk, const int *i_piv_bk, int npiv_bk, double efermi, double *sol_qp, int *nqp,
+              double *sol_qh, int *nqh, double *kolt3) {
+  if (!ctx || n_orb < 1 || n_orb > kMaxNI || !h || !u1 || !sol_qp || !sol_qh || !nqp || !nqh) return BCD_ERR_ARG;
+  nfw = std::max(nfw, 0);
+  nbk = std::max(nbk, 0);
+  if ((nfw && !rows_fw) || (nbk && !rows_bk)) return BCD_ERR_ARG;
+  if ((npiv_fw > 0 && !i_piv_fw) || (npiv_bk > 0 && !i_piv_bk) || npiv_fw > kMaxPiv || npiv_bk > kMaxPiv)
+    return BCD_ERR_ARG;
+  HTRY(hipSetDevice(ctx->device));
+  const int n = n_orb + nfw + nbk, Tf = ntot_den_fw + n_orb, Tb = ntot_den_bk + n_orb;
+  const bool in_lds = ((size_t)n * n + (size_t)(2 + n_orb) * n + n_orb + 16) * sizeof(double) <= (size_t)kLdsBytes;
+  DevBuf dh, du, dfw, dbk, dpf, dpb, dtri, dmat, de, dx, dc, dk, dj, dst;
+  BTRY(dh.alloc(sizeof(double) * n_orb * n_orb));
+  BTRY(du.alloc(sizeof(double) * n_orb * n_orb));
+  BTRY(dfw.alloc(sizeof(double) * (size_t)std::max(nfw, 1) * Tf));
+  BTRY(dbk.alloc(sizeof(double) * (size_t)std::max(nbk, 1) * Tb));
+  BTRY(dpf.alloc(sizeof(int) * (kMaxPiv + 2)));
+  BTRY(dpb.alloc(sizeof(int) * (kMaxPiv + 2)));
+  BTRY(dtri.alloc(sizeof(double) * (size_t)n * (2 + n_orb)));
+  if (!in_lds) BTRY(dmat.alloc(sizeof(double) * (size_t)n * n));
+  BTRY(de.alloc(sizeof(double) * n));
+  BTRY(dx.alloc(sizeof(double) * (size_t)n * n_orb));
+  BTRY(dc.alloc(sizeof(int) * 2));
+  BTRY(dk.alloc(sizeof(double) * 3));
+  BTRY(dj.alloc(sizeof(DysonJob)));
+  BTRY(dst.alloc(sizeof(int)));
+  if ((size_t)(2 + n_orb) * n * sizeof(double) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
+  HTRY(hipMemcpy(dh.p, h, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(du.p, u1, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
+  if (nfw) HTRY(hipMemcpy(dfw.p, rows_fw, sizeof(double) * (size_t)nfw * Tf, hipMemcpyHostToDevice));
+  if (nbk) HTRY(hipMemcpy(dbk.p, rows_bk, sizeof(double) * (size_t)nbk * Tb, hipMemcpyHostToDevice));
+  int pf[kMaxPiv + 2], pb[kMaxPiv + 2];
+  for (int k = 0; k < kMaxPiv + 2; ++k) { pf[k] = -100; pb[k] = -100; }
+  if (npiv_fw > 0) std::copy(i_piv_fw, i_piv_fw + npiv_fw + 2, pf);
+  if (npiv_bk > 0) std::copy(i_piv_bk, i_piv_bk + npiv_bk + 2, pb);
+  HTRY(hipMemcpy(dpf.p, pf, sizeof pf, hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(dpb.p, pb, sizeof pb, hipMemcpyHostToDevice));
+  HTRY(hipMemset(dst.p, 0, sizeof(int)));
+  DysonJob J;
+  std::memset(&J, 0, sizeof J);
+  J.nI = n_orb; J.j2 = j2; J.nfw = nfw; J.nbk = nbk; J.den_fw = ntot_den_fw; J.den_bk = ntot_den_bk;
+  J.npiv_fw = npiv_fw > 0 ? npiv_fw : 0; J.npiv_bk = npiv_bk > 0 ? npiv_bk : 0;
+  J.n = n; J.inst = -1; J.ch = 0; J.efermi = efermi;
+  J.h = (const double *)dh.p; J.u1 = (const double *)du.p;
+  J.rfw = (const double *)dfw.p; J.rbk = (const double *)dbk.p;
+  J.ipiv_fw = (const int *)dpf.p; J.ipiv_bk = (const int *)dpb.p;
+  J.tri = (double *)dtri.p; J.mat = (double *)dmat.p;
+  J.out_e = (double *)de.p; J.out_x = (double *)dx.p;
+  J.out_nqp = (int *)dc.p; J.out_nqh = (int *)dc.p + 1; J.kolt = (double *)dk.p;
+  HTRY(hipMemcpy(dj.p, &J, sizeof J, hipMemcpyHostToDevice));
+  const DysonJob *jp = (const DysonJob *)dj.p;
+  const size_t aux = ((size_t)(2 + n_orb) * n + n_orb + 16) * sizeof(double);
+  HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  if (in_lds)
+    hipLaunchKernelGGL(k_tridiag<true>, dim3(1), dim3(256), aux + (size_t)n * n * sizeof(double), ctx->stream, jp, 1);
+  else
+    hipLaunchKernelGGL(k_tridiag<false>, dim3(1), dim3(256), aux, ctx->stream, jp, 1);
+  hipLaunchKernelGGL(k_ql, dim3(1), dim3(64), (size_t)(2 + n_orb) * n * sizeof(double), ctx->stream, jp, 1,
+                     (int *)dst.p);
+  HTRY(hipGetLastError());
+  HTRY(hipStreamSynchronize(ctx->stream));
+  int st = 0, cnt[2] = {0, 0};
+  HTRY(hipMemcpy(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (st) return BCD_ERR_EIGEN;
+  HTRY(hipMemcpy(cnt, dc.p, sizeof cnt, hipMemcpyDeviceToHost));
+  std::vector<double> e(n), x((size_t)n * n_orb);
+  HTRY(hipMemcpy(e.data(), de.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(x.data(), dx.p, sizeof(double) * (size_t)n * n_orb, hipMemcpyDeviceToHost));
+  if (kolt3) HTRY(hipMemcpy(kolt3, dk.p, sizeof(double) * 3, hipMemcpyDeviceToHost));
+  const int w = n_orb + 2;
+  for (int k = 0; k < n; ++k) {
+    double *rec = k < cnt[0] ? sol_qp + (size_t)k * w : sol_qh + (size_t)(k - cnt[0]) * w;
+    double z = 0.0;
+    for (int r = 0; r < n_orb; ++r) { rec[2 + r] = x[(size_t)k * n_orb + r]; z += rec[2 + r] * rec[2 + r]; }
+    rec[0] = e[k];
+    rec[1] = z;
+  }
+  *nqp = cnt[0];
+  *nqh = cnt[1];
+  return BCD_OK;
+}
+
+int bcd_syevd_batched(bcd_ctx *ctx, int nmat, const int *n, const long *off, double *a, const long *woff,
+                      double *w) {
+  if (!ctx || nmat < 0 || (nmat && (!n || !off || !a || !woff || !w))) return BCD_ERR_ARG;
+  HTRY(hipSetDevice(ctx->device));
+  for (int k = 0; k < nmat; ++k) {
+    const int m = n[k];
+    if (m < 1) continue;
+    // every row of the orthogonal factor is tracked (nI = m): the tracked rows
+    // and work vectors sit in LDS, the matrix in LDS or HBM
+    const size_t aux = ((size_t)(2 + m) * m + m + 16) * sizeof(double);
+    if (aux > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
+    const bool in_lds = aux + (size_t)m * m * sizeof(double) <= (size_t)kLdsBytes;
+    DevBuf dh, du, dtri, dmat, de, dx, dc, dk, dj, dst;
+    BTRY(dh.alloc(sizeof(double) * m * m));
+    BTRY(du.alloc(sizeof(double) * m * m));
+    BTRY(dtri.alloc(sizeof(double) * (size_t)m * (2 + m)));
+    if (!in_lds) BTRY(dmat.alloc(sizeof(double) * (size_t)m * m));
+    BTRY(de.alloc(sizeof(double) * m));
+    BTRY(dx.alloc(sizeof(double) * (size_t)m * m));
+    BTRY(dc.alloc(sizeof(int) * 2));
+    BTRY(dk.alloc(sizeof(double) * 3));
+    BTRY(dj.alloc(sizeof(DysonJob)));
+    BTRY(dst.alloc(sizeof(int)));
+    HTRY(hipMemcpy(dh.p, a + off[k], sizeof(double) * m * m, hipMemcpyHostToDevice));
+    HTRY(hipMemset(du.p, 0, sizeof(double) * m * m));
+    HTRY(hipMemset(dst.p, 0, sizeof(int)));
+    DysonJob J;
+    std::memset(&J, 0, sizeof J);
+    J.nI = m; J.n = m; J.inst = -1; J.den_fw = J.den_bk = 2; J.efermi = HUGE_VAL;
+    J.h = (const double *)dh.p; J.u1 = (const double *)du.p;
+    J.tri = (double *)dtri.p; J.mat = (double *)dmat.p;
+    J.out_e = (double *)de.p; J.out_x = (double *)dx.p;
+    J.out_nqp = (int *)dc.p; J.out_nqh = (int *)dc.p + 1; J.kolt = (double *)dk.p;
+    HTRY(hipMemcpy(dj.p, &J, sizeof J, hipMemcpyHostToDevice));
+    const DysonJob *jp = (const DysonJob *)dj.p;
+    HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+    HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+    HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+    if (in_lds)
+      hipLaunchKernelGGL(k_tridiag<true>, dim3(1), dim3(256), aux + (size_t)m * m * sizeof(double), ctx->stream, jp, 1);
+    else
+      hipLaunchKernelGGL(k_tridiag<false>, dim3(1), dim3(256), aux, ctx->stream, jp, 1);
+    hipLaunchKernelGGL(k_ql, dim3(1), dim3(64), (size_t)(2 + m) * m * sizeof(double), ctx->stream, jp, 1,
+                       (int *)dst.p);
+    HTRY(hipGetLastError());
+    HTRY(hipStreamSynchronize(ctx->stream));
+    int st = 0;
+    HTRY(hipMemcpy(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) return BCD_ERR_EIGEN;
+    std::vector<double> e(m), x((size_t)m * m);
+    HTRY(hipMemcpy(e.data(), de.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HTRY(hipMemcpy(x.data(), dx.p, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost));
+    // all values sit below the infinite split point, so the records come out
+    // by descending value: reverse to LAPACK's ascending order
+    for (int i = 0; i < m; ++i) {
+      w[woff[k] + i] = e[m - 1 - i];
+      for (int r = 0; r < m; ++r) a[off[k] + (size_t)r * m + i] = x[(size_t)(m - 1 - i) * m + r];
+    }
+  }
+  return BCD_OK;
+}
+
+// ------------------------------------------------------------- batched solve
+
+int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_vpp *vpp, const bcd_prop *g0,
+                     const bcd_solve_params *par, bcd_batch **out) {
+  if (!out || n_inst < 1 || !ms || !vpp || !par) return BCD_ERR_ARG;
+  *out = nullptr;
+  BTRY(device_ok(device));
+  std::unique_ptr<bcd_batch> B(new (std::nothrow) bcd_batch);
+  if (!B) return BCD_ERR_NOMEM;
+  std::vector<const HostSystem *> sp;
+  std::vector<HostProp> gp;
+  for (int i = 0; i < n_inst; ++i) {
+    B->systems.emplace_back(new HostSystem);
+    BTRY(B->systems.back()->build(ms + i, vpp + i));
+    sp.push_back(B->systems.back().get());
+    if (g0) {
+      if (!prop_ok(*sp.back(), g0 + i)) return BCD_ERR_ARG;
+      gp.push_back(to_host_prop(*sp.back(), g0 + i));
+    } else {
+      gp.push_back(bare_prop(*sp.back()));
+    }
+  }
+  BTRY(B->b.setup(device, sp, *par, gp));
+  *out = B.release();
+  return BCD_OK;
+}
+
+void bcd_batch_destroy(bcd_batch *b) { delete b; }
+
+int bcd_batch_solve(bcd_batch *b) {
+  if (!b) return BCD_ERR_ARG;
+  return b->b.solve();
+}
+
+int bcd_batch_set_profile(bcd_batch *b, int on) {
+  if (!b) return BCD_ERR_ARG;
+  b->b.profile = on != 0;
+  return BCD_OK;
+}
+
+int bcd_batch_timings(bcd_batch *b, double *ms, int n) {
+  if (!b || !ms || n < 6) return BCD_ERR_ARG;
+  for (int k = 0; k < 6; ++k) ms[k] = b->b.stage_ms[k];
+  return BCD_OK;
+}
+
+int bcd_batch_stats(bcd_batch *b, int inst, int *hf_its, int *sc_its, double *last_diff, double *e_kolt,
+                    long *n_fw_poles, long *n_bk_poles) {
+  if (!b || inst < 0 || inst >= b->b.n_inst) return BCD_ERR_ARG;
+  Batch &B = b->b;
+  if (hf_its) *hf_its = B.hf_its.empty() ? 0 : B.hf_its[inst];
+  if (sc_its) *sc_its = B.sc_its.empty() ? 0 : B.sc_its[inst];
+  if (last_diff) *last_diff = B.h_is[inst].diff;
+  if (e_kolt) *e_kolt = B.h_is[inst].e_kolt;
+  if (n_fw_poles) *n_fw_poles = B.n_fw_tot.empty() ? 0 : B.n_fw_tot[inst];
+  if (n_bk_poles) *n_bk_poles = B.n_bk_tot.empty() ? 0 : B.n_bk_tot[inst];
+  return BCD_OK;
+}
+
+int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep,
+                     double *zp, double *fqp, double *eh, double *zh, double *fqh) {
+  if (!b) return BCD_ERR_ARG;
+  return b->b.result(inst, nspb, np, nh, n_qp, n_qh, ep, zp, fqp, eh, zh, fqh);
+}
+
+int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches) {
+  if (!b || k < 0 || k >= KT_N || !avg_ms || !launches) return BCD_ERR_ARG;
+  *launches = b->b.kcount[k];
+  *avg_ms = b->b.kcount[k] ? b->b.ktime[k] / b->b.kcount[k] : 0.0;
+  return BCD_OK;
+}
+
+int bcd_batch_work(bcd_batch *b, double *work, int n) {
+  if (!b || !work || n < 5) return BCD_ERR_ARG;
+  return b->b.work(work);
+}
+
+}  // extern "C"

@@ -1,0 +1,912 @@
+// bcd_kernels.hip -- gfx950 kernels of the second-order SCGF hot path (FP64).
+//
+// Stage map (reference files under source/ of BoccaDorata v1.0):
+//   k_wbuild, k_density, k_sigma_inf  Sigma^inf     Sp_Propagator_a.cpp:39-130
+//   k_sigma2_plan, k_sigma2           Sigma(2)      Self_energy_ext_Op_c.cpp:35-113, 323-561
+//   k_lanczos_prep, k_lanczos         Lanczos       Self_energy_ext_Op_e.cpp:55-232, Lanczos.cpp:44-286
+//   k_dyson_prep, k_tridiag, k_ql     Dyson         DysonMatrix.cpp:123-532
+//   k_finalize                        sweep control Main.cpp:403-451, Sp_Propagator.cpp:1189-1213,
+//                                                   DysonMatrix.cpp:619-635
+#include "bcd_kernels.hpp"
+
+namespace bcd {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__device__ __forceinline__ int wave_isum(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// Sum nv values over the whole block; every thread gets the totals.  red
+// must hold (blockDim/64)*nv doubles of LDS.  Deterministic order.
+__device__ void block_sum(double *vals, int nv, double *red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int k = 0; k < nv; ++k) vals[k] = wave_sum(vals[k]);
+  if (lane == 0)
+    for (int k = 0; k < nv; ++k) red[w * nv + k] = vals[k];
+  __syncthreads();
+  for (int k = 0; k < nv; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += red[i * nv + k];
+    vals[k] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int prop_cur(const PropView &P, int inst) {
+  return P.is ? P.is[inst].cur : 0;
+}
+
+// <ab;J|V|cd;J> from the dense J-blocks, with the exchange phase
+// (-1)^(ja+jb-J+1) for each pair given in reverse order and 0 for pairs that
+// are not stored (VppInt_t::get, Vpp_potential.cpp:557-607).
+__device__ double vget(const InstDev &I, int a, int b, int c, int d, int J) {
+  int ph = 0;
+  if (J > I.jmax) return 0.0;
+  if (a > b) { int t = a; a = b; b = t; ph += (I.j2[I.orb_sh[a]] + I.j2[I.orb_sh[b]]) / 2 - J + 1; }
+  if (c > d) { int t = c; c = d; d = t; ph += (I.j2[I.orb_sh[c]] + I.j2[I.orb_sh[d]]) / 2 - J + 1; }
+  const long base = (long)J * I.norb;
+  const int pa = I.pidx[(base + a) * I.norb + b];
+  const int pc = I.pidx[(base + c) * I.norb + d];
+  if (pa < 0 || pc < 0) return 0.0;
+  const int ca = pa >> 16, cc = pc >> 16;
+  if (ca != cc) return 0.0;
+  const double v = I.V[I.voff[ca] + (long)(pa & 0xffff) * I.npair[ca] + (pc & 0xffff)];
+  return (abs(ph) & 1) ? -v : v;
+}
+
+__device__ __forceinline__ bool tri_ok(int a2, int b2, int c2) {
+  const int s1 = a2 + b2 - c2, s2 = b2 + c2 - a2, s3 = c2 + a2 - b2;
+  return s1 >= 0 && s2 >= 0 && s3 >= 0 && !(s1 & 1) && !(s2 & 1) && !(s3 & 1);
+}
+
+// ------------------------------------------------------------- Sigma^inf
+
+// W[rc][q] = sum_J (2J+1)/(2j+1) f_rm f_cu V^J(r m; c u), rc = r(r+1)/2+c with
+// r >= c, q = qoff[v] + u*no_v + m; f = sqrt2 on the channel's own orbit and
+// that term dropped unless (j + j_v + J) is odd (Sp_Propagator_a.cpp:62-117).
+__global__ void k_wbuild(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
+                         double *W) {
+  const int gc = gc_begin + blockIdx.x;
+  if (gc >= gc_end) return;
+  const ChanDev C = chans[gc];
+  const InstDev I = insts[C.inst];
+  const int nI = C.nI, nrc = nI * (nI + 1) / 2, nq = I.nq;
+  for (long idx = threadIdx.x; idx < (long)nrc * nq; idx += blockDim.x) {
+    const int rc = (int)(idx / nq), q = (int)(idx % nq);
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= rc) ++r;
+    const int c = rc - r * (r + 1) / 2;
+    int v = 0;
+    while (v + 1 < I.nsubsh && I.qoff[v + 1] <= q) ++v;
+    const int nv = I.no[v], um = q - I.qoff[v], u = um / nv, m = um % nv;
+    const bool self = (v == C.ish);
+    const int ia = C.orb0 + r, ib = C.orb0 + c, im = I.orb0[v] + m, iu = I.orb0[v] + u;
+    const int j2I = C.j2, j2v = I.j2[v];
+    double acc = 0.0;
+    for (int J2 = abs(j2I - j2v); J2 <= j2I + j2v; J2 += 2) {
+      const int nJ = ((j2I + j2v + J2) / 2) % 2;
+      if (self && m == r && nJ != 1) continue;
+      if (self && u == c && nJ != 1) continue;
+      const double x2 = (self && m == r) ? kSqrt2 : 1.0;
+      const double x3 = (self && u == c) ? kSqrt2 : 1.0;
+      const double x1 = (double)(J2 + 1) / (double)(j2I + 1);
+      acc += x1 * (x2 * x3 * vget(I, ia, im, ib, iu, J2 / 2));
+    }
+    W[C.w_off + idx] = acc;
+  }
+}
+
+// rho_v(u,m) = sum over the quasiholes k of subshell v of Y_k(u) Y_k(m), in
+// stored order (the x6 sum of Sp_Propagator_a.cpp:103-110).
+__global__ void k_density(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
+                          PropView P, double *rho) {
+  const int gc = gc_begin + blockIdx.x;
+  if (gc >= gc_end) return;
+  const ChanDev C = chans[gc];
+  if (P.is && !P.is[C.inst].active) return;
+  const InstDev I = insts[C.inst];
+  const int cur = prop_cur(P, C.inst), nI = C.nI;
+  const int nqp = P.nqp[cur][gc], nqh = P.nqh[cur][gc];
+  const double *x = P.x[cur] + P.x_off[gc];
+  double *out = rho + I.rho_off + I.qoff[C.ish];
+  for (int idx = threadIdx.x; idx < nI * nI; idx += blockDim.x) {
+    const int u = idx / nI, m = idx % nI;
+    double s = 0.0;
+    for (int k = 0; k < nqh; ++k) s += x[(long)(nqp + k) * nI + u] * x[(long)(nqp + k) * nI + m];
+    out[idx] = s;
+  }
+}
+
+// h(r,c) = h(c,r) = sum_q W[rc][q] rho[q]: Sigma^inf of every channel.
+__global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
+                            const InstState *is, const double *W, const double *rho, double *h) {
+  const int gc = gc_begin + blockIdx.x;
+  if (gc >= gc_end) return;
+  const ChanDev C = chans[gc];
+  if (is && !is[C.inst].active) return;
+  const InstDev I = insts[C.inst];
+  const int nI = C.nI, nq = I.nq, lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const double *r0 = rho + I.rho_off;
+  for (int rc = w; rc < nI * (nI + 1) / 2; rc += nw) {
+    const double *wr = W + C.w_off + (long)rc * nq;
+    double s = 0.0;
+    for (int q = lane; q < nq; q += 64) s += wr[q] * r0[q];
+    s = wave_sum(s);
+    if (lane == 0) {
+      int r = 0;
+      while ((r + 1) * (r + 2) / 2 <= rc) ++r;
+      const int c = rc - r * (r + 1) / 2;
+      h[C.h_off + r * nI + c] = s;
+      h[C.h_off + c * nI + r] = s;
+    }
+  }
+}
+
+// ------------------------------------------------------------- Sigma(2)
+
+// Pole counts of each (J,a,b,g) block from the fragment counts of the current
+// propagator (Count_2p1h_2h1p_poles, Self_energy_ext_Op_c.cpp:76-90), row
+// offsets in build order, and a compact list of the non-empty blocks.
+__global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
+                              PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
+                              int *blk_bk_off, int *work_list, int *work_count, int only_ish) {
+  const int gc = gc_begin + blockIdx.x;
+  if (gc >= gc_end) return;
+  const ChanDev C = chans[gc];
+  if (only_ish >= 0 && C.ish != only_ish) {
+    if (threadIdx.x == 0) { cs[gc].mfw = 0; cs[gc].mbk = 0; }
+    return;
+  }
+  const InstDev I = insts[C.inst];
+  const int cur = prop_cur(P, C.inst), lane = threadIdx.x;
+  int base_fw = 0, base_bk = 0;
+  for (int b0 = 0; b0 < C.blk_cnt; b0 += 64) {
+    const int b = b0 + lane;
+    int cfw = 0, cbk = 0;
+    if (b < C.blk_cnt) {
+      const BlockDev B = blocks[C.blk_off + b];
+      const int ga = I.gc0 + B.a, gb = I.gc0 + B.b, gg = I.gc0 + B.g;
+      int na = P.nqp[cur][ga], nb = P.nqp[cur][gb], ng = P.nqh[cur][gg];
+      cfw = na * nb * ng;
+      if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cfw = (na * nb / 2) * ng; }
+      na = P.nqh[cur][ga]; nb = P.nqh[cur][gb]; ng = P.nqp[cur][gg];
+      cbk = na * nb * ng;
+      if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cbk = (na * nb / 2) * ng; }
+    }
+    // inclusive wave scans
+    int sfw = cfw, sbk = cbk;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int tf = __shfl_up(sfw, o, 64), tb = __shfl_up(sbk, o, 64);
+      if (lane >= o) { sfw += tf; sbk += tb; }
+    }
+    if (b < C.blk_cnt) {
+      blk_fw_off[C.blk_off + b] = base_fw + sfw - cfw;
+      blk_bk_off[C.blk_off + b] = base_bk + sbk - cbk;
+      if (cfw + cbk > 0) work_list[atomicAdd(work_count, 1)] = (int)(C.blk_off + b);
+    }
+    base_fw += __shfl(sfw, 63, 64);
+    base_bk += __shfl(sbk, 63, 64);
+  }
+  if (lane == 0) {
+    cs[gc].mfw = base_fw;
+    cs[gc].mbk = base_bk;
+  }
+}
+
+// The coupling build of Build_Dynamic_SlfEn_2ndOrder (Self_energy_ext_Op_c.cpp:
+// 380-539): per block the interaction part Vt(d; alpha,beta,gamma) is gathered
+// into LDS once, then every lane builds whole pole rows [E, sum m^2, m(d)] with
+// the reference's loop and operation order (contraction off), so rows match
+// the reference to the last bit.  One wave per block, grid-stride over the
+// compact list of non-empty blocks.
+__global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanDev *chans,
+                                               const BlockDev *blocks, PropView P,
+                                               const int *blk_fw_off, const int *blk_bk_off,
+                                               const int *work_list, const int *work_count,
+                                               const long *row_fw_base, const long *row_bk_base,
+                                               double *rows_fw, double *rows_bk) {
+#pragma clang fp contract(off)
+  __shared__ double vt[kMaxVt];
+  __shared__ int pa_[kMaxPairs], pb_[kMaxPairs];
+  const int lane = threadIdx.x;
+  const int total = *work_count;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int bidx = work_list[t];
+    const BlockDev B = blocks[bidx];
+    const ChanDev C = chans[B.gc];
+    const InstDev I = insts[C.inst];
+    const int cur = prop_cur(P, C.inst);
+    const int nI = C.nI, a = B.a, b = B.b, g = B.g, J = B.J, nJab = B.nJab, nJgd = B.nJgd;
+    const int noa = I.no[a], nob = I.no[b], nog = I.no[g];
+    const int ga = I.gc0 + a, gb = I.gc0 + b, gg = I.gc0 + g;
+    // (alpha, beta) pairs in the reference order (beta >= alpha + nJab when a == b)
+    if (lane == 0) {
+      int np = 0;
+      for (int al = 0; al < noa; ++al)
+        for (int be = (a == b) ? al + nJab : 0; be < nob; ++be) { pa_[np] = al; pb_[np] = be; ++np; }
+      pa_[kMaxPairs - 1] = np;
+    }
+    __syncthreads();
+    const int npair = pa_[kMaxPairs - 1];
+    const int nvt = nI * npair * nog;
+    const double x3 = sqrt((double)(J + J + 1) / (double)(1 + C.j2));
+    for (int e = lane; e < nvt; e += 64) {
+      const int ga_ = e % nog, pi = (e / nog) % npair, d = e / (nog * npair);
+      const int od = C.orb0 + d, og = I.orb0[g] + ga_;
+      double val = 0.0;
+      if (!(og == od && nJgd)) {
+        val = x3 * vget(I, I.orb0[a] + pa_[pi], I.orb0[b] + pb_[pi], od, og, J);
+        if (og == od) val *= kSqrt2;
+      }
+      vt[e] = val;
+    }
+    __syncthreads();
+    const double xJab = (double)(1 - 2 * nJab);
+    for (int side = 0; side < 2; ++side) {
+      // side 0: 2p1h from qp(a), qp(b), qh(g); side 1: 2h1p from qh(a), qh(b), qp(g)
+      const int cA = side ? P.nqh[cur][ga] : P.nqp[cur][ga];
+      const int cB = side ? P.nqh[cur][gb] : P.nqp[cur][gb];
+      const int cG = side ? P.nqp[cur][gg] : P.nqh[cur][gg];
+      const int oA = side ? P.nqp[cur][ga] : 0, oB = side ? P.nqp[cur][gb] : 0;
+      const int oG = side ? 0 : P.nqp[cur][gg];
+      int npole;
+      if (a == b) { int nb2 = cA + 1; if (nJab == 1) nb2 -= 2; npole = (cA * nb2 / 2) * cG; }
+      else npole = cA * cB * cG;
+      if (npole <= 0) continue;
+      const double *eA = P.e[cur] + P.e_off[ga] + oA, *eB = P.e[cur] + P.e_off[gb] + oB;
+      const double *eG = P.e[cur] + P.e_off[gg] + oG;
+      const double *xA = P.x[cur] + P.x_off[ga] + (long)oA * noa;
+      const double *xB = P.x[cur] + P.x_off[gb] + (long)oB * nob;
+      const double *xG = P.x[cur] + P.x_off[gg] + (long)oG * nog;
+      const int row_off = side ? blk_bk_off[bidx] : blk_fw_off[bidx];
+      double *rows = side ? rows_bk + row_bk_base[B.gc] : rows_fw + row_fw_base[B.gc];
+      const int w = 2 + nI;
+      for (int p = lane; p < npole; p += 64) {
+        int na, nb, ng = p % cG, q = p / cG;
+        if (a == b) {
+          na = 0;
+          int len = cA - nJab;
+          while (q >= len) { q -= len; ++na; --len; }
+          nb = na + nJab + q;
+        } else {
+          na = q / cB; nb = q % cB;
+        }
+        const double *ya = xA + (long)na * noa, *yb = xB + (long)nb * nob, *yg = xG + (long)ng * nog;
+        double *row = rows + (long)(row_off + p) * w;
+        double x4 = 1.0;
+        if (a == b && na == nb) x4 /= kSqrt2;
+        row[0] = eA[na] + eB[nb] - eG[ng];
+        double sum2 = 0.0;
+        int ie = 0;
+        for (int d = 0; d < nI; ++d) {
+          const int od = C.orb0 + d;
+          double x1 = 0.0;
+          for (int pi = 0; pi < npair; ++pi) {
+            const int al = pa_[pi], be = pb_[pi];
+            double x2 = ya[al] * yb[be];
+            if (a == b) {
+              x2 += xJab * ya[be] * yb[al];
+              if (al == be) x2 /= kSqrt2;
+            }
+            for (int gm = 0; gm < nog; ++gm, ++ie) {
+              if (I.orb0[g] + gm == od && nJgd) continue;
+              x1 += vt[ie] * x2 * yg[gm];
+            }
+          }
+          x1 *= x4;
+          row[2 + d] = x1;
+          sum2 += x1 * x1;
+        }
+        row[1] = sum2;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- Lanczos
+
+// Pivots and iteration counts of every channel from the current (HF)
+// propagator: quasiholes then quasiparticles by ascending energy with Z >= 60%
+// (SpProp_t::DysPivots, Sp_Propagator.cpp:1225-1289), lanc_n iterations per
+// pivot (kind 0) or lanc_n vectors per side split over the pivots (kind 1),
+// rescaled as Reduce_SelfEn_Lncz does when they exceed the pole count
+// (Self_energy_ext_Op_e.cpp:147-220).  Writes one LancJob per side.
+__global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P,
+                               ChanState *cs, double *piv, int lanc_kind, int lanc_n,
+                               const long *row_fw_base, const long *row_bk_base, double *rows_fw,
+                               double *rows_bk, double *lrows, double *lws, LancJob *jobs) {
+  const int gc = gc_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= gc_end) return;
+  const ChanDev C = chans[gc];
+  ChanState &S = cs[gc];
+  const int cur = prop_cur(P, C.inst), nI = C.nI;
+  const int nqp = P.nqp[cur][gc], nqh = P.nqh[cur][gc];
+  const double *e = P.e[cur] + P.e_off[gc];
+  const double *x = P.x[cur] + P.x_off[gc];
+  double *pv = piv + C.piv_off;
+  int npv = 0;
+  for (int side = 0; side < 2 && npv < kMaxPiv; ++side) {
+    const int cnt = side ? nqp : nqh, off = side ? 0 : nqp;
+    double x2 = -1.0e10;
+    int pick = cnt > 0 ? 1 : -100;
+    while (pick >= 0 && npv < kMaxPiv) {
+      double x1 = 1.0e10;
+      pick = -100;
+      for (int j = 0; j < cnt; ++j) {
+        const double ej = e[off + j];
+        double z = 0.0;
+        for (int k = 0; k < nI; ++k) z += x[(long)(off + j) * nI + k] * x[(long)(off + j) * nI + k];
+        if (ej < x1 && ej > x2 && 100. * z >= 60.0) { x1 = ej; pick = j; }
+      }
+      if (pick >= 0) {
+        x2 = x1;
+        for (int k = 0; k < nI; ++k) pv[npv * nI + k] = x[(long)(off + pick) * nI + k];
+        ++npv;
+      }
+    }
+  }
+  S.npiv_tot = npv;
+  S.den = npv + 1 > 2 ? npv + 1 : 2;
+  S.err = 0;
+  for (int side = 0; side < 2; ++side) {
+    int *itr = side ? S.itr_bk : S.itr_fw;
+    int *ipv = side ? S.ipiv_bk : S.ipiv_fw;
+    for (int k = 0; k < kMaxPiv + 2; ++k) ipv[k] = -100;
+    for (int k = 0; k < npv; ++k) itr[k] = lanc_kind == 0 ? lanc_n : lanc_n / npv + (k < lanc_n % npv ? 1 : 0);
+    const int M = side ? S.mbk : S.mfw;
+    int tot = 0;
+    for (int k = 0; k < npv; ++k) if (itr[k] > 0) tot += itr[k];
+    int nl = tot < M ? tot : M;
+    LancJob &Jb = jobs[2 * gc + side];
+    Jb.nLanc = 0;
+    Jb.inst = -1;
+    int *lout = side ? &S.lbk : &S.lfw;
+    *lout = 0;
+    if (M < 1 || nl < 1) { ipv[0] = 0; continue; }
+    if (tot > M) {
+      const double xs = (double)M / (double)tot;
+      for (int k = 0; k < npv; ++k) itr[k] = (int)(xs * itr[k] + 1.0);
+    }
+    int nLanc = 0;
+    for (int k = 0; k < npv; ++k) if (itr[k] > 0) nLanc += itr[k];
+    if (nLanc > M) nLanc = M;
+    const int cap = side ? C.lbk_cap : C.lfw_cap;
+    if (nLanc > (nl > 1 ? nl : 1) || nLanc > cap) { S.err = 1; continue; }
+    Jb.Min = side ? rows_bk + row_bk_base[gc] : rows_fw + row_fw_base[gc];
+    Jb.Tin = 2 + nI; Jb.NA_in = 2; Jb.NB = nI; Jb.M = M;
+    Jb.piv = pv; Jb.nt_pvts = npv; Jb.nLanc = nLanc; Jb.itr = itr;
+    Jb.Mout = lrows + (side ? C.lbk_off : C.lfw_off);
+    Jb.Tout = S.den + nI; Jb.NA_out = S.den;
+    Jb.i_piv_out = ipv; Jb.n_out = lout;
+    Jb.V = lws + (side ? C.lws_bk_off : C.lws_fw_off);
+  }
+}
+
+// Multi-pivot Lanczos on diag(E) with full reorthogonalisation in the
+// reference's order (Reduce_Mtx_Lncz, Lanczos.cpp:101-241): the current vector
+// lives in registers (PER elements per thread), earlier vectors in HBM; each
+// Gram-Schmidt projection (latest vector first) is one block-wide reduction.
+template <int NT, int PER>
+__global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, int njobs) {
+  __shared__ double red[(NT / 64) * kLancRed];
+  __shared__ double u[kMaxNB];
+  __shared__ int last[kMaxPiv + 1];
+  if ((int)blockIdx.x >= njobs) return;
+  const LancJob Jb = jobs[blockIdx.x];
+  if (Jb.nLanc <= 0) return;
+  const int tid = threadIdx.x, M = Jb.M, NB = Jb.NB, Tin = Jb.Tin, NA = Jb.NA_in;
+  const int Tout = Jb.Tout, NAo = Jb.NA_out, nLanc = Jb.nLanc;
+  const double *Min = Jb.Min;
+  for (long i = tid; i < (long)Tout * nLanc; i += NT) Jb.Mout[i] = 0.0;
+  double v[PER], vi[PER];
+  int i_pvt = -1, n_pvt = -1, j_next = 0;
+  int status = 0;
+  for (int j = 0; j < nLanc; ++j) {
+    const bool fresh = (j == j_next);
+    if (fresh) {
+      ++i_pvt;
+      while (i_pvt < Jb.nt_pvts && Jb.itr[i_pvt] < 1) ++i_pvt;
+      if (i_pvt >= Jb.nt_pvts) { status = -100; break; }
+      ++n_pvt;
+      j_next += Jb.itr[i_pvt];
+      if (tid == 0) {
+        last[n_pvt] = j_next - 1;
+        double x1 = 0.0;
+        for (int l = 0; l < NB; ++l) x1 += Jb.piv[i_pvt * NB + l] * Jb.piv[i_pvt * NB + l];
+        x1 = sqrt(x1);
+        for (int l = 0; l < NB; ++l) u[l] = Jb.piv[i_pvt * NB + l] / x1;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < PER; ++t) {
+        const int p = tid + NT * t;
+        double acc = 0.0;
+        if (p < M)
+          for (int l = 0; l < NB; ++l) acc += Min[(long)p * Tin + NA + l] * u[l];
+        v[t] = acc;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < PER; ++t) {
+        const int p = tid + NT * t;
+        v[t] = p < M ? v[t] * Min[(long)p * Tin] : 0.0;
+      }
+    }
+    // Gram-Schmidt against every earlier vector, latest first (Lanczos.cpp:164-173)
+    for (int i = j - 1; i >= 0; --i) {
+      const double *Vi = Jb.V + (long)i * M;
+      double s[2] = {0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < PER; ++t) {
+        const int p = tid + NT * t;
+        vi[t] = p < M ? Vi[p] : 0.0;
+        s[0] += vi[t] * v[t];
+        s[1] += vi[t] * vi[t];
+      }
+      block_sum(s, 2, red);
+#pragma unroll
+      for (int t = 0; t < PER; ++t) v[t] -= s[0] * vi[t] / s[1];
+    }
+    // a, b, g_i and the couplings m^L (Lanczos.cpp:181-216)
+    double s[kLancRed];
+    const int nred = 2 + NB + n_pvt;
+    for (int k = 0; k < nred; ++k) s[k] = 0.0;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int p = tid + NT * t;
+      if (p < M) {
+        const double E = Min[(long)p * Tin];
+        s[0] += E * v[t] * v[t];
+        s[1] += v[t] * v[t];
+        for (int l = 0; l < NB; ++l) s[2 + l] += Min[(long)p * Tin + NA + l] * v[t];
+      }
+    }
+    for (int i = 0; i < n_pvt; ++i) {
+      const double *Vl = Jb.V + (long)last[i] * M;
+#pragma unroll
+      for (int t = 0; t < PER; ++t) {
+        const int p = tid + NT * t;
+        if (p < M) s[2 + NB + i] += Min[(long)p * Tin] * Vl[p] * v[t];
+      }
+    }
+    block_sum(s, nred, red);
+    const double xa = s[0] / s[1], xb = sqrt(s[1]);
+    double *row = Jb.Mout + (long)j * Tout;
+    if (tid == 0) {
+      row[0] = xa;
+      row[1] = xb;
+      for (int l = 0; l < NB; ++l) row[NAo + l] = s[2 + l] / xb;
+      for (int i = 0; i < n_pvt; ++i) row[2 + i] = s[2 + NB + i] / xb;
+      if (fresh) row[1] = row[1 + n_pvt];
+    }
+    if (n_pvt + 2 + NB > Tout) { status = -200; break; }
+    double *Vj = Jb.V + (long)j * M;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int p = tid + NT * t;
+      v[t] /= xb;
+      if (p < M) Vj[p] = v[t];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (status == 0) {
+      ++n_pvt;
+      Jb.i_piv_out[0] = n_pvt;
+      Jb.i_piv_out[1] = 1;
+      for (int i = 0; i < n_pvt; ++i) Jb.i_piv_out[2 + i] = last[i] + 1;
+      if (last[n_pvt - 1] + 1 != nLanc && !(last[n_pvt - 1] + 1 > nLanc && M == nLanc)) status = -201;
+    }
+    *Jb.n_out = status == 0 ? nLanc : status;
+  }
+}
+
+template __global__ void k_lanczos<256, 8>(const LancJob *, int);
+template __global__ void k_lanczos<256, 24>(const LancJob *, int);
+template __global__ void k_lanczos<512, 24>(const LancJob *, int);
+template __global__ void k_lanczos<1024, 24>(const LancJob *, int);
+
+// ------------------------------------------------------------- Dyson
+
+// Fill the Dyson job of every channel for this sweep: static part, pole rows
+// (none, unreduced or Lanczos-reduced), Fermi energy of the sweep's input
+// propagator and the output slots of the other propagator buffer.
+__global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, const ChanState *cs,
+                             PropView P, int src, const double *h, const double *u1,
+                             const long *row_fw_base, const long *row_bk_base, const double *rows_fw,
+                             const double *rows_bk, const double *lrows, double *tri, double *mat,
+                             double *kolt, DysonJob *jobs) {
+  const int gc = gc_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= gc_end) return;
+  const ChanDev C = chans[gc];
+  DysonJob &J = jobs[gc - gc_begin];
+  const InstState *S = P.is ? P.is + C.inst : nullptr;
+  if (S && !S->active) { J.n = 0; return; }
+  const ChanState &Q = cs[gc];
+  const int cur = S ? S->cur : 0, nxt = 1 - cur, nI = C.nI;
+  J.nI = nI; J.j2 = C.j2; J.ch = C.ch; J.inst = C.inst;
+  J.h = h + C.h_off; J.u1 = u1 + C.u1_off;
+  if (src == 0) {
+    J.nfw = J.nbk = 0; J.den_fw = J.den_bk = 2; J.npiv_fw = J.npiv_bk = 0;
+    J.rfw = J.rbk = nullptr; J.ipiv_fw = J.ipiv_bk = nullptr;
+  } else if (src == 1) {
+    J.nfw = Q.mfw; J.nbk = Q.mbk; J.den_fw = J.den_bk = 2; J.npiv_fw = J.npiv_bk = 0;
+    J.rfw = rows_fw + row_fw_base[gc]; J.rbk = rows_bk + row_bk_base[gc];
+    J.ipiv_fw = J.ipiv_bk = nullptr;
+  } else {
+    J.nfw = Q.lfw > 0 ? Q.lfw : 0; J.nbk = Q.lbk > 0 ? Q.lbk : 0;
+    J.den_fw = J.den_bk = Q.den;
+    J.npiv_fw = J.nfw > 0 ? Q.ipiv_fw[0] : 0; J.npiv_bk = J.nbk > 0 ? Q.ipiv_bk[0] : 0;
+    J.rfw = lrows + C.lfw_off; J.rbk = lrows + C.lbk_off;
+    J.ipiv_fw = Q.ipiv_fw; J.ipiv_bk = Q.ipiv_bk;
+  }
+  J.n = nI + J.nfw + J.nbk;
+  if (J.n > C.cap) { J.n = 0; if (P.is) P.is[C.inst].err = 2; return; }
+  J.efermi = S ? S->ef[C.ch & 1] : 0.0;
+  J.tri = tri + C.tri_off;
+  J.mat = mat ? mat + C.mat_off : nullptr;
+  J.out_e = P.e[nxt] + P.e_off[gc];
+  J.out_x = P.x[nxt] + P.x_off[gc];
+  J.out_nqp = P.nqp[nxt] + gc;
+  J.out_nqh = P.nqh[nxt] + gc;
+  J.kolt = kolt + 3L * gc;
+}
+
+// Assemble the Dyson matrix (DysonMatrix.cpp:269-390) and reduce it to
+// tridiagonal form with Householder reflections (Golub & Van Loan, Alg.
+// 8.3.1), tracking only the first nI rows of the orthogonal factor, which are
+// all the quasiparticle amplitudes need.  Four waves per matrix; the matrix
+// sits in LDS (IN_LDS) or in a global scratch slab.
+template <bool IN_LDS>
+__global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs) {
+  extern __shared__ double sm[];
+  if ((int)blockIdx.x >= njobs) return;
+  const DysonJob J = jobs[blockIdx.x];
+  const int n = J.n;
+  if (n <= 0) return;
+  const int nI = J.nI, tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+  double *A = IN_LDS ? sm : J.mat;
+  double *aux = IN_LDS ? sm + (long)n * n : sm;
+  double *v = aux, *p = v + n, *Qh = p + n, *qv = Qh + (long)nI * n, *red = qv + nI;
+  double *td = J.tri, *te = td + n, *tq = te + n;
+  const long ld = n;
+  for (long i = tid; i < (long)n * n; i += blockDim.x) A[i] = 0.0;
+  for (long i = tid; i < (long)nI * n; i += blockDim.x) Qh[i] = ((i / n) == (i % n)) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int i = tid; i < nI * nI; i += blockDim.x) A[(i / nI) * ld + (i % nI)] = J.u1[i] + J.h[i];
+  for (int side = 0; side < 2; ++side) {
+    const int nd = side ? J.nbk : J.nfw, den = side ? J.den_bk : J.den_fw, T = den + nI;
+    const int base = nI + (side ? J.nfw : 0);
+    const double *rows = side ? J.rbk : J.rfw;
+    for (int i = tid; i < nd; i += blockDim.x) {
+      const double *row = rows + (long)i * T;
+      A[(base + i) * ld + base + i] = row[0];
+      for (int r = 0; r < nI; ++r) {
+        A[(base + i) * ld + r] = row[den + r];
+        A[(long)r * ld + base + i] = row[den + r];
+      }
+    }
+  }
+  __syncthreads();
+  for (int side = 0; side < 2; ++side) {
+    const int npiv = side ? J.npiv_bk : J.npiv_fw;
+    if (npiv <= 0) continue;
+    const int nd = side ? J.nbk : J.nfw, den = side ? J.den_bk : J.den_fw, T = den + nI;
+    const int base = nI + (side ? J.nfw : 0);
+    const double *rows = side ? J.rbk : J.rfw;
+    const int *ipv = side ? J.ipiv_bk : J.ipiv_fw;
+    for (int i = 1 + tid; i < nd; i += blockDim.x) {
+      const double b = rows[(long)i * T + 1];
+      A[(base + i) * ld + base + i - 1] = b;
+      A[(base + i - 1) * ld + base + i] = b;
+    }
+    __syncthreads();
+    // fishbone branches (DysonMatrix.cpp:303-333): branch nc starts at row
+    // i_piv[nc+1] and couples its predecessor to every later row
+    for (int nc = 1; nc <= npiv; ++nc) {
+      const int st = ipv[nc + 1];
+      if (st < 1 || st >= nd) break;
+      for (int r = st + tid; r < nd; r += blockDim.x) {
+        const double gv = rows[(long)r * T + 1 + nc];
+        A[(base + r) * ld + base + st - 1] = gv;
+        A[(base + st - 1) * ld + base + r] = gv;
+      }
+    }
+    __syncthreads();
+  }
+  // Householder reduction
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m0 = k + 1;
+    double s[1] = {0.0};
+    for (int j = m0 + tid; j < n; j += blockDim.x) { const double a = A[k * ld + j]; s[0] += a * a; }
+    block_sum(s, 1, red);
+    const double s2 = s[0], x0 = A[k * ld + m0], tail = s2 - x0 * x0;
+    if (tail <= 0.0) { if (tid == 0) te[k] = x0; continue; }
+    const double nrm = sqrt(s2), alpha = x0 > 0.0 ? -nrm : nrm, v0 = x0 - alpha;
+    const double beta = 2.0 / (tail + v0 * v0);
+    for (int j = m0 + tid; j < n; j += blockDim.x) v[j] = (j == m0) ? v0 : A[k * ld + j];
+    if (tid == 0) te[k] = alpha;
+    __syncthreads();
+    for (int i = m0 + w; i < n; i += nw) {
+      double acc = 0.0;
+      for (int j = m0 + lane; j < n; j += 64) acc += A[i * ld + j] * v[j];
+      acc = wave_sum(acc);
+      if (lane == 0) p[i] = beta * acc;
+    }
+    for (int r = w; r < nI; r += nw) {
+      double acc = 0.0;
+      for (int j = m0 + lane; j < n; j += 64) acc += Qh[(long)r * n + j] * v[j];
+      acc = wave_sum(acc);
+      if (lane == 0) qv[r] = beta * acc;
+    }
+    __syncthreads();
+    s[0] = 0.0;
+    for (int i = m0 + tid; i < n; i += blockDim.x) s[0] += p[i] * v[i];
+    block_sum(s, 1, red);
+    const double K = 0.5 * beta * s[0];
+    for (int i = m0 + tid; i < n; i += blockDim.x) p[i] -= K * v[i];
+    __syncthreads();
+    for (int i = m0 + w; i < n; i += nw) {
+      const double vi = v[i], wi = p[i];
+      for (int j = m0 + lane; j < n; j += 64) A[i * ld + j] -= vi * p[j] + wi * v[j];
+    }
+    for (int r = w; r < nI; r += nw) {
+      const double q = qv[r];
+      for (int j = m0 + lane; j < n; j += 64) Qh[(long)r * n + j] -= q * v[j];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += blockDim.x) {
+    td[i] = A[i * ld + i];
+    if (i == n - 2) te[i] = A[i * ld + i + 1];
+    if (i == n - 1) te[i] = 0.0;
+  }
+  for (long i = tid; i < (long)nI * n; i += blockDim.x) tq[i] = Qh[i];
+}
+
+template __global__ void k_tridiag<true>(const DysonJob *, int);
+template __global__ void k_tridiag<false>(const DysonJob *, int);
+
+// Implicit QL with shifts on the tridiagonal (Bowdler, Martin, Reinsch and
+// Wilkinson 1968), rotations applied to the nI tracked rows; then the split
+// at the Fermi energy, the descending sort of quasiparticles and quasiholes
+// (DysonMatrix.cpp:436-484) and the Calc_En partial sums (:542-594).  One wave
+// per matrix: the rotation chain is uniform across lanes, lanes < nI own one
+// tracked row each.
+__global__ __launch_bounds__(64) void k_ql(const DysonJob *jobs, int njobs, int *status) {
+  extern __shared__ double sm[];
+  if ((int)blockIdx.x >= njobs) return;
+  const DysonJob J = jobs[blockIdx.x];
+  const int n = J.n;
+  if (n <= 0) return;
+  const int nI = J.nI, lane = threadIdx.x;
+  double *d = sm, *e = d + n, *Q = e + n;
+  for (int i = lane; i < n; i += 64) { d[i] = J.tri[i]; e[i] = J.tri[n + i]; }
+  for (long i = lane; i < (long)nI * n; i += 64) Q[i] = J.tri[2L * n + i];
+  __syncthreads();
+  const double eps = 2.2204460492503131e-16;
+  int fail = 0;
+  for (int lo = 0; lo < n && !fail; ++lo) {
+    for (int it = 0;; ++it) {
+      int m = lo;
+      for (; m + 1 < n; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= eps * dd) break;
+      }
+      if (m == lo) break;
+      if (it >= 60) { fail = 1; break; }
+      double g = (d[lo + 1] - d[lo]) / (2.0 * e[lo]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[lo] + e[lo] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      bool defl = false;
+      // every lane runs the same scalar chain and stores the same values, so
+      // each lane's own program order keeps d and e consistent
+      for (int i = m - 1; i >= lo; --i) {
+        const double f = s * e[i], b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= pp;
+          e[m] = 0.0;
+          defl = true;
+          break;
+        }
+        s = f / r; c = g / r;
+        g = d[i + 1] - pp;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        pp = s * r;
+        d[i + 1] = g + pp;
+        g = c * r - b;
+        for (int rr = lane; rr < nI; rr += 64) {
+          double *row = Q + (long)rr * n;
+          const double t = row[i + 1];
+          row[i + 1] = s * row[i] + c * t;
+          row[i] = c * row[i] - s * t;
+        }
+      }
+      if (defl) continue;
+      d[lo] -= pp; e[lo] = g; e[m] = 0.0;
+    }
+  }
+  if (fail) { if (lane == 0) atomicAdd(status, 1); return; }
+  __syncthreads();
+  const double ef = J.efermi;
+  int cnt = 0;
+  for (int k = lane; k < n; k += 64) cnt += (d[k] <= ef);
+  const int nqh = wave_isum(cnt), nqp = n - nqh;
+  double kb[9];
+  for (int i = 0; i < 9; ++i) kb[i] = 0.0;
+  for (int k = lane; k < n; k += 64) {
+    const double dk = d[k];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) { const double dj = d[j]; rank += (dj < dk) || (dj == dk && j < k); }
+    const int slot = dk > ef ? (n - 1 - rank) : nqp + (nqh - 1 - rank);
+    J.out_e[slot] = dk;
+    double x1 = 0.0, x2 = 0.0;
+    for (int r = 0; r < nI; ++r) {
+      const double xr = Q[(long)r * n + k];
+      J.out_x[(long)slot * nI + r] = xr;
+      x1 += xr * xr;
+    }
+    if (dk <= ef) {
+      for (int r = 0; r < nI; ++r)
+        for (int c = 0; c < nI; ++c) x2 += J.u1[r * nI + c] * Q[(long)r * n + k] * Q[(long)c * n + k];
+      const int bk = x1 > 0.01 ? 0 : (x1 > 1.e-5 ? 1 : 2);
+      kb[3 * bk] += x2; kb[3 * bk + 1] += x1 * dk; kb[3 * bk + 2] += x1;
+    }
+  }
+  for (int i = 0; i < 9; ++i) kb[i] = wave_sum(kb[i]);
+  if (lane == 0) {
+    *J.out_nqp = nqp;
+    *J.out_nqh = nqh;
+    const double f = (double)J.j2 + 1.0;
+    J.kolt[0] = (kb[0] + kb[3] + kb[6]) * f;
+    J.kolt[1] = (kb[1] + kb[4] + kb[7]) * f;
+    J.kolt[2] = (kb[2] + kb[5] + kb[8]) * f;
+  }
+}
+
+// Stable rank sort of pole rows by column col (ascending, or descending when
+// desc): row i goes to position #{j : key_j before key_i}, ties by index.
+__global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const double ki = rows[(long)i * T + col];
+  int rank = 0;
+  for (int j = 0; j < M; ++j) {
+    const double kj = rows[(long)j * T + col];
+    rank += desc ? (kj > ki || (kj == ki && j < i)) : (kj < ki || (kj == ki && j < i));
+  }
+  for (int c = 0; c < T; ++c) out[(long)rank * T + c] = rows[(long)i * T + c];
+}
+
+// ------------------------------------------------------------- sweep control
+
+// One block per instance.  init = 1: Fermi energies of the current propagator
+// only.  Otherwise: compare the new propagator (buffer 1-cur) with the current
+// one by storage index (quick_n_dirty_diff, Sp_Propagator.cpp:1189-1213; sum or
+// max), add up the Koltun parts (Main.cpp:387-390), flip the buffers, stop the
+// instance when converged or at ItrMax (Main.cpp:216, 451), and set the Fermi
+// energies of the next sweep from the new propagator (Seek_EFermi,
+// DysonMatrix.cpp:619-635).
+__global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const ChanDev *chans,
+                                                  PropView P, const ChanState *cs, const double *kolt,
+                                                  int conv_kind, int init) {
+  __shared__ int pq[2][kMaxSub + 1], ph[2][kMaxSub + 1];
+  __shared__ double red[4 * 4];
+  const int inst = blockIdx.x, tid = threadIdx.x;
+  const InstDev I = insts[inst];
+  InstState &S = P.is[inst];
+  if (!init && !S.active) return;
+  const int ns = I.nsubsh, cur = S.cur, nxt = 1 - cur;
+  const int useb = init ? cur : nxt;
+  if (tid == 0) {
+    for (int b = 0; b < 2; ++b) {
+      pq[b][0] = 0; ph[b][0] = 0;
+      const int buf = b == 0 ? cur : nxt;
+      for (int s = 0; s < ns; ++s) {
+        pq[b][s + 1] = pq[b][s] + P.nqp[buf][I.gc0 + s];
+        ph[b][s + 1] = ph[b][s] + P.nqh[buf][I.gc0 + s];
+      }
+    }
+  }
+  __syncthreads();
+  // Fermi energies from buffer useb
+  {
+    double mn[2] = {1e300, 1e300}, mx[2] = {-1e300, -1e300};
+    for (int s = 0; s < ns; ++s) {
+      const int gc = I.gc0 + s, chs = I.ch[s] & 1;
+      const int np = P.nqp[useb][gc], nh = P.nqh[useb][gc];
+      const double *e = P.e[useb] + P.e_off[gc];
+      for (int k = tid; k < np; k += blockDim.x) mn[chs] = fmin(mn[chs], e[k]);
+      for (int k = tid; k < nh; k += blockDim.x) mx[chs] = fmax(mx[chs], e[np + k]);
+    }
+    double v4[4] = {-mn[0], -mn[1], mx[0], mx[1]};
+    for (int k = 0; k < 4; ++k) v4[k] = wave_max(v4[k]);
+    if ((tid & 63) == 0) for (int k = 0; k < 4; ++k) red[(tid >> 6) * 4 + k] = v4[k];
+    __syncthreads();
+    if (tid == 0) {
+      for (int k = 0; k < 4; ++k) {
+        double m = red[k];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, red[i * 4 + k]);
+        red[k] = m;
+      }
+      // first stored quasiparticle / quasihole of the propagator as the start
+      // values, as Seek_EFermi does
+      double e0p = 0.0, e0h = 0.0;
+      for (int s = 0; s < ns; ++s) if (P.nqp[useb][I.gc0 + s] > 0) { e0p = P.e[useb][P.e_off[I.gc0 + s]]; break; }
+      for (int s = 0; s < ns; ++s) {
+        const int gc = I.gc0 + s;
+        if (P.nqh[useb][gc] > 0) { e0h = P.e[useb][P.e_off[gc] + P.nqp[useb][gc]]; break; }
+      }
+      for (int c = 0; c < 2; ++c) {
+        const double x1 = fmin(e0p + 100., -red[c]), x2 = fmax(e0h - 100., red[2 + c]);
+        S.ef[c] = 0.5 * (x1 + x2);
+      }
+    }
+    __syncthreads();
+  }
+  if (init) {
+    if (tid == 0) { S.its = 0; S.active = 1; S.diff = 0.0; }
+    return;
+  }
+  // storage-index difference between buffers cur (old) and nxt (new)
+  double acc[2] = {0.0, 0.0};
+  for (int side = 0; side < 2; ++side) {
+    const int tot_old = side ? ph[0][ns] : pq[0][ns], tot_new = side ? ph[1][ns] : pq[1][ns];
+    const int tmax = tot_old > tot_new ? tot_old : tot_new;
+    for (int j = tid; j < tmax; j += blockDim.x) {
+      double eo = 0.0, en = 0.0;
+      for (int b = 0; b < 2; ++b) {
+        const int *pre = side ? ph[b] : pq[b];
+        const int tot = pre[ns];
+        if (j >= tot) continue;
+        int s = 0;
+        while (pre[s + 1] <= j) ++s;
+        const int gc = I.gc0 + s, buf = b == 0 ? cur : nxt;
+        const double val = P.e[buf][P.e_off[gc] + (side ? P.nqp[buf][gc] : 0) + (j - pre[s])];
+        if (b == 0) eo = val; else en = val;
+      }
+      const double dlt = fabs(eo - en);
+      acc[0] += dlt;
+      acc[1] = fmax(acc[1], dlt);
+    }
+  }
+  const double dsum = wave_sum(acc[0]), dmax = wave_max(acc[1]);
+  if ((tid & 63) == 0) { red[(tid >> 6) * 4] = dsum; red[(tid >> 6) * 4 + 1] = dmax; }
+  __syncthreads();
+  if (tid == 0) {
+    double sum = 0.0, mxd = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) { sum += red[i * 4]; mxd = fmax(mxd, red[i * 4 + 1]); }
+    double e1 = 0.0, es = 0.0, as = 0.0;
+    for (int s = 0; s < ns; ++s) {
+      const double *k3 = kolt + 3L * (I.gc0 + s);
+      e1 += k3[0]; es += (k3[0] + k3[1]) / 2.0; as += k3[2];
+    }
+    S.u1_tot = e1; S.e_kolt = es; S.a_tot = as;
+    const double diff = conv_kind == 1 ? mxd : sum;
+    S.diff = diff;
+    S.its += 1;
+    S.cur = nxt;
+    if (diff < S.conv || (S.itr_max >= 0 && S.its > S.itr_max)) S.active = 0;
+  }
+}
+
+}  // namespace bcd

@@ -1,0 +1,46 @@
+// bcd_kernels.hpp -- kernel limits and declarations shared by host and device code.
+#pragma once
+
+#include "bcd_device.hpp"
+
+namespace bcd {
+
+constexpr int kMaxVt = 2048;      // nI * (alpha,beta) pairs * gamma of one Sigma(2) block
+constexpr int kMaxPairs = 64;     // (alpha,beta) pairs of one block (+1 slot for the count)
+constexpr int kMaxPiv = 8;        // Lanczos pivots per channel
+constexpr int kMaxNB = 16;        // coupling columns of a Lanczos input row
+constexpr int kLancRed = 2 + kMaxNB + kMaxPiv;
+constexpr int kMaxNI = 16;        // orbits per subshell
+constexpr int kMaxSub = 64;       // subshells per instance
+
+__global__ void k_wbuild(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end, double *W);
+__global__ void k_density(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
+                          PropView P, double *rho);
+__global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
+                            const InstState *is, const double *W, const double *rho, double *h);
+__global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
+                              PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
+                              int *blk_bk_off, int *work_list, int *work_count, int only_ish);
+__global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const BlockDev *blocks, PropView P,
+                         const int *blk_fw_off, const int *blk_bk_off, const int *work_list,
+                         const int *work_count, const long *row_fw_base, const long *row_bk_base,
+                         double *rows_fw, double *rows_bk);
+__global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P, ChanState *cs,
+                               double *piv, int lanc_kind, int lanc_n, const long *row_fw_base,
+                               const long *row_bk_base, double *rows_fw, double *rows_bk, double *lrows,
+                               double *lws, LancJob *jobs);
+template <int NT, int PER>
+__global__ void k_lanczos(const LancJob *jobs, int njobs);
+__global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, const ChanState *cs,
+                             PropView P, int src, const double *h, const double *u1,
+                             const long *row_fw_base, const long *row_bk_base, const double *rows_fw,
+                             const double *rows_bk, const double *lrows, double *tri, double *mat,
+                             double *kolt, DysonJob *jobs);
+template <bool IN_LDS>
+__global__ void k_tridiag(const DysonJob *jobs, int njobs);
+__global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
+__global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out);
+__global__ void k_finalize(const InstDev *insts, const ChanDev *chans, PropView P, const ChanState *cs,
+                           const double *kolt, int conv_kind, int init);
+
+}  // namespace bcd

@@ -1,0 +1,138 @@
+// bcd_system.cpp -- host-side packing of the model space and the interaction.
+#include "bcd_system.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <map>
+
+namespace bcd {
+
+static bool tri_ok(int a2, int b2, int c2) {
+  const int s1 = a2 + b2 - c2, s2 = b2 + c2 - a2, s3 = c2 + a2 - b2;
+  return s1 >= 0 && s2 >= 0 && s3 >= 0 && !(s1 & 1) && !(s2 & 1) && !(s3 & 1);
+}
+
+int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
+  if (!ms || ms->nsubsh <= 0 || ms->nsubsh > kMaxSub) return BCD_ERR_ARG;
+  nsubsh = ms->nsubsh;
+  l.assign(ms->l, ms->l + nsubsh);
+  j2.assign(ms->j2, ms->j2 + nsubsh);
+  ip.assign(ms->ip, ms->ip + nsubsh);
+  ch.assign(ms->ch, ms->ch + nsubsh);
+  no.assign(ms->no, ms->no + nsubsh);
+  orb0.resize(nsubsh);
+  norb = 0;
+  nspb = 0;
+  jmax = 0;
+  for (int i = 0; i < nsubsh; ++i) {
+    if (no[i] <= 0 || no[i] > kMaxNI || ch[i] < 0 || ch[i] > 1) return BCD_ERR_ARG;
+    orb0[i] = norb;
+    norb += no[i];
+    if (no[i] > nspb) nspb = no[i];
+    if (j2[i] > jmax) jmax = j2[i];   // Calculate_Jch_bounds: Jmax = max 2j (Mod_Space.cpp:593)
+  }
+  if (norb >= kKeyAB || jmax >= kKeyJ) return BCD_ERR_ARG;
+  orb_n.assign(ms->orb_n, ms->orb_n + norb);
+  orb_F.assign(ms->orb_F, ms->orb_F + norb);
+  orb_e.assign(ms->orb_e, ms->orb_e + norb);
+  orb_sh.resize(norb);
+  for (int i = 0; i < nsubsh; ++i)
+    for (int k = 0; k < no[i]; ++k) orb_sh[orb0[i] + k] = i;
+  qoff.resize(nsubsh);
+  nq = 0;
+  for (int i = 0; i < nsubsh; ++i) { qoff[i] = nq; nq += no[i] * no[i]; }
+
+  // pair tables: pairs a <= b allowed by the triangle rule and Pauli
+  pidx.assign((size_t)(jmax + 1) * norb * norb, -1);
+  std::map<long, int> chan_id;
+  npair.clear();
+  for (int J = 0; J <= jmax; ++J)
+    for (int a = 0; a < norb; ++a)
+      for (int b = a; b < norb; ++b) {
+        const int sa = orb_sh[a], sb = orb_sh[b];
+        if (!tri_ok(j2[sa], j2[sb], 2 * J)) continue;
+        if (a == b && (J & 1)) continue;
+        const long key = ((long)J * 2 + (l[sa] + l[sb]) % 2) * 8 + (ch[sa] + ch[sb]);
+        auto it = chan_id.find(key);
+        int c;
+        if (it == chan_id.end()) {
+          c = (int)npair.size();
+          chan_id[key] = c;
+          npair.push_back(0);
+        } else {
+          c = it->second;
+        }
+        if (npair[c] >= 0xffff) return BCD_ERR_CAPACITY;
+        pidx[((size_t)J * norb + a) * norb + b] = (c << 16) | npair[c];
+        ++npair[c];
+      }
+  nchan2b = (int)npair.size();
+  voff.resize(nchan2b);
+  long tot = 0;
+  for (int c = 0; c < nchan2b; ++c) { voff[c] = tot; tot += (long)npair[c] * npair[c]; }
+  V.assign(tot, 0.0);
+  dropped = 0;
+  if (vpp && vpp->n_me > 0) {
+    for (long q = 0; q < vpp->n_me; ++q) {
+      const int k1 = vpp->iabJ[q], k2 = vpp->icdJ[q];
+      const int J = k1 % kKeyJ, J2 = k2 % kKeyJ;
+      const int a = (k1 / kKeyJ) / kKeyAB, b = (k1 / kKeyJ) % kKeyAB;
+      const int c = (k2 / kKeyJ) / kKeyAB, d = (k2 / kKeyJ) % kKeyAB;
+      if (J != J2 || J > jmax || a >= norb || b >= norb || c >= norb || d >= norb || a > b || c > d) {
+        ++dropped;
+        continue;
+      }
+      const int pa = pidx[((size_t)J * norb + a) * norb + b], pc = pidx[((size_t)J * norb + c) * norb + d];
+      if (pa < 0 || pc < 0 || (pa >> 16) != (pc >> 16)) { ++dropped; continue; }
+      const int cc = pa >> 16, ra = pa & 0xffff, rc = pc & 0xffff;
+      V[voff[cc] + (long)ra * npair[cc] + rc] = vpp->v[q];
+      V[voff[cc] + (long)rc * npair[cc] + ra] = vpp->v[q];
+    }
+  }
+
+  // Sigma(2) blocks per subshell
+  blocks.assign(nsubsh, {});
+  mbound_fw.assign(nsubsh, 0);
+  mbound_bk.assign(nsubsh, 0);
+  for (int I = 0; I < nsubsh; ++I) {
+    for (int J = 0; J <= jmax; ++J)
+      for (int a = 0; a < nsubsh; ++a)
+        for (int b = a; b < nsubsh; ++b) {
+          if (!tri_ok(j2[a], j2[b], 2 * J)) continue;
+          const int nJab = std::abs((j2[a] + j2[b]) / 2 - J + 1) % 2;
+          for (int g = 0; g < nsubsh; ++g) {
+            const int nJgd = std::abs((j2[g] + j2[I]) / 2 - J + 1) % 2;
+            if (ch[a] + ch[b] - ch[g] != ch[I]) continue;
+            if ((ip[a] + ip[b] + ip[g] + ip[I]) % 2 != 0) continue;
+            if (!tri_ok(j2[g], j2[I], 2 * J)) continue;
+            int np = 0;
+            for (int al = 0; al < no[a]; ++al) np += no[b] - ((a == b) ? al + nJab : 0);
+            if (np < 0) np = 0;
+            if (np >= kMaxPairs - 1 || (long)no[I] * np * no[g] > kMaxVt) return BCD_ERR_CAPACITY;
+            BlockDev B;
+            B.gc = I; B.a = a; B.b = b; B.g = g; B.J = J; B.nJab = nJab; B.nJgd = nJgd; B._pad = 0;
+            blocks[I].push_back(B);
+            const long pairs = (a == b) ? (long)no[a] * (no[a] + 1) / 2 : (long)no[a] * no[b];
+            mbound_fw[I] += pairs * no[g];
+            mbound_bk[I] += pairs * no[g];
+          }
+        }
+  }
+  return BCD_OK;
+}
+
+void HostSystem::u1(int ish, double t_fac, int ext_u1, double *out) const {
+  const int n = no[ish], ll = l[ish];
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) {
+      const int na = orb_n[orb0[ish] + r], nb = orb_n[orb0[ish] + c];
+      double t = 0.0;
+      if (na == nb) t = ((double)(na + nb + ll) + 1.5) / 2.0;
+      if (std::abs(na - nb) == 1) t = std::sqrt((double)(na + nb + 1) * (double)(na + nb + ll + ll + 2) / 4.0) / 2.0;
+      double v = t_fac * t;
+      if (ext_u1 == 1 && r == c) v += orb_e[orb0[ish] + r];
+      out[r * n + c] = v;
+    }
+}
+
+}  // namespace bcd

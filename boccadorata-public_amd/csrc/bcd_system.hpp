@@ -1,0 +1,31 @@
+// bcd_system.hpp -- host-side model space and interaction packing.
+#pragma once
+
+#include <vector>
+
+#include "bcd_kernels.hpp"
+#include "bcdor_amd.h"
+
+namespace bcd {
+
+// One system: ModSpace_t tables plus the interaction repacked from VppInt_t's
+// sorted key/value lists into dense symmetric blocks per two-body channel
+// (J, parity, charge), and the (J, a<=b, g) block list of the Sigma(2) build
+// of every subshell in the reference's loop order (Self_energy_ext_Op_c.cpp:380-395).
+struct HostSystem {
+  int nsubsh = 0, norb = 0, nspb = 0, jmax = 0, nchan2b = 0, nq = 0;
+  std::vector<int> l, j2, ip, ch, no, orb0, orb_sh, orb_n, orb_F, qoff;
+  std::vector<double> orb_e;
+  std::vector<int> pidx, npair;
+  std::vector<long> voff;
+  std::vector<double> V;
+  std::vector<std::vector<BlockDev>> blocks;   // [ish]
+  std::vector<long> mbound_fw, mbound_bk;      // pole-count bounds for HF-like propagators
+  long dropped = 0;                            // interaction keys outside the pair tables
+
+  int build(const bcd_modspace *ms, const bcd_vpp *vpp);
+  // one-body part U1 of subshell ish: t_fac * T_HO + ext_u1 * diag(spe)
+  void u1(int ish, double t_fac, int ext_u1, double *out) const;
+};
+
+}  // namespace bcd

@@ -1,0 +1,181 @@
+/*
+ * bcdor_amd.h -- C ABI of the MI355X (gfx950) second-order SCGF hot path.
+ *
+ * Plain pointers and sizes only.  Host arrays use the reference's own
+ * layouts (BoccaDorata v1.0, craolus/BoccaDorata-public, files under source/):
+ * the model space as ModSpace_t holds it, the interaction as VppInt_t stores
+ * it (packed keys + values), propagator fragments as SpProp_t stores them and
+ * self-energy pole rows as Wide_self_energy_t stores them, so a reference
+ * caller passes its existing arrays unchanged.  All arithmetic runs in FP64
+ * on the GPU; there is no CPU fallback: without a usable device every entry
+ * point returns BCD_ERR_DEVICE.
+ *
+ * Return codes: 0 on success, < 0 on error (bcd_error_string).  No entry
+ * point calls exit().  Calls on one context are not reentrant; different
+ * contexts may be used from different host threads.
+ */
+#ifndef BCDOR_AMD_H
+#define BCDOR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library error codes sit below -1000 so that they never collide with the
+ * reference's own Reduce_Mtx_Lncz codes (-1, -2, -100..-201), which
+ * bcd_reduce_mtx_lncz returns unchanged. */
+#define BCD_OK 0
+#define BCD_ERR_ARG (-1001)
+#define BCD_ERR_DEVICE (-1002)
+#define BCD_ERR_CAPACITY (-1003)
+#define BCD_ERR_LANCZOS (-1004)
+#define BCD_ERR_EIGEN (-1005)
+#define BCD_ERR_NOMEM (-1006)
+
+/* ModSpace_t arrays (Sp_classes.hh:31-71; filled by ModSpace_t(char*),
+ * Mod_Space.cpp:415-504).  Orbits are numbered subshell by subshell. */
+typedef struct bcd_modspace {
+  int nsubsh;
+  const int *l, *j2, *ip, *ch, *no;      /* [nsubsh]: l, 2j, parity, charge, orbits   */
+  const int *orb_n, *orb_F;               /* [norb]: radial n (Mor_n), hole flag (Mor_F) */
+  const double *orb_e;                    /* [norb]: spe column (Mor_xe)                */
+} bcd_modspace;
+
+/* VppInt_t storage (Int_classes.hh:20-33): n_me_stored unique elements with
+ * keys ((i1*1024+i2)*128+J), i1 <= i2, iabJ <= icdJ (Vpp_potential.cpp:82)
+ * and values Vpp_wrk, phases already applied as read_ascii does. */
+typedef struct bcd_vpp {
+  long n_me;
+  const int *iabJ, *icdJ;
+  const double *v;
+} bcd_vpp;
+
+/* SpProp_t fragments (Sp_classes.hh:78-102): per subshell np[i] quasiparticles
+ * then, in a separate array, nh[i] quasiholes, subshells in model-space
+ * order; overlaps fq*[k*nspb + n] (N_SPB_MX stride). */
+typedef struct bcd_prop {
+  int nspb;
+  const int *np, *nh;                     /* [nsubsh]                                  */
+  const double *ep, *fqp;                 /* [sum np], [sum np * nspb]                 */
+  const double *eh, *fqh;                 /* [sum nh], [sum nh * nspb]                 */
+} bcd_prop;
+
+typedef struct bcd_ctx bcd_ctx;
+typedef struct bcd_batch bcd_batch;
+
+int bcd_version(void);
+const char *bcd_error_string(int code);
+int bcd_device_count(void);
+
+/* ---------------------------------------------------------------- context
+ * A context holds one system on one device: the model-space tables and the
+ * interaction packed into dense J-blocks in HBM.  Replaces the set-up of
+ * ModSpace_t + VppInt_t (Load_ModSp / Load_interaction, Initialize.cpp:93-157). */
+int bcd_ctx_create(int device, const bcd_modspace *ms, const bcd_vpp *vpp, bcd_ctx **out);
+void bcd_ctx_destroy(bcd_ctx *ctx);
+
+/* Count_2p1h_2h1p_poles (Self_energy_ext_Op_c.cpp:35-113), icut == 0. */
+int bcd_count_poles(bcd_ctx *ctx, int ish, const bcd_prop *g, int *nfw, int *nbk);
+
+/* Build_Dynamic_SlfEn_2ndOrder (Self_energy_ext_Op_c.cpp:323-561), icut == 0:
+ * the 2p1h (fw) and 2h1p (bk) pole rows [E, sum_d m_d^2, m_0..m_{n-1}] of
+ * subshell ish in the reference's build order; row width 2 + no[ish]. */
+int bcd_sigma2_build(bcd_ctx *ctx, int ish, const bcd_prop *g,
+                     double *rows_fw, int cap_fw, double *rows_bk, int cap_bk,
+                     int *nfw, int *nbk);
+
+/* SpProp_t::ExtendedHartreeFock_Pot (Sp_Propagator_a.cpp:39-130): Sigma^inf of
+ * subshell ish from the quasihole densities of g; u is ndim x ndim row-major. */
+int bcd_hf_pot(bcd_ctx *ctx, int ish, const bcd_prop *g, double *u, int ndim);
+
+/* Reduce_Mtx_Lncz (Lanczos.cpp:44-286; declared in Run_vars.hh:193-195), same
+ * arguments and return value, with the pivot rows flattened to
+ * pivots[nt_pvts * NB_in] (normalised in place, as the reference does). */
+int bcd_reduce_mtx_lncz(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout,
+                        int *i_piv_out, int NA_in, int NB_in, int NPLS_in, const double *Min,
+                        int nt_pvts, double *pivots, int *n_itr);
+
+/* Wide_self_energy_t::Reduce_SelfEn_Lncz (Self_energy_ext_Op_e.cpp:55-232):
+ * sorts the unreduced rows in place (fw ascending, bk descending), rescales
+ * n_itr in place when it exceeds the pole count, reduces both sides.  Output
+ * rows have *ntot_den = max(2, nt_pvts+1) leading columns. */
+int bcd_reduce_selfen_lncz(bcd_ctx *ctx, int n_orb, int nfw, double *rows_fw, int nbk,
+                           double *rows_bk, int nt_pvts, double *pivots, int *n_itr_fw,
+                           int *n_itr_bk, double *out_fw, int *i_piv_fw, int *nfw_out,
+                           double *out_bk, int *i_piv_bk, int *nbk_out, int *ntot_den);
+
+/* DysonMatrix (DysonMatrix.cpp:123-532) without the propagator append: builds
+ * [h M_fw^T M_bk^T; M_fw D_fw 0; M_bk 0 D_bk] from the static part u1 + h and the
+ * pole rows (diagonal D for npiv == 0, fishbone for a Lanczos-reduced side),
+ * diagonalises it, splits at efermi and returns records [e, Z, X(0..n_orb-1)]
+ * sorted by descending e.  kolt3 (may be NULL) = Calc_En partial sums
+ * {e1b, esum, asum} (DysonMatrix.cpp:542-594). */
+int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1,
+              int ntot_den_fw, int nfw, const double *rows_fw, const int *i_piv_fw, int npiv_fw,
+              int ntot_den_bk, int nbk, const double *rows_bk, const int *i_piv_bk, int npiv_bk,
+              double efermi, double *sol_qp, int *nqp, double *sol_qh, int *nqh, double *kolt3);
+
+/* Batched symmetric eigensolver standing in for LAPACK dsyevd_('V','U', ...)
+ * (DysonMatrix.cpp:35-36, 412): nmat row-major symmetric matrices of orders
+ * n[k] at a + off[k]; eigenvalues ascending to w + woff[k]; eigenvectors
+ * overwrite the matrices column-wise. */
+int bcd_syevd_batched(bcd_ctx *ctx, int nmat, const int *n, const long *off, double *a,
+                      const long *woff, double *w);
+
+/* ------------------------------------------------------------ batched solve
+ * A batch holds n_inst independent systems (any mix of model spaces) resident
+ * on one device and runs the whole SCGF solve of each on the GPU: HF from the
+ * given (or bare) propagator, Sigma(2) from the HF result, optional Lanczos
+ * reduction, then the sc0 iteration of Sigma^inf + Dyson to convergence
+ * (the reference's HF=-1 -> MakeSelfEn 2nd [Lanczos] -> ExtSE=-1 chain,
+ * Run_itrs, Main.cpp:105-479). */
+typedef struct bcd_solve_params {
+  int mode;          /* 0: HF only; 1: HF then one 2nd-order sweep; 2: HF then sc0  */
+  int hf_itr_max;    /* ItrMax of the HF stage (< 0: until converged)               */
+  int sc_itr_max;    /* ItrMax of the sc0 stage (< 0: until converged)              */
+  int lanczos;       /* 1: Lanczos-reduce Sigma(2) before the Dyson sweeps          */
+  int lanc_kind;     /* 0: lanc_n iterations per pivot (MakePivots=lanc_n);
+                        1: lanc_n Krylov vectors per side split over the pivots     */
+  int lanc_n;
+  int conv_kind;     /* 0: sum_i |de_i| (quick_n_dirty_diff); 1: max_i |de_i|         */
+  double hf_conv;    /* HF stops when the measure drops below this                   */
+  double sc_conv;    /* sc0 stops when the measure drops below this                  */
+  double t_fac;      /* U1body_fac * hbar*omega multiplying T_HO (0: Trel=7)        */
+  int ext_u1;        /* 1: add diag(model-space spe) to U1 (ExtU1=1)                */
+} bcd_solve_params;
+
+/* g0 may be NULL (bare propagator, SpProp_t(ModSpace_t*, 1)); otherwise g0[i]
+ * is the starting propagator of instance i. */
+int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_vpp *vpp,
+                     const bcd_prop *g0, const bcd_solve_params *par, bcd_batch **out);
+void bcd_batch_destroy(bcd_batch *b);
+/* Run the full solve of every instance; inputs stay resident, so repeated
+ * calls redo the same work. */
+int bcd_batch_solve(bcd_batch *b);
+/* 1: time every kernel family with hipEvents during bcd_batch_solve (adds a
+ * synchronisation per launch); 0 (default): stage events only. */
+int bcd_batch_set_profile(bcd_batch *b, int on);
+/* Per-stage device time of the last solve in ms (hipEvents on the batch's
+ * stream): [0] W/Sigma^inf set-up, [1] HF loop, [2] Sigma(2) build,
+ * [3] Lanczos, [4] sc0 loop, [5] total.  n >= 6. */
+int bcd_batch_timings(bcd_batch *b, double *ms, int n);
+/* Iterations, last convergence measure and Koltun energy of instance i. */
+int bcd_batch_stats(bcd_batch *b, int inst, int *hf_its, int *sc_its, double *last_diff,
+                    double *e_kolt, long *n_fw_poles, long *n_bk_poles);
+/* Final propagator of instance i: first call with NULL arrays to get the
+ * totals, then with arrays sized sum np / sum nh (overlaps nspb-strided). */
+int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh,
+                     double *ep, double *zp, double *fqp, double *eh, double *zh, double *fqh);
+/* Average device duration (ms) and launch count of each kernel family over
+ * the last solve, for roofline accounting: names[k] "sigma2", "lanczos",
+ * "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild". */
+int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
+/* Algorithmic work of the last solve: [0] coupling FLOPs (2*sum poles*K*n),
+ * [1] Lanczos bytes (8*M*sum_j(2j+5+n+npiv)), [2] Dyson FLOPs (9 n^3 per
+ * eigensolve), [3] number of Dyson eigensolves, [4] sum of Dyson n^3. */
+int bcd_batch_work(bcd_batch *b, double *work, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

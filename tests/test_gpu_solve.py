@@ -1,0 +1,98 @@
+"""GPU parity of the whole SCGF solve (bcd_batch_*) against the oracle's
+restatement of Run_itrs on the same inputs.
+
+Bar: quasiparticle/quasihole energies 1e-10 relative (floor 1 MeV),
+spectroscopic factors 1e-10 absolute, the same fragment counts per subshell,
+the same number of sweeps, and instances of a batch identical to the same
+instances solved alone.
+
+Fragments with Z below helpers.ZSIG are compared through the Z-weighted
+spectral moments only: when a Lanczos chain runs past Krylov breakdown their
+energies are rounding-determined in the reference itself (see helpers.py).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from bcdor_amd import capi, synth
+from helpers import ZSIG_BY_NMAX, spectral_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_solve(ms, vpp, lanczos, conv_kind, hf_conv, sc_conv, sc_itr_max=-1, lanc_n=50):
+    S = po.System(ms, vpp)
+    ghf, st_hf = po.run(S, po.Prop.bare(S), "hf", itr_max=-1, conv_kind=conv_kind, conv=hf_conv)
+    g, st = po.run(S, ghf, "sc0", itr_max=sc_itr_max, lanczos=lanczos, lanc_kind=1, lanc_n=lanc_n,
+                   conv_kind=conv_kind, conv=sc_conv)
+    return g.to_propagator(ms.nsubsh), st_hf, st
+
+
+def check_prop(g, ref, tol=1e-10, zsig=1e-6):
+    assert np.array_equal(g.np_, ref.np_) and np.array_equal(g.nh, ref.nh), "fragment counts differ"
+    for i in range(len(g.np_)):
+        ep, zp, _, eh, zh, _ = g.subshell(i)
+        rp, rz, _, rh, rzh, _ = ref.subshell(i)
+        # quasiparticles and quasiholes separately, so the split at E_F is checked too
+        for a, b in ((np.stack([ep, zp], 1), np.stack([rp, rz], 1)), (np.stack([eh, zh], 1), np.stack([rh, rzh], 1))):
+            e_err, z_err, m_err = spectral_parity(a, b, tol, zsig=zsig)
+            assert e_err < tol and z_err < tol and m_err < tol, (i, e_err, z_err, m_err)
+
+
+def test_config1_full_sigma_sc0():
+    """Plain FP64 config 1: HF start, full (unreduced) Sigma(2), sc0 with the
+    reference's ItrMax=20 and sum|de| < 1e-7 criterion."""
+    ms, vpp = synth.make_system(3, 0)
+    par = capi.solve_params(mode=2, hf_itr_max=-1, sc_itr_max=20, lanczos=False, conv_kind=0, hf_conv=1e-7,
+                            sc_conv=1e-7)
+    b = capi.Batch([(ms, vpp)], par)
+    b.solve()
+    st = b.stats(0)
+    print("config1 full", st, b.timings(), flush=True)
+    ref, st_hf, st_sc = oracle_solve(ms, vpp, False, 0, 1e-7, 1e-7, sc_itr_max=20)
+    assert st["hf_its"] == st_hf.iterations and st["sc_its"] == st_sc.iterations
+    check_prop(b.result(0), ref)
+    assert abs(st["e_kolt"] - st_sc.e_kolt) < 1e-9 * max(1.0, abs(st_sc.e_kolt))
+
+
+def test_nmax5_lanczos_sc0():
+    """Lanczos sc0 at Nmax=5, where the reference is self-consistent under
+    pole reordering (tests/test_oracle_dyson.py::test_lanczos_breakdown_sensitivity)."""
+    ms, vpp = synth.make_system(5, 0)
+    par = capi.solve_params(mode=2, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1, hf_conv=1e-8, sc_conv=1e-8)
+    b = capi.Batch([(ms, vpp)], par)
+    b.solve()
+    st = b.stats(0)
+    print("nmax5 lanczos", st, b.timings(), flush=True)
+    ref, st_hf, st_sc = oracle_solve(ms, vpp, True, 1, 1e-8, 1e-8)
+    assert st["hf_its"] == st_hf.iterations and st["sc_its"] == st_sc.iterations
+    check_prop(b.result(0), ref, zsig=ZSIG_BY_NMAX[5])
+
+
+def test_config2_lanczos_sc0():
+    """Plain FP64 config 2: Nmax=7, 50 Krylov vectors per side, sc0 to
+    max|de| < 1e-8."""
+    ms, vpp = synth.make_system(7, 0)
+    par = capi.solve_params(mode=2, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1, hf_conv=1e-8, sc_conv=1e-8)
+    b = capi.Batch([(ms, vpp)], par)
+    b.solve()
+    st = b.stats(0)
+    print("config2", st, b.timings(), flush=True)
+    ref, st_hf, st_sc = oracle_solve(ms, vpp, True, 1, 1e-8, 1e-8)
+    assert st["hf_its"] == st_hf.iterations and st["sc_its"] == st_sc.iterations
+    check_prop(b.result(0), ref, zsig=ZSIG_BY_NMAX[7])
+
+
+def test_batch_equals_single():
+    """Instances of different sizes in one batch give exactly what they give alone."""
+    systems = [synth.make_system(nm, s) for nm, s in ((3, 1), (5, 2), (3, 3))]
+    par = capi.solve_params(mode=2, lanczos=True, lanc_kind=1, lanc_n=30, conv_kind=1, hf_conv=1e-8, sc_conv=1e-8)
+    b = capi.Batch(systems, par)
+    b.solve()
+    for i, sysm in enumerate(systems):
+        one = capi.Batch([sysm], par)
+        one.solve()
+        ga, gb = b.result(i), one.result(0)
+        assert np.array_equal(ga.ep, gb.ep) and np.array_equal(ga.eh, gb.eh), i
+        assert np.array_equal(ga.fqp, gb.fqp) and np.array_equal(ga.fqh, gb.fqh), i
+        assert b.stats(i) == one.stats(0)

@@ -1,0 +1,190 @@
+"""GPU parity of each stage entry point of the C ABI.
+
+Checks, per stage:
+* Sigma(2) rows (bcd_sigma2_build): bit-identical to the compiled reference's
+  golden rows (same loop and operation order, FP contraction off).
+* pole counts (bcd_count_poles): equal to the reference's.
+* Sigma^inf (bcd_hf_pot): 1e-13 relative (floor 1) to the reference's; the
+  GPU sums over J before the densities, so the last bits differ.
+* Lanczos (bcd_reduce_selfen_lncz): rows up to the first Krylov breakdown
+  (|b| < 1e-8, the reference's own flag) agree with the reference's to 1e-8
+  relative (block-wide reductions change the rounding of every projection,
+  which the recursion amplifies in late rows); past a breakdown the reference
+  keeps orthogonalising rounding noise, so those rows are
+  implementation-dependent.  Dyson spectra built from the GPU-reduced and the
+  reference-reduced self-energies agree (helpers.spectral_parity: Z >= 1e-6
+  solutions and Z-weighted moments to 1e-10) in every channel where no chain
+  broke down before the last pivot's chain.
+* Dyson (bcd_dyson): energies 1e-10 relative (floor 1), spectroscopic factors
+  1e-10 absolute, overlaps up to sign 1e-8, against the oracle; plus the
+  reference-held fixture bcdwk + HF -> sp_prop_O16_2nd_Lanc50 to print precision.
+* eigensolver (bcd_syevd_batched): LAPACK via numpy.linalg.eigh.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from bcdor_amd import capi
+from helpers import ZSIG_BY_NMAX, load_golden, rel_err, spectral_parity
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["c1_hf_split50", "c1_hf_piv10", "c1_dressed", "c2_hf_split50"]
+
+
+@pytest.fixture(scope="module", params=CASES)
+def case(request):
+    ms, vpp, g, z = load_golden(request.param)
+    print(f"case {request.param}", flush=True)
+    ctx = capi.Context(ms, vpp)
+    return request.param, ms, vpp, g, z, ctx
+
+
+def test_pole_counts(case):
+    name, ms, vpp, g, z, ctx = case
+    for i in z["channels"]:
+        assert ctx.count_poles(int(i), g) == tuple(z[f"count_{i}"]), (name, i)
+
+
+def test_sigma2_rows_bit_identical(case):
+    name, ms, vpp, g, z, ctx = case
+    for i in z["channels"]:
+        fw, bk = ctx.sigma2_build(int(i), g)
+        assert fw.shape == z[f"fw_{i}"].shape and bk.shape == z[f"bk_{i}"].shape, (name, i)
+        assert np.array_equal(fw, z[f"fw_{i}"]), (name, i, rel_err(fw, z[f"fw_{i}"]))
+        assert np.array_equal(bk, z[f"bk_{i}"]), (name, i, rel_err(bk, z[f"bk_{i}"]))
+
+
+def test_sigma_inf(case):
+    name, ms, vpp, g, z, ctx = case
+    for i in z["channels"]:
+        u = ctx.hf_pot(int(i), g)
+        assert rel_err(u, z[f"hf_{i}"]) < 1e-13, (name, i, rel_err(u, z[f"hf_{i}"]))
+
+
+def spectrum(n, j2, fw, pfw, bk, pbk, den):
+    h = np.diag(np.linspace(-30.0, 60.0, n))
+    sq, sh, _ = po.dyson(n, j2, h, np.zeros((n, n)), fw, pfw, int(pfw[0]) if fw.shape[0] else 0, bk, pbk,
+                         int(pbk[0]) if bk.shape[0] else 0, 0.0, den_fw=den, den_bk=den)
+    return np.concatenate([sq, sh])
+
+
+def breakdown(rows, ipiv):
+    """First row at which a chain of a reduced side broke down, as the
+    reference flags it (|b| < 1e-8 on a continuation row, Lanczos.cpp:220),
+    and whether that happened before the last pivot's chain started.  Past a
+    breakdown the reference continues from rounding noise: the rows, and the
+    spectrum when a later pivot's chain mixes with the noise, are then
+    determined by rounding."""
+    n = rows.shape[0]
+    if n == 0:
+        return 0, False
+    npv = int(ipiv[0])
+    starts = {0} | {int(x) for x in ipiv[2:2 + npv] if 0 <= x < n}
+    for j in range(n):
+        if j not in starts and abs(rows[j, 1]) < 1e-8:
+            last_start = max(starts)
+            return j, j < last_start
+    return n, False
+
+
+def test_lanczos_stage(case):
+    name, ms, vpp, g, z, ctx = case
+    clean = 0
+    for i in z["channels"]:
+        if f"lfw_{i}" not in z.files:
+            continue
+        n = int(ms.no[i])
+        ofw, pfw, obk, pbk, den, ifw, ibk, sfw, sbk = ctx.reduce_selfen_lncz(
+            n, z[f"fw_{i}"], z[f"bk_{i}"], z[f"piv_vec_{i}"], z[f"piv_fw_{i}"], z[f"piv_bk_{i}"])
+        assert np.array_equal(np.stack([ifw, ibk]), z[f"itr_{i}"]), (name, i)
+        assert ofw.shape == z[f"lfw_{i}"].shape and obk.shape == z[f"lbk_{i}"].shape, (name, i)
+        if ofw.shape[0]:
+            assert np.array_equal(pfw[:pfw[0] + 2], z[f"ipfw_{i}"]), (name, i)
+        if obk.shape[0]:
+            assert np.array_equal(pbk[:pbk[0] + 2], z[f"ipbk_{i}"]), (name, i)
+        mixed = False
+        for mine, ref, ipv in ((ofw, z[f"lfw_{i}"], z[f"ipfw_{i}"]), (obk, z[f"lbk_{i}"], z[f"ipbk_{i}"])):
+            k, mix = breakdown(ref, ipv)
+            mixed |= mix
+            assert rel_err(mine[:k], ref[:k]) < 1e-8, (name, i, k, rel_err(mine[:k], ref[:k]))
+        if mixed:
+            continue
+        clean += 1
+        s_gpu = spectrum(n, int(ms.j2[i]), ofw, pfw, obk, pbk, den)
+        s_ref = spectrum(n, int(ms.j2[i]), z[f"lfw_{i}"], z[f"ipfw_{i}"], z[f"lbk_{i}"], z[f"ipbk_{i}"], den)
+        e_err, z_err, m_err = spectral_parity(s_gpu, s_ref, zsig=ZSIG_BY_NMAX[int(z["nmax"])])
+        assert e_err < 1e-10 and z_err < 1e-10 and m_err < 1e-10, (name, i, e_err, z_err, m_err)
+    assert clean > 0
+
+
+def compare_records(a, b, tol_e=1e-10, tol_z=1e-10, tol_x=1e-8):
+    assert a.shape == b.shape
+    if a.shape[0] == 0:
+        return
+    assert rel_err(a[:, 0], b[:, 0]) < tol_e, rel_err(a[:, 0], b[:, 0])
+    assert np.max(np.abs(a[:, 1] - b[:, 1])) < tol_z
+    xa, xb = a[:, 2:], b[:, 2:]
+    sgn = np.sign(np.sum(xa * xb, 1)); sgn[sgn == 0] = 1
+    big = np.sum(xb * xb, 1) > 1e-6
+    assert np.max(np.abs(xa * sgn[:, None] - xb)[big], initial=0.0) < tol_x
+
+
+def test_dyson_config1_full_sigma():
+    from bcdor_amd import synth
+    ms, vpp = synth.make_system(3, 0)
+    S = po.System(ms, vpp)
+    ghf, _ = po.run(S, po.Prop.bare(S), "hf", itr_max=-1, conv_kind=0, conv=1e-12)
+    ctx = capi.Context(ms, vpp)
+    for i in range(ms.nsubsh):
+        n = int(ms.no[i])
+        fw, bk = po.build_sigma2(S, i, ghf)
+        h = po.hf_pot(S, i, ghf)
+        u1 = np.diag(ms.orb_e[ms.orb0[i]:ms.orb0[i] + n])
+        ef = po.seek_efermi(S, ghf, i)
+        a = po.dyson(n, int(ms.j2[i]), h, u1, fw, None, 0, bk, None, 0, ef)
+        b = ctx.dyson(n, int(ms.j2[i]), h, u1, fw, None, 0, bk, None, 0, ef)
+        print(f"config1 channel {i}: n={n + fw.shape[0] + bk.shape[0]}", flush=True)
+        compare_records(b[0], a[0])
+        compare_records(b[1], a[1])
+        assert rel_err(b[2], a[2]) < 1e-9
+
+
+def test_dyson_reference_fixture():
+    from test_oracle_dyson import efermi, fixture_channels, hf_matrix, kinetic
+    ms, ghf, g2, sig = fixture_channels()
+    ctx_ms, _ = ms, None
+    from bcdor_amd import synth
+    dummy = synth.Interaction(*(np.zeros(0, np.int32) for _ in range(5)), np.zeros(0))
+    ctx = capi.Context(ms, dummy)
+    worst = 0.0
+    for i in range(ms.nsubsh):
+        s = sig[i]
+        h = hf_matrix(ms, ghf, i)
+        u1 = kinetic(ms, i, 14.0)
+        ef = efermi(ms, ghf, i)
+        args = (int(ms.no[i]), int(ms.j2[i]), h - u1, u1, s.fw, s.i_piv_fw, int(s.i_piv_fw[0]), s.bk, s.i_piv_bk,
+                int(s.i_piv_bk[0]), ef)
+        a = po.dyson(*args, den_fw=s.ntot_den, den_bk=s.ntot_den)
+        b = ctx.dyson(*args, den_fw=s.ntot_den, den_bk=s.ntot_den)
+        compare_records(b[0], a[0])
+        compare_records(b[1], a[1])
+        ep, zp, fqp, eh, zh, fqh = g2.subshell(i)
+        worst = max(worst, np.max(np.abs(b[0][:, 0] - ep), initial=0), np.max(np.abs(b[1][:, 0] - eh), initial=0))
+    assert worst < 1e-5
+
+
+def test_syevd_batched():
+    from bcdor_amd import synth
+    ms, vpp = synth.make_system(1, 0)
+    ctx = capi.Context(ms, vpp)
+    rng = np.random.default_rng(3)
+    mats = []
+    for n in (1, 2, 5, 17, 40, 64):
+        a = rng.standard_normal((n, n)); mats.append(a + a.T)
+    out = ctx.syevd_batched(mats)
+    for a, (w, v) in zip(mats, out):
+        wl = np.linalg.eigh(a)[0]
+        assert np.max(np.abs(w - wl)) < 1e-12 * max(1.0, np.abs(wl).max())
+        assert np.max(np.abs(a @ v - v * w)) < 1e-11 * max(1.0, np.abs(wl).max())
+        assert np.max(np.abs(v.T @ v - np.eye(len(w)))) < 1e-12
