@@ -48,7 +48,7 @@ class BcdSolveParams(C.Structure):
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
            "bcd_count_poles", "bcd_sigma2_build", "bcd_hf_pot", "bcd_reduce_mtx_lncz", "bcd_reduce_selfen_lncz",
            "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_destroy", "bcd_batch_solve",
-           "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_result",
+           "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
            "bcd_batch_kernel_times", "bcd_batch_work"]
 
 KERNEL_NAMES = ["sigma2", "lanczos", "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild", "sigma2_plan"]
@@ -98,6 +98,7 @@ def lib():
         L.bcd_batch_set_profile.argtypes = [vp, C.c_int]
         L.bcd_batch_timings.argtypes = [vp, DP, C.c_int]
         L.bcd_batch_stats.argtypes = [vp, C.c_int, IP, IP, DP, DP, LP, LP]
+        L.bcd_batch_converged.argtypes = [vp, C.c_int]
         L.bcd_batch_result.argtypes = [vp, C.c_int, C.c_int, IP, IP, IP, IP, DP, DP, DP, DP, DP, DP]
         L.bcd_batch_kernel_times.argtypes = [vp, C.c_int, DP, LP]
         L.bcd_batch_work.argtypes = [vp, DP, C.c_int]
@@ -306,6 +307,9 @@ class Batch:
         check(lib().bcd_batch_stats(self.h, i, C.byref(a), C.byref(b), C.byref(d), C.byref(e), C.byref(f),
                                     C.byref(g)), "bcd_batch_stats")
         return dict(hf_its=a.value, sc_its=b.value, last_diff=d.value, e_kolt=e.value, n_fw=f.value, n_bk=g.value)
+
+    def converged(self, i) -> bool:
+        return check(lib().bcd_batch_converged(self.h, i), "bcd_batch_converged") == 1
 
     def result(self, i) -> Propagator:
         ms = self.systems[i][0]

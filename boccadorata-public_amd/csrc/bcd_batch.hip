@@ -335,7 +335,10 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   const int hard_cap = itr_max >= 0 ? itr_max + 1 : kMaxSweeps;
   int n_active = n_inst;
   for (int it = 0; it < hard_cap && n_active > 0; ++it) BTRY(sweep(src, n_active));
-  for (int i = 0; i < n_inst; ++i) (hf ? hf_its : sc_its)[i] = h_is[i].its;
+  for (int i = 0; i < n_inst; ++i) {
+    (hf ? hf_its : sc_its)[i] = h_is[i].its;
+    (hf ? hf_ok : sc_ok)[i] = h_is[i].diff < conv;
+  }
   return BCD_OK;
 }
 
@@ -490,6 +493,8 @@ int Batch::solve() {
   for (int k = 0; k < KT_N; ++k) { ktime[k] = 0.0; kcount[k] = 0; }
   hf_its.assign(n_inst, 0);
   sc_its.assign(n_inst, 0);
+  hf_ok.assign(n_inst, 0);
+  sc_ok.assign(n_inst, par.mode == 0 ? 1 : 0);
   HTRY(hipMemsetAsync(d_status, 0, sizeof(int), stream));
   HTRY(hipEventRecord(ev[0], stream));
   // start from the initial propagator in buffer 0

@@ -41,7 +41,7 @@ struct Batch {
   std::vector<BlockDev> h_blocks;
   std::vector<ChanState> h_cs;
   std::vector<InstState> h_is;
-  std::vector<int> hf_its, sc_its;
+  std::vector<int> hf_its, sc_its, hf_ok, sc_ok;
   std::vector<long> n_fw_tot, n_bk_tot;
 
   DevArena a_tab_i, a_tab_l, a_tab_d, a_inst, a_chan, a_blocks, a_work, a_prop, a_rows, a_lws, a_mat, a_jobs;

@@ -559,6 +559,11 @@ int bcd_batch_stats(bcd_batch *b, int inst, int *hf_its, int *sc_its, double *la
   return BCD_OK;
 }
 
+int bcd_batch_converged(bcd_batch *b, int inst) {
+  if (!b || inst < 0 || inst >= b->b.n_inst || b->b.hf_ok.empty()) return BCD_ERR_ARG;
+  return (b->b.hf_ok[inst] && b->b.sc_ok[inst]) ? 1 : 0;
+}
+
 int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep,
                      double *zp, double *fqp, double *eh, double *zh, double *fqh) {
   if (!b) return BCD_ERR_ARG;

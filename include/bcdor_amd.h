@@ -162,6 +162,9 @@ int bcd_batch_timings(bcd_batch *b, double *ms, int n);
 /* Iterations, last convergence measure and Koltun energy of instance i. */
 int bcd_batch_stats(bcd_batch *b, int inst, int *hf_its, int *sc_its, double *last_diff,
                     double *e_kolt, long *n_fw_poles, long *n_bk_poles);
+/* 1 when every stage of the last solve of instance i met its convergence
+ * criterion (rather than stopping at ItrMax), 0 otherwise, < 0 on error. */
+int bcd_batch_converged(bcd_batch *b, int inst);
 /* Final propagator of instance i: first call with NULL arrays to get the
  * totals, then with arrays sized sum np / sum nh (overlaps nspb-strided). */
 int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh,
