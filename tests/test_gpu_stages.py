@@ -6,12 +6,13 @@ Checks, per stage:
 * pole counts (bcd_count_poles): equal to the reference's.
 * Sigma^inf (bcd_hf_pot): 1e-13 relative (floor 1) to the reference's; the
   GPU sums over J before the densities, so the last bits differ.
-* Lanczos (bcd_reduce_selfen_lncz): rows up to the first Krylov breakdown
-  (|b| < 1e-8, the reference's own flag) agree with the reference's to 1e-8
-  relative (block-wide reductions change the rounding of every projection,
-  which the recursion amplifies in late rows); past a breakdown the reference
-  keeps orthogonalising rounding noise, so those rows are
-  implementation-dependent.  Dyson spectra built from the GPU-reduced and the
+* Lanczos (bcd_reduce_selfen_lncz): iteration counts and i_piv exactly; the
+  leading rows that the reference reproduces under rounding-level input noise
+  (reproducible_rows), and never past the first Krylov breakdown (|b| < 1e-8,
+  the reference's own flag), agree with the reference's to 1e-8 relative
+  (block-wide reductions change the rounding of every projection, which the
+  recursion amplifies in late rows; past a breakdown the reference keeps
+  orthogonalising rounding noise).  Dyson spectra built from the GPU-reduced and the
   reference-reduced self-energies agree (helpers.spectral_parity: Z >= 1e-6
   solutions and Z-weighted moments to 1e-10) in every channel where no chain
   broke down before the last pivot's chain.
@@ -88,6 +89,29 @@ def breakdown(rows, ipiv):
     return n, False
 
 
+def reproducible_rows(n, fw, bk, piv, itr_fw, itr_bk, ref_fw, ref_bk):
+    """Per side, the number of leading reduced rows that the reference
+    algorithm itself reproduces (to 1e-9) when its input couplings carry
+    rounding-level noise (1e-15 relative, two draws).  When a chain nears the
+    exhaustion of its Krylov space (L close to the number of distinct pole
+    energies) the late rows amplify rounding by many orders of magnitude in the
+    reference too; only the leading rows are pinned row by row, the rest
+    through the Dyson spectrum."""
+    out = [min(ref_fw.shape[0], 10 ** 9), min(ref_bk.shape[0], 10 ** 9)]
+    for seed in (1, 2):
+        rng = np.random.default_rng(seed)
+        f2, b2 = fw.copy(), bk.copy()
+        f2[:, 2:] *= 1.0 + 1e-15 * rng.standard_normal(f2[:, 2:].shape)
+        b2[:, 2:] *= 1.0 + 1e-15 * rng.standard_normal(b2[:, 2:].shape)
+        pf, _, pb, _, _, _, _ = po.reduce_selfen_lncz(n, f2, b2, piv, itr_fw, itr_bk)
+        for s, (mine, ref) in enumerate(((pf, ref_fw), (pb, ref_bk))):
+            for j in range(ref.shape[0]):
+                if rel_err(mine[j], ref[j]) > 1e-9:
+                    out[s] = min(out[s], j)
+                    break
+    return out
+
+
 def test_lanczos_stage(case):
     name, ms, vpp, g, z, ctx = case
     clean = 0
@@ -103,10 +127,15 @@ def test_lanczos_stage(case):
             assert np.array_equal(pfw[:pfw[0] + 2], z[f"ipfw_{i}"]), (name, i)
         if obk.shape[0]:
             assert np.array_equal(pbk[:pbk[0] + 2], z[f"ipbk_{i}"]), (name, i)
+        keep = reproducible_rows(n, z[f"fw_{i}"], z[f"bk_{i}"], z[f"piv_vec_{i}"], z[f"piv_fw_{i}"],
+                                 z[f"piv_bk_{i}"], z[f"lfw_{i}"], z[f"lbk_{i}"])
         mixed = False
-        for mine, ref, ipv in ((ofw, z[f"lfw_{i}"], z[f"ipfw_{i}"]), (obk, z[f"lbk_{i}"], z[f"ipbk_{i}"])):
+        for s, (mine, ref, ipv) in enumerate(((ofw, z[f"lfw_{i}"], z[f"ipfw_{i}"]),
+                                              (obk, z[f"lbk_{i}"], z[f"ipbk_{i}"]))):
             k, mix = breakdown(ref, ipv)
             mixed |= mix
+            k = min(k, keep[s])
+            assert k >= min(1, ref.shape[0]), (name, i, s, k)
             assert rel_err(mine[:k], ref[:k]) < 1e-8, (name, i, k, rel_err(mine[:k], ref[:k]))
         if mixed:
             continue
