@@ -13,8 +13,10 @@ Checks, per stage:
   (block-wide reductions change the rounding of every projection, which the
   recursion amplifies in late rows; past a breakdown the reference keeps
   orthogonalising rounding noise).  Dyson spectra built from the GPU-reduced and the
-  reference-reduced self-energies agree (helpers.spectral_parity: Z >= 1e-6
-  solutions and Z-weighted moments to 1e-10) in every channel where no chain
+  reference-reduced self-energies agree (helpers.spectral_parity: solutions
+  above the Z cut and Z-weighted moments) to 1e-10, or to ten times the
+  reference's own spread under that noise where it is larger (dressed input
+  propagators with many degenerate poles), in every channel where no chain
   broke down before the last pivot's chain.
 * Dyson (bcd_dyson): energies 1e-10 relative (floor 1), spectroscopic factors
   1e-10 absolute, overlaps up to sign 1e-8, against the oracle; plus the
@@ -89,21 +91,28 @@ def breakdown(rows, ipiv):
     return n, False
 
 
-def reproducible_rows(n, fw, bk, piv, itr_fw, itr_bk, ref_fw, ref_bk):
-    """Per side, the number of leading reduced rows that the reference
-    algorithm itself reproduces (to 1e-9) when its input couplings carry
-    rounding-level noise (1e-15 relative, two draws).  When a chain nears the
-    exhaustion of its Krylov space (L close to the number of distinct pole
-    energies) the late rows amplify rounding by many orders of magnitude in the
-    reference too; only the leading rows are pinned row by row, the rest
-    through the Dyson spectrum."""
-    out = [min(ref_fw.shape[0], 10 ** 9), min(ref_bk.shape[0], 10 ** 9)]
+def perturbed_reductions(n, fw, bk, piv, itr_fw, itr_bk):
+    """The reference algorithm's reduction of the same poles with
+    rounding-level noise (1e-15 relative, two draws) on the input couplings."""
+    out = []
     for seed in (1, 2):
         rng = np.random.default_rng(seed)
         f2, b2 = fw.copy(), bk.copy()
         f2[:, 2:] *= 1.0 + 1e-15 * rng.standard_normal(f2[:, 2:].shape)
         b2[:, 2:] *= 1.0 + 1e-15 * rng.standard_normal(b2[:, 2:].shape)
-        pf, _, pb, _, _, _, _ = po.reduce_selfen_lncz(n, f2, b2, piv, itr_fw, itr_bk)
+        out.append(po.reduce_selfen_lncz(n, f2, b2, piv, itr_fw, itr_bk))
+    return out
+
+
+def reproducible_rows(pert, ref_fw, ref_bk):
+    """Per side, the number of leading reduced rows that every perturbed
+    reduction reproduces to 1e-9.  When a chain nears the exhaustion of its
+    Krylov space (L close to the number of distinct pole energies) the late
+    rows amplify rounding by many orders of magnitude in the reference too;
+    only the leading rows are pinned row by row, the rest through the Dyson
+    spectrum."""
+    out = [ref_fw.shape[0], ref_bk.shape[0]]
+    for pf, _, pb, _, _, _, _ in pert:
         for s, (mine, ref) in enumerate(((pf, ref_fw), (pb, ref_bk))):
             for j in range(ref.shape[0]):
                 if rel_err(mine[j], ref[j]) > 1e-9:
@@ -113,6 +122,9 @@ def reproducible_rows(n, fw, bk, piv, itr_fw, itr_bk, ref_fw, ref_bk):
 
 
 def test_lanczos_stage(case):
+    """Spectra: the GPU-reduced self-energy must reproduce the reference's
+    Dyson spectrum to 1e-10, or to ten times the spread that rounding-level
+    input noise causes in the reference itself where that is larger."""
     name, ms, vpp, g, z, ctx = case
     clean = 0
     for i in z["channels"]:
@@ -127,8 +139,9 @@ def test_lanczos_stage(case):
             assert np.array_equal(pfw[:pfw[0] + 2], z[f"ipfw_{i}"]), (name, i)
         if obk.shape[0]:
             assert np.array_equal(pbk[:pbk[0] + 2], z[f"ipbk_{i}"]), (name, i)
-        keep = reproducible_rows(n, z[f"fw_{i}"], z[f"bk_{i}"], z[f"piv_vec_{i}"], z[f"piv_fw_{i}"],
-                                 z[f"piv_bk_{i}"], z[f"lfw_{i}"], z[f"lbk_{i}"])
+        pert = perturbed_reductions(n, z[f"fw_{i}"], z[f"bk_{i}"], z[f"piv_vec_{i}"], z[f"piv_fw_{i}"],
+                                    z[f"piv_bk_{i}"])
+        keep = reproducible_rows(pert, z[f"lfw_{i}"], z[f"lbk_{i}"])
         mixed = False
         for s, (mine, ref, ipv) in enumerate(((ofw, z[f"lfw_{i}"], z[f"ipfw_{i}"]),
                                               (obk, z[f"lbk_{i}"], z[f"ipbk_{i}"]))):
@@ -139,11 +152,18 @@ def test_lanczos_stage(case):
             assert rel_err(mine[:k], ref[:k]) < 1e-8, (name, i, k, rel_err(mine[:k], ref[:k]))
         if mixed:
             continue
+        zsig = ZSIG_BY_NMAX[int(z["nmax"])]
+        j2 = int(ms.j2[i])
+        s_ref = spectrum(n, j2, z[f"lfw_{i}"], z[f"ipfw_{i}"], z[f"lbk_{i}"], z[f"ipbk_{i}"], den)
+        spread = np.zeros(3)
+        for qf, qpf, qb, qpb, qden, _, _ in pert:
+            spread = np.maximum(spread, spectral_parity(spectrum(n, j2, qf, qpf, qb, qpb, qden), s_ref, zsig=zsig))
+        if not np.all(np.isfinite(spread)):
+            continue    # the reference's own solution count above zsig is rounding-determined here
         clean += 1
-        s_gpu = spectrum(n, int(ms.j2[i]), ofw, pfw, obk, pbk, den)
-        s_ref = spectrum(n, int(ms.j2[i]), z[f"lfw_{i}"], z[f"ipfw_{i}"], z[f"lbk_{i}"], z[f"ipbk_{i}"], den)
-        e_err, z_err, m_err = spectral_parity(s_gpu, s_ref, zsig=ZSIG_BY_NMAX[int(z["nmax"])])
-        assert e_err < 1e-10 and z_err < 1e-10 and m_err < 1e-10, (name, i, e_err, z_err, m_err)
+        tol = np.maximum(1e-10, 10.0 * spread)
+        got = spectral_parity(spectrum(n, j2, ofw, pfw, obk, pbk, den), s_ref, zsig=zsig)
+        assert np.all(np.array(got) < tol), (name, i, got, tol)
     assert clean > 0
 
 
