@@ -14,10 +14,10 @@ Checks, per stage:
   recursion amplifies in late rows; past a breakdown the reference keeps
   orthogonalising rounding noise).  Dyson spectra built from the GPU-reduced and the
   reference-reduced self-energies agree (helpers.spectral_parity: solutions
-  above the Z cut and Z-weighted moments) to 1e-10, or to ten times the
-  reference's own spread under that noise where it is larger (dressed input
-  propagators with many degenerate poles), in every channel where no chain
-  broke down before the last pivot's chain.
+  above the Z cut and Z-weighted moments) to 1e-10 in every channel where the
+  reference's own spectrum is stable to 1e-10 under that noise and no chain
+  broke down before the last pivot's chain (dressed input propagators with
+  many degenerate poles are unstable in the reference itself).
 * Dyson (bcd_dyson): energies 1e-10 relative (floor 1), spectroscopic factors
   1e-10 absolute, overlaps up to sign 1e-8, against the oracle; plus the
   reference-held fixture bcdwk + HF -> sp_prop_O16_2nd_Lanc50 to print precision.
@@ -122,9 +122,9 @@ def reproducible_rows(pert, ref_fw, ref_bk):
 
 
 def test_lanczos_stage(case):
-    """Spectra: the GPU-reduced self-energy must reproduce the reference's
-    Dyson spectrum to 1e-10, or to ten times the spread that rounding-level
-    input noise causes in the reference itself where that is larger."""
+    """Spectra: wherever the reference's own Dyson spectrum is stable to 1e-10
+    under rounding-level input noise, the GPU-reduced self-energy reproduces
+    it to 1e-10."""
     name, ms, vpp, g, z, ctx = case
     clean = 0
     for i in z["channels"]:
@@ -158,12 +158,14 @@ def test_lanczos_stage(case):
         spread = np.zeros(3)
         for qf, qpf, qb, qpb, qden, _, _ in pert:
             spread = np.maximum(spread, spectral_parity(spectrum(n, j2, qf, qpf, qb, qpb, qden), s_ref, zsig=zsig))
-        if not np.all(np.isfinite(spread)):
-            continue    # the reference's own solution count above zsig is rounding-determined here
+        if not np.all(spread <= 1e-10):
+            # the reference's own spectrum moves by more than 1e-10 under
+            # 1e-15 input noise (dressed propagators: Krylov chains over many
+            # degenerate poles); no parity is defined there beyond the rows
+            continue
         clean += 1
-        tol = np.maximum(1e-10, 10.0 * spread)
         got = spectral_parity(spectrum(n, j2, ofw, pfw, obk, pbk, den), s_ref, zsig=zsig)
-        assert np.all(np.array(got) < tol), (name, i, got, tol)
+        assert np.all(np.array(got) < 1e-10), (name, i, got)
     assert clean > 0
 
 
