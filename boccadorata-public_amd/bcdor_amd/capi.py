@@ -42,7 +42,7 @@ class BcdProp(C.Structure):
 class BcdSolveParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("hf_itr_max", C.c_int), ("sc_itr_max", C.c_int), ("lanczos", C.c_int),
                 ("lanc_kind", C.c_int), ("lanc_n", C.c_int), ("conv_kind", C.c_int), ("hf_conv", C.c_double),
-                ("sc_conv", C.c_double), ("t_fac", C.c_double), ("ext_u1", C.c_int)]
+                ("sc_conv", C.c_double), ("t_fac", C.c_double), ("ext_u1", C.c_int), ("dyson_solver", C.c_int)]
 
 
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
@@ -51,7 +51,8 @@ EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_creat
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
            "bcd_batch_kernel_times", "bcd_batch_work"]
 
-KERNEL_NAMES = ["sigma2", "lanczos", "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild", "sigma2_plan"]
+KERNEL_NAMES = ["sigma2", "lanczos", "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild", "sigma2_plan",
+                "arrow", "poles"]
 
 _LIB = None
 
@@ -258,9 +259,10 @@ class Context:
 
 
 def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1,
-                 hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1) -> BcdSolveParams:
+                 hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1, dyson_solver=0) -> BcdSolveParams:
+    """dyson_solver 0: arrowhead updates (default); 1: dense Householder + QL per sweep."""
     return BcdSolveParams(mode, hf_itr_max, sc_itr_max, int(lanczos), lanc_kind, lanc_n, conv_kind, hf_conv,
-                          sc_conv, t_fac, ext_u1)
+                          sc_conv, t_fac, ext_u1, dyson_solver)
 
 
 class Batch:

@@ -1,0 +1,106 @@
+// bcd_arrow.hip -- Dyson sweeps by successive arrowhead updates on gfx950.
+//
+// k_poles_prep + k_tridiag/k_ql (poles mode) diagonalise every
+// Lanczos-reduced side once per solve; each sweep then runs k_arrow_prep and
+// one k_arrow block per channel (DysonMatrix.cpp:123-532 for the matrix,
+// :436-484 for the split and ordering, :542-594 for Calc_En).
+#include "bcd_kernels.hpp"
+
+namespace bcd {
+
+namespace {
+
+struct DevBlock {
+  int *status;
+  __device__ int tid() const { return threadIdx.x; }
+  __device__ int nt() const { return blockDim.x; }
+  __device__ void sync() const { __syncthreads(); }
+  __device__ void fail() const { atomicAdd(status, 1); }
+};
+
+}  // namespace
+
+// One block per channel; the workspace sits in LDS (IN_LDS) or in the job's
+// global slab.  Blocks of inactive channels (n <= 0) leave at once.
+template <bool IN_LDS>
+__global__ __launch_bounds__(kArrowMaxThreads) void k_arrow(const ArrowJob *jobs, int njobs, int *status) {
+  extern __shared__ double sm[];
+  if ((int)blockIdx.x >= njobs) return;
+  const ArrowJob J = jobs[blockIdx.x];
+  if (J.n <= 0) return;
+  arrow_body(DevBlock{status}, J, IN_LDS ? sm : J.ws);
+}
+
+template __global__ void k_arrow<true>(const ArrowJob *, int, int *);
+template __global__ void k_arrow<false>(const ArrowJob *, int, int *);
+
+// Jobs that diagonalise both reduced sides of every channel: job 2*gc + side.
+__global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
+                             double *tri, double *poles, DysonJob *jobs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * n_gc) return;
+  const int gc = t >> 1, side = t & 1;
+  const ChanDev C = chans[gc];
+  const ChanState &Q = cs[gc];
+  DysonJob &J = jobs[t];
+  const int L = side ? Q.lbk : Q.lfw, capS = side ? C.lbk_cap : C.lfw_cap;
+  J.nI = C.nI; J.j2 = C.j2; J.ch = C.ch; J.inst = C.inst; J._pad = 0;
+  J.n = L > 0 ? L : 0;
+  J.nfw = J.n; J.nbk = 0;
+  J.den_fw = J.den_bk = Q.den;
+  J.npiv_fw = J.n > 0 ? (side ? Q.ipiv_bk[0] : Q.ipiv_fw[0]) : 0; J.npiv_bk = 0;
+  J.efermi = 0.0;
+  J.h = nullptr; J.u1 = nullptr;
+  J.rfw = lrows + (side ? C.lbk_off : C.lfw_off); J.rbk = nullptr;
+  J.ipiv_fw = side ? Q.ipiv_bk : Q.ipiv_fw; J.ipiv_bk = nullptr;
+  J.tri = tri + C.tri_off + (side ? (long)C.lfw_cap * (2 + C.nI) : 0);
+  J.mat = nullptr;
+  J.out_e = nullptr; J.out_x = nullptr; J.out_nqp = nullptr; J.out_nqh = nullptr; J.kolt = nullptr;
+  J.poles = 1;
+  J.ct_ld = capS;
+  J.out_lam = poles + C.pole_off + (side ? (long)(1 + C.nI) * C.lfw_cap : 0);
+  J.out_ct = J.out_lam + capS;
+}
+
+// The arrowhead job of every channel for this sweep: static part, pole
+// source (none, unreduced rows, or the diagonalised reduced sides), Fermi
+// energy of the sweep's input propagator, output slots in the other buffer.
+__global__ void k_arrow_prep(const ChanDev *chans, int n_gc, const ChanState *cs, PropView P, int src,
+                             const double *h, const double *u1, const long *row_fw_base,
+                             const long *row_bk_base, const double *rows_fw, const double *rows_bk,
+                             const double *poles, double *aws, double *kolt, ArrowJob *jobs) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= n_gc) return;
+  const ChanDev C = chans[gc];
+  ArrowJob &J = jobs[gc];
+  const InstState *S = P.is ? P.is + C.inst : nullptr;
+  if (S && !S->active) { J.n = 0; return; }
+  const ChanState &Q = cs[gc];
+  const int cur = S ? S->cur : 0, nxt = 1 - cur, nI = C.nI;
+  J.nI = nI; J.j2 = C.j2; J.inst = C.inst;
+  J.h = h + C.h_off; J.u1 = u1 + C.u1_off;
+  for (int s = 0; s < 2; ++s) { J.np[s] = 0; J.es[s] = 1; J.ci[s] = 1; J.cr[s] = 1; J.E[s] = nullptr; J.C[s] = nullptr; }
+  if (src == 1) {
+    const int T = 2 + nI;
+    J.np[0] = Q.mfw; J.np[1] = Q.mbk;
+    J.E[0] = rows_fw + row_fw_base[gc]; J.E[1] = rows_bk + row_bk_base[gc];
+    for (int s = 0; s < 2; ++s) { J.C[s] = J.E[s] + 2; J.es[s] = T; J.ci[s] = T; J.cr[s] = 1; }
+  } else if (src == 2) {
+    J.np[0] = Q.lfw > 0 ? Q.lfw : 0; J.np[1] = Q.lbk > 0 ? Q.lbk : 0;
+    J.E[0] = poles + C.pole_off;
+    J.C[0] = J.E[0] + C.lfw_cap; J.cr[0] = C.lfw_cap;
+    J.E[1] = poles + C.pole_off + (long)(1 + nI) * C.lfw_cap;
+    J.C[1] = J.E[1] + C.lbk_cap; J.cr[1] = C.lbk_cap;
+  }
+  J.n = nI + J.np[0] + J.np[1];
+  if (J.n > C.cap) { J.n = 0; if (P.is) P.is[C.inst].err = 2; return; }
+  J.efermi = S ? S->ef[C.ch & 1] : 0.0;
+  J.ws = aws ? aws + C.aws_off : nullptr;
+  J.out_e = P.e[nxt] + P.e_off[gc];
+  J.out_x = P.x[nxt] + P.x_off[gc];
+  J.out_nqp = P.nqp[nxt] + gc;
+  J.out_nqh = P.nqh[nxt] + gc;
+  J.kolt = kolt + 3L * gc;
+}
+
+}  // namespace bcd

@@ -1,0 +1,357 @@
+// bcd_arrow.hpp -- the Dyson eigenproblem of one channel by successive
+// arrowhead updates.
+//
+// The Dyson matrix of DysonMatrix.cpp:269-390,
+//
+//       [ h   C^T ]      h: nI x nI static part (U1 + Sigma^inf),
+//   A = [         ],     C: couplings of the m self-energy poles,
+//       [ C   L   ]      L: diagonal pole energies,
+//
+// holds the poles in diagonal form: for a Lanczos-reduced side the fishbone
+// block D is diagonalised once per solve (D = U L U^T, C = U^T M), which is a
+// similarity transform that leaves the first nI components of every
+// eigenvector -- the quasiparticle amplitudes -- unchanged.  The eigenpairs of
+// A then follow from nI arrowhead problems: head k (orbit k) joins the group
+// of already-diagonal directions through its coupling vector, the arrowhead
+// [alpha z^T; z diag(d)] is solved by its secular equation
+//
+//   f(x) = x - alpha + sum_i z_i^2 / (d_i - x) = 0,
+//
+// one root per interval of the sorted d, the eigenvectors come from couplings
+// recomputed from the roots (Loewner's formula; Gu and Eisenstat, SIAM J.
+// Matrix Anal. Appl. 16 (1995) 172), and only the nI head rows of the
+// eigenvectors plus the couplings of the heads still to come are carried
+// along.  Work per channel is O(nI^2 (m+nI)^2) instead of O((m+nI)^3), and
+// the roots are independent, so every thread of a block owns a few.
+//
+// Deflation: couplings below tol = 8 eps max(|alpha|, max|d|, |z|) drop out
+// (the direction is an eigenvector to within tol); two directions whose
+// energies differ by less than tol after a Givens rotation that moves the
+// whole coupling onto one of them are merged the same way.
+//
+// The body is a template over the block it runs on (thread index, thread
+// count, barrier, failure flag) so that the same code runs as a kernel and,
+// in the tests, on host threads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+
+namespace bcd {
+
+constexpr int kArrowNI = 16;       // largest nI (orbits per subshell)
+constexpr int kArrowMaxIt = 100;   // secular-equation iterations per root
+constexpr int kArrowMaxThreads = 256;
+constexpr int kArrowRed = 2 * kArrowMaxThreads + 16;
+
+// One channel's Dyson problem in arrowhead form.  Poles come from up to two
+// sources: energy i of source s is E[s][i*es[s]], its coupling to orbit r is
+// C[s][i*ci[s] + r*cr[s]].
+struct ArrowJob {
+  int nI, j2, n, inst;             // n = nI + np[0] + np[1]; n <= 0: nothing to do
+  int np[2], es[2], ci[2], cr[2];
+  const double *E[2], *C[2];
+  const double *h, *u1;            // static part = u1 + h (nI x nI, row-major)
+  double efermi;
+  double *ws;                      // workspace when it does not sit in LDS
+  double *out_e, *out_x;           // records: qp (descending e) then qh (descending e)
+  int *out_nqp, *out_nqh;
+  double *kolt;                    // Calc_En partial sums
+};
+
+__host__ __device__ inline size_t arrow_ws_doubles(int n, int nI) {
+  return (size_t)(8 + 4 * nI) * n + kArrowRed;
+}
+
+__host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
+  return arrow_ws_doubles(n, nI) * sizeof(double) + (3 * (size_t)n + 8) * sizeof(int);
+}
+
+// Root j (0..p) of the secular equation with sorted poles d[0..p-1] and
+// couplings z (all non-zero).  The root is returned as an offset tau from the
+// pole org it is closest to, so that x - d_i = tau - (d_i - d_org) keeps full
+// relative accuracy near the poles.  Returns the iterations used (kArrowMaxIt
+// if the iteration did not settle).
+__host__ __device__ inline int arrow_root(int j, int p, const double *d, const double *z, double alpha,
+                                          double znorm, int *org_out, double *tau_out) {
+  const double eps = 2.2204460492503131e-16;
+  int o, far;
+  double lo, hi;
+  if (j == 0) {
+    o = 0; far = -1;
+    lo = ((fmin(alpha, d[0]) - d[0]) - znorm) * (1.0 + 1e-7); hi = 0.0;
+  } else if (j == p) {
+    o = p - 1; far = -1;
+    lo = 0.0; hi = ((fmax(alpha, d[p - 1]) - d[p - 1]) + znorm) * (1.0 + 1e-7);
+  } else {
+    // which half of (d[j-1], d[j]) holds the root: f increases on the interval
+    const double a = d[j - 1], b = d[j], mid = 0.5 * (a + b);
+    double f = mid - alpha;
+    for (int i = 0; i < p; ++i) f += z[i] * z[i] / (d[i] - mid);
+    if (f >= 0.0) { o = j - 1; far = j; lo = 0.0; hi = mid - a; }
+    else { o = j; far = j - 1; lo = mid - b; hi = 0.0; }
+  }
+  const double dofs = d[o], zo2 = z[o] * z[o];
+  double tau = 0.5 * (lo + hi);
+  int it = 0;
+  for (; it < kArrowMaxIt; ++it) {
+    double S = 0.0, Sp = 0.0, mag = 0.0;
+    for (int i = 0; i < p; ++i) {
+      if (i == o) continue;
+      const double r = 1.0 / ((d[i] - dofs) - tau);
+      const double t = z[i] * z[i] * r;
+      S += t; Sp += t * r; mag += fabs(t);
+    }
+    const double near = zo2 / tau;
+    const double f = (dofs - alpha) + tau + S - near;
+    mag += fabs(dofs - alpha) + fabs(tau) + fabs(near);
+    if (fabs(f) <= 4.0 * eps * mag) break;
+    if (f < 0.0) lo = tau; else hi = tau;
+    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi))) break;
+    double tn;
+    if (far < 0) {
+      // model: a + b t - zo2 / t  (origin pole exact, the rest linearised)
+      const double b = 1.0 + Sp, a = (dofs - alpha) + tau + S - b * tau;
+      const double sq = sqrt(a * a + 4.0 * b * zo2);
+      if (tau > 0.0) tn = a > 0.0 ? 2.0 * zo2 / (a + sq) : (sq - a) / (2.0 * b);
+      else tn = a < 0.0 ? -2.0 * zo2 / (sq - a) : -(a + sq) / (2.0 * b);
+    } else {
+      // model: c + zo2 / (0 - t) + s / (dfar - t)  (both interval ends as
+      // poles; the far weight s takes every other term's slope)
+      const double dfar = d[far] - dofs, gap = dfar - tau;
+      const double s = (1.0 + Sp) * gap * gap;
+      const double c = f + near - s / gap;
+      const double Bq = c * dfar + zo2 + s, C0 = zo2 * dfar;
+      if (c == 0.0) {
+        tn = C0 / Bq;
+      } else {
+        const double disc = Bq * Bq - 4.0 * c * C0;
+        if (disc < 0.0) {
+          tn = 0.5 * (lo + hi);
+        } else {
+          const double sq = sqrt(disc), qq = Bq >= 0.0 ? Bq + sq : Bq - sq;
+          const double t1 = qq / (2.0 * c), t2 = qq != 0.0 ? 2.0 * C0 / qq : t1;
+          tn = (t1 > lo && t1 < hi) ? t1 : t2;
+        }
+      }
+    }
+    if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+    tau = tn;
+  }
+  *org_out = o;
+  *tau_out = tau;
+  return it;
+}
+
+// Block-wide sum of v (every thread passes its value; every thread gets the
+// total).  Fixed summation order.
+template <class Blk>
+__host__ __device__ inline double arrow_block_sum(const Blk &B, double *red, double v) {
+  const int tid = B.tid(), nt = B.nt();
+  red[tid] = v;
+  B.sync();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int t = 0; t < nt; ++t) s += red[t];
+    red[kArrowRed - 1] = s;
+  }
+  B.sync();
+  const double s = red[kArrowRed - 1];
+  B.sync();
+  return s;
+}
+
+// The whole channel: poles, nI arrowhead updates, split at the Fermi energy,
+// descending sort per side and the Calc_En partial sums (DysonMatrix.cpp:
+// 436-484, 542-594).  ws holds arrow_ws_bytes(J.n, J.nI) bytes.
+template <class Blk>
+__host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws) {
+  const int nI = J.nI, N = J.n, tid = B.tid(), nt = B.nt();
+  const int m = J.np[0] + J.np[1];
+  const long LN = N;
+  double *dA = ws, *dB = dA + N;
+  double *HA = dB + N, *HB = HA + nI * LN, *GA = HB + nI * LN, *GB = GA + nI * LN;
+  double *dn = GB + nI * LN, *zn = dn + N, *tau = zn + N, *zh = tau + N, *red = zh + N;
+  int *org = (int *)(red + kArrowRed), *nd = org + N, *dl = nd + N, *cnt = dl + N;
+  const double eps = 2.2204460492503131e-16;
+
+  if (tid == 0) cnt[2] = 0;
+  // poles (unsorted) into the B buffers
+  for (int i = tid; i < m; i += nt) {
+    const int s = i < J.np[0] ? 0 : 1, ii = s ? i - J.np[0] : i;
+    dB[i] = J.E[s][(long)ii * J.es[s]];
+    for (int r = 0; r < nI; ++r) GB[r * LN + i] = J.C[s][(long)ii * J.ci[s] + (long)r * J.cr[s]];
+  }
+  B.sync();
+  // ascending by energy, ties by position, into the A buffers
+  for (int i = tid; i < m; i += nt) {
+    const double v = dB[i];
+    int rk = 0;
+    for (int k = 0; k < m; ++k) { const double w = dB[k]; rk += (w < v) || (w == v && k < i); }
+    dA[rk] = v;
+    for (int r = 0; r < nI; ++r) { GA[r * LN + rk] = GB[r * LN + i]; HA[r * LN + rk] = 0.0; }
+  }
+  B.sync();
+
+  int g = m;
+  for (int k = 0; k < nI; ++k) {
+    const double alpha = J.u1[k * nI + k] + J.h[k * nI + k];
+    double *z = GA + k * LN;
+    double s2 = 0.0, dmx = 0.0;
+    for (int i = tid; i < g; i += nt) { s2 += z[i] * z[i]; dmx = fmax(dmx, fabs(dA[i])); }
+    const double znorm = sqrt(arrow_block_sum(B, red, s2));
+    red[tid] = dmx;
+    B.sync();
+    if (tid == 0) {
+      double mx = 0.0;
+      for (int t = 0; t < nt; ++t) mx = fmax(mx, red[t]);
+      red[kArrowRed - 2] = mx;
+    }
+    B.sync();
+    const double tol = 8.0 * eps * fmax(fmax(fabs(alpha), red[kArrowRed - 2]), znorm);
+    // deflation scan in energy order (one thread)
+    if (tid == 0) {
+      int p = 0, q = 0, prev = -1;
+      for (int i = 0; i < g; ++i) {
+        if (fabs(z[i]) <= tol) { dl[q++] = i; continue; }
+        if (prev >= 0) {
+          const double za = z[prev], zb = z[i], t = hypot(za, zb), c = za / t, sn = zb / t;
+          if (fabs((dA[i] - dA[prev]) * c * sn) <= tol) {
+            // rotate (prev, i): i carries the whole coupling, prev leaves
+            for (int r = 0; r < nI; ++r) {
+              const double a = HA[r * LN + prev], b = HA[r * LN + i];
+              HA[r * LN + prev] = sn * a - c * b;
+              HA[r * LN + i] = c * a + sn * b;
+            }
+            for (int r = k + 1; r < nI; ++r) {
+              const double a = GA[r * LN + prev], b = GA[r * LN + i];
+              GA[r * LN + prev] = sn * a - c * b;
+              GA[r * LN + i] = c * a + sn * b;
+            }
+            const double da = dA[prev], db = dA[i];
+            dA[prev] = sn * sn * da + c * c * db;
+            dA[i] = c * c * da + sn * sn * db;
+            z[prev] = 0.0;
+            z[i] = t;
+            --p;
+            dl[q++] = prev;
+          }
+        }
+        nd[p++] = i;
+        prev = i;
+      }
+      cnt[0] = p;
+      cnt[1] = q;
+    }
+    B.sync();
+    const int p = cnt[0], q = cnt[1];
+    for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i]]; }
+    B.sync();
+    // roots
+    for (int j = tid; j <= p; j += nt) {
+      if (p == 0) { org[j] = -1; tau[j] = 0.0; continue; }
+      const int its = arrow_root(j, p, dn, zn, alpha, znorm, &org[j], &tau[j]);
+      if (its >= kArrowMaxIt) cnt[2] = 1;
+    }
+    B.sync();
+    // couplings recomputed from the roots: zh_i^2 = -prod_j (d_i - x_j) /
+    // prod_{l != i} (d_i - d_l), paired along the interlacing x_i < d_i < x_{i+1}
+    for (int i = tid; i < p; i += nt) {
+      const double di = dn[i];
+      double v = -((di - dn[org[i]]) - tau[i]) * ((di - dn[org[i + 1]]) - tau[i + 1]);
+      for (int j = 0; j < i; ++j) v *= ((di - dn[org[j]]) - tau[j]) / (di - dn[j]);
+      for (int j = i + 2; j <= p; ++j) v *= ((di - dn[org[j]]) - tau[j]) / (di - dn[j - 1]);
+      zh[i] = copysign(sqrt(fmax(v, 0.0)), zn[i]);
+    }
+    B.sync();
+    // eigenvectors: roots to positions [0, p], deflated directions to [p+1, p+q]
+    for (int j = tid; j <= p; j += nt) {
+      double ah[kArrowNI], ag[kArrowNI];
+#pragma unroll
+      for (int r = 0; r < kArrowNI; ++r) {
+        ah[r] = (r == k) ? 1.0 : 0.0;
+        ag[r] = (r > k && r < nI) ? J.u1[r * nI + k] + J.h[r * nI + k] : 0.0;
+      }
+      double lam = alpha, nu2 = 1.0;
+      if (p > 0) {
+        const int o = org[j];
+        const double t = tau[j], dofs = dn[o];
+        lam = dofs + t;
+        for (int i = 0; i < p; ++i) {
+          const double w = zh[i] / (t - (dn[i] - dofs));
+          nu2 += w * w;
+          const long c = nd[i];
+#pragma unroll
+          for (int r = 0; r < kArrowNI; ++r) {
+            if (r < nI) ah[r] += w * HA[r * LN + c];
+            if (r > k && r < nI) ag[r] += w * GA[r * LN + c];
+          }
+        }
+      }
+      const double inv = 1.0 / sqrt(nu2);
+      dB[j] = lam;
+#pragma unroll
+      for (int r = 0; r < kArrowNI; ++r) {
+        if (r < nI) HB[r * LN + j] = ah[r] * inv;
+        if (r > k && r < nI) GB[r * LN + j] = ag[r] * inv;
+      }
+    }
+    for (int t = tid; t < q; t += nt) {
+      const long c = dl[t], pos = p + 1 + t;
+      dB[pos] = dA[c];
+      for (int r = 0; r < nI; ++r) HB[r * LN + pos] = HA[r * LN + c];
+      for (int r = k + 1; r < nI; ++r) GB[r * LN + pos] = GA[r * LN + c];
+    }
+    B.sync();
+    g += 1;
+    for (int i = tid; i < g; i += nt) {
+      const double v = dB[i];
+      int rk = 0;
+      for (int l = 0; l < g; ++l) { const double w = dB[l]; rk += (w < v) || (w == v && l < i); }
+      dA[rk] = v;
+      for (int r = 0; r < nI; ++r) HA[r * LN + rk] = HB[r * LN + i];
+      for (int r = k + 1; r < nI; ++r) GA[r * LN + rk] = GB[r * LN + i];
+    }
+    B.sync();
+  }
+
+  // dA ascending, amplitudes in HA: split at E_F, both sides descending
+  const double ef = J.efermi;
+  int c_le = 0;
+  for (int i = tid; i < N; i += nt) c_le += (dA[i] <= ef);
+  const int nqh = (int)(arrow_block_sum(B, red, (double)c_le) + 0.5), nqp = N - nqh;
+  double kb[9];
+  for (int b = 0; b < 9; ++b) kb[b] = 0.0;
+  for (int i = tid; i < N; i += nt) {
+    const double di = dA[i];
+    const long slot = di > ef ? (N - 1 - i) : nqp + (nqh - 1 - i);
+    J.out_e[slot] = di;
+    double x1 = 0.0;
+    for (int r = 0; r < nI; ++r) {
+      const double xr = HA[r * LN + i];
+      J.out_x[slot * nI + r] = xr;
+      x1 += xr * xr;
+    }
+    if (di <= ef) {
+      double x2 = 0.0;
+      for (int r = 0; r < nI; ++r)
+        for (int c = 0; c < nI; ++c) x2 += J.u1[r * nI + c] * HA[r * LN + i] * HA[c * LN + i];
+      const int bk = x1 > 0.01 ? 0 : (x1 > 1.e-5 ? 1 : 2);
+      kb[3 * bk] += x2; kb[3 * bk + 1] += x1 * di; kb[3 * bk + 2] += x1;
+    }
+  }
+  for (int b = 0; b < 9; ++b) kb[b] = arrow_block_sum(B, red, kb[b]);
+  if (tid == 0) {
+    *J.out_nqp = nqp;
+    *J.out_nqh = nqh;
+    const double f = (double)J.j2 + 1.0;
+    J.kolt[0] = (kb[0] + kb[3] + kb[6]) * f;
+    J.kolt[1] = (kb[1] + kb[4] + kb[7]) * f;
+    J.kolt[2] = (kb[2] + kb[5] + kb[8]) * f;
+    if (cnt[2]) B.fail();
+  }
+}
+
+}  // namespace bcd

@@ -39,7 +39,7 @@ Batch::~Batch() {
 
 std::vector<DevArena *> Batch::arenas() {
   return {&a_tab_i, &a_tab_l, &a_tab_d, &a_inst, &a_chan, &a_blocks, &a_work, &a_prop, &a_rows,
-          &a_lws, &a_mat, &a_jobs};
+          &a_lws, &a_mat, &a_jobs, &a_aws};
 }
 
 #define HTRY(x)                                   \
@@ -99,6 +99,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   nImax = 1;
   const bool dyn = par.mode >= 1;
   long o_u1 = 0, o_h = 0, o_w = 0, o_tri = 0, o_mat = 0, o_piv = 0, o_lr = 0, o_e = 0, o_x = 0, o_blk = 0;
+  long o_pole = 0, o_aws = 0;
   std::vector<long> e_off, x_off;
   std::vector<int> cap;
   for (int i = 0; i < n_inst; ++i) {
@@ -138,6 +139,8 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
       C.lfw_cap = par.lanczos ? (int)lfw : 0; C.lbk_cap = par.lanczos ? (int)lbk : 0;
       C.lfw_off = o_lr; o_lr += pad8((long)C.lfw_cap * width);
       C.lbk_off = o_lr; o_lr += pad8((long)C.lbk_cap * width);
+      C.pole_off = o_pole; o_pole += pad8((long)(1 + nI) * (C.lfw_cap + C.lbk_cap));
+      C.aws_off = o_aws; o_aws += pad8((long)((arrow_ws_bytes((int)cp, nI) + 7) / 8));
       e_off.push_back(o_e); o_e += pad8(cp);
       x_off.push_back(o_x); o_x += pad8(cp * nI);
       cap.push_back((int)cp);
@@ -188,11 +191,11 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   // work doubles: u1, h, W, rho, tri, piv, lrows, kolt ; ints: status, work_count
   l_u1 = 0; l_h = l_u1 + pad8(o_u1); l_w = l_h + pad8(o_h); l_rho = l_w + pad8(o_w);
   l_tri = l_rho + pad8(rho_tot); l_piv = l_tri + pad8(o_tri); l_lr = l_piv + pad8(o_piv);
-  l_kolt = l_lr + pad8(o_lr); l_end = l_kolt + pad8(3L * n_gc) + 16;
+  l_kolt = l_lr + pad8(o_lr); l_pole = l_kolt + pad8(3L * n_gc); l_end = l_pole + pad8(o_pole) + 16;
   BTRY(a_work.grow(sizeof(double) * l_end));
   double *W0 = (double *)a_work.ptr;
   d_u1 = W0 + l_u1; d_h = W0 + l_h; d_w = W0 + l_w; d_rho = W0 + l_rho; d_tri = W0 + l_tri;
-  d_piv = W0 + l_piv; d_lrows = W0 + l_lr; d_kolt = W0 + l_kolt;
+  d_piv = W0 + l_piv; d_lrows = W0 + l_lr; d_kolt = W0 + l_kolt; d_poles = W0 + l_pole;
   d_status = (int *)(W0 + l_end - 16);
   d_work_count = d_status + 2;
   HTRY(hipMemset(a_work.ptr, 0, sizeof(double) * l_end));
@@ -234,18 +237,29 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipMemcpy(pnqh_[2], hqh.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
 
   // jobs
-  BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) + 64));
+  BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) +
+                   pad8(sizeof(ArrowJob) * n_gc) + 64));
   d_djobs = (DysonJob *)a_jobs.ptr;
-  d_ljobs = (LancJob *)((char *)a_jobs.ptr + pad8(sizeof(DysonJob) * n_gc));
+  d_pjobs = d_djobs + n_gc;
+  d_ljobs = (LancJob *)((char *)a_jobs.ptr + pad8(sizeof(DysonJob) * 3 * n_gc));
+  d_ajobs = (ArrowJob *)((char *)d_ljobs + pad8(sizeof(LancJob) * 2 * n_gc));
   HTRY(hipMemset(a_jobs.ptr, 0, a_jobs.cap));
 
   // dense Dyson matrices in HBM when the largest one does not fit in LDS
-  if (tridiag_lds((int)capmax, true) > (size_t)kLdsBytes) BTRY(a_mat.grow(sizeof(double) * (mat_len + 16)));
+  if (par.dyson_solver == 1 && tridiag_lds((int)capmax, true) > (size_t)kLdsBytes)
+    BTRY(a_mat.grow(sizeof(double) * (mat_len + 16)));
+  // arrowhead workspaces in HBM when the largest one does not fit in LDS
+  aws_len = 0;
+  if (par.dyson_solver == 0 && arrow_ws_bytes((int)capmax, nImax) > (size_t)kLdsBytes) {
+    aws_len = o_aws;
+    BTRY(a_aws.grow(sizeof(double) * (aws_len + 16)));
+  }
 
   // kernel attributes for large dynamic LDS
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  HTRY(hipFuncSetAttribute((const void *)k_arrow<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   h_is.assign(n_inst, InstState());
   return BCD_OK;
 }
@@ -290,21 +304,36 @@ int Batch::sweep(int src, int &n_active) {
   kt_begin(KT_SIGINF);
   hipLaunchKernelGGL(k_sigma_inf, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_is, d_w, d_rho, d_h);
   kt_end(KT_SIGINF);
-  hipLaunchKernelGGL(k_dyson_prep, dim3(gb), dim3(128), 0, stream, d_chans, 0, n_gc, d_cs, P, src, d_h, d_u1,
-                     d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_lrows, d_tri,
-                     (double *)a_mat.ptr, d_kolt, d_djobs);
-  const int nmax = src == 0 ? nImax : (int)capmax;
-  const bool in_lds = tridiag_lds(nmax, true) <= (size_t)kLdsBytes;
-  kt_begin(KT_TRIDIAG);
-  if (in_lds)
-    hipLaunchKernelGGL(k_tridiag<true>, dim3(n_gc), dim3(256), tridiag_lds(nmax, true), stream, d_djobs, n_gc);
-  else
-    hipLaunchKernelGGL(k_tridiag<false>, dim3(n_gc), dim3(256), tridiag_lds(nmax, false), stream, d_djobs, n_gc);
-  kt_end(KT_TRIDIAG);
-  kt_begin(KT_QL);
-  hipLaunchKernelGGL(k_ql, dim3(n_gc), dim3(64), (size_t)(2 + nImax) * nmax * sizeof(double), stream, d_djobs,
-                     n_gc, d_status);
-  kt_end(KT_QL);
+  if (par.dyson_solver == 0) {
+    hipLaunchKernelGGL(k_arrow_prep, dim3(gb), dim3(128), 0, stream, d_chans, n_gc, d_cs, P, src, d_h, d_u1,
+                       d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_poles,
+                       aws_len ? (double *)a_aws.ptr : nullptr, d_kolt, d_ajobs);
+    const int nmax = src == 0 ? nImax : (int)capmax;
+    const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
+    kt_begin(KT_ARROW);
+    if (!aws_len)
+      hipLaunchKernelGGL(k_arrow<true>, dim3(n_gc), dim3(nt), arrow_ws_bytes(nmax, nImax), stream, d_ajobs, n_gc,
+                         d_status);
+    else
+      hipLaunchKernelGGL(k_arrow<false>, dim3(n_gc), dim3(nt), 0, stream, d_ajobs, n_gc, d_status);
+    kt_end(KT_ARROW);
+  } else {
+    hipLaunchKernelGGL(k_dyson_prep, dim3(gb), dim3(128), 0, stream, d_chans, 0, n_gc, d_cs, P, src, d_h, d_u1,
+                       d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_lrows, d_tri,
+                       (double *)a_mat.ptr, d_kolt, d_djobs);
+    const int nmax = src == 0 ? nImax : (int)capmax;
+    const bool in_lds = tridiag_lds(nmax, true) <= (size_t)kLdsBytes;
+    kt_begin(KT_TRIDIAG);
+    if (in_lds)
+      hipLaunchKernelGGL(k_tridiag<true>, dim3(n_gc), dim3(256), tridiag_lds(nmax, true), stream, d_djobs, n_gc);
+    else
+      hipLaunchKernelGGL(k_tridiag<false>, dim3(n_gc), dim3(256), tridiag_lds(nmax, false), stream, d_djobs, n_gc);
+    kt_end(KT_TRIDIAG);
+    kt_begin(KT_QL);
+    hipLaunchKernelGGL(k_ql, dim3(n_gc), dim3(64), (size_t)(2 + nImax) * nmax * sizeof(double), stream, d_djobs,
+                       n_gc, d_status);
+    kt_end(KT_QL);
+  }
   kt_begin(KT_FINAL);
   hipLaunchKernelGGL(k_finalize, dim3(n_inst), dim3(256), 0, stream, d_insts, d_chans, P, d_cs, d_kolt,
                      par.conv_kind, 0);
@@ -451,6 +480,24 @@ int Batch::lanczos() {
   return BCD_OK;
 }
 
+// Diagonalise every Lanczos-reduced side once (its fishbone block and
+// couplings are fixed for the whole sc0 stage): pole energies and rotated
+// couplings for the arrowhead sweeps.
+int Batch::decompose_poles() {
+  int lmax = 1;
+  for (const ChanDev &C : h_chans) lmax = std::max(lmax, std::max(C.lfw_cap, C.lbk_cap));
+  if (tridiag_lds(lmax, true) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
+  hipLaunchKernelGGL(k_poles_prep, dim3((2 * n_gc + 127) / 128), dim3(128), 0, stream, d_chans, n_gc, d_cs, d_lrows,
+                     d_tri, d_poles, d_pjobs);
+  kt_begin(KT_POLES);
+  hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, true), stream, d_pjobs, 2 * n_gc);
+  hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * lmax * sizeof(double), stream, d_pjobs,
+                     2 * n_gc, d_status);
+  kt_end(KT_POLES);
+  HTRY(hipGetLastError());
+  return BCD_OK;
+}
+
 // Algorithmic work of the last solve (see bcd_batch_work in bcdor_amd.h).
 int Batch::work(double *w) {
   for (int k = 0; k < 5; ++k) w[k] = 0.0;
@@ -513,6 +560,7 @@ int Batch::solve() {
     BTRY(build_sigma2(-1));
     HTRY(hipEventRecord(ev[3], stream));
     if (par.lanczos) BTRY(lanczos());
+    if (par.lanczos && par.dyson_solver == 0) BTRY(decompose_poles());
     HTRY(hipEventRecord(ev[4], stream));
     BTRY(stage(par.lanczos ? 2 : 1, par.mode == 1 ? 0 : par.sc_itr_max, par.sc_conv, false));
   } else {

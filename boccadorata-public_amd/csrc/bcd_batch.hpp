@@ -27,7 +27,10 @@ struct DevArena {
   void release();
 };
 
-enum KernelTimer { KT_SIGMA2, KT_LANCZOS, KT_TRIDIAG, KT_QL, KT_DENSITY, KT_SIGINF, KT_FINAL, KT_WBUILD, KT_PLAN, KT_N };
+enum KernelTimer {
+  KT_SIGMA2, KT_LANCZOS, KT_TRIDIAG, KT_QL, KT_DENSITY, KT_SIGINF, KT_FINAL, KT_WBUILD, KT_PLAN, KT_ARROW, KT_POLES,
+  KT_N
+};
 
 struct Batch {
   int device = 0;
@@ -44,7 +47,7 @@ struct Batch {
   std::vector<int> hf_its, sc_its, hf_ok, sc_ok;
   std::vector<long> n_fw_tot, n_bk_tot;
 
-  DevArena a_tab_i, a_tab_l, a_tab_d, a_inst, a_chan, a_blocks, a_work, a_prop, a_rows, a_lws, a_mat, a_jobs;
+  DevArena a_tab_i, a_tab_l, a_tab_d, a_inst, a_chan, a_blocks, a_work, a_prop, a_rows, a_lws, a_mat, a_jobs, a_aws;
   InstDev *d_insts = nullptr;
   InstState *d_is = nullptr;
   ChanDev *d_chans = nullptr;
@@ -55,13 +58,15 @@ struct Batch {
   int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_list = nullptr, *d_work_count = nullptr;
   int *d_status = nullptr;
   double *d_u1 = nullptr, *d_h = nullptr, *d_w = nullptr, *d_rho = nullptr, *d_tri = nullptr, *d_piv = nullptr;
-  double *d_lrows = nullptr, *d_kolt = nullptr, *d_rows_fw = nullptr, *d_rows_bk = nullptr;
-  long l_u1 = 0, l_h = 0, l_w = 0, l_rho = 0, l_tri = 0, l_piv = 0, l_lr = 0, l_kolt = 0, l_end = 0;
+  double *d_lrows = nullptr, *d_kolt = nullptr, *d_rows_fw = nullptr, *d_rows_bk = nullptr, *d_poles = nullptr;
+  long l_u1 = 0, l_h = 0, l_w = 0, l_rho = 0, l_tri = 0, l_piv = 0, l_lr = 0, l_kolt = 0, l_pole = 0, l_end = 0;
   double *pe_[3] = {nullptr, nullptr, nullptr}, *px_[3] = {nullptr, nullptr, nullptr};
   int *pnqp_[3] = {nullptr, nullptr, nullptr}, *pnqh_[3] = {nullptr, nullptr, nullptr};
   long prop_e_len = 0, prop_x_len = 0, prop_n_len = 0;
-  DysonJob *d_djobs = nullptr;
+  DysonJob *d_djobs = nullptr, *d_pjobs = nullptr;   // per-sweep dense jobs; side diagonalisation jobs
+  ArrowJob *d_ajobs = nullptr;
   LancJob *d_ljobs = nullptr;
+  long aws_len = 0;                                  // global arrowhead workspace (doubles), 0: LDS
 
   hipEvent_t ev[6] = {};
   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
@@ -88,6 +93,7 @@ struct Batch {
   int work(double *w);
   double coupling_flops = -1.0;
   int lanczos();
+  int decompose_poles();
   int solve();
   int result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep, double *zp, double *fqp,
              double *eh, double *zh, double *fqh);

@@ -364,23 +364,32 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
     return BCD_ERR_ARG;
   HTRY(hipSetDevice(ctx->device));
   const int n = n_orb + nfw + nbk, Tf = ntot_den_fw + n_orb, Tb = ntot_den_bk + n_orb;
-  const bool in_lds = ((size_t)n * n + (size_t)(2 + n_orb) * n + n_orb + 16) * sizeof(double) <= (size_t)kLdsBytes;
-  DevBuf dh, du, dfw, dbk, dpf, dpb, dtri, dmat, de, dx, dc, dk, dj, dst;
+  const int ns[2] = {nfw, nbk}, Ts[2] = {Tf, Tb}, dens[2] = {ntot_den_fw, ntot_den_bk};
+  const int npv[2] = {npiv_fw > 0 ? npiv_fw : 0, npiv_bk > 0 ? npiv_bk : 0};
+  // a reduced side (pivots given) is a fishbone block and is diagonalised
+  // first; an unreduced side is diagonal already
+  const int lside = std::max(npv[0] ? nfw : 0, npv[1] ? nbk : 0);
+  if (lside && ((size_t)(2 + n_orb) * lside + n_orb + 16 + (size_t)lside * lside) * sizeof(double) > (size_t)kLdsBytes)
+    return BCD_ERR_CAPACITY;
+  const size_t aws = arrow_ws_bytes(n, n_orb);
+  const bool a_lds = aws <= (size_t)kLdsBytes;
+  DevBuf dh, du, dfw, dbk, dpf, dpb, dtri, dpole, dws, de, dx, dc, dk, dj, dja, dst;
   BTRY(dh.alloc(sizeof(double) * n_orb * n_orb));
   BTRY(du.alloc(sizeof(double) * n_orb * n_orb));
   BTRY(dfw.alloc(sizeof(double) * (size_t)std::max(nfw, 1) * Tf));
   BTRY(dbk.alloc(sizeof(double) * (size_t)std::max(nbk, 1) * Tb));
   BTRY(dpf.alloc(sizeof(int) * (kMaxPiv + 2)));
   BTRY(dpb.alloc(sizeof(int) * (kMaxPiv + 2)));
-  BTRY(dtri.alloc(sizeof(double) * (size_t)n * (2 + n_orb)));
-  if (!in_lds) BTRY(dmat.alloc(sizeof(double) * (size_t)n * n));
+  BTRY(dtri.alloc(sizeof(double) * (size_t)std::max(lside, 1) * (2 + n_orb) * 2));
+  BTRY(dpole.alloc(sizeof(double) * (size_t)(1 + n_orb) * std::max(nfw + nbk, 1)));
+  if (!a_lds) BTRY(dws.alloc(aws));
   BTRY(de.alloc(sizeof(double) * n));
   BTRY(dx.alloc(sizeof(double) * (size_t)n * n_orb));
   BTRY(dc.alloc(sizeof(int) * 2));
   BTRY(dk.alloc(sizeof(double) * 3));
-  BTRY(dj.alloc(sizeof(DysonJob)));
+  BTRY(dj.alloc(sizeof(DysonJob) * 2));
+  BTRY(dja.alloc(sizeof(ArrowJob)));
   BTRY(dst.alloc(sizeof(int)));
-  if ((size_t)(2 + n_orb) * n * sizeof(double) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
   HTRY(hipMemcpy(dh.p, h, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
   HTRY(hipMemcpy(du.p, u1, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
   if (nfw) HTRY(hipMemcpy(dfw.p, rows_fw, sizeof(double) * (size_t)nfw * Tf, hipMemcpyHostToDevice));
@@ -392,29 +401,54 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   HTRY(hipMemcpy(dpf.p, pf, sizeof pf, hipMemcpyHostToDevice));
   HTRY(hipMemcpy(dpb.p, pb, sizeof pb, hipMemcpyHostToDevice));
   HTRY(hipMemset(dst.p, 0, sizeof(int)));
-  DysonJob J;
-  std::memset(&J, 0, sizeof J);
-  J.nI = n_orb; J.j2 = j2; J.nfw = nfw; J.nbk = nbk; J.den_fw = ntot_den_fw; J.den_bk = ntot_den_bk;
-  J.npiv_fw = npiv_fw > 0 ? npiv_fw : 0; J.npiv_bk = npiv_bk > 0 ? npiv_bk : 0;
-  J.n = n; J.inst = -1; J.ch = 0; J.efermi = efermi;
-  J.h = (const double *)dh.p; J.u1 = (const double *)du.p;
-  J.rfw = (const double *)dfw.p; J.rbk = (const double *)dbk.p;
-  J.ipiv_fw = (const int *)dpf.p; J.ipiv_bk = (const int *)dpb.p;
-  J.tri = (double *)dtri.p; J.mat = (double *)dmat.p;
-  J.out_e = (double *)de.p; J.out_x = (double *)dx.p;
-  J.out_nqp = (int *)dc.p; J.out_nqh = (int *)dc.p + 1; J.kolt = (double *)dk.p;
-  HTRY(hipMemcpy(dj.p, &J, sizeof J, hipMemcpyHostToDevice));
-  const DysonJob *jp = (const DysonJob *)dj.p;
-  const size_t aux = ((size_t)(2 + n_orb) * n + n_orb + 16) * sizeof(double);
+  const double *rows[2] = {(const double *)dfw.p, (const double *)dbk.p};
+  const int *ipv[2] = {(const int *)dpf.p, (const int *)dpb.p};
+  double *pole = (double *)dpole.p;
+  ArrowJob A;
+  std::memset(&A, 0, sizeof A);
+  DysonJob P[2];
+  int njob = 0;
+  long poff = 0;
+  for (int s = 0; s < 2; ++s) {
+    A.np[s] = ns[s];
+    if (npv[s] && ns[s] > 0) {
+      DysonJob &J = P[njob];
+      std::memset(&J, 0, sizeof J);
+      J.nI = n_orb; J.j2 = j2; J.n = ns[s]; J.inst = -1; J.poles = 1;
+      J.nfw = ns[s]; J.den_fw = J.den_bk = dens[s]; J.npiv_fw = npv[s];
+      J.rfw = rows[s]; J.ipiv_fw = ipv[s];
+      J.tri = (double *)dtri.p + (size_t)njob * lside * (2 + n_orb);
+      J.out_lam = pole + poff; J.out_ct = pole + poff + ns[s]; J.ct_ld = ns[s];
+      A.E[s] = J.out_lam; A.es[s] = 1; A.C[s] = J.out_ct; A.ci[s] = 1; A.cr[s] = ns[s];
+      poff += (long)(1 + n_orb) * ns[s];
+      ++njob;
+    } else {
+      A.E[s] = rows[s]; A.es[s] = Ts[s]; A.C[s] = rows[s] + dens[s]; A.ci[s] = Ts[s]; A.cr[s] = 1;
+    }
+  }
+  A.nI = n_orb; A.j2 = j2; A.n = n; A.inst = -1;
+  A.h = (const double *)dh.p; A.u1 = (const double *)du.p; A.efermi = efermi;
+  A.ws = a_lds ? nullptr : (double *)dws.p;
+  A.out_e = (double *)de.p; A.out_x = (double *)dx.p;
+  A.out_nqp = (int *)dc.p; A.out_nqh = (int *)dc.p + 1; A.kolt = (double *)dk.p;
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-  HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-  if (in_lds)
-    hipLaunchKernelGGL(k_tridiag<true>, dim3(1), dim3(256), aux + (size_t)n * n * sizeof(double), ctx->stream, jp, 1);
+  HTRY(hipFuncSetAttribute((const void *)k_arrow<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  if (njob) {
+    HTRY(hipMemcpy(dj.p, P, sizeof(DysonJob) * njob, hipMemcpyHostToDevice));
+    const DysonJob *jp = (const DysonJob *)dj.p;
+    const size_t aux = ((size_t)(2 + n_orb) * lside + n_orb + 16) * sizeof(double);
+    hipLaunchKernelGGL(k_tridiag<true>, dim3(njob), dim3(256), aux + (size_t)lside * lside * sizeof(double),
+                       ctx->stream, jp, njob);
+    hipLaunchKernelGGL(k_ql, dim3(njob), dim3(64), (size_t)(2 + n_orb) * lside * sizeof(double), ctx->stream, jp,
+                       njob, (int *)dst.p);
+  }
+  HTRY(hipMemcpyAsync(dja.p, &A, sizeof A, hipMemcpyHostToDevice, ctx->stream));
+  const int nt = n <= 64 ? 64 : (n <= 128 ? 128 : kArrowMaxThreads);
+  if (a_lds)
+    hipLaunchKernelGGL(k_arrow<true>, dim3(1), dim3(nt), aws, ctx->stream, (const ArrowJob *)dja.p, 1, (int *)dst.p);
   else
-    hipLaunchKernelGGL(k_tridiag<false>, dim3(1), dim3(256), aux, ctx->stream, jp, 1);
-  hipLaunchKernelGGL(k_ql, dim3(1), dim3(64), (size_t)(2 + n_orb) * n * sizeof(double), ctx->stream, jp, 1,
-                     (int *)dst.p);
+    hipLaunchKernelGGL(k_arrow<false>, dim3(1), dim3(nt), 0, ctx->stream, (const ArrowJob *)dja.p, 1, (int *)dst.p);
   HTRY(hipGetLastError());
   HTRY(hipStreamSynchronize(ctx->stream));
   int st = 0, cnt[2] = {0, 0};
@@ -506,6 +540,7 @@ int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_v
                      const bcd_solve_params *par, bcd_batch **out) {
   if (!out || n_inst < 1 || !ms || !vpp || !par) return BCD_ERR_ARG;
   *out = nullptr;
+  if (par->mode < 0 || par->mode > 2 || par->dyson_solver < 0 || par->dyson_solver > 1) return BCD_ERR_ARG;
   BTRY(device_ok(device));
   std::unique_ptr<bcd_batch> B(new (std::nothrow) bcd_batch);
   if (!B) return BCD_ERR_NOMEM;

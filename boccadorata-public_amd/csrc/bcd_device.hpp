@@ -45,6 +45,9 @@ struct ChanDev {
   long piv_off;                  // pivots [nI x nI]
   long tri_off;                  // d[cap], e[cap], Qhead[nI*cap]
   long mat_off;                  // dense Dyson matrix scratch (cap*cap) for the global-memory path
+  long pole_off;                 // diagonalised reduced sides: lam_fw[lfw_cap], Ct_fw[nI x lfw_cap],
+                                 // lam_bk[lbk_cap], Ct_bk[nI x lbk_cap]
+  long aws_off;                  // arrowhead workspace (doubles) for the global-memory path
 };
 
 // One (Jab, a <= b, g) block of the Sigma(2) build of channel gc; blocks of a
@@ -115,6 +118,12 @@ struct DysonJob {
   double *out_e, *out_x;         // records sorted: qp (descending) then qh (descending)
   int *out_nqp, *out_nqh;
   double *kolt;                  // 3 Calc_En partial sums
+  // poles != 0: diagonalise one reduced side alone, D = U diag(lam) U^T (its
+  // fishbone block in the fw slot, no static part).  The tracked rows start
+  // as the couplings M^T, so they end as M^T U: out_lam[k] = lam_k and
+  // out_ct[r * ct_ld + k] = (U^T M)[k][r], the coupling of pole k to orbit r.
+  int poles, ct_ld;
+  double *out_lam, *out_ct;
 };
 
 }  // namespace bcd

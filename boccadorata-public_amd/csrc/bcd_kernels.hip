@@ -540,7 +540,7 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
   if (S && !S->active) { J.n = 0; return; }
   const ChanState &Q = cs[gc];
   const int cur = S ? S->cur : 0, nxt = 1 - cur, nI = C.nI;
-  J.nI = nI; J.j2 = C.j2; J.ch = C.ch; J.inst = C.inst;
+  J.nI = nI; J.j2 = C.j2; J.ch = C.ch; J.inst = C.inst; J.poles = 0;
   J.h = h + C.h_off; J.u1 = u1 + C.u1_off;
   if (src == 0) {
     J.nfw = J.nbk = 0; J.den_fw = J.den_bk = 2; J.npiv_fw = J.npiv_bk = 0;
@@ -572,7 +572,8 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
 // tridiagonal form with Householder reflections (Golub & Van Loan, Alg.
 // 8.3.1), tracking only the first nI rows of the orthogonal factor, which are
 // all the quasiparticle amplitudes need.  Four waves per matrix; the matrix
-// sits in LDS (IN_LDS) or in a global scratch slab.
+// sits in LDS (IN_LDS) or in a global scratch slab.  In poles mode the same
+// reduction diagonalises one Lanczos-reduced side for the arrowhead solver.
 template <bool IN_LDS>
 __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs) {
   extern __shared__ double sm[];
@@ -586,20 +587,29 @@ __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs
   double *v = aux, *p = v + n, *Qh = p + n, *qv = Qh + (long)nI * n, *red = qv + nI;
   double *td = J.tri, *te = td + n, *tq = te + n;
   const long ld = n;
+  // poles mode: the matrix is one side's pole block alone and the tracked
+  // rows start as its couplings, so they end as U^T M
+  const bool poles = J.poles != 0;
+  const int hd = poles ? 0 : nI;
   for (long i = tid; i < (long)n * n; i += blockDim.x) A[i] = 0.0;
-  for (long i = tid; i < (long)nI * n; i += blockDim.x) Qh[i] = ((i / n) == (i % n)) ? 1.0 : 0.0;
+  for (long i = tid; i < (long)nI * n; i += blockDim.x) Qh[i] = (!poles && (i / n) == (i % n)) ? 1.0 : 0.0;
   __syncthreads();
-  for (int i = tid; i < nI * nI; i += blockDim.x) A[(i / nI) * ld + (i % nI)] = J.u1[i] + J.h[i];
+  if (!poles)
+    for (int i = tid; i < nI * nI; i += blockDim.x) A[(i / nI) * ld + (i % nI)] = J.u1[i] + J.h[i];
   for (int side = 0; side < 2; ++side) {
     const int nd = side ? J.nbk : J.nfw, den = side ? J.den_bk : J.den_fw, T = den + nI;
-    const int base = nI + (side ? J.nfw : 0);
+    const int base = hd + (side ? J.nfw : 0);
     const double *rows = side ? J.rbk : J.rfw;
     for (int i = tid; i < nd; i += blockDim.x) {
       const double *row = rows + (long)i * T;
       A[(base + i) * ld + base + i] = row[0];
       for (int r = 0; r < nI; ++r) {
-        A[(base + i) * ld + r] = row[den + r];
-        A[(long)r * ld + base + i] = row[den + r];
+        if (poles) {
+          Qh[(long)r * n + base + i] = row[den + r];
+        } else {
+          A[(base + i) * ld + r] = row[den + r];
+          A[(long)r * ld + base + i] = row[den + r];
+        }
       }
     }
   }
@@ -608,7 +618,7 @@ __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs
     const int npiv = side ? J.npiv_bk : J.npiv_fw;
     if (npiv <= 0) continue;
     const int nd = side ? J.nbk : J.nfw, den = side ? J.den_bk : J.den_fw, T = den + nI;
-    const int base = nI + (side ? J.nfw : 0);
+    const int base = hd + (side ? J.nfw : 0);
     const double *rows = side ? J.rbk : J.rfw;
     const int *ipv = side ? J.ipiv_bk : J.ipiv_fw;
     for (int i = 1 + tid; i < nd; i += blockDim.x) {
@@ -747,6 +757,13 @@ __global__ __launch_bounds__(64) void k_ql(const DysonJob *jobs, int njobs, int 
   }
   if (fail) { if (lane == 0) atomicAdd(status, 1); return; }
   __syncthreads();
+  if (J.poles) {
+    for (int k = lane; k < n; k += 64) {
+      J.out_lam[k] = d[k];
+      for (int r = 0; r < nI; ++r) J.out_ct[(long)r * J.ct_ld + k] = Q[(long)r * n + k];
+    }
+    return;
+  }
   const double ef = J.efermi;
   int cnt = 0;
   for (int k = lane; k < n; k += 64) cnt += (d[k] <= ef);

@@ -1,6 +1,7 @@
 // bcd_kernels.hpp -- kernel limits and declarations shared by host and device code.
 #pragma once
 
+#include "bcd_arrow.hpp"
 #include "bcd_device.hpp"
 
 namespace bcd {
@@ -39,6 +40,14 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
 template <bool IN_LDS>
 __global__ void k_tridiag(const DysonJob *jobs, int njobs);
 __global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
+template <bool IN_LDS>
+__global__ void k_arrow(const ArrowJob *jobs, int njobs, int *status);
+__global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
+                             double *tri, double *poles, DysonJob *jobs);
+__global__ void k_arrow_prep(const ChanDev *chans, int n_gc, const ChanState *cs, PropView P, int src,
+                             const double *h, const double *u1, const long *row_fw_base,
+                             const long *row_bk_base, const double *rows_fw, const double *rows_bk,
+                             const double *poles, double *aws, double *kolt, ArrowJob *jobs);
 __global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out);
 __global__ void k_finalize(const InstDev *insts, const ChanDev *chans, PropView P, const ChanState *cs,
                            const double *kolt, int conv_kind, int init);

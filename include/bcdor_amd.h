@@ -142,6 +142,10 @@ typedef struct bcd_solve_params {
   double sc_conv;    /* sc0 stops when the measure drops below this                  */
   double t_fac;      /* U1body_fac * hbar*omega multiplying T_HO (0: Trel=7)        */
   int ext_u1;        /* 1: add diag(model-space spe) to U1 (ExtU1=1)                */
+  int dyson_solver;  /* 0: arrowhead updates over the diagonalised pole blocks
+                        (each reduced side diagonalised once per solve);
+                        1: dense Householder + QL of the whole Dyson matrix per
+                        sweep.  Both give DysonMatrix's eigenpairs.              */
 } bcd_solve_params;
 
 /* g0 may be NULL (bare propagator, SpProp_t(ModSpace_t*, 1)); otherwise g0[i]

@@ -1,0 +1,88 @@
+// Runs bcd::arrow_body -- the code of the k_arrow kernel -- on host threads:
+// one std::thread per GPU thread of the block and a barrier for each
+// __syncthreads, so the tests can check the algorithm and every index on the
+// CPU (built with AddressSanitizer).
+//
+// stdin : int32 ncase; per case int32 nI, np0, np1, nt, j2; float64 efermi;
+//         float64 h[nI*nI], u1[nI*nI], E0[np0], C0[np0*nI], E1[np1], C1[np1*nI]
+//         (C rows = poles, columns = orbits)
+// stdout: per case int32 nqp, nqh, failed; float64 e[n], x[n*nI], kolt[3]
+#include <condition_variable>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "bcd_arrow.hpp"
+
+namespace {
+
+struct Barrier {
+  std::mutex mu;
+  std::condition_variable cv;
+  int n, waiting = 0;
+  long gen = 0;
+  explicit Barrier(int n_) : n(n_) {}
+  void wait() {
+    std::unique_lock<std::mutex> lk(mu);
+    const long g = gen;
+    if (++waiting == n) { waiting = 0; ++gen; cv.notify_all(); return; }
+    cv.wait(lk, [&] { return gen != g; });
+  }
+};
+
+struct HostBlock {
+  int t, n;
+  Barrier *bar;
+  int *failed;
+  int tid() const { return t; }
+  int nt() const { return n; }
+  void sync() const { bar->wait(); }
+  void fail() const { *failed = 1; }
+};
+
+template <class T>
+bool get(std::vector<T> &v, size_t n) {
+  v.resize(n);
+  return n == 0 || fread(v.data(), sizeof(T), n, stdin) == n;
+}
+
+}  // namespace
+
+int main() {
+  int32_t ncase = 0;
+  if (fread(&ncase, 4, 1, stdin) != 1) return 2;
+  for (int c = 0; c < ncase; ++c) {
+    int32_t hd[5];
+    double ef;
+    if (fread(hd, 4, 5, stdin) != 5 || fread(&ef, 8, 1, stdin) != 1) return 2;
+    const int nI = hd[0], np0 = hd[1], np1 = hd[2], nt = hd[3], j2 = hd[4], n = nI + np0 + np1;
+    if (nI < 1 || nI > bcd::kArrowNI || nt < 1 || nt > bcd::kArrowMaxThreads) return 3;
+    std::vector<double> h, u1, e0, c0, e1, c1;
+    if (!get(h, (size_t)nI * nI) || !get(u1, (size_t)nI * nI) || !get(e0, np0) || !get(c0, (size_t)np0 * nI) ||
+        !get(e1, np1) || !get(c1, (size_t)np1 * nI))
+      return 2;
+    std::vector<double> oe(n), ox((size_t)n * nI), kolt(3);
+    std::vector<char> ws(bcd::arrow_ws_bytes(n, nI));
+    int nqp = -1, nqh = -1, failed = 0;
+    bcd::ArrowJob J{};
+    J.nI = nI; J.j2 = j2; J.n = n; J.inst = -1;
+    J.np[0] = np0; J.np[1] = np1;
+    J.E[0] = e0.data(); J.E[1] = e1.data(); J.C[0] = c0.data(); J.C[1] = c1.data();
+    J.es[0] = J.es[1] = 1; J.ci[0] = J.ci[1] = nI; J.cr[0] = J.cr[1] = 1;
+    J.h = h.data(); J.u1 = u1.data(); J.efermi = ef;
+    J.out_e = oe.data(); J.out_x = ox.data(); J.out_nqp = &nqp; J.out_nqh = &nqh; J.kolt = kolt.data();
+    Barrier bar(nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&, t] { bcd::arrow_body(HostBlock{t, nt, &bar, &failed}, J, (double *)ws.data()); });
+    for (auto &x : th) x.join();
+    int32_t o[3] = {nqp, nqh, failed};
+    fwrite(o, 4, 3, stdout);
+    fwrite(oe.data(), 8, n, stdout);
+    fwrite(ox.data(), 8, (size_t)n * nI, stdout);
+    fwrite(kolt.data(), 8, 3, stdout);
+  }
+  return 0;
+}

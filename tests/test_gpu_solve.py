@@ -96,3 +96,23 @@ def test_batch_equals_single():
         assert np.array_equal(ga.ep, gb.ep) and np.array_equal(ga.eh, gb.eh), i
         assert np.array_equal(ga.fqp, gb.fqp) and np.array_equal(ga.fqh, gb.fqh), i
         assert b.stats(i) == one.stats(0)
+
+
+@pytest.mark.parametrize("nmax,lanczos", [(3, False), (5, True), (7, True)])
+def test_arrowhead_equals_dense(nmax, lanczos):
+    """The arrowhead Dyson solver (default) against the dense Householder + QL
+    solver on the whole solve: same sweep counts, and every fragment (no Z cut:
+    both read the same self-energy) to 1e-10."""
+    ms, vpp = synth.make_system(nmax, 0)
+    out = []
+    for solver in (0, 1):
+        par = capi.solve_params(mode=2, lanczos=lanczos, lanc_kind=1, lanc_n=50, conv_kind=1, hf_conv=1e-8,
+                                sc_conv=1e-8, sc_itr_max=40, dyson_solver=solver)
+        b = capi.Batch([(ms, vpp)], par)
+        b.solve()
+        print(f"nmax {nmax} solver {solver}", b.stats(0), b.timings(), flush=True)
+        out.append((b.result(0), b.stats(0)))
+    (ga, sa), (gb, sb) = out
+    assert sa["hf_its"] == sb["hf_its"] and sa["sc_its"] == sb["sc_its"]
+    check_prop(ga, gb, tol=1e-10, zsig=0.0)
+    assert abs(sa["e_kolt"] - sb["e_kolt"]) < 1e-9 * max(1.0, abs(sb["e_kolt"]))
