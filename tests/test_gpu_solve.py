@@ -101,8 +101,10 @@ def test_batch_equals_single():
 @pytest.mark.parametrize("nmax,lanczos", [(3, False), (5, True), (7, True)])
 def test_arrowhead_equals_dense(nmax, lanczos):
     """The arrowhead Dyson solver (default) against the dense Householder + QL
-    solver on the whole solve: same sweep counts, and every fragment (no Z cut:
-    both read the same self-energy) to 1e-10."""
+    solver on the whole solve: same sweep counts and fragments to 1e-10.  With
+    the full Sigma every fragment is compared; with Lanczos the HF stage's
+    last-bit differences reach the Lanczos input and move the small-Z
+    fragments as rounding does in the reference, so the Z cut applies."""
     ms, vpp = synth.make_system(nmax, 0)
     out = []
     for solver in (0, 1):
@@ -114,5 +116,5 @@ def test_arrowhead_equals_dense(nmax, lanczos):
         out.append((b.result(0), b.stats(0)))
     (ga, sa), (gb, sb) = out
     assert sa["hf_its"] == sb["hf_its"] and sa["sc_its"] == sb["sc_its"]
-    check_prop(ga, gb, tol=1e-10, zsig=0.0)
+    check_prop(ga, gb, tol=1e-10, zsig=ZSIG_BY_NMAX[nmax] if lanczos else 0.0)
     assert abs(sa["e_kolt"] - sb["e_kolt"]) < 1e-9 * max(1.0, abs(sb["e_kolt"]))
