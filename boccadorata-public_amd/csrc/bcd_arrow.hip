@@ -36,7 +36,7 @@ template __global__ void k_arrow<false>(const ArrowJob *, int, int *);
 
 // Jobs that diagonalise both reduced sides of every channel: job 2*gc + side.
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
-                             double *tri, double *poles, DysonJob *jobs) {
+                             double *tri, double *mat, double *poles, DysonJob *jobs) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 2 * n_gc) return;
   const int gc = t >> 1, side = t & 1;
@@ -54,7 +54,9 @@ __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs
   J.rfw = lrows + (side ? C.lbk_off : C.lfw_off); J.rbk = nullptr;
   J.ipiv_fw = side ? Q.ipiv_bk : Q.ipiv_fw; J.ipiv_bk = nullptr;
   J.tri = tri + C.tri_off + (side ? (long)C.lfw_cap * (2 + C.nI) : 0);
-  J.mat = nullptr;
+  // the channel's dense slab (cap^2 >= lfw_cap^2 + lbk_cap^2) when the side
+  // does not fit in LDS
+  J.mat = mat ? mat + C.mat_off + (side ? (long)C.lfw_cap * C.lfw_cap : 0) : nullptr;
   J.out_e = nullptr; J.out_x = nullptr; J.out_nqp = nullptr; J.out_nqh = nullptr; J.kolt = nullptr;
   J.poles = 1;
   J.ct_ld = capS;

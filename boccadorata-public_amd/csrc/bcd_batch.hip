@@ -245,8 +245,12 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_ajobs = (ArrowJob *)((char *)d_ljobs + pad8(sizeof(LancJob) * 2 * n_gc));
   HTRY(hipMemset(a_jobs.ptr, 0, a_jobs.cap));
 
-  // dense Dyson matrices in HBM when the largest one does not fit in LDS
-  if (par.dyson_solver == 1 && tridiag_lds((int)capmax, true) > (size_t)kLdsBytes)
+  // dense matrices in HBM when the largest one does not fit in LDS: the
+  // whole Dyson matrix (dense solver) or one reduced side (arrowhead solver)
+  int lside_max = 1;
+  for (const ChanDev &C : h_chans) lside_max = std::max(lside_max, std::max(C.lfw_cap, C.lbk_cap));
+  if ((par.dyson_solver == 1 && tridiag_lds((int)capmax, true) > (size_t)kLdsBytes) ||
+      (par.dyson_solver == 0 && par.lanczos && tridiag_lds(lside_max, true) > (size_t)kLdsBytes))
     BTRY(a_mat.grow(sizeof(double) * (mat_len + 16)));
   // arrowhead workspaces in HBM when the largest one does not fit in LDS
   aws_len = 0;
@@ -486,11 +490,18 @@ int Batch::lanczos() {
 int Batch::decompose_poles() {
   int lmax = 1;
   for (const ChanDev &C : h_chans) lmax = std::max(lmax, std::max(C.lfw_cap, C.lbk_cap));
-  if (tridiag_lds(lmax, true) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
+  if (tridiag_lds(lmax, false) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
+  const bool in_lds = tridiag_lds(lmax, true) <= (size_t)kLdsBytes;
+  if (!in_lds && !a_mat.ptr) return BCD_ERR_CAPACITY;
   hipLaunchKernelGGL(k_poles_prep, dim3((2 * n_gc + 127) / 128), dim3(128), 0, stream, d_chans, n_gc, d_cs, d_lrows,
-                     d_tri, d_poles, d_pjobs);
+                     d_tri, in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs);
   kt_begin(KT_POLES);
-  hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, true), stream, d_pjobs, 2 * n_gc);
+  if (in_lds)
+    hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, true), stream, d_pjobs,
+                       2 * n_gc);
+  else
+    hipLaunchKernelGGL(k_tridiag<false>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, false), stream, d_pjobs,
+                       2 * n_gc);
   hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * lmax * sizeof(double), stream, d_pjobs,
                      2 * n_gc, d_status);
   kt_end(KT_POLES);

@@ -369,11 +369,12 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   // a reduced side (pivots given) is a fishbone block and is diagonalised
   // first; an unreduced side is diagonal already
   const int lside = std::max(npv[0] ? nfw : 0, npv[1] ? nbk : 0);
-  if (lside && ((size_t)(2 + n_orb) * lside + n_orb + 16 + (size_t)lside * lside) * sizeof(double) > (size_t)kLdsBytes)
-    return BCD_ERR_CAPACITY;
+  const size_t t_aux = ((size_t)(2 + n_orb) * lside + n_orb + 16) * sizeof(double);
+  if (lside && t_aux > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
+  const bool t_lds = t_aux + (size_t)lside * lside * sizeof(double) <= (size_t)kLdsBytes;
   const size_t aws = arrow_ws_bytes(n, n_orb);
   const bool a_lds = aws <= (size_t)kLdsBytes;
-  DevBuf dh, du, dfw, dbk, dpf, dpb, dtri, dpole, dws, de, dx, dc, dk, dj, dja, dst;
+  DevBuf dh, du, dfw, dbk, dpf, dpb, dtri, dmat, dpole, dws, de, dx, dc, dk, dj, dja, dst;
   BTRY(dh.alloc(sizeof(double) * n_orb * n_orb));
   BTRY(du.alloc(sizeof(double) * n_orb * n_orb));
   BTRY(dfw.alloc(sizeof(double) * (size_t)std::max(nfw, 1) * Tf));
@@ -382,6 +383,7 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   BTRY(dpb.alloc(sizeof(int) * (kMaxPiv + 2)));
   BTRY(dtri.alloc(sizeof(double) * (size_t)std::max(lside, 1) * (2 + n_orb) * 2));
   BTRY(dpole.alloc(sizeof(double) * (size_t)(1 + n_orb) * std::max(nfw + nbk, 1)));
+  if (lside && !t_lds) BTRY(dmat.alloc(sizeof(double) * (size_t)lside * lside * 2));
   if (!a_lds) BTRY(dws.alloc(aws));
   BTRY(de.alloc(sizeof(double) * n));
   BTRY(dx.alloc(sizeof(double) * (size_t)n * n_orb));
@@ -418,6 +420,7 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
       J.nfw = ns[s]; J.den_fw = J.den_bk = dens[s]; J.npiv_fw = npv[s];
       J.rfw = rows[s]; J.ipiv_fw = ipv[s];
       J.tri = (double *)dtri.p + (size_t)njob * lside * (2 + n_orb);
+      J.mat = t_lds ? nullptr : (double *)dmat.p + (size_t)njob * lside * lside;
       J.out_lam = pole + poff; J.out_ct = pole + poff + ns[s]; J.ct_ld = ns[s];
       A.E[s] = J.out_lam; A.es[s] = 1; A.C[s] = J.out_ct; A.ci[s] = 1; A.cr[s] = ns[s];
       poff += (long)(1 + n_orb) * ns[s];
@@ -432,14 +435,17 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   A.out_e = (double *)de.p; A.out_x = (double *)dx.p;
   A.out_nqp = (int *)dc.p; A.out_nqh = (int *)dc.p + 1; A.kolt = (double *)dk.p;
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_arrow<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   if (njob) {
     HTRY(hipMemcpy(dj.p, P, sizeof(DysonJob) * njob, hipMemcpyHostToDevice));
     const DysonJob *jp = (const DysonJob *)dj.p;
-    const size_t aux = ((size_t)(2 + n_orb) * lside + n_orb + 16) * sizeof(double);
-    hipLaunchKernelGGL(k_tridiag<true>, dim3(njob), dim3(256), aux + (size_t)lside * lside * sizeof(double),
-                       ctx->stream, jp, njob);
+    if (t_lds)
+      hipLaunchKernelGGL(k_tridiag<true>, dim3(njob), dim3(256), t_aux + (size_t)lside * lside * sizeof(double),
+                         ctx->stream, jp, njob);
+    else
+      hipLaunchKernelGGL(k_tridiag<false>, dim3(njob), dim3(256), t_aux, ctx->stream, jp, njob);
     hipLaunchKernelGGL(k_ql, dim3(njob), dim3(64), (size_t)(2 + n_orb) * lside * sizeof(double), ctx->stream, jp,
                        njob, (int *)dst.p);
   }

@@ -43,7 +43,7 @@ __global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
 template <bool IN_LDS>
 __global__ void k_arrow(const ArrowJob *jobs, int njobs, int *status);
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
-                             double *tri, double *poles, DysonJob *jobs);
+                             double *tri, double *mat, double *poles, DysonJob *jobs);
 __global__ void k_arrow_prep(const ChanDev *chans, int n_gc, const ChanState *cs, PropView P, int src,
                              const double *h, const double *u1, const long *row_fw_base,
                              const long *row_bk_base, const double *rows_fw, const double *rows_bk,
