@@ -38,9 +38,12 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 
 
+PARAMS = dict(mode=2, hf_itr_max=HF_CAP, sc_itr_max=SC_CAP, lanczos=True, lanc_kind=1, lanc_n=LANC_N, conv_kind=1,
+              hf_conv=CONV, sc_conv=CONV, t_fac=0.0, ext_u1=1, dyson_solver=0)
+
+
 def params():
-    return capi.solve_params(mode=2, hf_itr_max=HF_CAP, sc_itr_max=SC_CAP, lanczos=True, lanc_kind=1, lanc_n=LANC_N,
-                             conv_kind=1, hf_conv=CONV, sc_conv=CONV, t_fac=0.0, ext_u1=1)
+    return capi.solve_params(**PARAMS)
 
 
 def screened_seeds(n):
@@ -75,14 +78,22 @@ def roofline(batch):
     totals = {k: v[0] * v[1] for k, v in kt.items()}
     dom = max(totals, key=totals.get)
     avg_ms, launches = kt[dom]
-    # algorithmic work per launch of each family:
-    #   tridiag: Householder reduction + accumulation, 8/3 n^3 per eigensolve
-    #   ql:      implicit QL with vector updates, the rest of 9 n^3
+    # algorithmic work of each family over the solve (divided by its launches
+    # below):
+    #   arrow:   arrowhead Dyson sweeps, FLOPs as counted by the kernel (9 per
+    #            pole per secular evaluation, 4 per Loewner factor, 5+2*rows
+    #            per pole per eigenvector), FP64
+    #   poles:   diagonalisation of the reduced sides, 9 L^3 per side, FP64
+    #   tridiag: dense solver's Householder reduction, 8/3 n^3 per eigensolve
+    #   ql:      dense solver's implicit QL with vector updates, rest of 9 n^3
     #   lanczos: 8*M*sum_j(2j+5+n+npiv) bytes per side
-    #   sigma2:  2*poles*K*n flops
+    #   sigma2:  2*poles*K*n FLOPs
+    dense = PARAMS.get("dyson_solver", 0) == 1
     per_launch = {
-        "tridiag": (work["sum_n3"] * 8.0 / 3.0, "mfma"),
-        "ql": (work["sum_n3"] * (9.0 - 8.0 / 3.0), "mfma"),
+        "arrow": (work["arrow_flops"], "mfma"),
+        "poles": (work["poles_flops"], "mfma"),
+        "tridiag": (work["sum_n3"] * 8.0 / 3.0 if dense else 0.0, "mfma"),
+        "ql": (work["sum_n3"] * (9.0 - 8.0 / 3.0) if dense else 0.0, "mfma"),
         "lanczos": (work["lanczos_bytes"], "hbm"),
         "sigma2": (work["coupling_flops"], "mfma"),
     }

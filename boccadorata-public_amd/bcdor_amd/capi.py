@@ -337,6 +337,7 @@ class Batch:
         return out
 
     def work(self):
-        w = np.zeros(5)
-        check(lib().bcd_batch_work(self.h, dp(w), 5), "bcd_batch_work")
-        return dict(coupling_flops=w[0], lanczos_bytes=w[1], dyson_flops=w[2], n_eig=w[3], sum_n3=w[4])
+        w = np.zeros(7)
+        check(lib().bcd_batch_work(self.h, dp(w), 7), "bcd_batch_work")
+        return dict(coupling_flops=w[0], lanczos_bytes=w[1], dyson_flops=w[2], n_eig=w[3], sum_n3=w[4],
+                    arrow_flops=w[5], poles_flops=w[6])

@@ -1,8 +1,8 @@
 // bcd_arrow.hip -- Dyson sweeps by successive arrowhead updates on gfx950.
 //
 // k_poles_prep + k_tridiag/k_ql (poles mode) diagonalise every
-// Lanczos-reduced side once per solve; each sweep then runs k_arrow_prep and
-// one k_arrow block per channel (DysonMatrix.cpp:123-532 for the matrix,
+// Lanczos-reduced side once per solve; each sweep then runs one
+// k_arrow_sweep block per channel (DysonMatrix.cpp:123-532 for the matrix,
 // :436-484 for the split and ordering, :542-594 for Calc_En).
 #include "bcd_kernels.hpp"
 
@@ -10,12 +10,46 @@ namespace bcd {
 
 namespace {
 
+__device__ __forceinline__ double shfl_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__device__ __forceinline__ double shfl_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// The block an arrowhead body runs on: reductions go through wave shuffles
+// and one LDS slot per wave.
 struct DevBlock {
   int *status;
+  double *work;                  // FLOP counter (may be null)
   __device__ int tid() const { return threadIdx.x; }
   __device__ int nt() const { return blockDim.x; }
   __device__ void sync() const { __syncthreads(); }
   __device__ void fail() const { atomicAdd(status, 1); }
+  __device__ void add_work(double f) const { if (work) atomicAdd(work, f); }
+  __device__ double sum(double *red, double v) const {
+    v = shfl_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    __syncthreads();
+    return s;
+  }
+  __device__ double max(double *red, double v) const {
+    v = shfl_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, red[w]);
+    __syncthreads();
+    return s;
+  }
 };
 
 }  // namespace
@@ -28,7 +62,7 @@ __global__ __launch_bounds__(kArrowMaxThreads) void k_arrow(const ArrowJob *jobs
   if ((int)blockIdx.x >= njobs) return;
   const ArrowJob J = jobs[blockIdx.x];
   if (J.n <= 0) return;
-  arrow_body(DevBlock{status}, J, IN_LDS ? sm : J.ws);
+  arrow_body(DevBlock{status, nullptr}, J, IN_LDS ? sm : J.ws);
 }
 
 template __global__ void k_arrow<true>(const ArrowJob *, int, int *);
@@ -64,45 +98,61 @@ __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs
   J.out_ct = J.out_lam + capS;
 }
 
-// The arrowhead job of every channel for this sweep: static part, pole
-// source (none, unreduced rows, or the diagonalised reduced sides), Fermi
-// energy of the sweep's input propagator, output slots in the other buffer.
-__global__ void k_arrow_prep(const ChanDev *chans, int n_gc, const ChanState *cs, PropView P, int src,
-                             const double *h, const double *u1, const long *row_fw_base,
-                             const long *row_bk_base, const double *rows_fw, const double *rows_bk,
-                             const double *poles, double *aws, double *kolt, ArrowJob *jobs) {
-  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gc >= n_gc) return;
-  const ChanDev C = chans[gc];
-  ArrowJob &J = jobs[gc];
-  const InstState *S = P.is ? P.is + C.inst : nullptr;
-  if (S && !S->active) { J.n = 0; return; }
-  const ChanState &Q = cs[gc];
+// The arrowhead job of channel gc for this sweep: static part, pole source
+// (none, unreduced rows, or the diagonalised reduced sides), Fermi energy of
+// the sweep's input propagator, output slots in the other buffer.  n = 0 for
+// channels of instances that no longer iterate.
+__device__ ArrowJob arrow_job(const ArrowSweep &a, int gc) {
+  const ChanDev &C = a.chans[gc];
+  ArrowJob J;
+  J.n = 0;
+  const InstState *S = a.P.is ? a.P.is + C.inst : nullptr;
+  if (S && !S->active) return J;
+  const ChanState &Q = a.cs[gc];
   const int cur = S ? S->cur : 0, nxt = 1 - cur, nI = C.nI;
   J.nI = nI; J.j2 = C.j2; J.inst = C.inst;
-  J.h = h + C.h_off; J.u1 = u1 + C.u1_off;
+  J.h = a.h + C.h_off; J.u1 = a.u1 + C.u1_off;
   for (int s = 0; s < 2; ++s) { J.np[s] = 0; J.es[s] = 1; J.ci[s] = 1; J.cr[s] = 1; J.E[s] = nullptr; J.C[s] = nullptr; }
-  if (src == 1) {
+  if (a.src == 1) {
     const int T = 2 + nI;
     J.np[0] = Q.mfw; J.np[1] = Q.mbk;
-    J.E[0] = rows_fw + row_fw_base[gc]; J.E[1] = rows_bk + row_bk_base[gc];
+    J.E[0] = a.rows_fw + a.row_fw_base[gc]; J.E[1] = a.rows_bk + a.row_bk_base[gc];
     for (int s = 0; s < 2; ++s) { J.C[s] = J.E[s] + 2; J.es[s] = T; J.ci[s] = T; J.cr[s] = 1; }
-  } else if (src == 2) {
+  } else if (a.src == 2) {
     J.np[0] = Q.lfw > 0 ? Q.lfw : 0; J.np[1] = Q.lbk > 0 ? Q.lbk : 0;
-    J.E[0] = poles + C.pole_off;
+    J.E[0] = a.poles + C.pole_off;
     J.C[0] = J.E[0] + C.lfw_cap; J.cr[0] = C.lfw_cap;
-    J.E[1] = poles + C.pole_off + (long)(1 + nI) * C.lfw_cap;
+    J.E[1] = a.poles + C.pole_off + (long)(1 + nI) * C.lfw_cap;
     J.C[1] = J.E[1] + C.lbk_cap; J.cr[1] = C.lbk_cap;
   }
   J.n = nI + J.np[0] + J.np[1];
-  if (J.n > C.cap) { J.n = 0; if (P.is) P.is[C.inst].err = 2; return; }
+  if (J.n > C.cap) {
+    J.n = 0;
+    if (a.P.is && threadIdx.x == 0) a.P.is[C.inst].err = 2;
+    return J;
+  }
   J.efermi = S ? S->ef[C.ch & 1] : 0.0;
-  J.ws = aws ? aws + C.aws_off : nullptr;
-  J.out_e = P.e[nxt] + P.e_off[gc];
-  J.out_x = P.x[nxt] + P.x_off[gc];
-  J.out_nqp = P.nqp[nxt] + gc;
-  J.out_nqh = P.nqh[nxt] + gc;
-  J.kolt = kolt + 3L * gc;
+  J.ws = a.aws ? a.aws + C.aws_off : nullptr;
+  J.out_e = a.P.e[nxt] + a.P.e_off[gc];
+  J.out_x = a.P.x[nxt] + a.P.x_off[gc];
+  J.out_nqp = a.P.nqp[nxt] + gc;
+  J.out_nqh = a.P.nqh[nxt] + gc;
+  J.kolt = a.kolt + 3L * gc;
+  return J;
 }
+
+// One Dyson sweep of every channel: one block per channel builds its job and
+// solves it.
+template <bool IN_LDS>
+__global__ __launch_bounds__(kArrowMaxThreads) void k_arrow_sweep(ArrowSweep a) {
+  extern __shared__ double sm[];
+  if ((int)blockIdx.x >= a.n_gc) return;
+  const ArrowJob J = arrow_job(a, blockIdx.x);
+  if (J.n <= 0) return;
+  arrow_body(DevBlock{a.status, a.work}, J, IN_LDS ? sm : J.ws);
+}
+
+template __global__ void k_arrow_sweep<true>(ArrowSweep);
+template __global__ void k_arrow_sweep<false>(ArrowSweep);
 
 }  // namespace bcd

@@ -72,8 +72,16 @@ __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
 // Root j (0..p) of the secular equation with sorted poles d[0..p-1] and
 // couplings z (all non-zero).  The root is returned as an offset tau from the
 // pole org it is closest to, so that x - d_i = tau - (d_i - d_org) keeps full
-// relative accuracy near the poles.  Returns the iterations used (kArrowMaxIt
-// if the iteration did not settle).
+// relative accuracy near the poles.
+//
+// Iteration: the poles on the origin's side of the root (psi) are modelled by
+// the origin pole alone and those on the far side plus the slope of the
+// linear term (phi) by the far pole, each weight matching its part's slope at
+// the current point (the "middle way" of R.-C. Li, LAPACK Working Note 89,
+// 1993); the extreme roots have no far pole and keep the linear term exact.
+// A bracket on tau falls back to bisection when a step leaves it.  Stops when
+// |f| is at the rounding level of its terms or the step is below an ulp of
+// tau.  Returns the iterations used (kArrowMaxIt if it did not settle).
 __host__ __device__ inline int arrow_root(int j, int p, const double *d, const double *z, double alpha,
                                           double znorm, int *org_out, double *tau_out) {
   const double eps = 2.2204460492503131e-16;
@@ -93,37 +101,40 @@ __host__ __device__ inline int arrow_root(int j, int p, const double *d, const d
     if (f >= 0.0) { o = j - 1; far = j; lo = 0.0; hi = mid - a; }
     else { o = j; far = j - 1; lo = mid - b; hi = 0.0; }
   }
-  const double dofs = d[o], zo2 = z[o] * z[o];
+  // poles i on the origin's side: i <= o when the origin is the left end,
+  // i >= o when it is the right end, all of them for an extreme root
+  const int s_lo = far < 0 ? 0 : (o < far ? 0 : o), s_hi = far < 0 ? p - 1 : (o < far ? o : p - 1);
+  const double dofs = d[o];
   double tau = 0.5 * (lo + hi);
   int it = 0;
   for (; it < kArrowMaxIt; ++it) {
-    double S = 0.0, Sp = 0.0, mag = 0.0;
+    double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, mag = 0.0;
     for (int i = 0; i < p; ++i) {
-      if (i == o) continue;
       const double r = 1.0 / ((d[i] - dofs) - tau);
       const double t = z[i] * z[i] * r;
-      S += t; Sp += t * r; mag += fabs(t);
+      if (i >= s_lo && i <= s_hi) { psi += t; dpsi += t * r; }
+      else { phi += t; dphi += t * r; }
+      mag += fabs(t);
     }
-    const double near = zo2 / tau;
-    const double f = (dofs - alpha) + tau + S - near;
-    mag += fabs(dofs - alpha) + fabs(tau) + fabs(near);
-    if (fabs(f) <= 4.0 * eps * mag) break;
+    const double f = (dofs - alpha) + tau + psi + phi;
+    mag += fabs(dofs - alpha) + fabs(tau);
+    if (fabs(f) <= 8.0 * eps * mag) break;
     if (f < 0.0) lo = tau; else hi = tau;
     if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi))) break;
+    const double so = dpsi * tau * tau;
     double tn;
     if (far < 0) {
-      // model: a + b t - zo2 / t  (origin pole exact, the rest linearised)
-      const double b = 1.0 + Sp, a = (dofs - alpha) + tau + S - b * tau;
-      const double sq = sqrt(a * a + 4.0 * b * zo2);
-      if (tau > 0.0) tn = a > 0.0 ? 2.0 * zo2 / (a + sq) : (sq - a) / (2.0 * b);
-      else tn = a < 0.0 ? -2.0 * zo2 / (sq - a) : -(a + sq) / (2.0 * b);
+      // a + t - so / t, with a + tau - so / tau = f
+      const double a = f - tau + so / tau;
+      const double sq = sqrt(a * a + 4.0 * so);
+      if (tau > 0.0) tn = a > 0.0 ? 2.0 * so / (a + sq) : 0.5 * (sq - a);
+      else tn = a < 0.0 ? -2.0 * so / (sq - a) : -0.5 * (a + sq);
     } else {
-      // model: c + zo2 / (0 - t) + s / (dfar - t)  (both interval ends as
-      // poles; the far weight s takes every other term's slope)
+      // c + so / (0 - t) + sf / (dfar - t)
       const double dfar = d[far] - dofs, gap = dfar - tau;
-      const double s = (1.0 + Sp) * gap * gap;
-      const double c = f + near - s / gap;
-      const double Bq = c * dfar + zo2 + s, C0 = zo2 * dfar;
+      const double sf = (dphi + 1.0) * gap * gap;
+      const double c = f + so / tau - sf / gap;
+      const double Bq = c * dfar + so + sf, C0 = so * dfar;
       if (c == 0.0) {
         tn = C0 / Bq;
       } else {
@@ -137,7 +148,14 @@ __host__ __device__ inline int arrow_root(int j, int p, const double *d, const d
         }
       }
     }
-    if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+    if (!(tn > lo && tn < hi)) {
+      // a step out of the bracket with |f| already near rounding is noise
+      if (fabs(f) <= 1e3 * eps * mag) break;
+      tn = 0.5 * (lo + hi);
+    } else if (fabs(tn - tau) <= 4.0 * eps * fabs(tau)) {
+      tau = tn;
+      break;
+    }
     tau = tn;
   }
   *org_out = o;
@@ -145,22 +163,17 @@ __host__ __device__ inline int arrow_root(int j, int p, const double *d, const d
   return it;
 }
 
-// Block-wide sum of v (every thread passes its value; every thread gets the
-// total).  Fixed summation order.
+// Block-wide sum and maximum of one value per thread; every thread gets the
+// result.  The block type supplies them (wave shuffles on the GPU, a serial
+// loop on host threads); both use a fixed order.
 template <class Blk>
 __host__ __device__ inline double arrow_block_sum(const Blk &B, double *red, double v) {
-  const int tid = B.tid(), nt = B.nt();
-  red[tid] = v;
-  B.sync();
-  if (tid == 0) {
-    double s = 0.0;
-    for (int t = 0; t < nt; ++t) s += red[t];
-    red[kArrowRed - 1] = s;
-  }
-  B.sync();
-  const double s = red[kArrowRed - 1];
-  B.sync();
-  return s;
+  return B.sum(red, v);
+}
+
+template <class Blk>
+__host__ __device__ inline double arrow_block_max(const Blk &B, double *red, double v) {
+  return B.max(red, v);
 }
 
 // The whole channel: poles, nI arrowhead updates, split at the Fermi energy,
@@ -178,6 +191,10 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
   const double eps = 2.2204460492503131e-16;
 
   if (tid == 0) cnt[2] = 0;
+  // algorithmic FLOPs of this thread's share: 9 per pole per secular-equation
+  // evaluation (4 for the bracketing midpoint), 4 per Loewner factor, and
+  // 5 + 2 per carried row per pole for each eigenvector
+  double flops = 0.0;
   // poles (unsorted) into the B buffers
   for (int i = tid; i < m; i += nt) {
     const int s = i < J.np[0] ? 0 : 1, ii = s ? i - J.np[0] : i;
@@ -202,15 +219,8 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     double s2 = 0.0, dmx = 0.0;
     for (int i = tid; i < g; i += nt) { s2 += z[i] * z[i]; dmx = fmax(dmx, fabs(dA[i])); }
     const double znorm = sqrt(arrow_block_sum(B, red, s2));
-    red[tid] = dmx;
-    B.sync();
-    if (tid == 0) {
-      double mx = 0.0;
-      for (int t = 0; t < nt; ++t) mx = fmax(mx, red[t]);
-      red[kArrowRed - 2] = mx;
-    }
-    B.sync();
-    const double tol = 8.0 * eps * fmax(fmax(fabs(alpha), red[kArrowRed - 2]), znorm);
+    const double dmax = arrow_block_max(B, red, dmx);
+    const double tol = 8.0 * eps * fmax(fmax(fabs(alpha), dmax), znorm);
     // deflation scan in energy order (one thread)
     if (tid == 0) {
       int p = 0, q = 0, prev = -1;
@@ -254,6 +264,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       if (p == 0) { org[j] = -1; tau[j] = 0.0; continue; }
       const int its = arrow_root(j, p, dn, zn, alpha, znorm, &org[j], &tau[j]);
       if (its >= kArrowMaxIt) cnt[2] = 1;
+      flops += 9.0 * p * (its + 1) + ((j > 0 && j < p) ? 4.0 * p : 0.0);
     }
     B.sync();
     // couplings recomputed from the roots: zh_i^2 = -prod_j (d_i - x_j) /
@@ -264,6 +275,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       for (int j = 0; j < i; ++j) v *= ((di - dn[org[j]]) - tau[j]) / (di - dn[j]);
       for (int j = i + 2; j <= p; ++j) v *= ((di - dn[org[j]]) - tau[j]) / (di - dn[j - 1]);
       zh[i] = copysign(sqrt(fmax(v, 0.0)), zn[i]);
+      flops += 4.0 * (p + 1);
     }
     B.sync();
     // eigenvectors: roots to positions [0, p], deflated directions to [p+1, p+q]
@@ -291,6 +303,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
         }
       }
       const double inv = 1.0 / sqrt(nu2);
+      flops += (5.0 + 2.0 * (2 * nI - k - 1)) * p;
       dB[j] = lam;
 #pragma unroll
       for (int r = 0; r < kArrowNI; ++r) {
@@ -343,6 +356,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     }
   }
   for (int b = 0; b < 9; ++b) kb[b] = arrow_block_sum(B, red, kb[b]);
+  flops = arrow_block_sum(B, red, flops);
   if (tid == 0) {
     *J.out_nqp = nqp;
     *J.out_nqh = nqh;
@@ -351,6 +365,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     J.kolt[1] = (kb[1] + kb[4] + kb[7]) * f;
     J.kolt[2] = (kb[2] + kb[5] + kb[8]) * f;
     if (cnt[2]) B.fail();
+    B.add_work(flops);
   }
 }
 

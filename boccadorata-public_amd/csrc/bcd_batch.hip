@@ -198,6 +198,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_piv = W0 + l_piv; d_lrows = W0 + l_lr; d_kolt = W0 + l_kolt; d_poles = W0 + l_pole;
   d_status = (int *)(W0 + l_end - 16);
   d_work_count = d_status + 2;
+  d_work = W0 + l_end - 8;
   HTRY(hipMemset(a_work.ptr, 0, sizeof(double) * l_end));
   HTRY(hipMemcpy(d_u1, u1.data(), sizeof(double) * o_u1, hipMemcpyHostToDevice));
 
@@ -237,12 +238,10 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipMemcpy(pnqh_[2], hqh.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
 
   // jobs
-  BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) +
-                   pad8(sizeof(ArrowJob) * n_gc) + 64));
+  BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) + 64));
   d_djobs = (DysonJob *)a_jobs.ptr;
   d_pjobs = d_djobs + n_gc;
   d_ljobs = (LancJob *)((char *)a_jobs.ptr + pad8(sizeof(DysonJob) * 3 * n_gc));
-  d_ajobs = (ArrowJob *)((char *)d_ljobs + pad8(sizeof(LancJob) * 2 * n_gc));
   HTRY(hipMemset(a_jobs.ptr, 0, a_jobs.cap));
 
   // dense matrices in HBM when the largest one does not fit in LDS: the
@@ -263,7 +262,8 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-  HTRY(hipFuncSetAttribute((const void *)k_arrow<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  HTRY(hipFuncSetAttribute((const void *)k_arrow_sweep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           kLdsBytes));
   h_is.assign(n_inst, InstState());
   return BCD_OK;
 }
@@ -299,7 +299,11 @@ void Batch::kt_end(int k) {
   kcount[k] += 1;
 }
 
-int Batch::sweep(int src, int &n_active) {
+// Queue one sweep (Sigma^inf from the current propagator, Dyson, convergence
+// bookkeeping) for every instance that still iterates; instances that have
+// stopped are skipped inside every kernel, so queued sweeps past an
+// instance's end change nothing.
+int Batch::sweep(int src) {
   PropView P = view(true);
   const int gb = (n_gc + 127) / 128;
   kt_begin(KT_DENSITY);
@@ -309,17 +313,18 @@ int Batch::sweep(int src, int &n_active) {
   hipLaunchKernelGGL(k_sigma_inf, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_is, d_w, d_rho, d_h);
   kt_end(KT_SIGINF);
   if (par.dyson_solver == 0) {
-    hipLaunchKernelGGL(k_arrow_prep, dim3(gb), dim3(128), 0, stream, d_chans, n_gc, d_cs, P, src, d_h, d_u1,
-                       d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_poles,
-                       aws_len ? (double *)a_aws.ptr : nullptr, d_kolt, d_ajobs);
+    ArrowSweep a;
+    a.chans = d_chans; a.n_gc = n_gc; a.src = src; a.cs = d_cs; a.P = P; a.h = d_h; a.u1 = d_u1;
+    a.row_fw_base = d_row_fw_base; a.row_bk_base = d_row_bk_base; a.rows_fw = d_rows_fw; a.rows_bk = d_rows_bk;
+    a.poles = d_poles; a.aws = aws_len ? (double *)a_aws.ptr : nullptr; a.kolt = d_kolt; a.status = d_status;
+    a.work = d_work;
     const int nmax = src == 0 ? nImax : (int)capmax;
     const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
     kt_begin(KT_ARROW);
     if (!aws_len)
-      hipLaunchKernelGGL(k_arrow<true>, dim3(n_gc), dim3(nt), arrow_ws_bytes(nmax, nImax), stream, d_ajobs, n_gc,
-                         d_status);
+      hipLaunchKernelGGL(k_arrow_sweep<true>, dim3(n_gc), dim3(nt), arrow_ws_bytes(nmax, nImax), stream, a);
     else
-      hipLaunchKernelGGL(k_arrow<false>, dim3(n_gc), dim3(nt), 0, stream, d_ajobs, n_gc, d_status);
+      hipLaunchKernelGGL(k_arrow_sweep<false>, dim3(n_gc), dim3(nt), 0, stream, a);
     kt_end(KT_ARROW);
   } else {
     hipLaunchKernelGGL(k_dyson_prep, dim3(gb), dim3(128), 0, stream, d_chans, 0, n_gc, d_cs, P, src, d_h, d_u1,
@@ -343,6 +348,12 @@ int Batch::sweep(int src, int &n_active) {
                      par.conv_kind, 0);
   kt_end(KT_FINAL);
   HTRY(hipGetLastError());
+  return BCD_OK;
+}
+
+// Per-instance flags after the queued sweeps: how many instances still
+// iterate, capacity and eigensolver failures.
+int Batch::check(int &n_active) {
   HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
   HTRY(hipStreamSynchronize(stream));
   n_active = 0;
@@ -366,8 +377,14 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
                      par.conv_kind, 1);
   HTRY(hipGetLastError());
   const int hard_cap = itr_max >= 0 ? itr_max + 1 : kMaxSweeps;
-  int n_active = n_inst;
-  for (int it = 0; it < hard_cap && n_active > 0; ++it) BTRY(sweep(src, n_active));
+  int n_active = n_inst, done = 0;
+  while (done < hard_cap && n_active > 0) {
+    // sweeps are queued kSweepChunk at a time between reads of the flags
+    const int k = std::min(kSweepChunk, hard_cap - done);
+    for (int q = 0; q < k; ++q) BTRY(sweep(src));
+    done += k;
+    BTRY(check(n_active));
+  }
   for (int i = 0; i < n_inst; ++i) {
     (hf ? hf_its : sc_its)[i] = h_is[i].its;
     (hf ? hf_ok : sc_ok)[i] = h_is[i].diff < conv;
@@ -511,8 +528,13 @@ int Batch::decompose_poles() {
 
 // Algorithmic work of the last solve (see bcd_batch_work in bcdor_amd.h).
 int Batch::work(double *w) {
-  for (int k = 0; k < 5; ++k) w[k] = 0.0;
+  for (int k = 0; k < 7; ++k) w[k] = 0.0;
   w[0] = coupling_flops;
+  if (d_work) HTRY(hipMemcpy(&w[5], d_work, sizeof(double), hipMemcpyDeviceToHost));
+  for (int gc = 0; gc < (int)h_cs.size() && par.lanczos && par.dyson_solver == 0; ++gc) {
+    const double lf = std::max(h_cs[gc].lfw, 0), lb = std::max(h_cs[gc].lbk, 0);
+    w[6] += 9.0 * (lf * lf * lf + lb * lb * lb);
+  }
   for (int gc = 0; gc < (int)h_cs.size(); ++gc) {
     const ChanDev &C = h_chans[gc];
     const ChanState &Q = h_cs[gc];
@@ -554,6 +576,7 @@ int Batch::solve() {
   hf_ok.assign(n_inst, 0);
   sc_ok.assign(n_inst, par.mode == 0 ? 1 : 0);
   HTRY(hipMemsetAsync(d_status, 0, sizeof(int), stream));
+  HTRY(hipMemsetAsync(d_work, 0, sizeof(double), stream));
   HTRY(hipEventRecord(ev[0], stream));
   // start from the initial propagator in buffer 0
   HTRY(hipMemcpyAsync(pe_[0], pe_[2], sizeof(double) * (prop_e_len + prop_x_len), hipMemcpyDeviceToDevice, stream));

@@ -12,6 +12,7 @@ namespace bcd {
 
 constexpr int kLdsBytes = 160 * 1024 - 1024;   // dynamic LDS ceiling used by the Dyson kernels
 constexpr int kMaxSweeps = 2000;
+constexpr int kSweepChunk = 4;                 // sweeps queued between host reads of the convergence flags
 
 // Starting propagator of one instance, fragments as SpProp_t stores them.
 struct HostProp {
@@ -57,6 +58,7 @@ struct Batch {
   BlockDev *d_blocks = nullptr;
   int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_list = nullptr, *d_work_count = nullptr;
   int *d_status = nullptr;
+  double *d_work = nullptr;                          // arrowhead FLOP counter of the current solve
   double *d_u1 = nullptr, *d_h = nullptr, *d_w = nullptr, *d_rho = nullptr, *d_tri = nullptr, *d_piv = nullptr;
   double *d_lrows = nullptr, *d_kolt = nullptr, *d_rows_fw = nullptr, *d_rows_bk = nullptr, *d_poles = nullptr;
   long l_u1 = 0, l_h = 0, l_w = 0, l_rho = 0, l_tri = 0, l_piv = 0, l_lr = 0, l_kolt = 0, l_pole = 0, l_end = 0;
@@ -64,7 +66,6 @@ struct Batch {
   int *pnqp_[3] = {nullptr, nullptr, nullptr}, *pnqh_[3] = {nullptr, nullptr, nullptr};
   long prop_e_len = 0, prop_x_len = 0, prop_n_len = 0;
   DysonJob *d_djobs = nullptr, *d_pjobs = nullptr;   // per-sweep dense jobs; side diagonalisation jobs
-  ArrowJob *d_ajobs = nullptr;
   LancJob *d_ljobs = nullptr;
   long aws_len = 0;                                  // global arrowhead workspace (doubles), 0: LDS
 
@@ -86,7 +87,8 @@ struct Batch {
   size_t tridiag_lds(int n, bool in_lds) const;
   void kt_begin(int k);
   void kt_end(int k);
-  int sweep(int src, int &n_active);
+  int sweep(int src);
+  int check(int &n_active);
   int stage(int src, int itr_max, double conv, bool hf);
   int build_sigma2(int only_ish);
   int prepare();

@@ -44,10 +44,21 @@ template <bool IN_LDS>
 __global__ void k_arrow(const ArrowJob *jobs, int njobs, int *status);
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
                              double *tri, double *mat, double *poles, DysonJob *jobs);
-__global__ void k_arrow_prep(const ChanDev *chans, int n_gc, const ChanState *cs, PropView P, int src,
-                             const double *h, const double *u1, const long *row_fw_base,
-                             const long *row_bk_base, const double *rows_fw, const double *rows_bk,
-                             const double *poles, double *aws, double *kolt, ArrowJob *jobs);
+// Everything one Dyson sweep over all channels reads (k_arrow_sweep).
+struct ArrowSweep {
+  const ChanDev *chans;
+  int n_gc, src;                 // src: 0 no dynamic Sigma, 1 unreduced rows, 2 diagonalised reduced sides
+  const ChanState *cs;
+  PropView P;
+  const double *h, *u1;
+  const long *row_fw_base, *row_bk_base;
+  const double *rows_fw, *rows_bk, *poles;
+  double *aws, *kolt;
+  int *status;
+  double *work;                  // accumulates the arrowhead FLOPs (may be null)
+};
+template <bool IN_LDS>
+__global__ void k_arrow_sweep(ArrowSweep a);
 __global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out);
 __global__ void k_finalize(const InstDev *insts, const ChanDev *chans, PropView P, const ChanState *cs,
                            const double *kolt, int conv_kind, int init);

@@ -174,12 +174,17 @@ int bcd_batch_converged(bcd_batch *b, int inst);
 int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh,
                      double *ep, double *zp, double *fqp, double *eh, double *zh, double *fqh);
 /* Average device duration (ms) and launch count of each kernel family over
- * the last solve, for roofline accounting: names[k] "sigma2", "lanczos",
- * "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild". */
+ * the last solve, for roofline accounting (hipEvents around every launch,
+ * only while profiling is on): k = 0 "sigma2", 1 "lanczos", 2 "tridiag",
+ * 3 "ql", 4 "density", 5 "sigma_inf", 6 "finalize", 7 "wbuild",
+ * 8 "sigma2_plan", 9 "arrow", 10 "poles". */
 int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
 /* Algorithmic work of the last solve: [0] coupling FLOPs (2*sum poles*K*n),
- * [1] Lanczos bytes (8*M*sum_j(2j+5+n+npiv)), [2] Dyson FLOPs (9 n^3 per
- * eigensolve), [3] number of Dyson eigensolves, [4] sum of Dyson n^3. */
+ * [1] Lanczos bytes (8*M*sum_j(2j+5+n+npiv)), [2] dense Dyson FLOPs (9 n^3
+ * per eigensolve), [3] number of Dyson eigensolves, [4] sum of Dyson n^3,
+ * [5] arrowhead-solver FLOPs as counted by the kernels, [6] FLOPs of the
+ * once-per-solve diagonalisation of the reduced sides (9 L^3 per side).
+ * n >= 7. */
 int bcd_batch_work(bcd_batch *b, double *work, int n);
 
 #ifdef __cplusplus

@@ -36,10 +36,22 @@ struct HostBlock {
   int t, n;
   Barrier *bar;
   int *failed;
+  double *acc;
   int tid() const { return t; }
   int nt() const { return n; }
   void sync() const { bar->wait(); }
   void fail() const { *failed = 1; }
+  void add_work(double) const {}
+  double sum(double *, double v) const { return reduce(v, false); }
+  double max(double *, double v) const { return reduce(v, true); }
+  double reduce(double v, bool mx) const {
+    acc[t] = v;
+    sync();
+    double s = mx ? acc[0] : 0.0;
+    for (int i = 0; i < n; ++i) s = mx ? (acc[i] > s ? acc[i] : s) : s + acc[i];
+    sync();
+    return s;
+  }
 };
 
 template <class T>
@@ -74,9 +86,11 @@ int main() {
     J.h = h.data(); J.u1 = u1.data(); J.efermi = ef;
     J.out_e = oe.data(); J.out_x = ox.data(); J.out_nqp = &nqp; J.out_nqh = &nqh; J.kolt = kolt.data();
     Barrier bar(nt);
+    std::vector<double> acc(nt);
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t)
-      th.emplace_back([&, t] { bcd::arrow_body(HostBlock{t, nt, &bar, &failed}, J, (double *)ws.data()); });
+      th.emplace_back(
+          [&, t] { bcd::arrow_body(HostBlock{t, nt, &bar, &failed, acc.data()}, J, (double *)ws.data()); });
     for (auto &x : th) x.join();
     int32_t o[3] = {nqp, nqh, failed};
     fwrite(o, 4, 3, stdout);
