@@ -238,10 +238,12 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipMemcpy(pnqh_[2], hqh.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
 
   // jobs
-  BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) + 64));
+  BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) +
+                   pad8(sizeof(int) * 2 * n_gc) + 64));
   d_djobs = (DysonJob *)a_jobs.ptr;
   d_pjobs = d_djobs + n_gc;
   d_ljobs = (LancJob *)((char *)a_jobs.ptr + pad8(sizeof(DysonJob) * 3 * n_gc));
+  d_lorder = (int *)((char *)d_ljobs + pad8(sizeof(LancJob) * 2 * n_gc));
   HTRY(hipMemset(a_jobs.ptr, 0, a_jobs.cap));
 
   // dense matrices in HBM when the largest one does not fit in LDS: the
@@ -479,19 +481,37 @@ int Batch::lanczos() {
                      par.lanc_kind, par.lanc_n, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_lrows,
                      (double *)a_lws.ptr, d_ljobs);
   HTRY(hipGetLastError());
-  int mmax = 0;
-  for (int gc = 0; gc < n_gc; ++gc) mmax = std::max(mmax, std::max(h_cs[gc].mfw, h_cs[gc].mbk));
+  // jobs longest first, grouped by the block size their M needs; one launch
+  // per group, the largest blocks first
+  std::vector<std::pair<int, int>> jm;
+  for (int gc = 0; gc < n_gc; ++gc)
+    for (int side = 0; side < 2; ++side) {
+      const int M = side ? h_cs[gc].mbk : h_cs[gc].mfw;
+      if (M > 0 && !lanczos_threads(M)) return BCD_ERR_CAPACITY;
+      jm.push_back({M, 2 * gc + side});
+    }
+  std::stable_sort(jm.begin(), jm.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+    return a.first > b.first;
+  });
+  std::vector<int> order(jm.size());
+  for (size_t k = 0; k < jm.size(); ++k) order[k] = jm[k].second;
+  HTRY(hipMemcpyAsync(d_lorder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
   kt_begin(KT_LANCZOS);
-  if (mmax <= 256 * 8)
-    hipLaunchKernelGGL((k_lanczos<256, 8>), dim3(2 * n_gc), dim3(256), 0, stream, d_ljobs, 2 * n_gc);
-  else if (mmax <= 256 * 24)
-    hipLaunchKernelGGL((k_lanczos<256, 24>), dim3(2 * n_gc), dim3(256), 0, stream, d_ljobs, 2 * n_gc);
-  else if (mmax <= 512 * 24)
-    hipLaunchKernelGGL((k_lanczos<512, 24>), dim3(2 * n_gc), dim3(512), 0, stream, d_ljobs, 2 * n_gc);
-  else if (mmax <= 1024 * 24)
-    hipLaunchKernelGGL((k_lanczos<1024, 24>), dim3(2 * n_gc), dim3(1024), 0, stream, d_ljobs, 2 * n_gc);
-  else
-    return BCD_ERR_CAPACITY;
+  size_t k0 = 0;
+  while (k0 < jm.size()) {
+    const int nt = jm[k0].first > 0 ? lanczos_threads(jm[k0].first) : 192;
+    size_t k1 = k0;
+    while (k1 < jm.size() && (jm[k1].first > 0 ? lanczos_threads(jm[k1].first) : 192) == nt) ++k1;
+    const int nj = (int)(k1 - k0);
+    const int *ord = d_lorder + k0;
+    switch (nt) {
+      case 192: hipLaunchKernelGGL(k_lanczos<192>, dim3(nj), dim3(192), 0, stream, d_ljobs, ord, nj); break;
+      case 384: hipLaunchKernelGGL(k_lanczos<384>, dim3(nj), dim3(384), 0, stream, d_ljobs, ord, nj); break;
+      case 768: hipLaunchKernelGGL(k_lanczos<768>, dim3(nj), dim3(768), 0, stream, d_ljobs, ord, nj); break;
+      default: hipLaunchKernelGGL(k_lanczos<1024>, dim3(nj), dim3(1024), 0, stream, d_ljobs, ord, nj); break;
+    }
+    k0 = k1;
+  }
   kt_end(KT_LANCZOS);
   HTRY(hipGetLastError());
   HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));

@@ -67,6 +67,7 @@ struct Batch {
   long prop_e_len = 0, prop_x_len = 0, prop_n_len = 0;
   DysonJob *d_djobs = nullptr, *d_pjobs = nullptr;   // per-sweep dense jobs; side diagonalisation jobs
   LancJob *d_ljobs = nullptr;
+  int *d_lorder = nullptr;                           // Lanczos jobs, longest first
   long aws_len = 0;                                  // global arrowhead workspace (doubles), 0: LDS
 
   hipEvent_t ev[6] = {};

@@ -149,16 +149,13 @@ int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout
   J.n_out = (int *)dn.p; J.V = (double *)dV.p; J.inst = -1;
   HTRY(hipMemcpy(djob.p, &J, sizeof J, hipMemcpyHostToDevice));
   const LancJob *jp = (const LancJob *)djob.p;
-  if (NPLS_in <= 256 * 8)
-    hipLaunchKernelGGL((k_lanczos<256, 8>), dim3(1), dim3(256), 0, ctx->stream, jp, 1);
-  else if (NPLS_in <= 256 * 24)
-    hipLaunchKernelGGL((k_lanczos<256, 24>), dim3(1), dim3(256), 0, ctx->stream, jp, 1);
-  else if (NPLS_in <= 512 * 24)
-    hipLaunchKernelGGL((k_lanczos<512, 24>), dim3(1), dim3(512), 0, ctx->stream, jp, 1);
-  else if (NPLS_in <= 1024 * 24)
-    hipLaunchKernelGGL((k_lanczos<1024, 24>), dim3(1), dim3(1024), 0, ctx->stream, jp, 1);
-  else
-    return BCD_ERR_CAPACITY;
+  switch (lanczos_threads(NPLS_in)) {
+    case 192: hipLaunchKernelGGL(k_lanczos<192>, dim3(1), dim3(192), 0, ctx->stream, jp, nullptr, 1); break;
+    case 384: hipLaunchKernelGGL(k_lanczos<384>, dim3(1), dim3(384), 0, ctx->stream, jp, nullptr, 1); break;
+    case 768: hipLaunchKernelGGL(k_lanczos<768>, dim3(1), dim3(768), 0, ctx->stream, jp, nullptr, 1); break;
+    case 1024: hipLaunchKernelGGL(k_lanczos<1024>, dim3(1), dim3(1024), 0, ctx->stream, jp, nullptr, 1); break;
+    default: return BCD_ERR_CAPACITY;
+  }
   HTRY(hipGetLastError());
   HTRY(hipStreamSynchronize(ctx->stream));
   int nout = 0;

@@ -398,24 +398,67 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
   }
 }
 
+// Sum NV values over an NT-thread block with one barrier: every thread gets
+// the totals.  Two LDS slots alternate (ph), so a slot is only rewritten after
+// every thread has passed the barrier of the reduction in between.  NV is a
+// compile-time count so that vals stays in registers.
+template <int NT, int NV>
+__device__ __forceinline__ void lanc_reduce(double *vals, double *red, int &ph) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double *slot = red + ph * (NT / 64) * kLancRed;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) vals[k] = wave_sum(vals[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) slot[w * kLancRed + k] = vals[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) s += slot[i * kLancRed + k];
+    vals[k] = s;
+  }
+  ph ^= 1;
+}
+
+// Raw buffer access: one per-lane 32-bit offset plus a wave-uniform scalar
+// offset, so a register tile of many elements needs no per-element address
+// registers.  Offsets are in bytes; callers keep them inside the buffer.
+constexpr int kBufWord3 = 0x00020000;   // gfx9-family buffer descriptor word 3
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, kBufWord3);
+}
+
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void buf_st(double x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, voff, soff, 0);
+}
+
 // Multi-pivot Lanczos on diag(E) with full reorthogonalisation in the
 // reference's order (Reduce_Mtx_Lncz, Lanczos.cpp:101-241): the current vector
-// lives in registers (PER elements per thread), earlier vectors in HBM; each
-// Gram-Schmidt projection (latest vector first) is one block-wide reduction.
+// lives in registers (PER elements per thread, element p = tid + NT*t), the
+// earlier vectors in HBM, each thread touching only its own elements of
+// them; each Gram-Schmidt projection, latest vector first, is one block-wide
+// reduction of <v_i, v> and <v_i, v_i> followed by v -= (x1/x2) v_i.
 template <int NT, int PER>
-__global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, int njobs) {
-  __shared__ double red[(NT / 64) * kLancRed];
-  __shared__ double u[kMaxNB];
-  __shared__ int last[kMaxPiv + 1];
-  if ((int)blockIdx.x >= njobs) return;
-  const LancJob Jb = jobs[blockIdx.x];
-  if (Jb.nLanc <= 0) return;
+__device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, double *u, int *last, double *part) {
   const int tid = threadIdx.x, M = Jb.M, NB = Jb.NB, Tin = Jb.Tin, NA = Jb.NA_in;
   const int Tout = Jb.Tout, NAo = Jb.NA_out, nLanc = Jb.nLanc;
   const double *Min = Jb.Min;
   for (long i = tid; i < (long)Tout * nLanc; i += NT) Jb.Mout[i] = 0.0;
+  // element p = tid + NT*t of a vector: byte offset 8*tid (lane) + 8*NT*t (scalar)
+  const __amdgpu_buffer_rsrc_t rV = buf_rsrc(Jb.V, 8L * nLanc * M), rM = buf_rsrc(Min, 8L * M * Tin);
+  const int vo = 8 * tid, mo = 8 * tid * Tin;
   double v[PER], vi[PER];
-  int i_pvt = -1, n_pvt = -1, j_next = 0;
+  int i_pvt = -1, n_pvt = -1, j_next = 0, ph = 0;
   int status = 0;
   for (int j = 0; j < nLanc; ++j) {
     const bool fresh = (j == j_next);
@@ -438,72 +481,86 @@ __global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, int njobs) 
         const int p = tid + NT * t;
         double acc = 0.0;
         if (p < M)
-          for (int l = 0; l < NB; ++l) acc += Min[(long)p * Tin + NA + l] * u[l];
+          for (int l = 0; l < NB; ++l) acc += buf_ld(rM, mo, 8 * (NT * t * Tin + NA + l)) * u[l];
         v[t] = acc;
       }
     } else {
 #pragma unroll
       for (int t = 0; t < PER; ++t) {
         const int p = tid + NT * t;
-        v[t] = p < M ? v[t] * Min[(long)p * Tin] : 0.0;
+        v[t] = p < M ? v[t] * buf_ld(rM, mo, 8 * NT * t * Tin) : 0.0;
       }
     }
     // Gram-Schmidt against every earlier vector, latest first (Lanczos.cpp:164-173)
     for (int i = j - 1; i >= 0; --i) {
-      const double *Vi = Jb.V + (long)i * M;
+      const int io = 8 * i * M;
       double s[2] = {0.0, 0.0};
 #pragma unroll
       for (int t = 0; t < PER; ++t) {
         const int p = tid + NT * t;
-        vi[t] = p < M ? Vi[p] : 0.0;
+        vi[t] = p < M ? buf_ld(rV, vo, io + 8 * NT * t) : 0.0;
         s[0] += vi[t] * v[t];
         s[1] += vi[t] * vi[t];
       }
-      block_sum(s, 2, red);
+      lanc_reduce<NT, 2>(s, red, ph);
+      const double c = s[0] / s[1];
 #pragma unroll
-      for (int t = 0; t < PER; ++t) v[t] -= s[0] * vi[t] / s[1];
+      for (int t = 0; t < PER; ++t) v[t] -= c * vi[t];
     }
-    // a, b, g_i and the couplings m^L (Lanczos.cpp:181-216)
-    double s[kLancRed];
-    const int nred = 2 + NB + n_pvt;
-    for (int k = 0; k < nred; ++k) s[k] = 0.0;
+    // a, b, g_i and the couplings m^L (Lanczos.cpp:181-216).  Values k:
+    // 0 sum E v^2, 1 sum v^2, 2..2+NB the couplings to the orbits, then the
+    // g_i of the pivots; one value at a time, each reduced over its wave
+    // into part[k][wave], then summed over the waves after one barrier.
+    const int nred = 2 + NB + n_pvt, lane = tid & 63, wv = tid >> 6;
+    for (int k = 0; k < nred; ++k) {
+      double acc = 0.0;
+      if (k < 2) {
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int p = tid + NT * t;
-      if (p < M) {
-        const double E = Min[(long)p * Tin];
-        s[0] += E * v[t] * v[t];
-        s[1] += v[t] * v[t];
-        for (int l = 0; l < NB; ++l) s[2 + l] += Min[(long)p * Tin + NA + l] * v[t];
+        for (int t = 0; t < PER; ++t) {
+          const int p = tid + NT * t;
+          if (p < M) acc += (k == 0 ? buf_ld(rM, mo, 8 * NT * t * Tin) * v[t] : v[t]) * v[t];
+        }
+      } else if (k < 2 + NB) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+          const int p = tid + NT * t;
+          if (p < M) acc += buf_ld(rM, mo, 8 * (NT * t * Tin + NA + k - 2)) * v[t];
+        }
+      } else {
+        const int lo = 8 * last[k - 2 - NB] * M;
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+          const int p = tid + NT * t;
+          if (p < M) acc += buf_ld(rV, vo, lo + 8 * NT * t) * buf_ld(rM, mo, 8 * NT * t * Tin) * v[t];
+        }
       }
+      acc = wave_sum(acc);
+      if (lane == 0) part[k * (NT / 64) + wv] = acc;
     }
-    for (int i = 0; i < n_pvt; ++i) {
-      const double *Vl = Jb.V + (long)last[i] * M;
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-      for (int t = 0; t < PER; ++t) {
-        const int p = tid + NT * t;
-        if (p < M) s[2 + NB + i] += Min[(long)p * Tin] * Vl[p] * v[t];
-      }
-    }
-    block_sum(s, nred, red);
-    const double xa = s[0] / s[1], xb = sqrt(s[1]);
+    for (int w = 0; w < NT / 64; ++w) { s0 += part[w]; s1 += part[NT / 64 + w]; }
+    const double xa = s0 / s1, xb = sqrt(s1);
     double *row = Jb.Mout + (long)j * Tout;
     if (tid == 0) {
       row[0] = xa;
       row[1] = xb;
-      for (int l = 0; l < NB; ++l) row[NAo + l] = s[2 + l] / xb;
-      for (int i = 0; i < n_pvt; ++i) row[2 + i] = s[2 + NB + i] / xb;
+      for (int k = 2; k < nred; ++k) {
+        double sk = 0.0;
+        for (int w = 0; w < NT / 64; ++w) sk += part[k * (NT / 64) + w];
+        if (k < 2 + NB) row[NAo + k - 2] = sk / xb;
+        else row[k - NB] = sk / xb;
+      }
       if (fresh) row[1] = row[1 + n_pvt];
     }
     if (n_pvt + 2 + NB > Tout) { status = -200; break; }
-    double *Vj = Jb.V + (long)j * M;
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int p = tid + NT * t;
       v[t] /= xb;
-      if (p < M) Vj[p] = v[t];
+      if (p < M) buf_st(v[t], rV, vo, 8 * (j * M + NT * t));
     }
-    __syncthreads();
   }
   if (tid == 0) {
     if (status == 0) {
@@ -517,10 +574,26 @@ __global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, int njobs) 
   }
 }
 
-template __global__ void k_lanczos<256, 8>(const LancJob *, int);
-template __global__ void k_lanczos<256, 24>(const LancJob *, int);
-template __global__ void k_lanczos<512, 24>(const LancJob *, int);
-template __global__ void k_lanczos<1024, 24>(const LancJob *, int);
+// One block of NT threads per job, jobs taken in the given order (longest
+// first); the register tile per thread is 8 or 24 elements, whichever holds
+// the job's M (the host keeps M <= 24*NT).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, const int *order, int njobs) {
+  __shared__ double red[2 * (NT / 64) * kLancRed];
+  __shared__ double u[kMaxNB];
+  __shared__ double part[kLancRed * (NT / 64)];
+  __shared__ int last[kMaxPiv + 1];
+  if ((int)blockIdx.x >= njobs) return;
+  const LancJob Jb = jobs[order ? order[blockIdx.x] : blockIdx.x];
+  if (Jb.nLanc <= 0 || Jb.M > kLancPer * NT) return;
+  if (Jb.M <= 8 * NT) lanczos_job<NT, 8>(Jb, red, u, last, part);
+  else lanczos_job<NT, kLancPer>(Jb, red, u, last, part);
+}
+
+template __global__ void k_lanczos<192>(const LancJob *, const int *, int);
+template __global__ void k_lanczos<384>(const LancJob *, const int *, int);
+template __global__ void k_lanczos<768>(const LancJob *, const int *, int);
+template __global__ void k_lanczos<1024>(const LancJob *, const int *, int);
 
 // ------------------------------------------------------------- Dyson
 

@@ -30,8 +30,16 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
                                double *piv, int lanc_kind, int lanc_n, const long *row_fw_base,
                                const long *row_bk_base, double *rows_fw, double *rows_bk, double *lrows,
                                double *lws, LancJob *jobs);
-template <int NT, int PER>
-__global__ void k_lanczos(const LancJob *jobs, int njobs);
+constexpr int kLancPer = 24;      // largest register tile (elements per thread) of k_lanczos
+template <int NT>
+__global__ void k_lanczos(const LancJob *jobs, const int *order, int njobs);
+// Threads per block that k_lanczos needs for a side of M poles (0: too many).
+inline int lanczos_threads(int M) {
+  const int nts[4] = {192, 384, 768, 1024};
+  for (int nt : nts)
+    if (M <= kLancPer * nt) return nt;
+  return 0;
+}
 __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, const ChanState *cs,
                              PropView P, int src, const double *h, const double *u1,
                              const long *row_fw_base, const long *row_bk_base, const double *rows_fw,
