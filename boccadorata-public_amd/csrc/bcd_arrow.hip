@@ -32,6 +32,14 @@ struct DevBlock {
   __device__ void sync() const { __syncthreads(); }
   __device__ void fail() const { atomicAdd(status, 1); }
   __device__ void add_work(double f) const { if (work) atomicAdd(work, f); }
+  // 1/x: the hardware estimate refined by two Newton steps (each doubles the
+  // correct bits), within an ulp for normal x
+  __device__ double rcp(double x) const {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+  }
   __device__ double sum(double *red, double v) const {
     v = shfl_sum(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -56,17 +64,19 @@ struct DevBlock {
 
 // One block per channel; the workspace sits in LDS (IN_LDS) or in the job's
 // global slab.  Blocks of inactive channels (n <= 0) leave at once.
-template <bool IN_LDS>
+template <bool IN_LDS, int NIMAX>
 __global__ __launch_bounds__(kArrowMaxThreads) void k_arrow(const ArrowJob *jobs, int njobs, int *status) {
   extern __shared__ double sm[];
   if ((int)blockIdx.x >= njobs) return;
   const ArrowJob J = jobs[blockIdx.x];
-  if (J.n <= 0) return;
-  arrow_body(DevBlock{status, nullptr}, J, IN_LDS ? sm : J.ws);
+  if (J.n <= 0 || J.nI > NIMAX) return;
+  arrow_body<NIMAX>(DevBlock{status, nullptr}, J, IN_LDS ? sm : J.ws);
 }
 
-template __global__ void k_arrow<true>(const ArrowJob *, int, int *);
-template __global__ void k_arrow<false>(const ArrowJob *, int, int *);
+#define BCD_ARROW_INST(L, N)                                                            \
+  template __global__ void k_arrow<L, N>(const ArrowJob *, int, int *);                 \
+  template __global__ void k_arrow_sweep<L, N>(ArrowSweep);
+
 
 // Jobs that diagonalise both reduced sides of every channel: job 2*gc + side.
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
@@ -143,16 +153,59 @@ __device__ ArrowJob arrow_job(const ArrowSweep &a, int gc) {
 
 // One Dyson sweep of every channel: one block per channel builds its job and
 // solves it.
-template <bool IN_LDS>
+template <bool IN_LDS, int NIMAX>
 __global__ __launch_bounds__(kArrowMaxThreads) void k_arrow_sweep(ArrowSweep a) {
   extern __shared__ double sm[];
   if ((int)blockIdx.x >= a.n_gc) return;
   const ArrowJob J = arrow_job(a, blockIdx.x);
-  if (J.n <= 0) return;
-  arrow_body(DevBlock{a.status, a.work}, J, IN_LDS ? sm : J.ws);
+  if (J.n <= 0 || J.nI > NIMAX) return;
+  arrow_body<NIMAX>(DevBlock{a.status, a.work}, J, IN_LDS ? sm : J.ws);
 }
 
-template __global__ void k_arrow_sweep<true>(ArrowSweep);
-template __global__ void k_arrow_sweep<false>(ArrowSweep);
+BCD_ARROW_INST(true, 4)
+BCD_ARROW_INST(true, 8)
+BCD_ARROW_INST(true, 16)
+BCD_ARROW_INST(false, 4)
+BCD_ARROW_INST(false, 8)
+BCD_ARROW_INST(false, 16)
+
+void launch_arrow_sweep(const ArrowSweep &a, int nimax, bool in_lds, int nt, size_t lds, hipStream_t st) {
+  const int nb = arrow_ni_bound(nimax);
+  const dim3 g(a.n_gc), b(nt);
+  if (in_lds) {
+    if (nb == 4) hipLaunchKernelGGL((k_arrow_sweep<true, 4>), g, b, lds, st, a);
+    else if (nb == 8) hipLaunchKernelGGL((k_arrow_sweep<true, 8>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_arrow_sweep<true, 16>), g, b, lds, st, a);
+  } else {
+    if (nb == 4) hipLaunchKernelGGL((k_arrow_sweep<false, 4>), g, b, 0, st, a);
+    else if (nb == 8) hipLaunchKernelGGL((k_arrow_sweep<false, 8>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_arrow_sweep<false, 16>), g, b, 0, st, a);
+  }
+}
+
+void launch_arrow(const ArrowJob *jobs, int njobs, int *status, int nimax, bool in_lds, int nt, size_t lds,
+                  hipStream_t st) {
+  const int nb = arrow_ni_bound(nimax);
+  const dim3 g(njobs), b(nt);
+  if (in_lds) {
+    if (nb == 4) hipLaunchKernelGGL((k_arrow<true, 4>), g, b, lds, st, jobs, njobs, status);
+    else if (nb == 8) hipLaunchKernelGGL((k_arrow<true, 8>), g, b, lds, st, jobs, njobs, status);
+    else hipLaunchKernelGGL((k_arrow<true, 16>), g, b, lds, st, jobs, njobs, status);
+  } else {
+    if (nb == 4) hipLaunchKernelGGL((k_arrow<false, 4>), g, b, 0, st, jobs, njobs, status);
+    else if (nb == 8) hipLaunchKernelGGL((k_arrow<false, 8>), g, b, 0, st, jobs, njobs, status);
+    else hipLaunchKernelGGL((k_arrow<false, 16>), g, b, 0, st, jobs, njobs, status);
+  }
+}
+
+// Allow the large dynamic LDS on every in-LDS arrowhead variant.
+int arrow_set_lds_limit(int bytes) {
+  const void *f[6] = {(const void *)k_arrow_sweep<true, 4>, (const void *)k_arrow_sweep<true, 8>,
+                      (const void *)k_arrow_sweep<true, 16>, (const void *)k_arrow<true, 4>,
+                      (const void *)k_arrow<true, 8>, (const void *)k_arrow<true, 16>};
+  for (const void *p : f)
+    if (hipFuncSetAttribute(p, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
+  return 0;
+}
 
 }  // namespace bcd

@@ -42,6 +42,9 @@
 namespace bcd {
 
 constexpr int kArrowNI = 16;       // largest nI (orbits per subshell)
+
+// The smallest accumulator bound of the compiled ones (4, 8, 16) that holds nI.
+__host__ __device__ inline int arrow_ni_bound(int nI) { return nI <= 4 ? 4 : (nI <= 8 ? 8 : 16); }
 constexpr int kArrowMaxIt = 100;   // secular-equation iterations per root
 constexpr int kArrowMaxThreads = 256;
 constexpr int kArrowRed = 2 * kArrowMaxThreads + 16;
@@ -79,48 +82,63 @@ __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
 // linear term (phi) by the far pole, each weight matching its part's slope at
 // the current point (the "middle way" of R.-C. Li, LAPACK Working Note 89,
 // 1993); the extreme roots have no far pole and keep the linear term exact.
-// A bracket on tau falls back to bisection when a step leaves it.  Stops when
+// For an interior root the first evaluation, at the middle of the interval,
+// both picks the origin (the nearer end) and supplies the first step.  A
+// bracket on tau falls back to bisection when a step leaves it.  Stops when
 // |f| is at the rounding level of its terms or the step is below an ulp of
-// tau.  Returns the iterations used (kArrowMaxIt if it did not settle).
-__host__ __device__ inline int arrow_root(int j, int p, const double *d, const double *z, double alpha,
-                                          double znorm, int *org_out, double *tau_out) {
+// tau.  B.rcp(x) is 1/x to within an ulp.  Returns the iterations used
+// (kArrowMaxIt if it did not settle).
+template <class Blk>
+__host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const double *d, const double *z,
+                                          double alpha, double znorm, int *org_out, double *tau_out) {
   const double eps = 2.2204460492503131e-16;
   int o, far;
-  double lo, hi;
-  if (j == 0) {
-    o = 0; far = -1;
-    lo = ((fmin(alpha, d[0]) - d[0]) - znorm) * (1.0 + 1e-7); hi = 0.0;
-  } else if (j == p) {
-    o = p - 1; far = -1;
-    lo = 0.0; hi = ((fmax(alpha, d[p - 1]) - d[p - 1]) + znorm) * (1.0 + 1e-7);
+  double lo, hi, tau;
+  // poles i on the origin's side: i <= j-1 or i >= j for an interior root
+  // (whichever end the origin is), all of them for an extreme root
+  int s_lo = 0, s_hi = p - 1;
+  double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, mag = 0.0, f = 0.0;
+  if (j == 0 || j == p) {
+    o = j == 0 ? 0 : p - 1; far = -1;
+    if (j == 0) { lo = ((fmin(alpha, d[0]) - d[0]) - znorm) * (1.0 + 1e-7); hi = 0.0; }
+    else { lo = 0.0; hi = ((fmax(alpha, d[p - 1]) - d[p - 1]) + znorm) * (1.0 + 1e-7); }
+    tau = 0.5 * (lo + hi);
   } else {
-    // which half of (d[j-1], d[j]) holds the root: f increases on the interval
     const double a = d[j - 1], b = d[j], mid = 0.5 * (a + b);
-    double f = mid - alpha;
-    for (int i = 0; i < p; ++i) f += z[i] * z[i] / (d[i] - mid);
-    if (f >= 0.0) { o = j - 1; far = j; lo = 0.0; hi = mid - a; }
-    else { o = j; far = j - 1; lo = mid - b; hi = 0.0; }
-  }
-  // poles i on the origin's side: i <= o when the origin is the left end,
-  // i >= o when it is the right end, all of them for an extreme root
-  const int s_lo = far < 0 ? 0 : (o < far ? 0 : o), s_hi = far < 0 ? p - 1 : (o < far ? o : p - 1);
-  const double dofs = d[o];
-  double tau = 0.5 * (lo + hi);
-  int it = 0;
-  for (; it < kArrowMaxIt; ++it) {
-    double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, mag = 0.0;
+    double sl = 0.0, dsl = 0.0, sr = 0.0, dsr = 0.0;
     for (int i = 0; i < p; ++i) {
-      const double r = 1.0 / ((d[i] - dofs) - tau);
-      const double t = z[i] * z[i] * r;
-      if (i >= s_lo && i <= s_hi) { psi += t; dpsi += t * r; }
-      else { phi += t; dphi += t * r; }
+      const double r = B.rcp(d[i] - mid), t = z[i] * z[i] * r;
+      if (i < j) { sl += t; dsl += t * r; } else { sr += t; dsr += t * r; }
       mag += fabs(t);
     }
-    const double f = (dofs - alpha) + tau + psi + phi;
-    mag += fabs(dofs - alpha) + fabs(tau);
-    if (fabs(f) <= 8.0 * eps * mag) break;
-    if (f < 0.0) lo = tau; else hi = tau;
-    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi))) break;
+    f = (mid - alpha) + sl + sr;
+    if (f >= 0.0) {
+      o = j - 1; far = j; lo = 0.0; hi = mid - a; tau = hi;
+      s_lo = 0; s_hi = j - 1; psi = sl; dpsi = dsl; phi = sr; dphi = dsr;
+    } else {
+      o = j; far = j - 1; lo = mid - b; hi = 0.0; tau = lo;
+      s_lo = j; s_hi = p - 1; psi = sr; dpsi = dsr; phi = sl; dphi = dsl;
+    }
+    mag += fabs(d[o] - alpha) + fabs(tau);
+  }
+  const double dofs = d[o];
+  int it = 0;
+  for (; it < kArrowMaxIt; ++it) {
+    if (it > 0 || far < 0) {
+      psi = dpsi = phi = dphi = mag = 0.0;
+      for (int i = 0; i < p; ++i) {
+        const double r = B.rcp((d[i] - dofs) - tau);
+        const double t = z[i] * z[i] * r;
+        if (i >= s_lo && i <= s_hi) { psi += t; dpsi += t * r; }
+        else { phi += t; dphi += t * r; }
+        mag += fabs(t);
+      }
+      f = (dofs - alpha) + tau + psi + phi;
+      mag += fabs(dofs - alpha) + fabs(tau);
+      if (fabs(f) <= 8.0 * eps * mag) break;
+      if (f < 0.0) lo = tau; else hi = tau;
+      if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi))) break;
+    }
     const double so = dpsi * tau * tau;
     double tn;
     if (far < 0) {
@@ -178,8 +196,9 @@ __host__ __device__ inline double arrow_block_max(const Blk &B, double *red, dou
 
 // The whole channel: poles, nI arrowhead updates, split at the Fermi energy,
 // descending sort per side and the Calc_En partial sums (DysonMatrix.cpp:
-// 436-484, 542-594).  ws holds arrow_ws_bytes(J.n, J.nI) bytes.
-template <class Blk>
+// 436-484, 542-594).  ws holds arrow_ws_bytes(J.n, J.nI) bytes; NIMAX >= J.nI
+// bounds the per-thread accumulators.
+template <int NIMAX, class Blk>
 __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws) {
   const int nI = J.nI, N = J.n, tid = B.tid(), nt = B.nt();
   const int m = J.np[0] + J.np[1];
@@ -192,8 +211,8 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
 
   if (tid == 0) cnt[2] = 0;
   // algorithmic FLOPs of this thread's share: 9 per pole per secular-equation
-  // evaluation (4 for the bracketing midpoint), 4 per Loewner factor, and
-  // 5 + 2 per carried row per pole for each eigenvector
+  // evaluation, 4 per Loewner factor, and 5 + 2 per carried row per pole for
+  // each eigenvector
   double flops = 0.0;
   // poles (unsorted) into the B buffers
   for (int i = tid; i < m; i += nt) {
@@ -262,9 +281,9 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     // roots
     for (int j = tid; j <= p; j += nt) {
       if (p == 0) { org[j] = -1; tau[j] = 0.0; continue; }
-      const int its = arrow_root(j, p, dn, zn, alpha, znorm, &org[j], &tau[j]);
+      const int its = arrow_root(B, j, p, dn, zn, alpha, znorm, &org[j], &tau[j]);
       if (its >= kArrowMaxIt) cnt[2] = 1;
-      flops += 9.0 * p * (its + 1) + ((j > 0 && j < p) ? 4.0 * p : 0.0);
+      flops += 9.0 * p * (its + 1);
     }
     B.sync();
     // couplings recomputed from the roots: zh_i^2 = -prod_j (d_i - x_j) /
@@ -272,17 +291,17 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     for (int i = tid; i < p; i += nt) {
       const double di = dn[i];
       double v = -((di - dn[org[i]]) - tau[i]) * ((di - dn[org[i + 1]]) - tau[i + 1]);
-      for (int j = 0; j < i; ++j) v *= ((di - dn[org[j]]) - tau[j]) / (di - dn[j]);
-      for (int j = i + 2; j <= p; ++j) v *= ((di - dn[org[j]]) - tau[j]) / (di - dn[j - 1]);
+      for (int j = 0; j < i; ++j) v *= ((di - dn[org[j]]) - tau[j]) * B.rcp(di - dn[j]);
+      for (int j = i + 2; j <= p; ++j) v *= ((di - dn[org[j]]) - tau[j]) * B.rcp(di - dn[j - 1]);
       zh[i] = copysign(sqrt(fmax(v, 0.0)), zn[i]);
       flops += 4.0 * (p + 1);
     }
     B.sync();
     // eigenvectors: roots to positions [0, p], deflated directions to [p+1, p+q]
     for (int j = tid; j <= p; j += nt) {
-      double ah[kArrowNI], ag[kArrowNI];
+      double ah[NIMAX], ag[NIMAX];
 #pragma unroll
-      for (int r = 0; r < kArrowNI; ++r) {
+      for (int r = 0; r < NIMAX; ++r) {
         ah[r] = (r == k) ? 1.0 : 0.0;
         ag[r] = (r > k && r < nI) ? J.u1[r * nI + k] + J.h[r * nI + k] : 0.0;
       }
@@ -292,11 +311,11 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
         const double t = tau[j], dofs = dn[o];
         lam = dofs + t;
         for (int i = 0; i < p; ++i) {
-          const double w = zh[i] / (t - (dn[i] - dofs));
+          const double w = zh[i] * B.rcp(t - (dn[i] - dofs));
           nu2 += w * w;
           const long c = nd[i];
 #pragma unroll
-          for (int r = 0; r < kArrowNI; ++r) {
+          for (int r = 0; r < NIMAX; ++r) {
             if (r < nI) ah[r] += w * HA[r * LN + c];
             if (r > k && r < nI) ag[r] += w * GA[r * LN + c];
           }
@@ -306,7 +325,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       flops += (5.0 + 2.0 * (2 * nI - k - 1)) * p;
       dB[j] = lam;
 #pragma unroll
-      for (int r = 0; r < kArrowNI; ++r) {
+      for (int r = 0; r < NIMAX; ++r) {
         if (r < nI) HB[r * LN + j] = ah[r] * inv;
         if (r > k && r < nI) GB[r * LN + j] = ag[r] * inv;
       }

@@ -264,8 +264,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-  HTRY(hipFuncSetAttribute((const void *)k_arrow_sweep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                           kLdsBytes));
+  if (arrow_set_lds_limit(kLdsBytes)) return BCD_ERR_DEVICE;
   h_is.assign(n_inst, InstState());
   return BCD_OK;
 }
@@ -323,10 +322,7 @@ int Batch::sweep(int src) {
     const int nmax = src == 0 ? nImax : (int)capmax;
     const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
     kt_begin(KT_ARROW);
-    if (!aws_len)
-      hipLaunchKernelGGL(k_arrow_sweep<true>, dim3(n_gc), dim3(nt), arrow_ws_bytes(nmax, nImax), stream, a);
-    else
-      hipLaunchKernelGGL(k_arrow_sweep<false>, dim3(n_gc), dim3(nt), 0, stream, a);
+    launch_arrow_sweep(a, nImax, !aws_len, nt, aws_len ? 0 : arrow_ws_bytes(nmax, nImax), stream);
     kt_end(KT_ARROW);
   } else {
     hipLaunchKernelGGL(k_dyson_prep, dim3(gb), dim3(128), 0, stream, d_chans, 0, n_gc, d_cs, P, src, d_h, d_u1,

@@ -434,7 +434,7 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-  HTRY(hipFuncSetAttribute((const void *)k_arrow<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  if (arrow_set_lds_limit(kLdsBytes)) return BCD_ERR_DEVICE;
   if (njob) {
     HTRY(hipMemcpy(dj.p, P, sizeof(DysonJob) * njob, hipMemcpyHostToDevice));
     const DysonJob *jp = (const DysonJob *)dj.p;
@@ -448,10 +448,7 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   }
   HTRY(hipMemcpyAsync(dja.p, &A, sizeof A, hipMemcpyHostToDevice, ctx->stream));
   const int nt = n <= 64 ? 64 : (n <= 128 ? 128 : kArrowMaxThreads);
-  if (a_lds)
-    hipLaunchKernelGGL(k_arrow<true>, dim3(1), dim3(nt), aws, ctx->stream, (const ArrowJob *)dja.p, 1, (int *)dst.p);
-  else
-    hipLaunchKernelGGL(k_arrow<false>, dim3(1), dim3(nt), 0, ctx->stream, (const ArrowJob *)dja.p, 1, (int *)dst.p);
+  launch_arrow((const ArrowJob *)dja.p, 1, (int *)dst.p, n_orb, a_lds, nt, a_lds ? aws : 0, ctx->stream);
   HTRY(hipGetLastError());
   HTRY(hipStreamSynchronize(ctx->stream));
   int st = 0, cnt[2] = {0, 0};

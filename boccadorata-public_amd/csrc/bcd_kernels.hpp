@@ -48,7 +48,7 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
 template <bool IN_LDS>
 __global__ void k_tridiag(const DysonJob *jobs, int njobs);
 __global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
-template <bool IN_LDS>
+template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow(const ArrowJob *jobs, int njobs, int *status);
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
                              double *tri, double *mat, double *poles, DysonJob *jobs);
@@ -65,8 +65,14 @@ struct ArrowSweep {
   int *status;
   double *work;                  // accumulates the arrowhead FLOPs (may be null)
 };
-template <bool IN_LDS>
+template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow_sweep(ArrowSweep a);
+// Launch one arrowhead kernel variant: the workspace in LDS or HBM, the
+// accumulator bound from nimax (arrow_ni_bound).
+void launch_arrow_sweep(const ArrowSweep &a, int nimax, bool in_lds, int nt, size_t lds, hipStream_t st);
+void launch_arrow(const ArrowJob *jobs, int njobs, int *status, int nimax, bool in_lds, int nt, size_t lds,
+                  hipStream_t st);
+int arrow_set_lds_limit(int bytes);
 __global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out);
 __global__ void k_finalize(const InstDev *insts, const ChanDev *chans, PropView P, const ChanState *cs,
                            const double *kolt, int conv_kind, int init);

@@ -42,6 +42,7 @@ struct HostBlock {
   void sync() const { bar->wait(); }
   void fail() const { *failed = 1; }
   void add_work(double) const {}
+  double rcp(double x) const { return 1.0 / x; }
   double sum(double *, double v) const { return reduce(v, false); }
   double max(double *, double v) const { return reduce(v, true); }
   double reduce(double v, bool mx) const {
@@ -90,7 +91,7 @@ int main() {
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t)
       th.emplace_back(
-          [&, t] { bcd::arrow_body(HostBlock{t, nt, &bar, &failed, acc.data()}, J, (double *)ws.data()); });
+          [&, t] { bcd::arrow_body<bcd::kArrowNI>(HostBlock{t, nt, &bar, &failed, acc.data()}, J, (double *)ws.data()); });
     for (auto &x : th) x.join();
     int32_t o[3] = {nqp, nqh, failed};
     fwrite(o, 4, 3, stdout);
