@@ -188,6 +188,7 @@ def main():
     print(f"# timed {args.steps} steps in {dt:.3f} s; last stages {stages}", file=sys.stderr, flush=True)
     rl, kernels, work = roofline(batch)
     print(f"# kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
+    print(f"# arrowhead phases (block-ms) {json.dumps(work['arrow_phase_ms'])}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(seeds, args.cpu_seconds)
     solves = converged
     line = {

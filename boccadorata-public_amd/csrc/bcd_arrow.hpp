@@ -210,6 +210,9 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
   const double eps = 2.2204460492503131e-16;
 
   if (tid == 0) cnt[2] = 0;
+  // phase timers (profiling only): 0 load and sort, 7 norms, 1 deflation,
+  // 2 roots, 3 Loewner, 4 eigenvectors, 5 reorder, 6 split and output
+  unsigned long long tk = B.lap(-1, 0);
   // algorithmic FLOPs of this thread's share: 9 per pole per secular-equation
   // evaluation, 4 per Loewner factor, and 5 + 2 per carried row per pole for
   // each eigenvector
@@ -232,6 +235,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
   B.sync();
 
   int g = m;
+  tk = B.lap(0, tk);
   for (int k = 0; k < nI; ++k) {
     const double alpha = J.u1[k * nI + k] + J.h[k * nI + k];
     double *z = GA + k * LN;
@@ -240,6 +244,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     const double znorm = sqrt(arrow_block_sum(B, red, s2));
     const double dmax = arrow_block_max(B, red, dmx);
     const double tol = 8.0 * eps * fmax(fmax(fabs(alpha), dmax), znorm);
+    tk = B.lap(7, tk);
     // deflation scan in energy order (one thread)
     if (tid == 0) {
       int p = 0, q = 0, prev = -1;
@@ -278,6 +283,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     const int p = cnt[0], q = cnt[1];
     for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i]]; }
     B.sync();
+    tk = B.lap(1, tk);
     // roots
     for (int j = tid; j <= p; j += nt) {
       if (p == 0) { org[j] = -1; tau[j] = 0.0; continue; }
@@ -286,6 +292,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       flops += 9.0 * p * (its + 1);
     }
     B.sync();
+    tk = B.lap(2, tk);
     // couplings recomputed from the roots: zh_i^2 = -prod_j (d_i - x_j) /
     // prod_{l != i} (d_i - d_l), paired along the interlacing x_i < d_i < x_{i+1}
     for (int i = tid; i < p; i += nt) {
@@ -297,6 +304,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       flops += 4.0 * (p + 1);
     }
     B.sync();
+    tk = B.lap(3, tk);
     // eigenvectors: roots to positions [0, p], deflated directions to [p+1, p+q]
     for (int j = tid; j <= p; j += nt) {
       double ah[NIMAX], ag[NIMAX];
@@ -337,6 +345,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       for (int r = k + 1; r < nI; ++r) GB[r * LN + pos] = GA[r * LN + c];
     }
     B.sync();
+    tk = B.lap(4, tk);
     g += 1;
     for (int i = tid; i < g; i += nt) {
       const double v = dB[i];
@@ -347,8 +356,10 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       for (int r = k + 1; r < nI; ++r) GA[r * LN + rk] = GB[r * LN + i];
     }
     B.sync();
+    tk = B.lap(5, tk);
   }
 
+  tk = B.lap(6, tk);
   // dA ascending, amplitudes in HA: split at E_F, both sides descending
   const double ef = J.efermi;
   int c_le = 0;
@@ -386,6 +397,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     if (cnt[2]) B.fail();
     B.add_work(flops);
   }
+  B.lap(6, tk);
 }
 
 }  // namespace bcd

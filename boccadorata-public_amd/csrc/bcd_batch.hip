@@ -191,7 +191,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   // work doubles: u1, h, W, rho, tri, piv, lrows, kolt ; ints: status, work_count
   l_u1 = 0; l_h = l_u1 + pad8(o_u1); l_w = l_h + pad8(o_h); l_rho = l_w + pad8(o_w);
   l_tri = l_rho + pad8(rho_tot); l_piv = l_tri + pad8(o_tri); l_lr = l_piv + pad8(o_piv);
-  l_kolt = l_lr + pad8(o_lr); l_pole = l_kolt + pad8(3L * n_gc); l_end = l_pole + pad8(o_pole) + 16;
+  l_kolt = l_lr + pad8(o_lr); l_pole = l_kolt + pad8(3L * n_gc); l_end = l_pole + pad8(o_pole) + 32;
   BTRY(a_work.grow(sizeof(double) * l_end));
   double *W0 = (double *)a_work.ptr;
   d_u1 = W0 + l_u1; d_h = W0 + l_h; d_w = W0 + l_w; d_rho = W0 + l_rho; d_tri = W0 + l_tri;
@@ -199,6 +199,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_status = (int *)(W0 + l_end - 16);
   d_work_count = d_status + 2;
   d_work = W0 + l_end - 8;
+  d_prof = (unsigned long long *)(W0 + l_end - 32);
   HTRY(hipMemset(a_work.ptr, 0, sizeof(double) * l_end));
   HTRY(hipMemcpy(d_u1, u1.data(), sizeof(double) * o_u1, hipMemcpyHostToDevice));
 
@@ -319,6 +320,7 @@ int Batch::sweep(int src) {
     a.row_fw_base = d_row_fw_base; a.row_bk_base = d_row_bk_base; a.rows_fw = d_rows_fw; a.rows_bk = d_rows_bk;
     a.poles = d_poles; a.aws = aws_len ? (double *)a_aws.ptr : nullptr; a.kolt = d_kolt; a.status = d_status;
     a.work = d_work;
+    a.prof = profile ? d_prof : nullptr;
     const int nmax = src == 0 ? nImax : (int)capmax;
     const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
     kt_begin(KT_ARROW);
@@ -543,8 +545,15 @@ int Batch::decompose_poles() {
 }
 
 // Algorithmic work of the last solve (see bcd_batch_work in bcdor_amd.h).
-int Batch::work(double *w) {
-  for (int k = 0; k < 7; ++k) w[k] = 0.0;
+int Batch::work(double *w, int n) {
+  for (int k = 0; k < n; ++k) w[k] = 0.0;
+  if (n >= 15) {
+    unsigned long long t[8];
+    int khz = 0;
+    HTRY(hipMemcpy(t, d_prof, sizeof t, hipMemcpyDeviceToHost));
+    HTRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    for (int k = 0; k < 8; ++k) w[7 + k] = khz > 0 ? (double)t[k] / (double)khz : 0.0;
+  }
   w[0] = coupling_flops;
   if (d_work) HTRY(hipMemcpy(&w[5], d_work, sizeof(double), hipMemcpyDeviceToHost));
   for (int gc = 0; gc < (int)h_cs.size() && par.lanczos && par.dyson_solver == 0; ++gc) {
@@ -593,6 +602,7 @@ int Batch::solve() {
   sc_ok.assign(n_inst, par.mode == 0 ? 1 : 0);
   HTRY(hipMemsetAsync(d_status, 0, sizeof(int), stream));
   HTRY(hipMemsetAsync(d_work, 0, sizeof(double), stream));
+  HTRY(hipMemsetAsync(d_prof, 0, 8 * sizeof(unsigned long long), stream));
   HTRY(hipEventRecord(ev[0], stream));
   // start from the initial propagator in buffer 0
   HTRY(hipMemcpyAsync(pe_[0], pe_[2], sizeof(double) * (prop_e_len + prop_x_len), hipMemcpyDeviceToDevice, stream));

@@ -59,6 +59,7 @@ struct Batch {
   int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_list = nullptr, *d_work_count = nullptr;
   int *d_status = nullptr;
   double *d_work = nullptr;                          // arrowhead FLOP counter of the current solve
+  unsigned long long *d_prof = nullptr;              // arrowhead phase timers (profiling)
   double *d_u1 = nullptr, *d_h = nullptr, *d_w = nullptr, *d_rho = nullptr, *d_tri = nullptr, *d_piv = nullptr;
   double *d_lrows = nullptr, *d_kolt = nullptr, *d_rows_fw = nullptr, *d_rows_bk = nullptr, *d_poles = nullptr;
   long l_u1 = 0, l_h = 0, l_w = 0, l_rho = 0, l_tri = 0, l_piv = 0, l_lr = 0, l_kolt = 0, l_pole = 0, l_end = 0;
@@ -93,7 +94,7 @@ struct Batch {
   int stage(int src, int itr_max, double conv, bool hf);
   int build_sigma2(int only_ish);
   int prepare();
-  int work(double *w);
+  int work(double *w, int n);
   double coupling_flops = -1.0;
   int lanczos();
   int decompose_poles();

@@ -614,7 +614,7 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches) 
 
 int bcd_batch_work(bcd_batch *b, double *work, int n) {
   if (!b || !work || n < 7) return BCD_ERR_ARG;
-  return b->b.work(work);
+  return b->b.work(work, n >= 15 ? 15 : 7);
 }
 
 }  // extern "C"

@@ -184,7 +184,9 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
  * per eigensolve), [3] number of Dyson eigensolves, [4] sum of Dyson n^3,
  * [5] arrowhead-solver FLOPs as counted by the kernels, [6] FLOPs of the
  * once-per-solve diagonalisation of the reduced sides (9 L^3 per side).
- * n >= 7. */
+ * n >= 7.  With n >= 15 and profiling on, [7..14] are the arrowhead kernel's
+ * block time (ms summed over blocks) per phase: load and sort, deflation,
+ * roots, Loewner couplings, eigenvectors, reorder, split and output, norms. */
 int bcd_batch_work(bcd_batch *b, double *work, int n);
 
 #ifdef __cplusplus

@@ -43,6 +43,7 @@ struct HostBlock {
   void fail() const { *failed = 1; }
   void add_work(double) const {}
   double rcp(double x) const { return 1.0 / x; }
+  unsigned long long lap(int, unsigned long long) const { return 0; }
   double sum(double *, double v) const { return reduce(v, false); }
   double max(double *, double v) const { return reduce(v, true); }
   double reduce(double v, bool mx) const {
