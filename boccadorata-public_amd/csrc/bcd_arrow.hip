@@ -49,6 +49,23 @@ struct DevBlock {
     r = fma(r, fma(-x, r, 1.0), r);
     return r;
   }
+  // exclusive prefix sum of v over the block (thread order); total in tot
+  __device__ int scan(int *tmp, int v, int &tot) const {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    int off = 0, t = 0;
+    for (int i = 0; i < nw; ++i) { const int c = tmp[i]; off += i < w ? c : 0; t += c; }
+    __syncthreads();
+    tot = t;
+    return off + x - v;
+  }
   __device__ double sum(double *red, double v) const {
     v = shfl_sum(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;

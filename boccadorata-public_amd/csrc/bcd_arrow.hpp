@@ -94,9 +94,10 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
   const double eps = 2.2204460492503131e-16;
   int o, far;
   double lo, hi, tau;
-  // poles i on the origin's side: i <= j-1 or i >= j for an interior root
-  // (whichever end the origin is), all of them for an extreme root
-  int s_lo = 0, s_hi = p - 1;
+  // poles on the origin's side: below sp (psi_left) or from sp on; an extreme
+  // root has them all on one side (sp = p)
+  int sp = p;
+  bool psi_left = true;
   double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, mag = 0.0, f = 0.0;
   if (j == 0 || j == p) {
     o = j == 0 ? 0 : p - 1; far = -1;
@@ -112,12 +113,13 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
       mag += fabs(t);
     }
     f = (mid - alpha) + sl + sr;
+    sp = j;
     if (f >= 0.0) {
       o = j - 1; far = j; lo = 0.0; hi = mid - a; tau = hi;
-      s_lo = 0; s_hi = j - 1; psi = sl; dpsi = dsl; phi = sr; dphi = dsr;
+      psi = sl; dpsi = dsl; phi = sr; dphi = dsr;
     } else {
       o = j; far = j - 1; lo = mid - b; hi = 0.0; tau = lo;
-      s_lo = j; s_hi = p - 1; psi = sr; dpsi = dsr; phi = sl; dphi = dsl;
+      psi_left = false; psi = sr; dpsi = dsr; phi = sl; dphi = dsl;
     }
     mag += fabs(d[o] - alpha) + fabs(tau);
   }
@@ -125,14 +127,18 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
   int it = 0;
   for (; it < kArrowMaxIt; ++it) {
     if (it > 0 || far < 0) {
-      psi = dpsi = phi = dphi = mag = 0.0;
-      for (int i = 0; i < p; ++i) {
-        const double r = B.rcp((d[i] - dofs) - tau);
-        const double t = z[i] * z[i] * r;
-        if (i >= s_lo && i <= s_hi) { psi += t; dpsi += t * r; }
-        else { phi += t; dphi += t * r; }
-        mag += fabs(t);
+      double sl = 0.0, dsl = 0.0, sr = 0.0, dsr = 0.0;
+      mag = 0.0;
+      for (int i = 0; i < sp; ++i) {
+        const double r = B.rcp((d[i] - dofs) - tau), t = z[i] * z[i] * r;
+        sl += t; dsl += t * r; mag += fabs(t);
       }
+      for (int i = sp; i < p; ++i) {
+        const double r = B.rcp((d[i] - dofs) - tau), t = z[i] * z[i] * r;
+        sr += t; dsr += t * r; mag += fabs(t);
+      }
+      if (psi_left) { psi = sl; dpsi = dsl; phi = sr; dphi = dsr; }
+      else { psi = sr; dpsi = dsr; phi = sl; dphi = dsl; }
       f = (dofs - alpha) + tau + psi + phi;
       mag += fabs(dofs - alpha) + fabs(tau);
       if (fabs(f) <= 8.0 * eps * mag) break;
@@ -245,41 +251,69 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     const double dmax = arrow_block_max(B, red, dmx);
     const double tol = 8.0 * eps * fmax(fmax(fabs(alpha), dmax), znorm);
     tk = B.lap(7, tk);
-    // deflation scan in energy order (one thread)
-    if (tid == 0) {
-      int p = 0, q = 0, prev = -1;
-      for (int i = 0; i < g; ++i) {
-        if (fabs(z[i]) <= tol) { dl[q++] = i; continue; }
-        if (prev >= 0) {
-          const double za = z[prev], zb = z[i], t = hypot(za, zb), c = za / t, sn = zb / t;
-          if (fabs((dA[i] - dA[prev]) * c * sn) <= tol) {
-            // rotate (prev, i): i carries the whole coupling, prev leaves
-            for (int r = 0; r < nI; ++r) {
-              const double a = HA[r * LN + prev], b = HA[r * LN + i];
-              HA[r * LN + prev] = sn * a - c * b;
-              HA[r * LN + i] = c * a + sn * b;
-            }
-            for (int r = k + 1; r < nI; ++r) {
-              const double a = GA[r * LN + prev], b = GA[r * LN + i];
-              GA[r * LN + prev] = sn * a - c * b;
-              GA[r * LN + i] = c * a + sn * b;
-            }
-            const double da = dA[prev], db = dA[i];
-            dA[prev] = sn * sn * da + c * c * db;
-            dA[i] = c * c * da + sn * sn * db;
-            z[prev] = 0.0;
-            z[i] = t;
-            --p;
-            dl[q++] = prev;
-          }
+    // deflation, in energy order: couplings below tol leave at once (kept
+    // indices compacted to nd, the others to dl, by a prefix scan); then
+    // neighbouring kept directions whose energies differ by less than tol
+    // after moving the whole coupling onto one of them merge.  Merges chain
+    // (a merge changes the next pair's values), so they run on one thread,
+    // but only when some neighbouring pair qualifies on the original values
+    // -- if none does, the chain makes no merge either.
+    {
+      int kept = 0, gone = 0;
+      for (int b0 = 0; b0 < g; b0 += nt) {
+        const int i = b0 + tid;
+        const int keep = (i < g && fabs(z[i]) > tol) ? 1 : 0;
+        int tot = 0;
+        const int pre = B.scan((int *)red, keep, tot);
+        if (i < g) {
+          if (keep) nd[kept + pre] = i;
+          else dl[gone + (tid - pre)] = i;
         }
-        nd[p++] = i;
-        prev = i;
+        kept += tot;
+        gone += (g - b0 < nt ? g - b0 : nt) - tot;
       }
-      cnt[0] = p;
-      cnt[1] = q;
+      if (tid == 0) { cnt[0] = kept; cnt[1] = gone; cnt[3] = 0; }
+      B.sync();
+      for (int a = tid; a + 1 < kept; a += nt) {
+        const int i0 = nd[a], i1 = nd[a + 1];
+        const double za = z[i0], zb = z[i1], t = hypot(za, zb);
+        if (fabs((dA[i1] - dA[i0]) * (za / t) * (zb / t)) <= tol) cnt[3] = 1;
+      }
+      B.sync();
+      if (cnt[3] && tid == 0) {
+        int p = 0, q = gone;
+        for (int a = 0; a < kept; ++a) {
+          const int i = nd[a], prev = p > 0 ? nd[p - 1] : -1;
+          if (prev >= 0) {
+            const double za = z[prev], zb = z[i], t = hypot(za, zb), c = za / t, sn = zb / t;
+            if (fabs((dA[i] - dA[prev]) * c * sn) <= tol) {
+              // rotate (prev, i): i carries the whole coupling, prev leaves
+              for (int r = 0; r < nI; ++r) {
+                const double x = HA[r * LN + prev], y = HA[r * LN + i];
+                HA[r * LN + prev] = sn * x - c * y;
+                HA[r * LN + i] = c * x + sn * y;
+              }
+              for (int r = k + 1; r < nI; ++r) {
+                const double x = GA[r * LN + prev], y = GA[r * LN + i];
+                GA[r * LN + prev] = sn * x - c * y;
+                GA[r * LN + i] = c * x + sn * y;
+              }
+              const double da = dA[prev], db = dA[i];
+              dA[prev] = sn * sn * da + c * c * db;
+              dA[i] = c * c * da + sn * sn * db;
+              z[prev] = 0.0;
+              z[i] = t;
+              --p;
+              dl[q++] = prev;
+            }
+          }
+          nd[p++] = i;
+        }
+        cnt[0] = p;
+        cnt[1] = q;
+      }
+      B.sync();
     }
-    B.sync();
     const int p = cnt[0], q = cnt[1];
     for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i]]; }
     B.sync();

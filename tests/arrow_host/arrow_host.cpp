@@ -44,6 +44,15 @@ struct HostBlock {
   void add_work(double) const {}
   double rcp(double x) const { return 1.0 / x; }
   unsigned long long lap(int, unsigned long long) const { return 0; }
+  int scan(int *, int v, int &tot) const {
+    acc[t] = v;
+    sync();
+    int pre = 0;
+    tot = 0;
+    for (int i = 0; i < n; ++i) { pre += i < t ? (int)acc[i] : 0; tot += (int)acc[i]; }
+    sync();
+    return pre;
+  }
   double sum(double *, double v) const { return reduce(v, false); }
   double max(double *, double v) const { return reduce(v, true); }
   double reduce(double v, bool mx) const {
