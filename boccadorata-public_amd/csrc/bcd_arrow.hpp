@@ -47,7 +47,7 @@ constexpr int kArrowNI = 16;       // largest nI (orbits per subshell)
 __host__ __device__ inline int arrow_ni_bound(int nI) { return nI <= 4 ? 4 : (nI <= 8 ? 8 : 16); }
 constexpr int kArrowMaxIt = 100;   // secular-equation iterations per root
 constexpr int kArrowMaxThreads = 256;
-constexpr int kArrowRed = 2 * kArrowMaxThreads + 16;
+constexpr int kArrowRed = 16;      // reduction scratch (doubles): one slot per wave
 
 // One channel's Dyson problem in arrowhead form.  Poles come from up to two
 // sources: energy i of source s is E[s][i*es[s]], its coupling to orbit r is
@@ -65,7 +65,7 @@ struct ArrowJob {
 };
 
 __host__ __device__ inline size_t arrow_ws_doubles(int n, int nI) {
-  return (size_t)(8 + 4 * nI) * n + kArrowRed;
+  return (size_t)(6 + 4 * nI) * n + kArrowRed;
 }
 
 __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
@@ -73,7 +73,7 @@ __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
 }
 
 // Root j (0..p) of the secular equation with sorted poles d[0..p-1] and
-// couplings z (all non-zero).  The root is returned as an offset tau from the
+// squared couplings zz (all non-zero).  The root is returned as an offset tau from the
 // pole org it is closest to, so that x - d_i = tau - (d_i - d_org) keeps full
 // relative accuracy near the poles.
 //
@@ -89,7 +89,7 @@ __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
 // tau.  B.rcp(x) is 1/x to within an ulp.  Returns the iterations used
 // (kArrowMaxIt if it did not settle).
 template <class Blk>
-__host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const double *d, const double *z,
+__host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const double *d, const double *zz,
                                           double alpha, double znorm, int *org_out, double *tau_out) {
   const double eps = 2.2204460492503131e-16;
   int o, far;
@@ -107,11 +107,16 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
   } else {
     const double a = d[j - 1], b = d[j], mid = 0.5 * (a + b);
     double sl = 0.0, dsl = 0.0, sr = 0.0, dsr = 0.0;
-    for (int i = 0; i < p; ++i) {
-      const double r = B.rcp(d[i] - mid), t = z[i] * z[i] * r;
-      if (i < j) { sl += t; dsl += t * r; } else { sr += t; dsr += t * r; }
-      mag += fabs(t);
+    for (int i = 0; i < j; ++i) {
+      const double r = B.rcp(d[i] - mid), t = zz[i] * r;
+      sl += t; dsl += t * r;
     }
+    for (int i = j; i < p; ++i) {
+      const double r = B.rcp(d[i] - mid), t = zz[i] * r;
+      sr += t; dsr += t * r;
+    }
+    // terms left of the root are all negative, right of it all positive
+    mag = fabs(sl) + fabs(sr);
     f = (mid - alpha) + sl + sr;
     sp = j;
     if (f >= 0.0) {
@@ -128,15 +133,15 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
   for (; it < kArrowMaxIt; ++it) {
     if (it > 0 || far < 0) {
       double sl = 0.0, dsl = 0.0, sr = 0.0, dsr = 0.0;
-      mag = 0.0;
       for (int i = 0; i < sp; ++i) {
-        const double r = B.rcp((d[i] - dofs) - tau), t = z[i] * z[i] * r;
-        sl += t; dsl += t * r; mag += fabs(t);
+        const double r = B.rcp((d[i] - dofs) - tau), t = zz[i] * r;
+        sl += t; dsl += t * r;
       }
       for (int i = sp; i < p; ++i) {
-        const double r = B.rcp((d[i] - dofs) - tau), t = z[i] * z[i] * r;
-        sr += t; dsr += t * r; mag += fabs(t);
+        const double r = B.rcp((d[i] - dofs) - tau), t = zz[i] * r;
+        sr += t; dsr += t * r;
       }
+      mag = fabs(sl) + fabs(sr);
       if (psi_left) { psi = sl; dpsi = dsl; phi = sr; dphi = dsr; }
       else { psi = sr; dpsi = dsr; phi = sl; dphi = dsl; }
       f = (dofs - alpha) + tau + psi + phi;
@@ -315,13 +320,15 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       B.sync();
     }
     const int p = cnt[0], q = cnt[1];
-    for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i]]; }
+    // zh holds the squared couplings for the roots until the Loewner step
+    // overwrites it
+    for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i]]; zh[i] = zn[i] * zn[i]; }
     B.sync();
     tk = B.lap(1, tk);
     // roots
     for (int j = tid; j <= p; j += nt) {
       if (p == 0) { org[j] = -1; tau[j] = 0.0; continue; }
-      const int its = arrow_root(B, j, p, dn, zn, alpha, znorm, &org[j], &tau[j]);
+      const int its = arrow_root(B, j, p, dn, zh, alpha, znorm, &org[j], &tau[j]);
       if (its >= kArrowMaxIt) cnt[2] = 1;
       flops += 9.0 * p * (its + 1);
     }
