@@ -91,7 +91,7 @@ struct DevBlock {
 // One block per channel; the workspace sits in LDS (IN_LDS) or in the job's
 // global slab.  Blocks of inactive channels (n <= 0) leave at once.
 template <bool IN_LDS, int NIMAX>
-__global__ __launch_bounds__(kArrowMaxThreads) void k_arrow(const ArrowJob *jobs, int njobs, int *status) {
+__global__ __launch_bounds__(kArrowMaxThreads, 4) void k_arrow(const ArrowJob *jobs, int njobs, int *status) {
   extern __shared__ double sm[];
   if ((int)blockIdx.x >= njobs) return;
   const ArrowJob J = jobs[blockIdx.x];
@@ -180,7 +180,7 @@ __device__ ArrowJob arrow_job(const ArrowSweep &a, int gc) {
 // One Dyson sweep of every channel: one block per channel builds its job and
 // solves it.
 template <bool IN_LDS, int NIMAX>
-__global__ __launch_bounds__(kArrowMaxThreads) void k_arrow_sweep(ArrowSweep a) {
+__global__ __launch_bounds__(kArrowMaxThreads, 4) void k_arrow_sweep(ArrowSweep a) {
   extern __shared__ double sm[];
   if ((int)blockIdx.x >= a.n_gc) return;
   const ArrowJob J = arrow_job(a, blockIdx.x);

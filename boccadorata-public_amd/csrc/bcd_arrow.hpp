@@ -88,6 +88,29 @@ __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
 // |f| is at the rounding level of its terms or the step is below an ulp of
 // tau.  B.rcp(x) is 1/x to within an ulp.  Returns the iterations used
 // (kArrowMaxIt if it did not settle).
+// Sums of zz_i r_i and zz_i r_i^2 over i in [i0, i1), r_i = 1/((d_i - dofs) -
+// tau), in two interleaved partial sums so that consecutive poles' reciprocal
+// chains overlap.
+template <class Blk>
+__host__ __device__ inline void arrow_sums(const Blk &B, const double *d, const double *zz, int i0, int i1,
+                                           double dofs, double tau, double &s, double &ds) {
+  double s0 = 0.0, s1 = 0.0, q0 = 0.0, q1 = 0.0;
+  int i = i0;
+#pragma unroll 2
+  for (; i + 1 < i1; i += 2) {
+    const double r0 = B.rcp((d[i] - dofs) - tau), r1 = B.rcp((d[i + 1] - dofs) - tau);
+    const double t0 = zz[i] * r0, t1 = zz[i + 1] * r1;
+    s0 += t0; q0 += t0 * r0;
+    s1 += t1; q1 += t1 * r1;
+  }
+  if (i < i1) {
+    const double r0 = B.rcp((d[i] - dofs) - tau), t0 = zz[i] * r0;
+    s0 += t0; q0 += t0 * r0;
+  }
+  s = s0 + s1;
+  ds = q0 + q1;
+}
+
 template <class Blk>
 __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const double *d, const double *zz,
                                           double alpha, double znorm, int *org_out, double *tau_out) {
@@ -106,15 +129,9 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
     tau = 0.5 * (lo + hi);
   } else {
     const double a = d[j - 1], b = d[j], mid = 0.5 * (a + b);
-    double sl = 0.0, dsl = 0.0, sr = 0.0, dsr = 0.0;
-    for (int i = 0; i < j; ++i) {
-      const double r = B.rcp(d[i] - mid), t = zz[i] * r;
-      sl += t; dsl += t * r;
-    }
-    for (int i = j; i < p; ++i) {
-      const double r = B.rcp(d[i] - mid), t = zz[i] * r;
-      sr += t; dsr += t * r;
-    }
+    double sl, dsl, sr, dsr;
+    arrow_sums(B, d, zz, 0, j, mid, 0.0, sl, dsl);
+    arrow_sums(B, d, zz, j, p, mid, 0.0, sr, dsr);
     // terms left of the root are all negative, right of it all positive
     mag = fabs(sl) + fabs(sr);
     f = (mid - alpha) + sl + sr;
@@ -132,15 +149,9 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
   int it = 0;
   for (; it < kArrowMaxIt; ++it) {
     if (it > 0 || far < 0) {
-      double sl = 0.0, dsl = 0.0, sr = 0.0, dsr = 0.0;
-      for (int i = 0; i < sp; ++i) {
-        const double r = B.rcp((d[i] - dofs) - tau), t = zz[i] * r;
-        sl += t; dsl += t * r;
-      }
-      for (int i = sp; i < p; ++i) {
-        const double r = B.rcp((d[i] - dofs) - tau), t = zz[i] * r;
-        sr += t; dsr += t * r;
-      }
+      double sl, dsl, sr, dsr;
+      arrow_sums(B, d, zz, 0, sp, dofs, tau, sl, dsl);
+      arrow_sums(B, d, zz, sp, p, dofs, tau, sr, dsr);
       mag = fabs(sl) + fabs(sr);
       if (psi_left) { psi = sl; dpsi = dsl; phi = sr; dphi = dsr; }
       else { psi = sr; dpsi = dsr; phi = sl; dphi = dsl; }
@@ -239,6 +250,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
   for (int i = tid; i < m; i += nt) {
     const double v = dB[i];
     int rk = 0;
+#pragma unroll 4
     for (int k = 0; k < m; ++k) { const double w = dB[k]; rk += (w < v) || (w == v && k < i); }
     dA[rk] = v;
     for (int r = 0; r < nI; ++r) { GA[r * LN + rk] = GB[r * LN + i]; HA[r * LN + rk] = 0.0; }
@@ -339,8 +351,25 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     for (int i = tid; i < p; i += nt) {
       const double di = dn[i];
       double v = -((di - dn[org[i]]) - tau[i]) * ((di - dn[org[i + 1]]) - tau[i + 1]);
-      for (int j = 0; j < i; ++j) v *= ((di - dn[org[j]]) - tau[j]) * B.rcp(di - dn[j]);
-      for (int j = i + 2; j <= p; ++j) v *= ((di - dn[org[j]]) - tau[j]) * B.rcp(di - dn[j - 1]);
+      // two interleaved partial products (pairs of ratios near 1, no overflow)
+      double v1 = 1.0;
+      {
+        int jj = 0;
+#pragma unroll 2
+        for (; jj + 1 < i; jj += 2) {
+          v *= ((di - dn[org[jj]]) - tau[jj]) * B.rcp(di - dn[jj]);
+          v1 *= ((di - dn[org[jj + 1]]) - tau[jj + 1]) * B.rcp(di - dn[jj + 1]);
+        }
+        if (jj < i) v *= ((di - dn[org[jj]]) - tau[jj]) * B.rcp(di - dn[jj]);
+        jj = i + 2;
+#pragma unroll 2
+        for (; jj + 1 <= p; jj += 2) {
+          v *= ((di - dn[org[jj]]) - tau[jj]) * B.rcp(di - dn[jj - 1]);
+          v1 *= ((di - dn[org[jj + 1]]) - tau[jj + 1]) * B.rcp(di - dn[jj]);
+        }
+        if (jj <= p) v *= ((di - dn[org[jj]]) - tau[jj]) * B.rcp(di - dn[jj - 1]);
+      }
+      v *= v1;
       zh[i] = copysign(sqrt(fmax(v, 0.0)), zn[i]);
       flops += 4.0 * (p + 1);
     }
@@ -359,6 +388,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
         const int o = org[j];
         const double t = tau[j], dofs = dn[o];
         lam = dofs + t;
+#pragma unroll 2
         for (int i = 0; i < p; ++i) {
           const double w = zh[i] * B.rcp(t - (dn[i] - dofs));
           nu2 += w * w;
@@ -391,6 +421,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     for (int i = tid; i < g; i += nt) {
       const double v = dB[i];
       int rk = 0;
+#pragma unroll 4
       for (int l = 0; l < g; ++l) { const double w = dB[l]; rk += (w < v) || (w == v && l < i); }
       dA[rk] = v;
       for (int r = 0; r < nI; ++r) HA[r * LN + rk] = HB[r * LN + i];

@@ -907,15 +907,17 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
   if (!init && !S.active) return;
   const int ns = I.nsubsh, cur = S.cur, nxt = 1 - cur;
   const int useb = init ? cur : nxt;
-  if (tid == 0) {
-    for (int b = 0; b < 2; ++b) {
-      pq[b][0] = 0; ph[b][0] = 0;
-      const int buf = b == 0 ? cur : nxt;
-      for (int s = 0; s < ns; ++s) {
-        pq[b][s + 1] = pq[b][s] + P.nqp[buf][I.gc0 + s];
-        ph[b][s + 1] = ph[b][s] + P.nqh[buf][I.gc0 + s];
-      }
-    }
+  // fragment counts of both buffers into LDS in parallel, prefix sums from LDS
+  for (int t = tid; t < 2 * ns; t += blockDim.x) {
+    const int b = t / ns, s = t % ns, buf = b == 0 ? cur : nxt;
+    pq[b][s + 1] = P.nqp[buf][I.gc0 + s];
+    ph[b][s + 1] = P.nqh[buf][I.gc0 + s];
+  }
+  __syncthreads();
+  if (tid < 2) {
+    int *q = pq[tid], *h = ph[tid];
+    q[0] = 0; h[0] = 0;
+    for (int s = 0; s < ns; ++s) { q[s + 1] += q[s]; h[s + 1] += h[s]; }
   }
   __syncthreads();
   // Fermi energies from buffer useb
@@ -940,11 +942,14 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
       }
       // first stored quasiparticle / quasihole of the propagator as the start
       // values, as Seek_EFermi does
+      // counts of buffer useb are in LDS slot bu (prefix differences)
+      const int bu = useb == cur ? 0 : 1;
       double e0p = 0.0, e0h = 0.0;
-      for (int s = 0; s < ns; ++s) if (P.nqp[useb][I.gc0 + s] > 0) { e0p = P.e[useb][P.e_off[I.gc0 + s]]; break; }
+      for (int s = 0; s < ns; ++s)
+        if (pq[bu][s + 1] > pq[bu][s]) { e0p = P.e[useb][P.e_off[I.gc0 + s]]; break; }
       for (int s = 0; s < ns; ++s) {
         const int gc = I.gc0 + s;
-        if (P.nqh[useb][gc] > 0) { e0h = P.e[useb][P.e_off[gc] + P.nqp[useb][gc]]; break; }
+        if (ph[bu][s + 1] > ph[bu][s]) { e0h = P.e[useb][P.e_off[gc] + (pq[bu][s + 1] - pq[bu][s])]; break; }
       }
       for (int c = 0; c < 2; ++c) {
         const double x1 = fmin(e0p + 100., -red[c]), x2 = fmax(e0h - 100., red[2 + c]);
@@ -982,14 +987,18 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
   const double dsum = wave_sum(acc[0]), dmax = wave_max(acc[1]);
   if ((tid & 63) == 0) { red[(tid >> 6) * 4] = dsum; red[(tid >> 6) * 4 + 1] = dmax; }
   __syncthreads();
-  if (tid == 0) {
-    double sum = 0.0, mxd = 0.0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) { sum += red[i * 4]; mxd = fmax(mxd, red[i * 4 + 1]); }
-    double e1 = 0.0, es = 0.0, as = 0.0;
-    for (int s = 0; s < ns; ++s) {
+  // Koltun sums over the subshells: first wave, one subshell per lane
+  double e1 = 0.0, es = 0.0, as = 0.0;
+  if (tid < 64) {
+    for (int s = tid; s < ns; s += 64) {
       const double *k3 = kolt + 3L * (I.gc0 + s);
       e1 += k3[0]; es += (k3[0] + k3[1]) / 2.0; as += k3[2];
     }
+    e1 = wave_sum(e1); es = wave_sum(es); as = wave_sum(as);
+  }
+  if (tid == 0) {
+    double sum = 0.0, mxd = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) { sum += red[i * 4]; mxd = fmax(mxd, red[i * 4 + 1]); }
     S.u1_tot = e1; S.e_kolt = es; S.a_tot = as;
     const double diff = conv_kind == 1 ? mxd : sum;
     S.diff = diff;
