@@ -27,13 +27,13 @@ __device__ __forceinline__ double shfl_max(double x) {
 struct DevBlock {
   int *status;
   double *work;                  // FLOP counter (may be null)
-  unsigned long long *prof;      // per-phase block time in wall-clock ticks (may be null)
+  unsigned long long *prof;      // this block's 8 phase-time slots in wall-clock ticks (may be null)
   // ends phase k (k < 0: starts timing) begun at t0; thread 0 adds the
-  // elapsed ticks of its block
+  // elapsed ticks to its block's own slot
   __device__ unsigned long long lap(int k, unsigned long long t0) const {
     if (!prof) return 0;
     const unsigned long long t = wall_clock64();
-    if (k >= 0 && threadIdx.x == 0) atomicAdd(&prof[k], t - t0);
+    if (k >= 0 && threadIdx.x == 0) prof[k] += t - t0;
     return t;
   }
   __device__ int tid() const { return threadIdx.x; }
@@ -185,7 +185,8 @@ __global__ __launch_bounds__(kArrowMaxThreads, 4) void k_arrow_sweep(ArrowSweep 
   if ((int)blockIdx.x >= a.n_gc) return;
   const ArrowJob J = arrow_job(a, blockIdx.x);
   if (J.n <= 0 || J.nI > NIMAX) return;
-  arrow_body<NIMAX>(DevBlock{a.status, a.work, a.prof}, J, IN_LDS ? sm : J.ws);
+  arrow_body<NIMAX>(DevBlock{a.status, a.work, a.prof ? a.prof + 8L * blockIdx.x : nullptr}, J,
+                    IN_LDS ? sm : J.ws);
 }
 
 BCD_ARROW_INST(true, 4)

@@ -89,26 +89,28 @@ __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
 // tau.  B.rcp(x) is 1/x to within an ulp.  Returns the iterations used
 // (kArrowMaxIt if it did not settle).
 // Sums of zz_i r_i and zz_i r_i^2 over i in [i0, i1), r_i = 1/((d_i - dofs) -
-// tau), in two interleaved partial sums so that consecutive poles' reciprocal
-// chains overlap.
+// tau), in four interleaved partial sums so that the reciprocal chains of
+// consecutive poles overlap.
 template <class Blk>
 __host__ __device__ inline void arrow_sums(const Blk &B, const double *d, const double *zz, int i0, int i1,
                                            double dofs, double tau, double &s, double &ds) {
-  double s0 = 0.0, s1 = 0.0, q0 = 0.0, q1 = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
   int i = i0;
-#pragma unroll 2
-  for (; i + 1 < i1; i += 2) {
+  for (; i + 3 < i1; i += 4) {
     const double r0 = B.rcp((d[i] - dofs) - tau), r1 = B.rcp((d[i + 1] - dofs) - tau);
-    const double t0 = zz[i] * r0, t1 = zz[i + 1] * r1;
+    const double r2 = B.rcp((d[i + 2] - dofs) - tau), r3 = B.rcp((d[i + 3] - dofs) - tau);
+    const double t0 = zz[i] * r0, t1 = zz[i + 1] * r1, t2 = zz[i + 2] * r2, t3 = zz[i + 3] * r3;
     s0 += t0; q0 += t0 * r0;
     s1 += t1; q1 += t1 * r1;
+    s2 += t2; q2 += t2 * r2;
+    s3 += t3; q3 += t3 * r3;
   }
-  if (i < i1) {
+  for (; i < i1; ++i) {
     const double r0 = B.rcp((d[i] - dofs) - tau), t0 = zz[i] * r0;
     s0 += t0; q0 += t0 * r0;
   }
-  s = s0 + s1;
-  ds = q0 + q1;
+  s = (s0 + s1) + (s2 + s3);
+  ds = (q0 + q1) + (q2 + q3);
 }
 
 template <class Blk>

@@ -39,7 +39,7 @@ Batch::~Batch() {
 
 std::vector<DevArena *> Batch::arenas() {
   return {&a_tab_i, &a_tab_l, &a_tab_d, &a_inst, &a_chan, &a_blocks, &a_work, &a_prop, &a_rows,
-          &a_lws, &a_mat, &a_jobs, &a_aws};
+          &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof};
 }
 
 #define HTRY(x)                                   \
@@ -199,7 +199,8 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_status = (int *)(W0 + l_end - 16);
   d_work_count = d_status + 2;
   d_work = W0 + l_end - 8;
-  d_prof = (unsigned long long *)(W0 + l_end - 32);
+  BTRY(a_prof.grow(sizeof(unsigned long long) * 8 * (size_t)n_gc));
+  d_prof = (unsigned long long *)a_prof.ptr;
   HTRY(hipMemset(a_work.ptr, 0, sizeof(double) * l_end));
   HTRY(hipMemcpy(d_u1, u1.data(), sizeof(double) * o_u1, hipMemcpyHostToDevice));
 
@@ -548,11 +549,15 @@ int Batch::decompose_poles() {
 int Batch::work(double *w, int n) {
   for (int k = 0; k < n; ++k) w[k] = 0.0;
   if (n >= 15) {
-    unsigned long long t[8];
+    std::vector<unsigned long long> t(8 * (size_t)n_gc);
     int khz = 0;
-    HTRY(hipMemcpy(t, d_prof, sizeof t, hipMemcpyDeviceToHost));
+    HTRY(hipMemcpy(t.data(), d_prof, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
     HTRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
-    for (int k = 0; k < 8; ++k) w[7 + k] = khz > 0 ? (double)t[k] / (double)khz : 0.0;
+    for (int k = 0; k < 8; ++k) {
+      double tk = 0.0;
+      for (int gc = 0; gc < n_gc; ++gc) tk += (double)t[8L * gc + k];
+      w[7 + k] = khz > 0 ? tk / (double)khz : 0.0;
+    }
   }
   w[0] = coupling_flops;
   if (d_work) HTRY(hipMemcpy(&w[5], d_work, sizeof(double), hipMemcpyDeviceToHost));
@@ -602,7 +607,7 @@ int Batch::solve() {
   sc_ok.assign(n_inst, par.mode == 0 ? 1 : 0);
   HTRY(hipMemsetAsync(d_status, 0, sizeof(int), stream));
   HTRY(hipMemsetAsync(d_work, 0, sizeof(double), stream));
-  HTRY(hipMemsetAsync(d_prof, 0, 8 * sizeof(unsigned long long), stream));
+  HTRY(hipMemsetAsync(d_prof, 0, 8 * sizeof(unsigned long long) * n_gc, stream));
   HTRY(hipEventRecord(ev[0], stream));
   // start from the initial propagator in buffer 0
   HTRY(hipMemcpyAsync(pe_[0], pe_[2], sizeof(double) * (prop_e_len + prop_x_len), hipMemcpyDeviceToDevice, stream));

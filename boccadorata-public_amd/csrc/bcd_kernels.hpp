@@ -64,7 +64,7 @@ struct ArrowSweep {
   double *aws, *kolt;
   int *status;
   double *work;                  // accumulates the arrowhead FLOPs (may be null)
-  unsigned long long *prof;      // 8 per-phase block-time counters (may be null)
+  unsigned long long *prof;      // 8 phase-time counters per channel (may be null)
 };
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow_sweep(ArrowSweep a);
