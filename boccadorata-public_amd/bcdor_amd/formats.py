@@ -7,6 +7,7 @@ reading them back (the reference reads with %lf) reproduces every bit.
 from __future__ import annotations
 
 import dataclasses
+import os
 import struct
 
 import numpy as np
@@ -204,6 +205,11 @@ def read_pivots(path: str, ms: ModelSpace) -> dict:
 # ---------------------------------------------------------------- bcdwk Sigma
 @dataclasses.dataclass
 class SelfEnergyFile:
+    """One channel's dynamic self-energy as Wide_self_energy_t stores it
+    (SlfEn_classes.hh:14-42): pole rows [E, b or sum m^2, g..., m(0..norb-1)] of
+    width ntot_den + ntot_orb per side, and i_piv = [n_pvt, 1, starts...] for a
+    Lanczos-reduced side (empty for a diagonal side).  The header fields
+    describe the channel (Lambda = 0: one partial wave, the subshell itself)."""
     ish: int
     ntot_den: int
     ntot_orb: int
@@ -211,6 +217,151 @@ class SelfEnergyFile:
     bk: np.ndarray
     i_piv_fw: np.ndarray
     i_piv_bk: np.ndarray
+    j2: int = 0
+    ip: int = 0
+    ch: int = 0
+    alloc_fw: int = -1    # N_PLS_FW_alloc (-1: max(1, row count), as Allocate_Dynamic_SlfEn sets it)
+    alloc_bk: int = -1
+    i_pwv: tuple = ()
+    nst_pwv: tuple = ()
+    norb_pwv: tuple = ()
+    isp_msp: tuple = ()   # model-space orbit of each row column
+    mor_n: tuple = ()     # radial quantum number of each of those orbits
+
+    @property
+    def ntot_dim(self) -> int:
+        return self.ntot_den + self.ntot_orb
+
+
+def selfenergy_for_channel(ms: ModelSpace, ish: int, fw: np.ndarray, bk: np.ndarray, ntot_den: int,
+                           i_piv_fw=None, i_piv_bk=None) -> SelfEnergyFile:
+    """A SelfEnergyFile for subshell ish of ms with the header tables that
+    set_clj(ish) fills for Lambda = 0 (Self_energy_ext_Op_a.cpp:223-282)."""
+    no = int(ms.no[ish])
+    o0 = int(ms.orb0[ish])
+    orbs = tuple(range(o0, o0 + no))
+    empty = np.zeros(0, np.int32)
+    return SelfEnergyFile(ish, int(ntot_den), no, np.asarray(fw, np.float64).reshape(-1, ntot_den + no),
+                          np.asarray(bk, np.float64).reshape(-1, ntot_den + no),
+                          empty if i_piv_fw is None else np.asarray(i_piv_fw, np.int32),
+                          empty if i_piv_bk is None else np.asarray(i_piv_bk, np.int32),
+                          int(ms.j2[ish]), int(ms.ip[ish]), int(ms.ch[ish]), -1, -1, (ish,), (0,), (no,), orbs,
+                          tuple(int(ms.orb_n[k]) for k in orbs))
+
+
+def selfenergy_filename(storedir: str, label: str, j2: int, ip: int, ch: int, fb: str = "", order: str = "Dyn",
+                        lam: int = 0, parity: int = 0, charge: int = 0) -> str:
+    """Wide_self_energy_t::get_filename (Self_energy_ext_Op_d.cpp:43-52):
+    <dir>/<label>_<order>[_T=<lam><+|-><charge>]_J2pi=<2j><+|->_dt=<ch><fb>.bin"""
+    sign = "+-"
+    stem = f"{label}_{order}" if (lam, parity, charge) == (0, 0, 0) else \
+        f"{label}_{order}_T={lam}{sign[parity]}_{charge}"
+    return os.path.join(storedir, f"{stem}_J2pi={j2}{sign[ip]}_dt={ch}{fb}.bin")
+
+
+def _piv_len(ipv) -> int:
+    return int(ipv[0]) if len(ipv) else 0
+
+
+def _header_ints(sef: SelfEnergyFile, nfw: int, nbk: int) -> bytes:
+    # Allocate_Dynamic_SlfEn (Self_energy_ext_Op_a.cpp:186-187) keeps at least one row
+    afw = sef.alloc_fw if sef.alloc_fw >= 0 else max(1, sef.fw.shape[0])
+    abk = sef.alloc_bk if sef.alloc_bk >= 0 else max(1, sef.bk.shape[0])
+    npw = len(sef.i_pwv)
+    h = [nfw, nbk, afw, abk, sef.ish, sef.ntot_den, sef.ntot_orb, sef.ntot_dim, npw, sef.j2, sef.ip, sef.ch]
+    tab = list(sef.i_pwv) + list(sef.nst_pwv) + list(sef.norb_pwv) + list(sef.isp_msp) + list(sef.mor_n)
+    if len(sef.isp_msp) != sef.ntot_orb or len(sef.mor_n) != sef.ntot_orb:
+        raise ValueError("isp_msp and mor_n need one entry per orbit column")
+    return struct.pack(f"<{12 + len(tab)}i", *h, *tab)
+
+
+def write_bcdwk(path: str, sef: SelfEnergyFile, i_fwbk: int = 0) -> None:
+    """Wide_self_energy_t::save_SelfEn_bin (Self_energy_ext_Op_d.cpp:55-133):
+    12-int header and tables, the 3-int Lanczos descriptor (longest i_piv + 3,
+    and whether each side is one Lanczos block), then per side the rows and,
+    for a reduced side, [N_PLS, i_piv[0 .. i_piv[0]+1]].  i_fwbk > 0 writes the
+    forward side only, < 0 the backward side only."""
+    nfw = sef.fw.shape[0] if i_fwbk >= 0 else 0
+    nbk = sef.bk.shape[0] if i_fwbk <= 0 else 0
+    pf, pb = _piv_len(sef.i_piv_fw), _piv_len(sef.i_piv_bk)
+    n1 = 0
+    if pf > 0:
+        n1 = max(n1, pf + 3)
+    if pb > 0:
+        n1 = max(n1, pb + 3)
+    out = [_header_ints(sef, nfw, nbk), struct.pack("<3i", n1, int(pf > 0), int(pb > 0))]
+    if i_fwbk >= 0:
+        out.append(np.ascontiguousarray(sef.fw, "<f8").tobytes())
+        if pf > 0:
+            out.append(struct.pack(f"<{pf + 3}i", sef.fw.shape[0], *[int(x) for x in sef.i_piv_fw[:pf + 2]]))
+    if i_fwbk <= 0:
+        out.append(np.ascontiguousarray(sef.bk, "<f8").tobytes())
+        if pb > 0:
+            out.append(struct.pack(f"<{pb + 3}i", sef.bk.shape[0], *[int(x) for x in sef.i_piv_bk[:pb + 2]]))
+    with open(path, "wb") as f:
+        f.write(b"".join(out))
+
+
+def _read_header(buf: bytes, off: int = 0):
+    h = struct.unpack_from("<12i", buf, off)
+    npw, norb = h[8], h[6]
+    if npw < 0 or norb < 0 or h[7] != h[5] + h[6]:
+        raise ValueError("not a bcdwk self-energy header")
+    tab = struct.unpack_from(f"<{3 * npw + 2 * norb}i", buf, off + 48)
+    return h, tab, off + 48 + 4 * len(tab)
+
+
+def _sef_from_header(h, tab, fw, bk, ipf, ipb) -> SelfEnergyFile:
+    npw, norb = h[8], h[6]
+    return SelfEnergyFile(h[4], h[5], h[6], fw, bk, ipf, ipb, h[9], h[10], h[11], h[2], h[3],
+                          tuple(tab[:npw]), tuple(tab[npw:2 * npw]), tuple(tab[2 * npw:3 * npw]),
+                          tuple(tab[3 * npw:3 * npw + norb]), tuple(tab[3 * npw + norb:]))
+
+
+def _read_side(buf, off, n, ndim, reduced):
+    rows = np.frombuffer(buf, "<f8", n * ndim, off).reshape(n, ndim).copy()
+    off += 8 * n * ndim
+    ipv = np.zeros(0, np.int32)
+    if reduced:
+        # load_SelfEn_bin_c (Self_energy_ext_Op_d.cpp:183-193): N_PLS, then i_piv
+        _npls, npv = struct.unpack_from("<2i", buf, off)
+        if npv < 0 or npv + 2 > ndim:
+            raise ValueError("bad pivot block in bcdwk file")
+        ipv = np.asarray(struct.unpack_from(f"<{npv + 2}i", buf, off + 4), np.int32)
+        off += 4 * (npv + 3)
+    return rows, ipv, off
+
+
+def read_bcdwk(path: str) -> SelfEnergyFile:
+    """The layout write_bcdwk (save_SelfEn_bin) produces, as load_SelfEn_bin
+    reads it (Self_energy_ext_Op_d.cpp:195-244); a side written empty comes
+    back with no rows.  A one-side file (i_fwbk != 0) keeps, in its 3-int
+    descriptor, the block flag and pivot length of the side it leaves out, as
+    save_SelfEn_bin writes them; only the written side's pivot block is read."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    h, tab, off = _read_header(buf)
+    npiv_max, bfw, bbk = struct.unpack_from("<3i", buf, off)
+    off += 12
+    if bfw not in (0, 1) or bbk not in (0, 1):
+        raise ValueError(f"{path}: more than one Lanczos block per side is not supported")
+    ndim = h[7]
+    fw, ipf, off = _read_side(buf, off, h[0], ndim, bfw == 1 and h[0] > 0)
+    bk, ipb, off = _read_side(buf, off, h[1], ndim, bbk == 1 and h[1] > 0)
+    if off != len(buf):
+        raise ValueError(f"{path}: {len(buf) - off} trailing bytes")
+    return _sef_from_header(h, tab, fw, bk, ipf, ipb)
+
+
+def read_bcdwk_split(path_fw: str, path_bk: str) -> SelfEnergyFile:
+    """The split form load_SelfEn_bin falls back to (Self_energy_ext_Op_d.cpp:
+    245-290): forward rows from the -Fw file, backward rows from the -Bk file,
+    each file in the write_bcdwk layout with one side."""
+    a, b = read_bcdwk(path_fw), read_bcdwk(path_bk)
+    if (a.ish, a.ntot_dim) != (b.ish, b.ntot_dim):
+        raise ValueError("forward and backward files belong to different channels")
+    a.bk, a.i_piv_bk, a.alloc_bk = b.bk, b.i_piv_bk, b.alloc_bk
+    return a
 
 
 def read_bcdwk_old(path: str) -> SelfEnergyFile:
@@ -219,14 +370,20 @@ def read_bcdwk_old(path: str) -> SelfEnergyFile:
     Lanczos descriptor; after each side's rows, [n_piv, i_piv[NTOT_DIM]]."""
     with open(path, "rb") as f:
         buf = f.read()
-    ints = lambda off, n: struct.unpack_from(f"<{n}i", buf, off)
-    h = ints(0, 12)
-    nfw, nbk, _afw, _abk, ish, den, norb, ndim, npwv = h[:9]
-    off = 48 + 4 * (3 * npwv + 2 * norb)
+    h, tab, off = _read_header(buf)
+    nfw, nbk, ndim = h[0], h[1], h[7]
     fw = np.frombuffer(buf, "<f8", nfw * ndim, off).reshape(nfw, ndim).copy(); off += 8 * nfw * ndim
-    _npf = ints(off, 1)[0]; ipf = np.asarray(ints(off + 4, ndim), np.int32); off += 4 * (1 + ndim)
+    ipf = np.asarray(struct.unpack_from(f"<{ndim}i", buf, off + 4), np.int32); off += 4 * (1 + ndim)
     bk = np.frombuffer(buf, "<f8", nbk * ndim, off).reshape(nbk, ndim).copy(); off += 8 * nbk * ndim
-    _npb = ints(off, 1)[0]; ipb = np.asarray(ints(off + 4, ndim), np.int32); off += 4 * (1 + ndim)
+    ipb = np.asarray(struct.unpack_from(f"<{ndim}i", buf, off + 4), np.int32); off += 4 * (1 + ndim)
     if off != len(buf):
         raise ValueError(f"{path}: {len(buf) - off} trailing bytes, not the old bcdwk layout")
-    return SelfEnergyFile(ish, den, norb, fw, bk, ipf, ipb)
+    return _sef_from_header(h, tab, fw, bk, ipf, ipb)
+
+
+def read_bcdwk_any(path: str) -> SelfEnergyFile:
+    """Either layout: the current one (write_bcdwk), else the older one."""
+    try:
+        return read_bcdwk(path)
+    except (ValueError, struct.error):
+        return read_bcdwk_old(path)

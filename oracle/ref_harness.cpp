@@ -4,7 +4,7 @@
 // (built from /root/reference/source by oracle/Makefile, target `ref`); it
 // only drives reference functions and prints their outputs with %.17g.
 //
-//   ref_harness <model_space> <Vpp.bcd> <sp_prop> <pivots|-> <out.txt>
+//   ref_harness <model_space> <Vpp.bcd> <sp_prop> <pivots|-> <out.txt> [bcdwk_dir]
 //
 // Per subshell it records: the pole counts (Count_2p1h_2h1p_poles,
 // Self_energy_ext_Op_c.cpp:35-113), Sigma^inf (Build_ExtendedHartreeFock ->
@@ -13,6 +13,10 @@
 // (Build_Dynamic_SlfEn_2ndOrder, Self_energy_ext_Op_c.cpp:323-561) and, with a
 // pivots file, the Lanczos-reduced rows (Reduce_SelfEn_Lncz,
 // Self_energy_ext_Op_e.cpp:55-232 -> Reduce_Mtx_Lncz, Lanczos.cpp:44-286).
+// With bcdwk_dir it also stores each channel's Sigma(2) with the reference's
+// own save_SelfEn_bin (Self_energy_ext_Op_d.cpp:55-133): the unreduced one as
+// "Unred" (both sides), "UnredFw" (i_FwBk=+1) and "UnredBk" (i_FwBk=-1), and
+// the Lanczos-reduced one likewise as "Lanc", "LancFw" and "LancBk".
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,6 +27,17 @@
 // the reference headers expect std names in scope
 using namespace std;
 #include "BcDor-Run_vars.hh"
+
+static void save_three(Wide_self_energy_t &s, const char *dir, const char *label) {
+  char lab[64];
+  strcpy(s.STOREDIR, dir);
+  const int side[3] = {0, 1, -1};
+  const char *suffix[3] = {"", "Fw", "Bk"};
+  for (int k = 0; k < 3; ++k) {
+    sprintf(lab, "%s%s", label, suffix[k]);
+    s.save_SelfEn_bin(side[k], lab);
+  }
+}
 
 static void put_rows(FILE *f, const char *tag, int ish, int n, int w, const double *r) {
   fprintf(f, "%s %d %d %d\n", tag, ish, n, w);
@@ -67,7 +82,7 @@ static int read_pivots(const char *fn, int ish, int no, vector<double> &vec, vec
 
 int main(int argc, char **argv) {
   if (argc < 6) {
-    fprintf(stderr, "usage: %s msp vpp prop pivots|- out\n", argv[0]);
+    fprintf(stderr, "usage: %s msp vpp prop pivots|- out [bcdwk_dir]\n", argv[0]);
     return 2;
   }
   BcDor_Init();
@@ -79,6 +94,11 @@ int main(int argc, char **argv) {
   VppInt_t vpp(argv[2], &msp);
   SpProp_t g(argv[3], &msp);
   const bool lanc = strcmp(argv[4], "-") != 0;
+  const char *wkdir = argc > 6 ? argv[6] : 0;
+  if (wkdir && strlen(wkdir) >= 100) {
+    fprintf(stderr, "bcdwk_dir longer than STOREDIR holds\n");
+    return 2;
+  }
   FILE *out = fopen(argv[5], "w");
   fprintf(out, "nme %d\n", vpp.n_me_stored);
 
@@ -95,6 +115,7 @@ int main(int argc, char **argv) {
     sf.Build_Dynamic_SlfEn_2ndOrder(&n3, &n4, 1, 0, 0);
     put_rows(out, "fw", ish, sf.N_PLS_fw, sf.NTOT_DIM, sf.Sig_dyn_fw);
     put_rows(out, "bk", ish, sf.N_PLS_bk, sf.NTOT_DIM, sf.Sig_dyn_bk);
+    if (wkdir) save_three(sf, wkdir, "Unred");
 
     if (lanc) {
       vector<double> pv;
@@ -115,6 +136,7 @@ int main(int argc, char **argv) {
       fprintf(out, " |");
       for (int k = 0; k < sl.n_piv_bk + 2 && sl.N_PLS_bk > 0; ++k) fprintf(out, " %d", sl.i_piv_bk[k]);
       fprintf(out, "\n");
+      if (wkdir) save_three(sl, wkdir, "Lanc");
     }
   }
   fclose(out);

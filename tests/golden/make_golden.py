@@ -6,6 +6,10 @@ Each case writes tests/golden/<case>.npz with the inputs (model space,
 propagator, pivots; the interaction is regenerated from its seed and checked
 against the stored digest) and the reference outputs per subshell: pole
 counts, Sigma^inf, Sigma(2) rows in build order and Lanczos-reduced rows.
+For the channels a case lists in `bcdwk`, the files the reference's own
+save_SelfEn_bin wrote for them (unreduced "Unred*" and reduced "Lanc*", both
+sides and each side alone, or the labels the case names) are kept as
+tests/golden/bcdwk/<case>/*.bin.
 The input propagators come from the oracle (HF) or are random; they are
 written to the reference's propagator format with %.17g, so the reference
 reads exactly the same numbers.
@@ -14,6 +18,7 @@ from __future__ import annotations
 
 import hashlib
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -110,7 +115,11 @@ def random_propagator(ms, rng, nfrag=2) -> formats.Propagator:
                               (fp ** 2).sum(1), fp, np.array(eh), (fh ** 2).sum(1), fh)
 
 
-def run_case(name, nmax, seed, prop_kind, piv_kind, piv_n, keep=None, store_v=True):
+BCDWK_LABELS = ("Unred", "UnredFw", "UnredBk", "Lanc", "LancFw", "LancBk")
+
+
+def run_case(name, nmax, seed, prop_kind, piv_kind, piv_n, keep=None, store_v=True, bcdwk=(),
+             bcdwk_labels=BCDWK_LABELS):
     ms, vpp = synth.make_system(nmax, seed)
     S = po.System(ms, vpp)
     if prop_kind == "hf":
@@ -131,8 +140,17 @@ def run_case(name, nmax, seed, prop_kind, piv_kind, piv_n, keep=None, store_v=Tr
         formats.write_vpp_bcd(f_v, ms, vpp)
         formats.write_propagator(f_g, ms, g)
         formats.write_pivots(f_p, ms, piv)
-        subprocess.run([HARNESS, f_ms, f_v, f_g, f_p, f_o], check=True, cwd=td)
+        wk = os.path.join(td, "wk")
+        os.mkdir(wk)
+        subprocess.run([HARNESS, f_ms, f_v, f_g, f_p, f_o, wk], check=True, cwd=td)
         res = parse(f_o)
+        if bcdwk:
+            dst = os.path.join(OUT, "bcdwk", name)
+            os.makedirs(dst, exist_ok=True)
+            for i in bcdwk:
+                for lab in bcdwk_labels:
+                    src = formats.selfenergy_filename(wk, lab, int(ms.j2[i]), int(ms.ip[i]), int(ms.ch[i]))
+                    shutil.copy(src, dst)
     keep = range(ms.nsubsh) if keep is None else keep
     data = dict(nmax=nmax, seed=seed, v_digest=v_digest(vpp), nme_ref=res["nme"],
                 prop_np=g.np_, prop_nh=g.nh, prop_ep=g.ep, prop_fqp=g.fqp, prop_eh=g.eh, prop_fqh=g.fqh,
@@ -156,9 +174,9 @@ def run_case(name, nmax, seed, prop_kind, piv_kind, piv_n, keep=None, store_v=Tr
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit("build the harness first: make -C oracle ref")
-    run_case("c1_hf_split50", 3, 0, "hf", 1, 50)
+    run_case("c1_hf_split50", 3, 0, "hf", 1, 50, bcdwk=(0, 9))
     run_case("c1_hf_piv10", 3, 0, "hf", 0, 10)
-    run_case("c1_dressed", 3, 5, "random", 0, 0)
+    run_case("c1_dressed", 3, 5, "random", 0, 0, bcdwk=(11, 12), bcdwk_labels=BCDWK_LABELS[3:])
     run_case("c2_hf_split50", 7, 0, "hf", 1, 50, keep=[0, 2, 7, 14, 15, 22, 29], store_v=False)
 
 
