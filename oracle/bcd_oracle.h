@@ -23,6 +23,9 @@
  *                            numpy eigh on the assembled matrix, one-orbit secular equation
  *   orc_run ................ Koltun energies of the reference fixtures, HF fixed point,
  *                            golden first-iteration Sigma + Dyson chains
+ *   orc_s6j ................ exact rational 6j values (tests/test_oracle_2b.py)
+ *   orc_pandya, orc_ladd_*,
+ *   orc_ph_* ............... golden vectors of the compiled reference (gII_b.cpp, PolProp_b.cpp)
  * The Dyson stage of the reference (DysonMatrix.cpp) links LAPACK dsyevd_, an
  * external library, so it is not built from the reference sources; its parity
  * rests on the reference-held fixture and on the closed-form checks above.
@@ -137,6 +140,25 @@ int orc_run(const orc_sys *s, const orc_prop *gin, const orc_params *par,
             const int *piv_n, const int *piv_off, const int *piv_ioff,
             const double *piv_vec, const int *itr_fw, const int *itr_bk,
             orc_prop *gout, orc_stats *st);
+
+/* Two-body propagator matrices (bcd_oracle_2b.c): the pp/hh ladder of
+ * gII_prop_t (gII_b.cpp:62-563) and the ph ring of Pol_prop_t
+ * (PolProp_b.cpp:43-614).  Matrices are stored with entry (c, r) at
+ * Sab[c*ld + r]. */
+double orc_s6j(int ja, int jb, int jc, int la, int lb, int lc);
+double orc_pandya(const orc_sys *s, int ia, int ib, int ig, int id, int Jf);
+int orc_ladd_basis(const orc_sys *s, const orc_prop *g, int Jf, int dT, int pif, int icut, int ntcut,
+                   const int *n1p, const int *n1h, int cap_pp, int cap_hh, int *n1_pp, int *n2_pp,
+                   double *e_pp, int *k1_hh, int *k2_hh, double *e_hh, int *npp, int *nhh);
+int orc_ladd_fill(const orc_sys *s, const orc_prop *g, int Jf, int npp, const int *n1_pp, const int *n2_pp,
+                  const double *e_pp, int nhh, const int *k1_hh, const int *k2_hh, const double *e_hh,
+                  int mode, double *Sab, int ld, double *free_poles, int add_free_poles);
+int orc_ph_basis(const orc_sys *s, const orc_prop *g, int Jf, int dT, int pif, int icut, int ntcut,
+                 const int *n1p, const int *n1h, int cap_ph, int cap_hp, int *n_ph, int *k_ph, double *e_ph,
+                 int *n_hp, int *k_hp, double *e_hp, int *nph, int *nhp);
+int orc_ph_fill(const orc_sys *s, const orc_prop *g, int Jf, int dT, int nph, const int *n_ph, const int *k_ph,
+                const double *e_ph, int nhp, const int *n_hp, const int *k_hp, const double *e_hp, int mode,
+                double *Sab, int ld, double *free_poles, int add_free_poles);
 
 /* accessors for ctypes */
 int orc_prop_counts(const orc_prop *p, int *n_qp, int *n_qh, int *nspb);

@@ -78,6 +78,18 @@ def lib():
                                 C.c_double, F64P, C.POINTER(C.c_int), F64P, C.POINTER(C.c_int), F64P]
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [vp, vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, C.POINTER(Stats)]
+        L.orc_s6j.restype = C.c_double; L.orc_s6j.argtypes = [C.c_int] * 6
+        L.orc_pandya.restype = C.c_double; L.orc_pandya.argtypes = [vp] + [C.c_int] * 5
+        ip = C.POINTER(C.c_int)
+        for f in (L.orc_ladd_basis, L.orc_ph_basis):
+            f.restype = C.c_int
+            f.argtypes = [vp, vp] + [C.c_int] * 5 + [vp, vp, C.c_int, C.c_int] + [vp] * 6 + [ip, ip]
+        L.orc_ladd_fill.restype = C.c_int
+        L.orc_ladd_fill.argtypes = [vp, vp, C.c_int, C.c_int, I32P, I32P, F64P, C.c_int, I32P, I32P, F64P,
+                                    C.c_int, F64P, C.c_int, F64P, C.c_int]
+        L.orc_ph_fill.restype = C.c_int
+        L.orc_ph_fill.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, I32P, I32P, F64P, C.c_int, I32P, I32P, F64P,
+                                  C.c_int, F64P, C.c_int, F64P, C.c_int]
         _LIB = L
     return _LIB
 
@@ -257,3 +269,89 @@ def run(sysm: System, g: Prop, mode="hf", itr_max=-1, lanczos=False, lanc_kind=1
     if rc:
         raise RuntimeError(f"orc_run failed ({rc})")
     return out, st
+
+
+# ---------------------------------------------------------------- two-body
+def s6j(ja, jb, jc, la, lb, lc) -> float:
+    """6j symbol with all arguments doubled (ang_mom_functions::s6j)."""
+    return lib().orc_s6j(ja, jb, jc, la, lb, lc)
+
+
+def pandya(sysm: System, a, b, g, d, J) -> float:
+    return lib().orc_pandya(sysm.h, a, b, g, d, J)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _basis(fn, sysm, g, J, dT, pif, icut, ntcut, n1p, n1h, names):
+    L = lib()
+    n1p = None if n1p is None else np.ascontiguousarray(n1p, np.int32)
+    n1h = None if n1h is None else np.ascontiguousarray(n1h, np.int32)
+    a, b = C.c_int(), C.c_int()
+    rc = fn(sysm.h, g.h, J, dT, pif, icut, ntcut, _ptr(n1p), _ptr(n1h), 0, 0, *([None] * 6),
+            C.byref(a), C.byref(b))
+    if rc < 0:
+        raise RuntimeError("basis count failed")
+    na, nb = a.value, b.value
+    out = [np.zeros(max(na, 1), np.int32), np.zeros(max(na, 1), np.int32), np.zeros(max(na, 1)),
+           np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1))]
+    rc = fn(sysm.h, g.h, J, dT, pif, icut, ntcut, _ptr(n1p), _ptr(n1h), na, nb, *[_ptr(x) for x in out],
+            C.byref(a), C.byref(b))
+    if rc < 0:
+        raise RuntimeError("basis build failed")
+    cut = [na, na, na, nb, nb, nb]
+    return {k: v[:n] for k, v, n in zip(names, out, cut)}
+
+
+LADD_KEYS = ("n1_pp", "n2_pp", "e_pp", "k1_hh", "k2_hh", "e_hh")
+PH_KEYS = ("n_ph", "k_ph", "e_ph", "n_hp", "k_hp", "e_hp")
+
+
+def ladd_basis(sysm: System, g: Prop, J, dT, pif, icut=0, ntcut=0, n1p=None, n1h=None) -> dict:
+    """gII_prop_t::Build_LaddRPA_basis: dict of the 2qp and 2qh pair lists."""
+    return _basis(lib().orc_ladd_basis, sysm, g, J, dT, pif, icut, ntcut, n1p, n1h, LADD_KEYS)
+
+
+def ph_basis(sysm: System, g: Prop, J, dT, pif, icut=0, ntcut=0, n1p=None, n1h=None) -> dict:
+    """Pol_prop_t::Build_phRPA_basis: dict of the qp-qh and qh-qp lists."""
+    return _basis(lib().orc_ph_basis, sysm, g, J, dT, pif, icut, ntcut, n1p, n1h, PH_KEYS)
+
+
+def _arr(x, t):
+    x = np.ascontiguousarray(x, t)
+    return x if x.size else np.zeros(1, t)
+
+
+def ladd_fill(sysm: System, g: Prop, J, basis: dict, mode: int, add_free_poles=True, ld=None, fill=0.0):
+    """gII_prop_t::Fill_LaddRPA_mtx; returns (matrix rows [ndim x ld], free poles [ndim]),
+    both from buffers prefilled with `fill`."""
+    npp, nhh = len(basis["e_pp"]), len(basis["e_hh"])
+    ndim = (npp if mode != 2 else 0) + (nhh if mode != 1 else 0)
+    ld = max(ndim, 1) if ld is None else ld
+    sab = np.full(max(ndim, 1) * ld, fill); fp = np.full(max(ndim, 1), fill)
+    b = basis
+    rc = lib().orc_ladd_fill(sysm.h, g.h, J, npp, _arr(b["n1_pp"], np.int32), _arr(b["n2_pp"], np.int32),
+                             _arr(b["e_pp"], np.float64), nhh, _arr(b["k1_hh"], np.int32),
+                             _arr(b["k2_hh"], np.int32), _arr(b["e_hh"], np.float64), mode, sab, ld, fp,
+                             int(add_free_poles))
+    if rc:
+        raise RuntimeError(f"orc_ladd_fill failed ({rc})")
+    return sab[:ndim * ld].reshape(ndim, ld), fp[:ndim]
+
+
+def ph_fill(sysm: System, g: Prop, J, dT, basis: dict, mode: int, add_free_poles=True, ld=None, fill=0.0):
+    """Pol_prop_t::Fill_phRPA_mtx; returns (matrix rows [ndim x ld], free poles [ndim]),
+    both from buffers prefilled with `fill`."""
+    nph, nhp = len(basis["e_ph"]), len(basis["e_hp"])
+    ndim = nph + (nhp if mode != 1 else 0)
+    ld = max(ndim, 1) if ld is None else ld
+    sab = np.full(max(ndim, 1) * ld, fill); fp = np.full(max(ndim, 1), fill)
+    b = basis
+    rc = lib().orc_ph_fill(sysm.h, g.h, J, dT, nph, _arr(b["n_ph"], np.int32), _arr(b["k_ph"], np.int32),
+                           _arr(b["e_ph"], np.float64), nhp, _arr(b["n_hp"], np.int32), _arr(b["k_hp"], np.int32),
+                           _arr(b["e_hp"], np.float64), mode, sab, ld, fp, int(add_free_poles))
+    if rc:
+        raise RuntimeError(f"orc_ph_fill failed ({rc})")
+    return sab[:ndim * ld].reshape(ndim, ld), fp[:ndim]

@@ -1,6 +1,6 @@
 """Write golden vectors of the compiled reference (oracle/_ref/ref_harness).
 
-Run in the build container (it needs /root/reference to build the harness):
+Run in the build container (it needs /root/reference to build the harnesses):
     make -C oracle ref && python tests/golden/make_golden.py
 Each case writes tests/golden/<case>.npz with the inputs (model space,
 propagator, pivots; the interaction is regenerated from its seed and checked
@@ -171,13 +171,86 @@ def run_case(name, nmax, seed, prop_kind, piv_kind, piv_n, keep=None, store_v=Tr
     print(f"{name}: n_me={res['nme']} channels={len(list(keep))}")
 
 
+HARNESS_2B = os.path.join(ROOT, "oracle", "_ref", "ref_harness_2b")
+
+
+def parse_2b(path, jkeep):
+    """ref_harness_2b output -> flat dict of arrays.  Bases of every channel
+    ("lb_J_pi_dT_<field>", "rb_..."; "lc<icut>_..." / "rc<icut>_..." under the
+    n1 cut), and the matrices of channels J <= jkeep as
+    "lm_J_pi_dT_<mode>_<add>" [ndim x ld] and "lf_..." (free poles)."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    out = {}
+    p = 0
+    cut = None
+    ints = lambda s: np.array([int(x) for x in s.split()], np.int32)
+    dbls = lambda s: np.array([float(x) for x in s.split()], np.float64)
+    while p < len(lines):
+        t = lines[p].split()
+        p += 1
+        if not t:
+            continue
+        if t[0] == "bounds":
+            out["bounds"] = np.array([int(x) for x in t[1:]], np.int32)
+        elif t[0] in ("lcut", "rcut"):
+            cut = int(t[1])
+        elif t[0] in ("lbasis", "rbasis"):
+            J, pi, dT = t[1:4]
+            pre = ("l" if t[0] == "lbasis" else "r") + ("b" if cut is None else f"c{cut}")
+            keys = ("n1_pp", "n2_pp", "e_pp", "k1_hh", "k2_hh", "e_hh") if t[0] == "lbasis" else \
+                ("n_ph", "k_ph", "e_ph", "n_hp", "k_hp", "e_hp")
+            for k, key in enumerate(keys):
+                out[f"{pre}_{J}_{pi}_{dT}_{key}"] = (dbls if key.startswith("e_") else ints)(lines[p + k])
+            p += 6
+            cut = None
+        elif t[0] in ("lmat", "rmat"):
+            J, pi, dT, mode, add, ndim, ld, rc = (int(x) for x in t[1:])
+            rows = [dbls(lines[p + r]) for r in range(ndim)]
+            fp = dbls(lines[p + ndim])
+            p += ndim + 1
+            if J <= jkeep:
+                key = f"{t[0][0]}m_{J}_{pi}_{dT}_{mode}_{add}"
+                out[key] = np.array(rows).reshape(ndim, ld)
+                out[key.replace("m_", "f_", 1)] = fp
+                out[key.replace("m_", "rc_", 1)] = np.array([rc], np.int32)
+    return out
+
+
+def run_case_2b(name, nmax, seed, prop_kind, jmodes, jkeep, nfrag=2):
+    """Ladder and ring bases/matrices of the compiled reference for one
+    propagator (HF from the oracle, or random with nfrag fragments per side
+    and subshell)."""
+    ms, vpp = synth.make_system(nmax, seed)
+    if prop_kind == "hf":
+        S = po.System(ms, vpp)
+        gh, _ = po.run(S, po.Prop.bare(S), "hf", itr_max=-1, conv_kind=0, conv=1e-7)
+        g = gh.to_propagator(ms.nsubsh)
+    else:
+        g = random_propagator(ms, np.random.default_rng(2000 + seed), nfrag)
+    with tempfile.TemporaryDirectory() as td:
+        f_ms, f_v, f_g, f_o = (os.path.join(td, x) for x in ("msp", "v.bcd", "prop", "out"))
+        formats.write_model_space(f_ms, ms)
+        formats.write_vpp_bcd(f_v, ms, vpp)
+        formats.write_propagator(f_g, ms, g)
+        subprocess.run([HARNESS_2B, f_ms, f_v, f_g, f_o, str(jmodes)], check=True, cwd=td,
+                       stdout=subprocess.DEVNULL)
+        res = parse_2b(f_o, jkeep)
+    data = dict(nmax=nmax, seed=seed, v_digest=v_digest(vpp), prop_np=g.np_, prop_nh=g.nh, prop_ep=g.ep,
+                prop_fqp=g.fqp, prop_eh=g.eh, prop_fqh=g.fqh, jkeep=jkeep, jmodes=jmodes, **res)
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **data)
+    print(f"{name}: {sum(1 for k in res if k[1] == 'm')} matrices")
+
+
 def main():
-    if not os.path.exists(HARNESS):
-        raise SystemExit("build the harness first: make -C oracle ref")
+    if not (os.path.exists(HARNESS) and os.path.exists(HARNESS_2B)):
+        raise SystemExit("build the harnesses first: make -C oracle ref")
     run_case("c1_hf_split50", 3, 0, "hf", 1, 50, bcdwk=(0, 9))
     run_case("c1_hf_piv10", 3, 0, "hf", 0, 10)
     run_case("c1_dressed", 3, 5, "random", 0, 0, bcdwk=(11, 12), bcdwk_labels=BCDWK_LABELS[3:])
     run_case("c2_hf_split50", 7, 0, "hf", 1, 50, keep=[0, 2, 7, 14, 15, 22, 29], store_v=False)
+    run_case_2b("c1_hf_2b", 3, 0, "hf", jmodes=3, jkeep=7)
+    run_case_2b("c1_dressed_2b", 3, 5, "random", jmodes=0, jkeep=1)
 
 
 if __name__ == "__main__":
