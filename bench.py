@@ -152,6 +152,147 @@ def cpu_baseline(seeds, seconds=15.0, threads=None):
                       f"{wall:.1f} s wall; mean {np.mean(done):.2f} s per solve"}
 
 
+# ----------------------------------------------------------- two-body matrices
+T2_NMAX = 7
+T2_FRAGS = 4          # quasiparticle and quasihole fragments per subshell
+
+
+def t2_propagator(ms, seed=1):
+    """A dressed propagator: T2_FRAGS quasiparticles (e in [20, 120]) and
+    quasiholes (e in [-60, 0]) per subshell with N(0, 0.25) overlaps."""
+    from bcdor_amd import formats
+    rng = np.random.default_rng(seed)
+    nspb = ms.nspb
+    ep, eh, fp, fh = [], [], [], []
+    for i in range(ms.nsubsh):
+        for kind in (0, 1):
+            for _ in range(T2_FRAGS):
+                x = np.zeros(nspb)
+                x[:ms.no[i]] = rng.standard_normal(ms.no[i]) * 0.5
+                (fp if kind == 0 else fh).append(x)
+                (ep if kind == 0 else eh).append(rng.uniform(20, 120) if kind == 0 else rng.uniform(-60, 0))
+    fp, fh = np.array(fp), np.array(fh)
+    n = np.full(ms.nsubsh, T2_FRAGS, np.int32)
+    return formats.Propagator(n, n.copy(), np.array(ep), (fp ** 2).sum(1), fp, np.array(eh), (fh ** 2).sum(1), fh)
+
+
+def t2_cpu_baseline(ms, vpp, g, t, seconds):
+    """The oracle's ladder/ring fills (the reference's fourfold orbit loops)
+    on a sample of the build's channels, smallest first, one channel per
+    thread, until the time budget is spent; the rate in orbit-quadruple terms
+    per second is extrapolated to the whole build."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    S = po.System(ms, vpp)
+    P = po.Prop.from_propagator(S, g)
+    ms_no = np.asarray(ms.no)
+    sub_p = np.repeat(np.arange(ms.nsubsh), np.asarray(g.np_))
+    sub_h = np.repeat(np.arange(ms.nsubsh), np.asarray(g.nh))
+
+    def terms(inf, b):
+        # orbit terms of every entry: rows x cols of no(a)*no(b) products
+        if inf["kind"] == 0:
+            wa = ms_no[sub_p[b["n1_pp"]]] * ms_no[sub_p[b["n2_pp"]]]
+            wb = ms_no[sub_h[b["k1_hh"]]] * ms_no[sub_h[b["k2_hh"]]]
+        else:
+            wa = ms_no[sub_p[b["n_ph"]]] * ms_no[sub_h[b["k_ph"]]]
+            wb = ms_no[sub_p[b["n_hp"]]] * ms_no[sub_h[b["k_hp"]]]
+        w = np.concatenate([wa, wb]).astype(np.float64)
+        return float(w.sum() ** 2)
+    jobs = []
+    total_terms = 0.0
+    for k in range(t.n):
+        inf = t.info(k)
+        b = t.basis(k)
+        tt = terms(inf, b)
+        total_terms += tt
+        jobs.append((tt, inf, b))
+    jobs.sort(key=lambda x: x[0])
+    threads = min(16, os.cpu_count() or 1)
+    lock = threading.Lock()
+    done = []
+    t_end = time.perf_counter() + seconds
+    nxt = [0]
+
+    def worker():
+        while time.perf_counter() < t_end:
+            with lock:
+                if nxt[0] >= len(jobs):
+                    return
+                tt, inf, b = jobs[nxt[0]]
+                nxt[0] += 1
+            if inf["kind"] == 0:
+                po.ladd_fill(S, P, inf["J"], b, inf["mode"], False)
+            else:
+                po.ph_fill(S, P, inf["J"], inf["dT"], b, inf["mode"], False)
+            with lock:
+                done.append(tt)
+
+    t0 = time.perf_counter()
+    pool = [threading.Thread(target=worker) for _ in range(threads)]
+    for th in pool:
+        th.start()
+    for th in pool:
+        th.join()
+    wall = time.perf_counter() - t0
+    rate = sum(done) / wall
+    return {"value": rate / total_terms, "unit": "builds/s", "cores": threads, "kind": "port",
+            "sample": f"{len(done)} of {t.n} channel matrices (smallest first), {sum(done):.3g} of "
+                      f"{total_terms:.3g} orbit terms in {wall:.1f} s wall; builds/s = terms/s / terms per build"}
+
+
+def main_twobody(args):
+    ms, vpp = synth.make_system(T2_NMAX, 0)
+    g = t2_propagator(ms)
+    ctx = capi.Context(ms, vpp)
+    t0 = time.perf_counter()
+    t = capi.TwoBody(ctx, g, ladd_mode=3, ring_mode=3, add_free_poles=False)
+    print(f"# setup {time.perf_counter() - t0:.1f} s: {t.n} matrices", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        t.build()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t.build()
+    dt = time.perf_counter() - t0
+    t.set_profile(True)
+    t.build()
+    kt = t.kernel_times()
+    t.set_profile(False)
+    work = t.work()
+    print(f"# kernels {json.dumps(kt)}", file=sys.stderr, flush=True)
+    dom = max(kt, key=lambda k: kt[k][0])
+    avg_ms = kt[dom][0]
+    flops = {"t2_gemm1": work["flops_gemm1"], "t2_gemm2": work["flops_gemm2"]}.get(dom)
+    rl = None
+    if flops:
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        rl = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+              "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom, "avg_launch_ms": avg_ms,
+              "launches": kt[dom][1]}
+    cpu = None if args.no_cpu else t2_cpu_baseline(ms, vpp, g, t, args.cpu_seconds)
+    line = {
+        "metric": "ladder + ring (D)RPA matrix sets built per second (every two-body channel, i_RPAcalc=3)",
+        "value": args.steps / dt,
+        "unit": "builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (Nmax={T2_NMAX} seed 0 interaction; {T2_FRAGS} qp + {T2_FRAGS} qh random fragments "
+                f"per subshell)",
+        "config": {"workload": f"twobody_ladder_ring_nmax{T2_NMAX}_f{T2_FRAGS}", "matrices": t.n,
+                   "gemm_flops_per_build": work["flops_gemm1"] + work["flops_gemm2"],
+                   "matrix_bytes_per_build": work["bytes_out"]},
+        "roofline": rl,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +301,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("scgf", "twobody"), default="scgf",
+                    help="scgf: the converged-solve benchmark (default); twobody: ladder + ring matrices")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; shard instances across GPUs "
@@ -167,6 +310,8 @@ def main():
         sys.exit(2)
     if capi.lib().bcd_device_count() < 1:
         raise SystemExit("no GPU visible: the HIP path has no CPU fallback")
+    if args.workload == "twobody":
+        return main_twobody(args)
 
     t0 = time.perf_counter()
     seeds, dropped = screened_seeds(args.batch)

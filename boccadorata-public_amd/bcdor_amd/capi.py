@@ -49,7 +49,10 @@ EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_creat
            "bcd_count_poles", "bcd_sigma2_build", "bcd_hf_pot", "bcd_reduce_mtx_lncz", "bcd_reduce_selfen_lncz",
            "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_destroy", "bcd_batch_solve",
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
-           "bcd_batch_kernel_times", "bcd_batch_work"]
+           "bcd_batch_kernel_times", "bcd_batch_work", "bcd_ladd_count", "bcd_ladd_basis", "bcd_ladd_fill",
+           "bcd_ph_count", "bcd_ph_basis", "bcd_ph_fill", "bcd_t2_create", "bcd_t2_destroy", "bcd_t2_count",
+           "bcd_t2_info", "bcd_t2_basis", "bcd_t2_build", "bcd_t2_matrix", "bcd_t2_set_profile",
+           "bcd_t2_kernel_times", "bcd_t2_work"]
 
 KERNEL_NAMES = ["sigma2", "lanczos", "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild", "sigma2_plan",
                 "arrow", "poles"]
@@ -103,6 +106,25 @@ def lib():
         L.bcd_batch_result.argtypes = [vp, C.c_int, C.c_int, IP, IP, IP, IP, DP, DP, DP, DP, DP, DP]
         L.bcd_batch_kernel_times.argtypes = [vp, C.c_int, DP, LP]
         L.bcd_batch_work.argtypes = [vp, DP, C.c_int]
+        PP = C.POINTER(BcdProp)
+        for f in (L.bcd_ladd_count, L.bcd_ph_count):
+            f.argtypes = [vp, PP] + [C.c_int] * 5 + [IP, IP, IP, IP]
+        for f in (L.bcd_ladd_basis, L.bcd_ph_basis):
+            f.argtypes = [vp, PP] + [C.c_int] * 5 + [IP, IP, C.c_int, C.c_int, IP, IP, DP, IP, IP, DP, IP, IP]
+        for f in (L.bcd_ladd_fill, L.bcd_ph_fill):
+            f.argtypes = [vp, PP, C.c_int, C.c_int, C.c_int, C.c_int, IP, IP, DP, C.c_int, IP, IP, DP, C.c_int, DP,
+                          C.c_int, DP, C.c_int]
+        L.bcd_t2_create.argtypes = [vp, PP, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+        L.bcd_t2_destroy.argtypes = [vp]
+        L.bcd_t2_destroy.restype = None
+        L.bcd_t2_count.argtypes = [vp]
+        L.bcd_t2_info.argtypes = [vp, C.c_int, IP]
+        L.bcd_t2_basis.argtypes = [vp, C.c_int, IP, IP, DP, IP, IP, DP]
+        L.bcd_t2_build.argtypes = [vp]
+        L.bcd_t2_matrix.argtypes = [vp, C.c_int, DP, DP]
+        L.bcd_t2_set_profile.argtypes = [vp, C.c_int]
+        L.bcd_t2_kernel_times.argtypes = [vp, C.c_int, DP, LP]
+        L.bcd_t2_work.argtypes = [vp, DP, C.c_int]
         _LIB = L
     return _LIB
 
@@ -158,7 +180,147 @@ def prop_struct(g: Propagator, nspb: int, keep: _Keep) -> BcdProp:
                    dp(keep.d(np.append(fqh.reshape(-1), 0.0))))
 
 
-class Context:
+LADD_KEYS = ("n1_pp", "n2_pp", "e_pp", "k1_hh", "k2_hh", "e_hh")
+PH_KEYS = ("n_ph", "k_ph", "e_ph", "n_hp", "k_hp", "e_hp")
+T2_KERNELS = ("t2_zero", "t2_amp", "t2_pandya", "t2_gemm1", "t2_gemm2", "t2_diag")
+_NULL_I = C.cast(None, IP)
+_NULL_D = C.cast(None, DP)
+
+
+def _opt_i(a, keep):
+    return _NULL_I if a is None else ip(keep.i(a))
+
+
+def _fill_dims(kind, mode, na, nb):
+    if kind == 0:
+        return (na if mode != 2 else 0) + (nb if mode != 1 else 0)
+    return na + (nb if mode != 1 else 0)
+
+
+class _TwoBodyMixin:
+    """Ladder (gII_prop_t) and ring (Pol_prop_t) bases and matrices of one
+    channel through the C ABI; bases are dicts keyed as the reference's members."""
+
+    def _basis(self, kind, g, J, dT, pif, icut, ntcut, n1p, n1h):
+        L = lib()
+        keep = _Keep()
+        gs = prop_struct(g, self.ms.nspb, keep)
+        a, b = C.c_int(), C.c_int()
+        cnt = L.bcd_ladd_count if kind == 0 else L.bcd_ph_count
+        check(cnt(self.h, C.byref(gs), J, dT, pif, icut, ntcut, _opt_i(n1p, keep), _opt_i(n1h, keep),
+                  C.byref(a), C.byref(b)), "basis count")
+        na, nb = a.value, b.value
+        arr = [np.zeros(max(na, 1), np.int32), np.zeros(max(na, 1), np.int32), np.zeros(max(na, 1)),
+               np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1))]
+        fn = L.bcd_ladd_basis if kind == 0 else L.bcd_ph_basis
+        check(fn(self.h, C.byref(gs), J, dT, pif, icut, ntcut, _opt_i(n1p, keep), _opt_i(n1h, keep), na, nb,
+                 ip(arr[0]), ip(arr[1]), dp(arr[2]), ip(arr[3]), ip(arr[4]), dp(arr[5]), C.byref(a), C.byref(b)),
+              "basis build")
+        keys = LADD_KEYS if kind == 0 else PH_KEYS
+        return {k: v[:(na if i < 3 else nb)] for i, (k, v) in enumerate(zip(keys, arr))}
+
+    def ladd_basis(self, g, J, dT, pif, icut=0, ntcut=0, n1p=None, n1h=None):
+        return self._basis(0, g, J, dT, pif, icut, ntcut, n1p, n1h)
+
+    def ph_basis(self, g, J, dT, pif, icut=0, ntcut=0, n1p=None, n1h=None):
+        return self._basis(1, g, J, dT, pif, icut, ntcut, n1p, n1h)
+
+    def _fill(self, kind, g, J, dT, pif, basis, mode, add, ld, fill, raw):
+        keys = LADD_KEYS if kind == 0 else PH_KEYS
+        b = [basis[k] for k in keys]
+        na, nb = len(b[2]), len(b[5])
+        ndim = _fill_dims(kind, mode, na, nb)
+        ld = max(ndim, 1) if ld is None else ld
+        rows = max(ndim, 1)
+        sab = np.full(rows * max(ld, 1), fill)
+        fp = np.full(rows, fill)
+        keep = _Keep()
+        gs = prop_struct(g, self.ms.nspb, keep)
+        z = lambda a, f: f(keep.i(a) if f is ip else keep.d(a)) if len(a) else (_NULL_I if f is ip else _NULL_D)
+        fn = lib().bcd_ladd_fill if kind == 0 else lib().bcd_ph_fill
+        rc = fn(self.h, C.byref(gs), J, dT, pif, na, z(b[0], ip), z(b[1], ip), z(b[2], dp), nb, z(b[3], ip),
+                z(b[4], ip), z(b[5], dp), mode, dp(sab), ld, dp(fp), int(add))
+        if raw:
+            return rc
+        check(rc, "bcd_ladd_fill" if kind == 0 else "bcd_ph_fill")
+        return sab[:ndim * ld].reshape(ndim, ld), fp[:ndim]
+
+    def ladd_fill(self, g, J, dT, pif, basis, mode, add_free_poles=True, ld=None, fill=0.0, raw=False):
+        """gII_prop_t::Fill_LaddRPA_mtx: (matrix rows [ndim x ld], free poles), both
+        from buffers prefilled with `fill`; raw=True returns the status code."""
+        return self._fill(0, g, J, dT, pif, basis, mode, add_free_poles, ld, fill, raw)
+
+    def ph_fill(self, g, J, dT, pif, basis, mode, add_free_poles=True, ld=None, fill=0.0, raw=False):
+        """Pol_prop_t::Fill_phRPA_mtx, as ladd_fill."""
+        return self._fill(1, g, J, dT, pif, basis, mode, add_free_poles, ld, fill, raw)
+
+
+class TwoBody:
+    """Every ladder and ring matrix of a propagator (bcd_t2), resident on the device."""
+
+    def __init__(self, ctx, g: Propagator, ladd_mode=3, ring_mode=3, add_free_poles=False):
+        L = lib()
+        keep = _Keep()
+        h = C.c_void_p()
+        check(L.bcd_t2_create(ctx.h, C.byref(prop_struct(g, ctx.ms.nspb, keep)), ladd_mode, ring_mode,
+                              int(add_free_poles), C.byref(h)), "bcd_t2_create")
+        self.h = h
+        self.ctx = ctx     # keeps the context alive
+        self.n = check(L.bcd_t2_count(self.h), "bcd_t2_count")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bcd_t2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self, k):
+        v = np.zeros(10, np.int32)
+        check(lib().bcd_t2_info(self.h, k, ip(v)), "bcd_t2_info")
+        return dict(zip(("kind", "J", "pi", "dT", "mode", "nf", "nb", "ld", "na", "nb_basis"), (int(x) for x in v)))
+
+    def basis(self, k):
+        """The basis of matrix k as a dict keyed like the reference's members."""
+        inf = self.info(k)
+        na, nb = inf["na"], inf["nb_basis"]
+        arr = [np.zeros(max(na, 1), np.int32), np.zeros(max(na, 1), np.int32), np.zeros(max(na, 1)),
+               np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1))]
+        check(lib().bcd_t2_basis(self.h, k, ip(arr[0]), ip(arr[1]), dp(arr[2]), ip(arr[3]), ip(arr[4]),
+                                 dp(arr[5])), "bcd_t2_basis")
+        keys = LADD_KEYS if inf["kind"] == 0 else PH_KEYS
+        return {key: v[:(na if i < 3 else nb)] for i, (key, v) in enumerate(zip(keys, arr))}
+
+    def build(self):
+        check(lib().bcd_t2_build(self.h), "bcd_t2_build")
+
+    def matrix(self, k):
+        inf = self.info(k)
+        ndim, ld = inf["nf"] + inf["nb"], inf["ld"]
+        sab = np.zeros(max(ndim, 1) * ld)
+        fp = np.full(max(ndim, 1), np.nan)
+        check(lib().bcd_t2_matrix(self.h, k, dp(sab), dp(fp)), "bcd_t2_matrix")
+        return sab[:ndim * ld].reshape(ndim, ld), fp[:ndim]
+
+    def set_profile(self, on: bool):
+        check(lib().bcd_t2_set_profile(self.h, int(on)), "bcd_t2_set_profile")
+
+    def kernel_times(self):
+        out = {}
+        for k, name in enumerate(T2_KERNELS):
+            a, n = C.c_double(), C.c_long()
+            check(lib().bcd_t2_kernel_times(self.h, k, C.byref(a), C.byref(n)), "bcd_t2_kernel_times")
+            out[name] = (a.value, n.value)
+        return out
+
+    def work(self):
+        w = np.zeros(3)
+        check(lib().bcd_t2_work(self.h, dp(w), 3), "bcd_t2_work")
+        return {"flops_gemm1": w[0], "flops_gemm2": w[1], "bytes_out": w[2]}
+
+
+class Context(_TwoBodyMixin):
     """One system on one device (bcd_ctx): stage-level entry points."""
 
     def __init__(self, ms: ModelSpace, vpp: Interaction, device: int = 0):

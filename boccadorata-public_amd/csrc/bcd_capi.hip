@@ -13,16 +13,10 @@
 #include <vector>
 
 #include "bcd_batch.hpp"
+#include "bcd_ctx.hpp"
 #include "bcdor_amd.h"
 
 using namespace bcd;
-
-struct bcd_ctx {
-  int device = 0;
-  HostSystem sys;
-  hipStream_t stream = nullptr;
-  DevArena scratch;
-};
 
 struct bcd_batch {
   std::vector<std::unique_ptr<HostSystem>> systems;
@@ -219,6 +213,7 @@ int bcd_ctx_create(int device, const bcd_modspace *ms, const bcd_vpp *vpp, bcd_c
 
 void bcd_ctx_destroy(bcd_ctx *ctx) {
   if (!ctx) return;
+  ctx->t2.reset();
   ctx->scratch.release();
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -51,24 +51,6 @@ __device__ __forceinline__ int prop_cur(const PropView &P, int inst) {
   return P.is ? P.is[inst].cur : 0;
 }
 
-// <ab;J|V|cd;J> from the dense J-blocks, with the exchange phase
-// (-1)^(ja+jb-J+1) for each pair given in reverse order and 0 for pairs that
-// are not stored (VppInt_t::get, Vpp_potential.cpp:557-607).
-__device__ double vget(const InstDev &I, int a, int b, int c, int d, int J) {
-  int ph = 0;
-  if (J > I.jmax) return 0.0;
-  if (a > b) { int t = a; a = b; b = t; ph += (I.j2[I.orb_sh[a]] + I.j2[I.orb_sh[b]]) / 2 - J + 1; }
-  if (c > d) { int t = c; c = d; d = t; ph += (I.j2[I.orb_sh[c]] + I.j2[I.orb_sh[d]]) / 2 - J + 1; }
-  const long base = (long)J * I.norb;
-  const int pa = I.pidx[(base + a) * I.norb + b];
-  const int pc = I.pidx[(base + c) * I.norb + d];
-  if (pa < 0 || pc < 0) return 0.0;
-  const int ca = pa >> 16, cc = pc >> 16;
-  if (ca != cc) return 0.0;
-  const double v = I.V[I.voff[ca] + (long)(pa & 0xffff) * I.npair[ca] + (pc & 0xffff)];
-  return (abs(ph) & 1) ? -v : v;
-}
-
 __device__ __forceinline__ bool tri_ok(int a2, int b2, int c2) {
   const int s1 = a2 + b2 - c2, s2 = b2 + c2 - a2, s3 = c2 + a2 - b2;
   return s1 >= 0 && s2 >= 0 && s3 >= 0 && !(s1 & 1) && !(s2 & 1) && !(s3 & 1);

@@ -1,0 +1,786 @@
+// bcd_twobody.hip -- ladder (gII_prop_t) and ring (Pol_prop_t) matrices on gfx950.
+//
+//   k_t2_zero   clears the extent Fill_*_mtx clears (gII_b.cpp:350-352, PolProp_b.cpp:310-312)
+//   k_t2_sixj   (2J'+1) {ja jb J; jg jd J'} tables (Ang_momenta.cpp:150-182) per J
+//   k_t2_amp    folded amplitude matrices T (one row per basis entry)
+//   k_t2_pandya Pandya-transformed interaction per ring channel (PolProp_b.cpp:569-614)
+//   k_t2_gemm   FP64 MFMA GEMMs G = T K and G T^T with the block epilogue
+//   k_t2_diag   free-pole energies on the diagonal (gII_b.cpp:552-556, PolProp_b.cpp:542-556)
+#include "bcd_twobody.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "bcdor_amd.h"
+
+namespace bcd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTM = 64, kTN = 64, kTK = 16, kTPad = 1;
+constexpr int kBin = 60;          // Pascal table rows (covers 2j <= 2*27)
+constexpr long kZeroChunk = 1 << 16;
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_t2_zero(const ZeroTask *tasks, const int2 *chunks) {
+  const int2 c = chunks[blockIdx.x];
+  const ZeroTask t = tasks[c.x];
+  const long lo = (long)c.y * kZeroChunk;
+  const long hi = min(t.n, lo + kZeroChunk);
+  for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) t.out[i] = 0.0;
+}
+
+// {a/2 b/2 c/2; d/2 e/2 f/2} in the binomial form of Ang_momenta.cpp:150-182.
+__device__ double s6j_dev(const double *bn, int ja, int jb, int jc, int la, int lb, int lc) {
+  auto tri = [](int a, int b, int c) {
+    const int s1 = a + b - c, s2 = b + c - a, s3 = c + a - b;
+    return s1 >= 0 && s2 >= 0 && s3 >= 0 && !((s1 | s2 | s3) & 1);
+  };
+  if (!tri(ja, jb, jc) || !tri(ja, lb, lc) || !tri(la, jb, lc) || !tri(la, lb, jc)) return 0.0;
+  const int mt = (ja + jb + jc) / 2 + 1, ma = (ja + lb + lc) / 2 + 1;
+  const int mb = (la + jb + lc) / 2 + 1, mc = (la + lb + jc) / 2 + 1;
+  const int na = mt - ja, nb = mt - jb, nc = mt - jc, ka = ma - lc, kb = mb - lc, kc = mc - jc;
+  auto B = [bn](int n, int k) { return bn[n * kBin + k]; };
+  double norm = B(mt, ja + 1) * B(ja, nc - 1) /
+                (B(ma, ja + 1) * B(ja, ka - 1) * B(mb, la + 1) * B(la, kb - 1) * B(mc, la + 1) * B(la, kc - 1));
+  norm = sqrt(norm) / (la + 1.0);
+  const int lo = max(max(mt, ma), max(mb, mc));
+  const int hi = min(min(ma + na, mb + nb), mc + nc);
+  double x = 0.0;
+  for (int l = lo; l < hi; ++l) x = -x + B(l, mt) * B(na - 1, l - ma) * B(nb - 1, l - mb) * B(nc - 1, l - mc);
+  return (double)(1 - 2 * (hi & 1)) * norm * x;
+}
+
+// tab[((slot * nj^4 + quad) * nJp + Jp)] = (2Jp+1) {ja jb J; jg jd Jp}, quad = ((a*nj+b)*nj+g)*nj+d
+__global__ void k_t2_sixj(double *tab, const int *Js, int nslot, int nj, const int *jv, int nJp, const double *bn) {
+  const long nq = (long)nj * nj * nj * nj;
+  const long total = (long)nslot * nq * nJp;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int Jp = (int)(i % nJp);
+    long q = (i / nJp) % nq;
+    const int slot = (int)(i / (nq * nJp));
+    const int d = (int)(q % nj); q /= nj;
+    const int g = (int)(q % nj); q /= nj;
+    const int b = (int)(q % nj); q /= nj;
+    const int a = (int)q;
+    tab[i] = (double)(2 * Jp + 1) * s6j_dev(bn, jv[a], jv[b], 2 * Js[slot], jv[g], jv[d], 2 * Jp);
+  }
+}
+
+// One block per (task, row): T[r][p] for all pairs p of the task.
+__global__ void k_t2_amp(const AmpTask *tasks, const int2 *rows, InstDev I, FragDev F) {
+  const int2 tr = rows[blockIdx.x];
+  const AmpTask t = tasks[tr.x];
+  const int r = tr.y;
+  const int u = t.u[r], w = t.w[r];
+  const int su = F.sub[t.ukind][u], sw = F.sub[t.wkind][w];
+  const double *xu = F.fq[t.ukind] + (long)u * F.nspb;
+  const double *xw = F.fq[t.wkind] + (long)w * F.nspb;
+  const int u0 = I.orb0[su], w0 = I.orb0[sw];
+  double *out = t.T + (long)r * t.ld;
+  for (int p = threadIdx.x; p < t.npair; p += blockDim.x) {
+    const int pr = t.pair[p];
+    const int a = pr >> 16, b = pr & 0xffff;
+    const int sa = I.orb_sh[a], sb = I.orb_sh[b];
+    double v = 0.0;
+    if (t.kind == 0) {
+      if (sa == su && sb == sw) {
+        v = xu[a - u0] * xw[b - w0];
+        if (a == b) v *= kSqrt2;
+      }
+      if (a != b && sb == su && sa == sw) {
+        const double x = xu[b - u0] * xw[a - w0];
+        const int ph = (I.j2[sa] + I.j2[sb]) / 2 - t.J + 1;
+        v += (abs(ph) & 1) ? -x : x;
+      }
+    } else if (t.kind == 1) {
+      if (sa == su && sb == sw) v = xu[a - u0] * xw[b - w0];
+    } else {
+      if (sb == su && sa == sw) v = xu[b - u0] * xw[a - w0];
+    }
+    out[p] = v;
+  }
+}
+
+// One block per (task, row p): K[p][q] for all q.
+__global__ void k_t2_pandya(const PandyaTask *tasks, const int2 *rows, InstDev I, const double *sixj,
+                            const int *jidx, int nj, int nJp) {
+  const int2 tr = rows[blockIdx.x];
+  const PandyaTask t = tasks[tr.x];
+  const int p = tr.y;
+  const int i = t.pair[p] >> 16, j = t.pair[p] & 0xffff;
+  const int ji = I.j2[I.orb_sh[i]], jj = I.j2[I.orb_sh[j]];
+  const long nq = (long)nj * nj * nj * nj;
+  const double *W = sixj + (long)t.slot * nq * nJp;
+  const int xi = jidx[ji], xj = jidx[jj];
+  double *out = t.K + (long)p * t.ld;
+  for (int q = threadIdx.x; q < t.npair; q += blockDim.x) {
+    const int k = t.pair[q] >> 16, l = t.pair[q] & 0xffff;
+    const int jk = I.j2[I.orb_sh[k]], jl = I.j2[I.orb_sh[l]];
+    const double *w = W + ((((long)xi * nj + xj) * nj + jidx[jk]) * nj + jidx[jl]) * nJp;
+    const int lo = max(abs(ji - jl), abs(jj - jk)), hi = min(ji + jl, jj + jk);
+    double v = 0.0;
+    for (int J2 = lo; J2 <= hi; J2 += 2) v -= w[J2 / 2] * vget(I, i, l, k, j, J2 / 2);
+    if (i == l) v *= kSqrt2;
+    if (k == j) v *= kSqrt2;
+    out[q] = v;
+  }
+}
+
+// C = A B^T on FP64 MFMA (v_mfma_f64_16x16x4f64).  One 64x64 tile per block of
+// four waves, each wave 32x32 (2x2 MFMA tiles); K in steps of 16 through a
+// double-buffered LDS stage.  Operand layout of the MFMA: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; it returns C[4*(l/16)+v][l%16], v = 0..3.
+template <bool EPI>
+__global__ __launch_bounds__(256) void k_t2_gemm(const GemmTask *tasks, const int4 *tiles) {
+  const int4 tl = tiles[blockIdx.x];
+  const GemmTask t = tasks[tl.x];
+  const int m0 = tl.y * kTM, n0 = tl.z * kTN;
+  __shared__ double As[2][kTM][kTK + kTPad];
+  __shared__ double Bs[2][kTN][kTK + kTPad];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int lr = tid >> 2, lk = (tid & 3) * 4;
+  const int gra = m0 + lr, grb = n0 + lr;
+  const double *Ar = t.A + (long)gra * t.lda;
+  const double *Br = t.B + (long)grb * t.ldb;
+  double ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + lk + q;
+      ra[q] = (gra < t.M && k < t.K) ? Ar[k] : 0.0;
+      rb[q] = (grb < t.N && k < t.K) ? Br[k] : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[buf][lr][lk + q] = ra[q];
+      Bs[buf][lr][lk + q] = rb[q];
+    }
+  };
+  d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  const int nk = (t.K + kTK - 1) / kTK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * kTK);
+#pragma unroll
+    for (int kk = 0; kk < kTK / 4; ++kk) {
+      const int kc = kk * 4 + (lane >> 4);
+      double a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[cur][wm + i * 16 + (lane & 15)][kc];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = Bs[cur][wn + j * 16 + (lane & 15)][kc];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + j * 16 + (lane & 15);
+      if (col >= t.N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+        if (row >= t.M) continue;
+        double y = acc[i][j][v];
+        if (!EPI) {
+          t.C[(long)row * t.ldc + col] = y;
+          continue;
+        }
+        if (t.sym && row > col) continue;
+        if (t.x3) y = t.x3[col] * y;
+        if (t.rs && t.rs[row]) y /= kSqrt2;
+        if (t.cs && t.cs[col]) y /= kSqrt2;
+        for (int k = 0; k < t.nw; ++k) {
+          const long C = t.tr[k] ? t.c0[k] + row : t.c0[k] + col;
+          const long R = t.tr[k] ? t.r0[k] + col : t.r0[k] + row;
+          t.out[C * t.ld + R] = t.sgn[k] < 0 ? -y : y;
+          if (t.sym && row != col) {
+            const long Cm = t.tr[k] ? t.c0[k] + col : t.c0[k] + row;
+            const long Rm = t.tr[k] ? t.r0[k] + row : t.r0[k] + col;
+            t.out[Cm * t.ld + Rm] = t.sgn[k] < 0 ? -y : y;
+          }
+        }
+      }
+    }
+}
+
+__global__ void k_t2_diag(const DiagTask *tasks, int ntask) {
+  for (int k = blockIdx.x; k < ntask; k += gridDim.x) {
+    const DiagTask t = tasks[k];
+    for (int i = threadIdx.x; i < t.n; i += blockDim.x) {
+      double &d = t.out[(long)(t.k0 + i) * (t.ld + 1)];
+      d = t.sgn < 0 ? d - t.e[i] : d + t.e[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------- system
+
+namespace {
+
+bool tri2(int a, int b, int c) {
+  const int s1 = a + b - c, s2 = b + c - a, s3 = c + a - b;
+  return s1 >= 0 && s2 >= 0 && s3 >= 0 && !((s1 | s2 | s3) & 1);
+}
+
+template <class T>
+int dev_copy(std::vector<void *> &bufs, const T *src, size_t n, T **dst) {
+  void *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return BCD_ERR_NOMEM;
+  bufs.push_back(p);
+  if (n && hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return BCD_ERR_DEVICE;
+  *dst = (T *)p;
+  return BCD_OK;
+}
+
+}  // namespace
+
+void T2System::release() {
+  for (void *p : bufs) (void)hipFree(p);
+  bufs.clear();
+  d_pairs = nullptr;
+  d_jidx = nullptr;
+}
+
+int T2System::upload(const HostSystem &S) {
+  release();
+  std::memset(&I, 0, sizeof I);
+  I.nsubsh = S.nsubsh; I.norb = S.norb; I.nspb = S.nspb; I.jmax = S.jmax; I.nchan2b = S.nchan2b;
+  int *p;
+  long *lp;
+  double *dp;
+  int rc;
+#define UP(field, vec)                                              \
+  if ((rc = dev_copy(bufs, vec.data(), vec.size(), &p)) != BCD_OK) \
+    return rc;                                                      \
+  I.field = p;
+  UP(l, S.l) UP(j2, S.j2) UP(ip, S.ip) UP(ch, S.ch) UP(no, S.no) UP(orb0, S.orb0) UP(orb_sh, S.orb_sh)
+  UP(pidx, S.pidx) UP(npair, S.npair)
+#undef UP
+  if ((rc = dev_copy(bufs, S.voff.data(), S.voff.size(), &lp)) != BCD_OK) return rc;
+  I.voff = lp;
+  if ((rc = dev_copy(bufs, S.V.data(), S.V.size(), &dp)) != BCD_OK) return rc;
+  I.V = dp;
+  // pair list of every dense V block, and the (J, parity, charge) -> block map
+  pairs.assign(S.nchan2b, {});
+  chan_of.assign((size_t)(S.jmax + 1) * 2 * 8, -1);
+  for (int J = 0; J <= S.jmax; ++J)
+    for (int a = 0; a < S.norb; ++a)
+      for (int b = a; b < S.norb; ++b) {
+        const int pe = S.pidx[((size_t)J * S.norb + a) * S.norb + b];
+        if (pe < 0) continue;
+        const int c = pe >> 16, row = pe & 0xffff;
+        std::vector<int> &pl = pairs[c];
+        if ((int)pl.size() <= row) pl.resize(row + 1, -1);
+        pl[row] = (a << 16) | b;
+        const int sa = S.orb_sh[a], sb = S.orb_sh[b];
+        chan_of[((size_t)J * 2 + (S.l[sa] + S.l[sb]) % 2) * 8 + (S.ch[sa] + S.ch[sb])] = c;
+      }
+  std::vector<int> flat;
+  pair_off.assign(S.nchan2b + 1, 0);
+  for (int c = 0; c < S.nchan2b; ++c) {
+    pair_off[c] = (long)flat.size();
+    flat.insert(flat.end(), pairs[c].begin(), pairs[c].end());
+  }
+  pair_off[S.nchan2b] = (long)flat.size();
+  if ((rc = dev_copy(bufs, flat.data(), flat.size(), &d_pairs)) != BCD_OK) return rc;
+  // distinct 2j values
+  int jmax2 = 0;
+  for (int s = 0; s < S.nsubsh; ++s) jmax2 = std::max(jmax2, S.j2[s]);
+  jidx.assign(jmax2 + 1, -1);
+  nj = 0;
+  for (int v = 0; v <= jmax2; ++v)
+    for (int s = 0; s < S.nsubsh; ++s)
+      if (S.j2[s] == v) { jidx[v] = nj++; break; }
+  if ((rc = dev_copy(bufs, jidx.data(), jidx.size(), &d_jidx)) != BCD_OK) return rc;
+  return BCD_OK;
+}
+
+// -------------------------------------------------------------------- bases
+
+namespace {
+
+// icut == 0 keeps every pair; otherwise (n1(u)+n1(w))*icut >= ntcut, with
+// ntcut -> 1-ntcut for icut < 0 (gII_b.cpp:67-70, PolProp_b.cpp:48-50).
+bool cut_keep(int icut, int ntcut, int n1u, int n1w) {
+  if (icut == 0) return true;
+  if (icut < 0) ntcut = 1 - ntcut;
+  return (n1u + n1w) * icut >= ntcut;
+}
+
+}  // namespace
+
+int t2_ladd_basis(const HostSystem &S, const HostFrags &F, int J, int dT, int pi, int icut, int ntcut,
+                  const int *n1p, const int *n1h, T2Job &job) {
+  job.kind = 0; job.J = J; job.dT = dT; job.pi = pi;
+  job.a1.clear(); job.a2.clear(); job.b1.clear(); job.b2.clear(); job.ea.clear(); job.eb.clear();
+  for (int side = 0; side < 2; ++side) {
+    const int n = F.n[side];
+    const int *n1 = side ? n1h : n1p;
+    if (icut && !n1) return BCD_ERR_ARG;
+    std::vector<int> &x1 = side ? job.b1 : job.a1, &x2 = side ? job.b2 : job.a2;
+    std::vector<double> &e = side ? job.eb : job.ea;
+    for (int u = 0; u < n; ++u)
+      for (int w = u; w < n; ++w) {
+        if (icut && !cut_keep(icut, ntcut, n1[u], n1[w])) continue;
+        const int su = F.sub[side][u], sw = F.sub[side][w];
+        if ((pi + S.ip[su] + S.ip[sw]) % 2) continue;
+        if (!tri2(S.j2[su], S.j2[sw], 2 * J)) continue;
+        if (u == w && (std::abs((S.j2[su] + S.j2[sw]) / 2 - J + 1) % 2)) continue;
+        if (S.ch[su] + S.ch[sw] != dT) continue;
+        x1.push_back(u);
+        x2.push_back(w);
+        e.push_back(F.e[side][u] + F.e[side][w]);
+      }
+  }
+  return BCD_OK;
+}
+
+int t2_ph_basis(const HostSystem &S, const HostFrags &F, int J, int dT, int pi, int icut, int ntcut,
+                const int *n1p, const int *n1h, T2Job &job) {
+  job.kind = 1; job.J = J; job.dT = dT; job.pi = pi;
+  job.a1.clear(); job.a2.clear(); job.b1.clear(); job.b2.clear(); job.ea.clear(); job.eb.clear();
+  if (icut && (!n1p || !n1h)) return BCD_ERR_ARG;
+  for (int u = 0; u < F.n[0]; ++u)
+    for (int w = 0; w < F.n[1]; ++w) {
+      if (icut && !cut_keep(icut, ntcut, n1p[u], n1h[w])) continue;
+      const int sa = F.sub[0][u], sb = F.sub[1][w];
+      if ((pi + S.ip[sa] + S.ip[sb]) % 2) continue;
+      if (!tri2(S.j2[sa], S.j2[sb], 2 * J)) continue;
+      const double e = F.e[0][u] - F.e[1][w];
+      if (S.ch[sa] - S.ch[sb] == dT) { job.a1.push_back(u); job.a2.push_back(w); job.ea.push_back(e); }
+      if (S.ch[sb] - S.ch[sa] == dT) { job.b1.push_back(u); job.b2.push_back(w); job.eb.push_back(e); }
+    }
+  return BCD_OK;
+}
+
+int t2_job_dims(T2Job &j) {
+  const int na = (int)j.ea.size(), nb = (int)j.eb.size();
+  const double untouched = std::nan("");
+  if (j.mode < 0 || j.mode > 4 || (j.kind == 1 && j.mode == 2)) return -100;
+  if (j.kind == 0) {
+    j.nf = j.mode == 2 ? 0 : na;
+    j.nb = j.mode == 1 ? 0 : nb;
+    j.free_poles.assign(j.nf + j.nb, untouched);
+    if (j.mode != 4) {
+      if (j.mode != 2)
+        for (int k = 0; k < na; ++k) j.free_poles[k] = j.ea[k];
+      if (j.mode != 1)
+        for (int k = 0; k < nb; ++k) j.free_poles[j.nf + k] = j.eb[k];
+    }
+  } else {
+    j.nf = na;
+    j.nb = j.mode == 1 ? 0 : nb;
+    j.free_poles.assign(j.nf + j.nb, untouched);
+    if (j.mode != 4) {
+      const bool copied = j.dT == 0 && j.mode != 1, direct = j.dT != 0 && j.mode != 1;
+      for (int k = 0; k < na; ++k) {
+        j.free_poles[k] = j.ea[k];
+        if (copied) j.free_poles[j.nf + k] = j.ea[k];
+      }
+      if (direct)
+        for (int k = 0; k < nb; ++k) j.free_poles[j.nf + k] = j.eb[k];
+    }
+  }
+  return j.nf + j.nb;
+}
+
+// --------------------------------------------------------------------- plan
+
+namespace {
+
+// Staging of one plan: a dry pass counts, the real pass hands out device
+// addresses and fills the host copies of the uploaded arrays.
+struct Bump {
+  bool dry = true;
+  int *di = nullptr;
+  double *dd = nullptr, *dws = nullptr, *dout = nullptr;
+  std::vector<int> hi;
+  std::vector<double> hd;
+  long ni = 0, nd = 0, nws = 0, nout = 0;
+  static long up(long n) { return (n + 31) & ~31L; }
+  const int *ints(const int *src, long n) {
+    const long o = ni;
+    ni += up(n);
+    if (dry) return nullptr;
+    std::copy(src, src + n, hi.begin() + o);
+    return di + o;
+  }
+  const double *dbls(const double *src, long n) {
+    const long o = nd;
+    nd += up(n);
+    if (dry) return nullptr;
+    std::copy(src, src + n, hd.begin() + o);
+    return dd + o;
+  }
+  double *ws(long n) {
+    const long o = nws;
+    nws += up(n);
+    return dry ? nullptr : dws + o;
+  }
+  double *out(long n) {
+    const long o = nout;
+    nout += up(n);
+    return dry ? nullptr : dout + o;
+  }
+};
+
+struct FragSubs {
+  const std::vector<int> &p, &h;
+};
+
+struct Emit {
+  const T2System &S;
+  const HostSystem &H;
+  Bump &B;
+  std::vector<AmpTask> &amp;
+  std::vector<PandyaTask> &pan;
+  std::vector<GemmTask> &g1, &g2;
+  std::vector<DiagTask> &diag;
+  std::vector<ZeroTask> &zero;
+  std::vector<int> &sixj_J;
+  double &f1, &f2, &nout;
+
+  int slot(int J) {
+    for (size_t k = 0; k < sixj_J.size(); ++k)
+      if (sixj_J[k] == J) return (int)k;
+    sixj_J.push_back(J);
+    return (int)sixj_J.size() - 1;
+  }
+
+  double *amp_task(int kind, int J, const std::vector<int> &u, const std::vector<int> &w, int ukind, int wkind,
+                   const int *pair, int npair) {
+    AmpTask t;
+    std::memset(&t, 0, sizeof t);
+    t.kind = kind; t.J = J; t.nrows = (int)u.size(); t.npair = npair; t.ld = npair;
+    t.ukind = ukind; t.wkind = wkind;
+    t.u = B.ints(u.data(), (long)u.size());
+    t.w = B.ints(w.data(), (long)w.size());
+    t.pair = pair;
+    t.T = B.ws((long)t.nrows * npair);
+    amp.push_back(t);
+    return t.T;
+  }
+
+  double *gemm1(const double *A, int M, const double *K, int P) {
+    GemmTask t;
+    std::memset(&t, 0, sizeof t);
+    t.M = M; t.N = P; t.K = P; t.lda = P; t.ldb = P; t.ldc = P;
+    t.A = A; t.B = K;
+    t.C = B.ws((long)M * P);
+    g1.push_back(t);
+    f1 += 2.0 * M * P * P;
+    return t.C;
+  }
+
+  void gemm2(const double *G, int M, const double *Tc, int N, int P, double *out, int ld, const int *rs,
+             const int *cs, const double *x3, int nw, const int (&w)[2][4], bool sym = false) {
+    GemmTask t;
+    std::memset(&t, 0, sizeof t);
+    t.M = M; t.N = N; t.K = P; t.lda = P; t.ldb = P; t.sym = sym;
+    t.A = G; t.B = Tc; t.out = out; t.ld = ld; t.nw = nw;
+    for (int k = 0; k < nw; ++k) {
+      t.c0[k] = w[k][0]; t.r0[k] = w[k][1]; t.tr[k] = w[k][2]; t.sgn[k] = w[k][3];
+    }
+    t.rs = rs; t.cs = cs; t.x3 = x3;
+    g2.push_back(t);
+    f2 += (sym ? 1.0 : 2.0) * M * N * P;
+    nout += (double)nw * M * N;
+  }
+
+  void energies(double *out, int ld, const std::vector<double> &e, int k0, int sgn) {
+    if (e.empty()) return;
+    DiagTask d;
+    d.out = out; d.ld = ld; d.k0 = k0; d.n = (int)e.size(); d.sgn = sgn;
+    d.e = B.dbls(e.data(), (long)e.size());
+    diag.push_back(d);
+  }
+
+  const int *eq_flags(const std::vector<int> &u, const std::vector<int> &w) {
+    std::vector<int> f(u.size());
+    for (size_t k = 0; k < u.size(); ++k) f[k] = u[k] == w[k];
+    return B.ints(f.data(), (long)f.size());
+  }
+
+  // x3 of PolProp_b.cpp:330-331: -1 when (|2j_g - 2j_d| / 2) is odd, for the
+  // (qp g, qh d) of each column entry
+  const double *x3_of(const std::vector<int> &g, const std::vector<int> &d, const FragSubs &F) {
+    std::vector<double> x(g.size());
+    for (size_t k = 0; k < g.size(); ++k) {
+      const int sg = F.p[g[k]], sd = F.h[d[k]];
+      x[k] = ((std::abs(H.j2[sg] - H.j2[sd]) / 2) % 2) ? -1.0 : 1.0;
+    }
+    return B.dbls(x.data(), (long)x.size());
+  }
+
+
+  void ring_pairs(int J, int pi, int dT, std::vector<int> &out) const {
+    out.clear();
+    for (int i = 0; i < H.norb; ++i)
+      for (int j = 0; j < H.norb; ++j) {
+        const int si = H.orb_sh[i], sj = H.orb_sh[j];
+        if ((pi + H.ip[si] + H.ip[sj]) % 2) continue;
+        if (!tri2(H.j2[si], H.j2[sj], 2 * J)) continue;
+        if (H.ch[si] - H.ch[sj] != dT) continue;
+        out.push_back((i << 16) | j);
+      }
+  }
+};
+
+}  // namespace
+
+void T2Plan::release() {
+  for (void *p : bufs) (void)hipFree(p);
+  bufs.clear();
+  d_out = nullptr;
+  d_sixj = nullptr;
+}
+
+int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, hipStream_t st) {
+  release();
+  for (T2Job &j : jobs)
+    if (t2_job_dims(j) < 0) return BCD_ERR_ARG;
+  Bump B;
+  std::vector<int> ring_list;
+  auto emit_all = [&](std::vector<int> &pairs_host_out) {
+    amp.clear(); pan.clear(); g1.clear(); g2.clear(); diag.clear(); zero.clear(); sixj_J.clear();
+    flops_g1 = flops_g2 = out_doubles = 0.0;
+    pairs_host_out.clear();
+    Emit E{S, H, B, amp, pan, g1, g2, diag, zero, sixj_J, flops_g1, flops_g2, out_doubles};
+    for (T2Job &j : jobs) {
+      const int ntot = j.nf + j.nb;
+      if (j.ld < std::max(ntot, 1)) j.ld = std::max(ntot, 1);
+      double *out = B.out((long)std::max(ntot, 1) * j.ld);
+      j.out_off = B.dry ? 0 : (long)(out - B.dout);
+      if (ntot > 0) zero.push_back(ZeroTask{out, (long)(ntot - 1) * j.ld + ntot});
+      const int nf = j.nf, ld = j.ld;
+      const bool add = j.add || j.mode == 4;
+      if (j.kind == 0) {
+        const bool make_pp = j.mode != 2, make_hh = j.mode != 1, make_b = j.mode == 3;
+        if (j.mode != 4) {
+          const size_t key = ((size_t)j.J * 2 + j.pi) * 8 + j.dT;
+          const int c = (j.J <= H.jmax && j.dT >= 0 && j.dT < 8) ? S.chan_of[key] : -1;
+          const int P = c >= 0 ? H.npair[c] : 0;
+          if (P > 0) {
+            const int *pairs = S.d_pairs + S.pair_off[c];
+            const double *K = S.I.V + H.voff[c];
+            const int npp = (int)j.ea.size(), nhh = (int)j.eb.size();
+            const double *Tpp = nullptr, *Thh = nullptr, *Gpp = nullptr, *Ghh = nullptr;
+            const int *eqp = nullptr, *eqh = nullptr;
+            if ((make_pp || make_b) && npp) {
+              Tpp = E.amp_task(0, j.J, j.a1, j.a2, 0, 0, pairs, P);
+              eqp = E.eq_flags(j.a1, j.a2);
+            }
+            if ((make_hh || make_b) && nhh) {
+              Thh = E.amp_task(0, j.J, j.b1, j.b2, 1, 1, pairs, P);
+              eqh = E.eq_flags(j.b1, j.b2);
+            }
+            if (make_pp && npp) Gpp = E.gemm1(Tpp, npp, K, P);
+            if ((make_hh || make_b) && nhh) Ghh = E.gemm1(Thh, nhh, K, P);
+            if (make_pp && npp)
+              E.gemm2(Gpp, npp, Tpp, npp, P, out, ld, eqp, eqp, nullptr, 1, {{0, 0, 0, 1}, {0, 0, 0, 0}}, true);
+            if (make_b && npp && nhh)
+              E.gemm2(Ghh, nhh, Tpp, npp, P, out, ld, eqh, eqp, nullptr, 2, {{0, nf, 0, -1}, {nf, 0, 1, 1}});
+            if (make_hh && nhh)
+              E.gemm2(Ghh, nhh, Thh, nhh, P, out, ld, eqh, eqh, nullptr, 1, {{nf, nf, 0, -1}, {0, 0, 0, 0}}, true);
+          }
+        }
+        if (add) {
+          if (make_pp) E.energies(out, ld, j.ea, 0, 1);
+          if (make_hh) E.energies(out, ld, j.eb, nf, 1);
+        }
+      } else {
+        const bool direct = j.dT != 0 && j.mode != 1, copied = j.dT == 0 && j.mode != 1;
+        const bool make_b = j.mode == 3, t0 = copied && make_b;
+        const int nph = (int)j.ea.size(), nhp = (int)j.eb.size();
+        FragSubs FS{F.sub[0], F.sub[1]};
+        if (j.mode != 4) {
+          std::vector<int> qp, qm;
+          E.ring_pairs(j.J, j.pi, j.dT, qp);
+          const int Pp = (int)qp.size();
+          if (Pp > 0 && nph > 0) {
+            const int *dqp = B.ints(qp.data(), Pp);
+            PandyaTask pt{j.J, Pp, Pp, E.slot(j.J), dqp, B.ws((long)Pp * Pp)};
+            pan.push_back(pt);
+            const double *Tph = E.amp_task(1, j.J, j.a1, j.a2, 0, 1, dqp, Pp);
+            const double *Gph = E.gemm1(Tph, nph, pt.K, Pp);
+            if (copied)
+              E.gemm2(Gph, nph, Tph, nph, Pp, out, ld, nullptr, nullptr, nullptr, 2, {{0, 0, 0, 1}, {nf, nf, 0, -1}},
+                      true);
+            else
+              E.gemm2(Gph, nph, Tph, nph, Pp, out, ld, nullptr, nullptr, nullptr, 1, {{0, 0, 0, 1}, {0, 0, 0, 0}},
+                      true);
+            if (t0) {
+              const double *Tt = E.amp_task(2, j.J, j.a1, j.a2, 0, 1, dqp, Pp);
+              E.gemm2(Gph, nph, Tt, nph, Pp, out, ld, nullptr, nullptr, E.x3_of(j.a1, j.a2, FS), 2,
+                      {{0, nf, 0, -1}, {nf, 0, 0, 1}});
+            }
+            if (direct && make_b && nhp > 0) {
+              const double *Tt = E.amp_task(2, j.J, j.b1, j.b2, 0, 1, dqp, Pp);
+              E.gemm2(Gph, nph, Tt, nhp, Pp, out, ld, nullptr, nullptr, E.x3_of(j.b1, j.b2, FS), 2,
+                      {{nf, 0, 0, 1}, {0, nf, 1, -1}});
+            }
+          }
+          if (direct && nhp > 0) {
+            E.ring_pairs(j.J, j.pi, -j.dT, qm);
+            const int Pm = (int)qm.size();
+            if (Pm > 0) {
+              const int *dqm = B.ints(qm.data(), Pm);
+              PandyaTask pt{j.J, Pm, Pm, E.slot(j.J), dqm, B.ws((long)Pm * Pm)};
+              pan.push_back(pt);
+              const double *Thp = E.amp_task(1, j.J, j.b1, j.b2, 0, 1, dqm, Pm);
+              const double *Ghp = E.gemm1(Thp, nhp, pt.K, Pm);
+              E.gemm2(Ghp, nhp, Thp, nhp, Pm, out, ld, nullptr, nullptr, nullptr, 1, {{nf, nf, 0, -1}, {0, 0, 0, 0}},
+                      true);
+            }
+          }
+        }
+        if (add) {
+          E.energies(out, ld, j.ea, 0, 1);
+          if (j.dT == 0 && j.mode != 1) E.energies(out, ld, j.ea, nf, -1);
+          if (direct) E.energies(out, ld, j.eb, nf, -1);
+        }
+      }
+    }
+  };
+  std::vector<int> unused;
+  B.dry = true;
+  emit_all(unused);
+  // allocate
+  auto dmalloc = [&](size_t bytes, void **p) {
+    if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return BCD_ERR_NOMEM;
+    bufs.push_back(*p);
+    return BCD_OK;
+  };
+  void *pi, *pd, *pw, *po;
+  int rc;
+  if ((rc = dmalloc(sizeof(int) * B.ni, &pi)) || (rc = dmalloc(sizeof(double) * B.nd, &pd)) ||
+      (rc = dmalloc(sizeof(double) * B.nws, &pw)) || (rc = dmalloc(sizeof(double) * B.nout, &po)))
+    return rc;
+  const long ni = B.ni, nd = B.nd;
+  B = Bump();
+  B.dry = false;
+  B.di = (int *)pi; B.dd = (double *)pd; B.dws = (double *)pw; B.dout = (double *)po;
+  B.hi.assign(ni, 0);
+  B.hd.assign(nd, 0.0);
+  emit_all(unused);
+  d_out = B.dout;
+  out_len = B.nout;
+  if (hipMemcpyAsync(pi, B.hi.data(), sizeof(int) * B.hi.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(pd, B.hd.data(), sizeof(double) * B.hd.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+    return BCD_ERR_DEVICE;
+  // 6j tables: the Racah sum of arguments up to 2j = jmax2 (orbits) and
+  // 2J, 2J' <= 2 jmax2 reaches binomial rows 2 jmax2 + 1
+  if (!sixj_J.empty() && 2 * ((int)S.jidx.size() - 1) + 2 >= kBin) return BCD_ERR_CAPACITY;
+  if (!sixj_J.empty()) {
+    std::vector<double> bn(kBin * kBin, 0.0);
+    for (int n = 0; n < kBin; ++n) { bn[n * kBin] = 1.0; bn[n * kBin + n] = 1.0; }
+    for (int n = 1; n < kBin - 1; ++n)
+      for (int k = 1; k <= n; ++k) bn[(n + 1) * kBin + k] = bn[n * kBin + k - 1] + bn[n * kBin + k];
+    std::vector<int> jv(S.nj);
+    for (size_t v = 0; v < S.jidx.size(); ++v)
+      if (S.jidx[v] >= 0) jv[S.jidx[v]] = (int)v;
+    double *dbn;
+    int *djv, *dJs;
+    std::vector<void *> tmp;
+    if ((rc = dev_copy(bufs, bn.data(), bn.size(), &dbn)) || (rc = dev_copy(bufs, jv.data(), jv.size(), &djv)) ||
+        (rc = dev_copy(bufs, sixj_J.data(), sixj_J.size(), &dJs)))
+      return rc;
+    const int nJp = H.jmax + 1;
+    const long n = (long)sixj_J.size() * S.nj * S.nj * S.nj * S.nj * nJp;
+    void *p6;
+    if ((rc = dmalloc(sizeof(double) * n, &p6))) return rc;
+    d_sixj = (double *)p6;
+    hipLaunchKernelGGL(k_t2_sixj, dim3((unsigned)std::min<long>((n + 255) / 256, 4096)), dim3(256), 0, st, d_sixj,
+                       dJs, (int)sixj_J.size(), S.nj, djv, nJp, dbn);
+    if (hipGetLastError() != hipSuccess) return BCD_ERR_DEVICE;
+  }
+  // task lists and launch tables
+  auto up_tasks = [&](auto &vec, auto **dst) {
+    using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+    T *p = nullptr;
+    int r = dev_copy(bufs, vec.data(), vec.size(), &p);
+    *dst = p;
+    return r;
+  };
+  std::vector<int2> arows, prows, zchunks;
+  for (size_t k = 0; k < amp.size(); ++k)
+    for (int r = 0; r < amp[k].nrows; ++r) arows.push_back(int2{(int)k, r});
+  for (size_t k = 0; k < pan.size(); ++k)
+    for (int r = 0; r < pan[k].npair; ++r) prows.push_back(int2{(int)k, r});
+  for (size_t k = 0; k < zero.size(); ++k)
+    for (long c = 0; c * kZeroChunk < zero[k].n; ++c) zchunks.push_back(int2{(int)k, (int)c});
+  auto tiles = [](const std::vector<GemmTask> &g) {
+    std::vector<int4> t;
+    for (size_t k = 0; k < g.size(); ++k)
+      for (int m = 0; m * kTM < g[k].M; ++m)
+        for (int n = g[k].sym ? m : 0; n * kTN < g[k].N; ++n) t.push_back(int4{(int)k, m, n, 0});
+    return t;
+  };
+  std::vector<int4> t1 = tiles(g1), t2 = tiles(g2);
+  int2 *dar, *dpr, *dzc;
+  int4 *dt1, *dt2;
+  if ((rc = up_tasks(amp, &d_amp)) || (rc = up_tasks(pan, &d_pan)) || (rc = up_tasks(g1, &d_g1)) ||
+      (rc = up_tasks(g2, &d_g2)) || (rc = up_tasks(diag, &d_diag)) || (rc = up_tasks(zero, &d_zero)) ||
+      (rc = dev_copy(bufs, arows.data(), arows.size(), &dar)) ||
+      (rc = dev_copy(bufs, prows.data(), prows.size(), &dpr)) ||
+      (rc = dev_copy(bufs, zchunks.data(), zchunks.size(), &dzc)) ||
+      (rc = dev_copy(bufs, t1.data(), t1.size(), &dt1)) || (rc = dev_copy(bufs, t2.data(), t2.size(), &dt2)))
+    return rc;
+  d_amp_rows = (int *)dar; amp_rows = (long)arows.size();
+  d_pan_rows = (int *)dpr; pan_rows = (long)prows.size();
+  d_zero_chunks = dzc; zero_chunks = (long)zchunks.size();
+  d_t1 = (int *)dt1; n_t1 = (long)t1.size();
+  d_t2 = (int *)dt2; n_t2 = (long)t2.size();
+  if (hipStreamSynchronize(st) != hipSuccess) return BCD_ERR_DEVICE;
+  return BCD_OK;
+}
+
+int T2Plan::launch(const T2System &S, const FragDev &frag, hipStream_t st, hipEvent_t *ev) const {
+  auto mark = [&](int k) {
+    if (ev) (void)hipEventRecord(ev[k], st);
+  };
+  mark(0);
+  if (zero_chunks)
+    hipLaunchKernelGGL(k_t2_zero, dim3((unsigned)zero_chunks), dim3(256), 0, st, d_zero, d_zero_chunks);
+  mark(1);
+  if (amp_rows)
+    hipLaunchKernelGGL(k_t2_amp, dim3((unsigned)amp_rows), dim3(128), 0, st, d_amp, (const int2 *)d_amp_rows, S.I,
+                       frag);
+  mark(2);
+  if (pan_rows)
+    hipLaunchKernelGGL(k_t2_pandya, dim3((unsigned)pan_rows), dim3(128), 0, st, d_pan, (const int2 *)d_pan_rows,
+                       S.I, d_sixj, S.d_jidx, S.nj, S.I.jmax + 1);
+  mark(3);
+  if (n_t1) hipLaunchKernelGGL(k_t2_gemm<false>, dim3((unsigned)n_t1), dim3(256), 0, st, d_g1, (const int4 *)d_t1);
+  mark(4);
+  if (n_t2) hipLaunchKernelGGL(k_t2_gemm<true>, dim3((unsigned)n_t2), dim3(256), 0, st, d_g2, (const int4 *)d_t2);
+  mark(5);
+  if (!diag.empty())
+    hipLaunchKernelGGL(k_t2_diag, dim3((unsigned)std::min<size_t>(diag.size(), 1024)), dim3(128), 0, st, d_diag,
+                       (int)diag.size());
+  mark(6);
+  return hipGetLastError() == hipSuccess ? BCD_OK : BCD_ERR_DEVICE;
+}
+
+}  // namespace bcd

@@ -189,6 +189,80 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
  * roots, Loewner couplings, eigenvectors, reorder, split and output, norms. */
 int bcd_batch_work(bcd_batch *b, double *work, int n);
 
+/* ------------------------------------------ two-body propagator matrices
+ * The pp/hh ladder of gII_prop_t (gII_b.cpp:62-563) and the ph ring of
+ * Pol_prop_t (PolProp_b.cpp:43-614) built from the fragments of g, as
+ * CoupledCluster_t uses them (CCD_a.cpp:158-240).  Fragment indices count
+ * quasiparticles (quasiholes) over all subshells in storage order, as
+ * SpProp_t's n_qp (n_qh) arrays do.  A channel is (J, dT, pif): total angular
+ * momentum, total charge (ladder) or charge difference (ring), parity.
+ * n1p / n1h are SpProp_t's per-fragment labels for the selection
+ * (icut, ntcut); they may be NULL when icut == 0.
+ * Matrices are stored as the reference stores them: entry (c, r) at
+ * Sab[c * ndim_vec_alloc + r]; a call writes exactly the extent the
+ * reference writes ([0, (ndim-1)*ndim_vec_alloc + ndim)).  free_poles (may be
+ * NULL) receives the entries Fill_*_mtx writes.  i_RPAcalc: 0 (D)TDA of both
+ * blocks, 1 first block only, 2 second block only (ladder), 3 (D)RPA, 4 free
+ * propagator.  Returns 0, -100 (invalid i_RPAcalc), -101 (ndim_vec_alloc < 1),
+ * -102 (ndim > ndim_vec_alloc, where the reference stops the program) or a
+ * BCD_ERR_ code.  Every call is independent: the reference keeps the
+ * free-propagator switch of i_RPAcalc = 4 in a file-scope static that stays
+ * set for the rest of the process; this library does not. */
+
+/* gII_prop_t::Count_LaddRPA_basis (gII_b.cpp:62-113) */
+int bcd_ladd_count(bcd_ctx *ctx, const bcd_prop *g, int J, int dT, int pif, int icut, int ntcut,
+                   const int *n1p, const int *n1h, int *npp, int *nhh);
+/* gII_prop_t::Build_LaddRPA_basis (gII_b.cpp:115-200): 2qp pairs na <= nb,
+ * then 2qh pairs ka <= kb, with e_pp = ep[na] + ep[nb], e_hh = eh[ka] + eh[kb]. */
+int bcd_ladd_basis(bcd_ctx *ctx, const bcd_prop *g, int J, int dT, int pif, int icut, int ntcut,
+                   const int *n1p, const int *n1h, int cap_pp, int cap_hh, int *n1_pp, int *n2_pp,
+                   double *e_pp, int *k1_hh, int *k2_hh, double *e_hh, int *npp, int *nhh);
+/* gII_prop_t::Fill_LaddRPA_mtx (gII_b.cpp:288-563) */
+int bcd_ladd_fill(bcd_ctx *ctx, const bcd_prop *g, int J, int dT, int pif, int npp, const int *n1_pp,
+                  const int *n2_pp, const double *e_pp, int nhh, const int *k1_hh, const int *k2_hh,
+                  const double *e_hh, int i_RPAcalc, double *Sab, int ndim_vec_alloc,
+                  double *free_poles, int add_free_poles);
+/* Pol_prop_t::Count_phRPA_basis (PolProp_b.cpp:43-94) */
+int bcd_ph_count(bcd_ctx *ctx, const bcd_prop *g, int J, int dT, int pif, int icut, int ntcut,
+                 const int *n1p, const int *n1h, int *nph, int *nhp);
+/* Pol_prop_t::Build_phRPA_basis (PolProp_b.cpp:96-175): every (qp na, qh kb),
+ * ph when ch(a) - ch(b) == dT, hp when ch(b) - ch(a) == dT, e = ep[na] - eh[kb]. */
+int bcd_ph_basis(bcd_ctx *ctx, const bcd_prop *g, int J, int dT, int pif, int icut, int ntcut,
+                 const int *n1p, const int *n1h, int cap_ph, int cap_hp, int *n_ph, int *k_ph,
+                 double *e_ph, int *n_hp, int *k_hp, double *e_hp, int *nph, int *nhp);
+/* Pol_prop_t::Fill_phRPA_mtx (PolProp_b.cpp:255-563) */
+int bcd_ph_fill(bcd_ctx *ctx, const bcd_prop *g, int J, int dT, int pif, int nph, const int *n_ph,
+                const int *k_ph, const double *e_ph, int nhp, const int *n_hp, const int *k_hp,
+                const double *e_hp, int i_RPAcalc, double *Sab, int ndim_vec_alloc,
+                double *free_poles, int add_free_poles);
+
+/* Every channel at once (the loops of CoupledCluster_t, CCD_a.cpp:158-240):
+ * for J = 0..Jmax and both parities, the ladder of each total charge and the
+ * ring of each charge difference whose basis is not empty, with i_RPAcalc
+ * ladd_mode / ring_mode (-1: leave that family out).  The fragments and all
+ * matrices stay resident on the device; bcd_t2_build recomputes them. */
+typedef struct bcd_t2 bcd_t2;
+int bcd_t2_create(bcd_ctx *ctx, const bcd_prop *g, int ladd_mode, int ring_mode, int add_free_poles,
+                  bcd_t2 **out);
+void bcd_t2_destroy(bcd_t2 *t);
+int bcd_t2_count(const bcd_t2 *t);
+/* info[10] = {kind (0 ladder, 1 ring), J, pif, dT, i_RPAcalc, n_f, n_b, ld, n_a, n_b_basis}:
+ * the matrix has dimension n_f + n_b and leading dimension ld; the basis has
+ * n_a entries in its first block (2qp / ph) and n_b_basis in its second. */
+int bcd_t2_info(const bcd_t2 *t, int k, int *info);
+/* basis of matrix k: ladder n1_pp n2_pp e_pp k1_hh k2_hh e_hh, ring n_ph k_ph
+ * e_ph n_hp k_hp e_hp (NULL to skip an array) */
+int bcd_t2_basis(const bcd_t2 *t, int k, int *a1, int *a2, double *ea, int *b1, int *b2, double *eb);
+int bcd_t2_build(bcd_t2 *t);
+int bcd_t2_matrix(const bcd_t2 *t, int k, double *Sab, double *free_poles);
+/* 1: time each kernel family of bcd_t2_build with hipEvents (resets the sums) */
+int bcd_t2_set_profile(bcd_t2 *t, int on);
+/* k = 0 "t2_zero", 1 "t2_amp", 2 "t2_pandya", 3 "t2_gemm1", 4 "t2_gemm2", 5 "t2_diag" */
+int bcd_t2_kernel_times(const bcd_t2 *t, int k, double *avg_ms, long *launches);
+/* work[0] FLOPs of the G = T K products, [1] of the G T^T products (2 M N K
+ * each, unpadded), [2] bytes of matrix entries written by the products; n >= 3 */
+int bcd_t2_work(const bcd_t2 *t, double *work, int n);
+
 #ifdef __cplusplus
 }
 #endif
