@@ -176,69 +176,79 @@ def t2_propagator(ms, seed=1):
     return formats.Propagator(n, n.copy(), np.array(ep), (fp ** 2).sum(1), fp, np.array(eh), (fh ** 2).sum(1), fh)
 
 
-def t2_cpu_baseline(ms, vpp, g, t, seconds):
-    """The oracle's ladder/ring fills (the reference's fourfold orbit loops)
-    on a sample of the build's channels, smallest first, one channel per
-    thread, until the time budget is spent; the rate in orbit-quadruple terms
-    per second is extrapolated to the whole build."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-    S = po.System(ms, vpp)
-    P = po.Prop.from_propagator(S, g)
+def t2_orbit_terms(ms, g, t):
+    """Orbit terms of the reference's fill per matrix (entries x no(a) no(b)
+    no(g) no(d)), keyed by (kind, J, pi, dT): the work measure used to carry
+    a timed sample of channels over to the whole build."""
     ms_no = np.asarray(ms.no)
     sub_p = np.repeat(np.arange(ms.nsubsh), np.asarray(g.np_))
     sub_h = np.repeat(np.arange(ms.nsubsh), np.asarray(g.nh))
-
-    def terms(inf, b):
-        # orbit terms of every entry: rows x cols of no(a)*no(b) products
+    out = {}
+    for k in range(t.n):
+        inf = t.info(k)
+        b = t.basis(k)
         if inf["kind"] == 0:
             wa = ms_no[sub_p[b["n1_pp"]]] * ms_no[sub_p[b["n2_pp"]]]
             wb = ms_no[sub_h[b["k1_hh"]]] * ms_no[sub_h[b["k2_hh"]]]
         else:
             wa = ms_no[sub_p[b["n_ph"]]] * ms_no[sub_h[b["k_ph"]]]
             wb = ms_no[sub_p[b["n_hp"]]] * ms_no[sub_h[b["k_hp"]]]
-        w = np.concatenate([wa, wb]).astype(np.float64)
-        return float(w.sum() ** 2)
-    jobs = []
-    total_terms = 0.0
-    for k in range(t.n):
-        inf = t.info(k)
-        b = t.basis(k)
-        tt = terms(inf, b)
-        total_terms += tt
-        jobs.append((tt, inf, b))
-    jobs.sort(key=lambda x: x[0])
-    threads = min(16, os.cpu_count() or 1)
-    lock = threading.Lock()
-    done = []
-    t_end = time.perf_counter() + seconds
-    nxt = [0]
+        w = float(np.concatenate([wa, wb]).astype(np.float64).sum())
+        out[(inf["kind"], inf["J"], inf["pi"], inf["dT"])] = w * w
+    return out
 
-    def worker():
-        while time.perf_counter() < t_end:
-            with lock:
-                if nxt[0] >= len(jobs):
-                    return
-                tt, inf, b = jobs[nxt[0]]
-                nxt[0] += 1
+
+def t2_cpu_baseline(ms, vpp, g, t, seconds):
+    """CPU baseline of the two-body build: the compiled reference
+    (oracle/_ref/ref_harness_2b, gII_b.cpp and PolProp_b.cpp as they are)
+    fills the mode-3 matrices channel by channel in CoupledCluster_t's loop
+    order for a bounded time, single-threaded (the reference keeps its state
+    in file-scope statics).  Its rate in orbit terms per second is carried
+    over to the whole build.  Without the harness the oracle's restatement is
+    timed the same way (kind "port")."""
+    import subprocess
+    import tempfile
+    from bcdor_amd import formats
+    terms = t2_orbit_terms(ms, g, t)
+    total = sum(terms.values())
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness_2b")
+    done, wall, kind = [], 0.0, "reference"
+    if os.path.exists(harness):
+        with tempfile.TemporaryDirectory() as td:
+            formats.write_model_space(os.path.join(td, "msp"), ms)
+            formats.write_vpp_bcd(os.path.join(td, "v.bcd"), ms, vpp)
+            formats.write_propagator(os.path.join(td, "prop"), ms, g)
+            subprocess.run([harness, os.path.join(td, "msp"), os.path.join(td, "v.bcd"), os.path.join(td, "prop"),
+                            os.path.join(td, "times"), "time", str(seconds)], check=True, cwd=td,
+                           stdout=subprocess.DEVNULL)
+            with open(os.path.join(td, "times")) as f:
+                for line in f:
+                    kd, J, pi, dT, _n, sec = line.split()
+                    done.append(terms[(int(kd), int(J), int(pi), int(dT))])
+                    wall += float(sec)
+    else:
+        kind = "port"
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        S = po.System(ms, vpp)
+        P = po.Prop.from_propagator(S, g)
+        t_end = time.perf_counter() + seconds
+        for k in range(t.n):
+            if time.perf_counter() > t_end:
+                break
+            inf = t.info(k)
+            b = t.basis(k)
+            t0 = time.perf_counter()
             if inf["kind"] == 0:
-                po.ladd_fill(S, P, inf["J"], b, inf["mode"], False)
+                po.ladd_fill(S, P, inf["J"], b, 3, False)
             else:
-                po.ph_fill(S, P, inf["J"], inf["dT"], b, inf["mode"], False)
-            with lock:
-                done.append(tt)
-
-    t0 = time.perf_counter()
-    pool = [threading.Thread(target=worker) for _ in range(threads)]
-    for th in pool:
-        th.start()
-    for th in pool:
-        th.join()
-    wall = time.perf_counter() - t0
-    rate = sum(done) / wall
-    return {"value": rate / total_terms, "unit": "builds/s", "cores": threads, "kind": "port",
-            "sample": f"{len(done)} of {t.n} channel matrices (smallest first), {sum(done):.3g} of "
-                      f"{total_terms:.3g} orbit terms in {wall:.1f} s wall; builds/s = terms/s / terms per build"}
+                po.ph_fill(S, P, inf["J"], inf["dT"], b, 3, False)
+            wall += time.perf_counter() - t0
+            done.append(terms[(inf["kind"], inf["J"], inf["pi"], inf["dT"])])
+    rate = sum(done) / wall if wall > 0 else 0.0
+    return {"value": rate / total, "unit": "builds/s", "cores": 1, "kind": kind,
+            "sample": f"{len(done)} of {t.n} channel matrices in loop order ({sum(done):.3g} of {total:.3g} "
+                      f"orbit terms) in {wall:.1f} s of fills; builds/s = (terms/s) / terms per build"}
 
 
 def main_twobody(args):

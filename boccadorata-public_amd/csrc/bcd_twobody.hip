@@ -132,8 +132,9 @@ __global__ void k_t2_pandya(const PandyaTask *tasks, const int2 *rows, InstDev I
 
 // C = A B^T on FP64 MFMA (v_mfma_f64_16x16x4f64).  One 64x64 tile per block of
 // four waves, each wave 32x32 (2x2 MFMA tiles); K in steps of 16 through a
-// double-buffered LDS stage.  Operand layout of the MFMA: lane l supplies
-// A[l%16][l/16] and B[l/16][l%16]; it returns C[4*(l/16)+v][l%16], v = 0..3.
+// double-buffered LDS stage.  Operand layout of the MFMA (measured on gfx950):
+// lane l supplies A[l%16][l/16] and B[l/16][l%16] and receives
+// C[l/16 + 4v][l%16] in item v = 0..3.
 template <bool EPI>
 __global__ __launch_bounds__(256) void k_t2_gemm(const GemmTask *tasks, const int4 *tiles) {
   const int4 tl = tiles[blockIdx.x];
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void k_t2_gemm(const GemmTask *tasks, const in
       if (col >= t.N) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+        const int row = m0 + wm + i * 16 + (lane >> 4) + 4 * v;
         if (row >= t.M) continue;
         double y = acc[i][j][v];
         if (!EPI) {
@@ -460,6 +461,7 @@ struct Emit {
   std::vector<ZeroTask> &zero;
   std::vector<int> &sixj_J;
   double &f1, &f2, &nout;
+  long area = 0;                   // output entries the current job's epilogues write
 
   int slot(int J) {
     for (size_t k = 0; k < sixj_J.size(); ++k)
@@ -506,6 +508,7 @@ struct Emit {
     g2.push_back(t);
     f2 += (sym ? 1.0 : 2.0) * M * N * P;
     nout += (double)nw * M * N;
+    area += (long)nw * M * N;
   }
 
   void energies(double *out, int ld, const std::vector<double> &e, int k0, int sgn) {
@@ -572,7 +575,7 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
       if (j.ld < std::max(ntot, 1)) j.ld = std::max(ntot, 1);
       double *out = B.out((long)std::max(ntot, 1) * j.ld);
       j.out_off = B.dry ? 0 : (long)(out - B.dout);
-      if (ntot > 0) zero.push_back(ZeroTask{out, (long)(ntot - 1) * j.ld + ntot});
+      E.area = 0;
       const int nf = j.nf, ld = j.ld;
       const bool add = j.add || j.mode == 4;
       if (j.kind == 0) {
@@ -661,6 +664,11 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
           if (direct) E.energies(out, ld, j.eb, nf, -1);
         }
       }
+      // the extent Fill_*_mtx clears, unless the products write all of it
+      // (every block present and no padding between rows); the clearing
+      // kernel runs before the products
+      if (ntot > 0 && !(E.area == (long)ntot * ntot && ld == ntot))
+        zero.push_back(ZeroTask{out, (long)(ntot - 1) * ld + ntot});
     }
   };
   std::vector<int> unused;

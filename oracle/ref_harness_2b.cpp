@@ -6,6 +6,12 @@
 // reference functions and prints their outputs with %.17g.
 //
 //   ref_harness_2b <model_space> <Vpp.bcd> <sp_prop> <out.txt> <jmodes>
+//   ref_harness_2b <model_space> <Vpp.bcd> <sp_prop> <out.txt> time <seconds>
+//
+// The second form is the CPU baseline of the two-body benchmark: it builds
+// the mode-3 matrices of every channel with a non-empty basis (ladder, then
+// ring, in the CoupledCluster_t loop order) until <seconds> have passed and
+// prints one line per finished channel, "kind J pi dT ndim seconds".
 //
 // For every two-body channel (J <= Jmax, parity, charge) it prints the ladder
 // basis (gII_prop_t::Build_LaddRPA_basis, gII_b.cpp:115-200) and the ring basis
@@ -22,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <fstream>
 #include <iostream>
 #include <vector>
@@ -78,9 +85,47 @@ static void put_fill(FILE *f, const char *tag, T &X, int J, int pi, int dT, int 
   put_dbls(f, &fp[0], ndim);
 }
 
+static double now_s() {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+// Timed builds of every channel's mode-3 matrices (no output of the matrices).
+static void time_builds(FILE *out, ModSpace_t &msp, VppInt_t &vpp, SpProp_t &g, double budget) {
+  const double t_end = now_s() + budget;
+  for (int J = 0; J <= msp.Jmax; ++J)
+    for (int pi = 0; pi < 2; ++pi) {
+      for (int dT = msp.ch_pp_mn; dT <= msp.ch_pp_mx; ++dT) {
+        if (now_s() > t_end) return;
+        gII_prop_t L(&msp, &g, &vpp);
+        L.Build_LaddRPA_basis(J, dT, pi);
+        const int n = (L.n_pp_basis > 0 ? L.n_pp_basis : 0) + (L.n_hh_basis > 0 ? L.n_hh_basis : 0);
+        if (n == 0) continue;
+        vector<double> buf((size_t)n * n), fp(n);
+        const double t0 = now_s();
+        L.Fill_LaddRPA_mtx(3, &buf[0], n, &fp[0], false);
+        fprintf(out, "0 %d %d %d %d %.6f\n", J, pi, dT, n, now_s() - t0);
+        fflush(out);
+      }
+      for (int dT = msp.ch_ph_mn; dT <= msp.ch_ph_mx; ++dT) {
+        if (now_s() > t_end) return;
+        Pol_prop_t R(&msp, &g, &vpp);
+        R.Build_phRPA_basis(J, dT, pi);
+        const int n = (R.n_ph_basis > 0 ? R.n_ph_basis : 0) + (R.n_hp_basis > 0 ? R.n_hp_basis : 0);
+        if (n == 0) continue;
+        vector<double> buf((size_t)n * n), fp(n);
+        const double t0 = now_s();
+        R.Fill_phRPA_mtx(3, &buf[0], n, &fp[0], false);
+        fprintf(out, "1 %d %d %d %d %.6f\n", J, pi, dT, n, now_s() - t0);
+        fflush(out);
+      }
+    }
+}
+
 int main(int argc, char **argv) {
   if (argc < 6) {
-    fprintf(stderr, "usage: %s msp vpp prop out jmodes\n", argv[0]);
+    fprintf(stderr, "usage: %s msp vpp prop out jmodes | time seconds\n", argv[0]);
     return 2;
   }
   BcDor_Init();
@@ -91,8 +136,14 @@ int main(int argc, char **argv) {
   msp.Calculate_Jch_bounds();
   VppInt_t vpp(argv[2], &msp);
   SpProp_t g(argv[3], &msp);
-  const int jmodes = atoi(argv[5]);
   FILE *out = fopen(argv[4], "w");
+  if (strcmp(argv[5], "time") == 0) {
+    time_builds(out, msp, vpp, g, argc > 6 ? atof(argv[6]) : 10.0);
+    fclose(out);
+    cout.rdbuf(keep);
+    return 0;
+  }
+  const int jmodes = atoi(argv[5]);
   fprintf(out, "bounds %d %d %d %d %d\n", msp.Jmax, msp.ch_pp_mn, msp.ch_pp_mx, msp.ch_ph_mn, msp.ch_ph_mx);
 
   for (int pass = 0; pass < 2; ++pass) {     // pass 1: the free-propagator mode only
