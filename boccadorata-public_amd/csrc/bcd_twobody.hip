@@ -19,7 +19,9 @@ namespace bcd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTM = 64, kTN = 64, kTK = 16, kTPad = 1;
+constexpr int kTK = 16, kTPad = 1;   // K step of the GEMMs
+constexpr int kBigTile = 128;   // block tile of tasks whose both dimensions reach it (else 64)
+constexpr bool kUseBigTiles = false;
 constexpr int kBin = 60;          // Pascal table rows (covers 2j <= 2*27)
 constexpr long kZeroChunk = 1 << 16;
 
@@ -130,78 +132,82 @@ __global__ void k_t2_pandya(const PandyaTask *tasks, const int2 *rows, InstDev I
   }
 }
 
-// C = A B^T on FP64 MFMA (v_mfma_f64_16x16x4f64).  One 64x64 tile per block of
-// four waves, each wave 32x32 (2x2 MFMA tiles); K in steps of 16 through a
-// double-buffered LDS stage.  Operand layout of the MFMA (measured on gfx950):
-// lane l supplies A[l%16][l/16] and B[l/16][l%16] and receives
-// C[l/16 + 4v][l%16] in item v = 0..3.
-template <bool EPI>
+// C = A B^T on FP64 MFMA (v_mfma_f64_16x16x4f64).  Block of four waves in a
+// 2x2 arrangement; each wave owns TI x TI MFMA tiles of 16x16, so the block
+// tile is (32 TI) x (32 TI): 64 for TI = 2, 128 for TI = 4.  K advances in
+// steps of TK through a double-buffered LDS stage.  Operand layout of the
+// MFMA (measured on gfx950): lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// and receives C[l/16 + 4v][l%16] in item v = 0..3.
+template <bool EPI, int TI, int TK>
 __global__ __launch_bounds__(256) void k_t2_gemm(const GemmTask *tasks, const int4 *tiles) {
+  constexpr int BT = 32 * TI;                 // block tile edge
+  constexpr int LPT = BT * TK / 256;          // doubles per thread per operand per K step
+  constexpr int TPR = TK / LPT;               // threads per tile row
   const int4 tl = tiles[blockIdx.x];
   const GemmTask t = tasks[tl.x];
-  const int m0 = tl.y * kTM, n0 = tl.z * kTN;
-  __shared__ double As[2][kTM][kTK + kTPad];
-  __shared__ double Bs[2][kTN][kTK + kTPad];
+  const int m0 = tl.y * BT, n0 = tl.z * BT;
+  __shared__ double As[2][BT][TK + kTPad];
+  __shared__ double Bs[2][BT][TK + kTPad];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const int lr = tid >> 2, lk = (tid & 3) * 4;
-  const int gra = m0 + lr, grb = n0 + lr;
-  const double *Ar = t.A + (long)gra * t.lda;
-  const double *Br = t.B + (long)grb * t.ldb;
-  double ra[4], rb[4];
+  const int wm = (wave >> 1) * 16 * TI, wn = (wave & 1) * 16 * TI;
+  const int lr = tid / TPR, lk = (tid % TPR) * LPT;
+  const double *Ar = t.A + (long)(m0 + lr) * t.lda;
+  const double *Br = t.B + (long)(n0 + lr) * t.ldb;
+  double ra[LPT], rb[LPT];
   auto gload = [&](int k0) {
+    const double2 *pa = (const double2 *)(Ar + k0 + lk), *pb = (const double2 *)(Br + k0 + lk);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int k = k0 + lk + q;
-      ra[q] = (gra < t.M && k < t.K) ? Ar[k] : 0.0;
-      rb[q] = (grb < t.N && k < t.K) ? Br[k] : 0.0;
+    for (int q = 0; q < LPT / 2; ++q) {
+      const double2 x = pa[q], y = pb[q];
+      ra[2 * q] = x.x; ra[2 * q + 1] = x.y;
+      rb[2 * q] = y.x; rb[2 * q + 1] = y.y;
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < LPT; ++q) {
       As[buf][lr][lk + q] = ra[q];
       Bs[buf][lr][lk + q] = rb[q];
     }
   };
-  d4 acc[2][2];
+  d4 acc[TI][TI];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  const int nk = (t.K + kTK - 1) / kTK;
+    for (int j = 0; j < TI; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  const int nk = (t.K + TK - 1) / TK;
   gload(0);
   sstore(0);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * kTK);
+    if (kt + 1 < nk) gload((kt + 1) * TK);
 #pragma unroll
-    for (int kk = 0; kk < kTK / 4; ++kk) {
+    for (int kk = 0; kk < TK / 4; ++kk) {
       const int kc = kk * 4 + (lane >> 4);
-      double a[2], b[2];
+      double a[TI], b[TI];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[cur][wm + i * 16 + (lane & 15)][kc];
+      for (int i = 0; i < TI; ++i) a[i] = As[cur][wm + i * 16 + (lane & 15)][kc];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[cur][wn + j * 16 + (lane & 15)][kc];
+      for (int j = 0; j < TI; ++j) b[j] = Bs[cur][wn + j * 16 + (lane & 15)][kc];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TI; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TI; ++j) {
       const int col = n0 + wn + j * 16 + (lane & 15);
-      if (col >= t.N) continue;
+      if (col >= t.Nv) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = m0 + wm + i * 16 + (lane >> 4) + 4 * v;
-        if (row >= t.M) continue;
+        if (row >= t.Mv) continue;
         double y = acc[i][j][v];
         if (!EPI) {
           t.C[(long)row * t.ldc + col] = y;
@@ -470,26 +476,38 @@ struct Emit {
     return (int)sixj_J.size() - 1;
   }
 
+  // Operand padding: T [r64(rows) x r16(P)], K [r64(P) x r16(P)],
+  // G [r64(rows) x r64(P)]; the workspace is zeroed once, and the padding is
+  // never written except by GEMM1, which writes exact zeros there.
+  static int r16(int n) { return (n + kTK - 1) / kTK * kTK; }          // K padding
+  static int r64(int n) {                                              // row padding: the block tile
+    const int bt = kUseBigTiles ? kBigTile : kBigTile / 2;
+    return (n + bt - 1) / bt * bt;
+  }
+
   double *amp_task(int kind, int J, const std::vector<int> &u, const std::vector<int> &w, int ukind, int wkind,
                    const int *pair, int npair) {
     AmpTask t;
     std::memset(&t, 0, sizeof t);
-    t.kind = kind; t.J = J; t.nrows = (int)u.size(); t.npair = npair; t.ld = npair;
+    t.kind = kind; t.J = J; t.nrows = (int)u.size(); t.npair = npair; t.ld = r16(npair);
     t.ukind = ukind; t.wkind = wkind;
     t.u = B.ints(u.data(), (long)u.size());
     t.w = B.ints(w.data(), (long)w.size());
     t.pair = pair;
-    t.T = B.ws((long)t.nrows * npair);
+    t.T = B.ws((long)r64(t.nrows) * t.ld);
     amp.push_back(t);
     return t.T;
   }
 
+  double *kmat(int P) { return B.ws((long)r64(P) * r16(P)); }
+
   double *gemm1(const double *A, int M, const double *K, int P) {
     GemmTask t;
     std::memset(&t, 0, sizeof t);
-    t.M = M; t.N = P; t.K = P; t.lda = P; t.ldb = P; t.ldc = P;
+    t.M = r64(M); t.N = r64(P); t.K = r16(P); t.lda = r16(P); t.ldb = r16(P); t.ldc = r64(P);
+    t.Mv = t.M; t.Nv = t.N;
     t.A = A; t.B = K;
-    t.C = B.ws((long)M * P);
+    t.C = B.ws((long)t.M * t.ldc);
     g1.push_back(t);
     f1 += 2.0 * M * P * P;
     return t.C;
@@ -499,7 +517,8 @@ struct Emit {
              const int *cs, const double *x3, int nw, const int (&w)[2][4], bool sym = false) {
     GemmTask t;
     std::memset(&t, 0, sizeof t);
-    t.M = M; t.N = N; t.K = P; t.lda = P; t.ldb = P; t.sym = sym;
+    t.M = r64(M); t.N = r64(N); t.K = r16(P); t.lda = r64(P); t.ldb = r16(P); t.sym = sym;
+    t.Mv = M; t.Nv = N;
     t.A = G; t.B = Tc; t.out = out; t.ld = ld; t.nw = nw;
     for (int k = 0; k < nw; ++k) {
       t.c0[k] = w[k][0]; t.r0[k] = w[k][1]; t.tr[k] = w[k][2]; t.sgn[k] = w[k][3];
@@ -566,7 +585,7 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
   Bump B;
   std::vector<int> ring_list;
   auto emit_all = [&](std::vector<int> &pairs_host_out) {
-    amp.clear(); pan.clear(); g1.clear(); g2.clear(); diag.clear(); zero.clear(); sixj_J.clear();
+    amp.clear(); pan.clear(); g1.clear(); g2.clear(); diag.clear(); zero.clear(); sixj_J.clear(); vcopy.clear();
     flops_g1 = flops_g2 = out_doubles = 0.0;
     pairs_host_out.clear();
     Emit E{S, H, B, amp, pan, g1, g2, diag, zero, sixj_J, flops_g1, flops_g2, out_doubles};
@@ -586,7 +605,8 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
           const int P = c >= 0 ? H.npair[c] : 0;
           if (P > 0) {
             const int *pairs = S.d_pairs + S.pair_off[c];
-            const double *K = S.I.V + H.voff[c];
+            double *K = E.kmat(P);
+            vcopy.push_back(VCopy{K, S.I.V + H.voff[c], P, Emit::r16(P)});
             const int npp = (int)j.ea.size(), nhh = (int)j.eb.size();
             const double *Tpp = nullptr, *Thh = nullptr, *Gpp = nullptr, *Ghh = nullptr;
             const int *eqp = nullptr, *eqh = nullptr;
@@ -623,7 +643,7 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
           const int Pp = (int)qp.size();
           if (Pp > 0 && nph > 0) {
             const int *dqp = B.ints(qp.data(), Pp);
-            PandyaTask pt{j.J, Pp, Pp, E.slot(j.J), dqp, B.ws((long)Pp * Pp)};
+            PandyaTask pt{j.J, Pp, Emit::r16(Pp), E.slot(j.J), dqp, E.kmat(Pp)};
             pan.push_back(pt);
             const double *Tph = E.amp_task(1, j.J, j.a1, j.a2, 0, 1, dqp, Pp);
             const double *Gph = E.gemm1(Tph, nph, pt.K, Pp);
@@ -649,7 +669,7 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
             const int Pm = (int)qm.size();
             if (Pm > 0) {
               const int *dqm = B.ints(qm.data(), Pm);
-              PandyaTask pt{j.J, Pm, Pm, E.slot(j.J), dqm, B.ws((long)Pm * Pm)};
+              PandyaTask pt{j.J, Pm, Emit::r16(Pm), E.slot(j.J), dqm, E.kmat(Pm)};
               pan.push_back(pt);
               const double *Thp = E.amp_task(1, j.J, j.b1, j.b2, 0, 1, dqm, Pm);
               const double *Ghp = E.gemm1(Thp, nhp, pt.K, Pm);
@@ -695,8 +715,13 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
   d_out = B.dout;
   out_len = B.nout;
   if (hipMemcpyAsync(pi, B.hi.data(), sizeof(int) * B.hi.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(pd, B.hd.data(), sizeof(double) * B.hd.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+      hipMemcpyAsync(pd, B.hd.data(), sizeof(double) * B.hd.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemsetAsync(pw, 0, sizeof(double) * std::max<long>(B.nws, 1), st) != hipSuccess)
     return BCD_ERR_DEVICE;
+  for (const VCopy &v : vcopy)
+    if (hipMemcpy2DAsync(v.dst, sizeof(double) * v.ld, v.src, sizeof(double) * v.P, sizeof(double) * v.P, v.P,
+                         hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return BCD_ERR_DEVICE;
   // 6j tables: the Racah sum of arguments up to 2j = jmax2 (orbits) and
   // 2J, 2J' <= 2 jmax2 reaches binomial rows 2 jmax2 + 1
   if (!sixj_J.empty() && 2 * ((int)S.jidx.size() - 1) + 2 >= kBin) return BCD_ERR_CAPACITY;
@@ -738,14 +763,23 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
     for (int r = 0; r < pan[k].npair; ++r) prows.push_back(int2{(int)k, r});
   for (size_t k = 0; k < zero.size(); ++k)
     for (long c = 0; c * kZeroChunk < zero[k].n; ++c) zchunks.push_back(int2{(int)k, (int)c});
-  auto tiles = [](const std::vector<GemmTask> &g) {
+  // tiles of the 64 (small) and 128 (big) kernels; big tiles for tasks whose
+  // both dimensions reach 128
+  auto tiles = [](const std::vector<GemmTask> &g, bool big) {
     std::vector<int4> t;
-    for (size_t k = 0; k < g.size(); ++k)
-      for (int m = 0; m * kTM < g[k].M; ++m)
-        for (int n = g[k].sym ? m : 0; n * kTN < g[k].N; ++n) t.push_back(int4{(int)k, m, n, 0});
+    for (size_t k = 0; k < g.size(); ++k) {
+      if ((kUseBigTiles && std::min(g[k].M, g[k].N) >= kBigTile) != big) continue;
+      const int bt = big ? kBigTile : kBigTile / 2;
+      for (int m = 0; m * bt < g[k].M; ++m)
+        for (int n = g[k].sym ? m : 0; n * bt < g[k].N; ++n) t.push_back(int4{(int)k, m, n, 0});
+    }
     return t;
   };
-  std::vector<int4> t1 = tiles(g1), t2 = tiles(g2);
+  std::vector<int4> t1 = tiles(g1, false), t2 = tiles(g2, false), t1b = tiles(g1, true), t2b = tiles(g2, true);
+  n_t1b = (long)t1b.size();
+  n_t2b = (long)t2b.size();
+  t1.insert(t1.end(), t1b.begin(), t1b.end());
+  t2.insert(t2.end(), t2b.begin(), t2b.end());
   int2 *dar, *dpr, *dzc;
   int4 *dt1, *dt2;
   if ((rc = up_tasks(amp, &d_amp)) || (rc = up_tasks(pan, &d_pan)) || (rc = up_tasks(g1, &d_g1)) ||
@@ -758,8 +792,8 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
   d_amp_rows = (int *)dar; amp_rows = (long)arows.size();
   d_pan_rows = (int *)dpr; pan_rows = (long)prows.size();
   d_zero_chunks = dzc; zero_chunks = (long)zchunks.size();
-  d_t1 = (int *)dt1; n_t1 = (long)t1.size();
-  d_t2 = (int *)dt2; n_t2 = (long)t2.size();
+  d_t1 = (int *)dt1; n_t1 = (long)t1.size() - n_t1b;
+  d_t2 = (int *)dt2; n_t2 = (long)t2.size() - n_t2b;
   if (hipStreamSynchronize(st) != hipSuccess) return BCD_ERR_DEVICE;
   return BCD_OK;
 }
@@ -780,9 +814,12 @@ int T2Plan::launch(const T2System &S, const FragDev &frag, hipStream_t st, hipEv
     hipLaunchKernelGGL(k_t2_pandya, dim3((unsigned)pan_rows), dim3(128), 0, st, d_pan, (const int2 *)d_pan_rows,
                        S.I, d_sixj, S.d_jidx, S.nj, S.I.jmax + 1);
   mark(3);
-  if (n_t1) hipLaunchKernelGGL(k_t2_gemm<false>, dim3((unsigned)n_t1), dim3(256), 0, st, d_g1, (const int4 *)d_t1);
+  const int4 *t1 = (const int4 *)d_t1, *t2 = (const int4 *)d_t2;
+  if (n_t1b) hipLaunchKernelGGL((k_t2_gemm<false, 4, kTK>), dim3((unsigned)n_t1b), dim3(256), 0, st, d_g1, t1 + n_t1);
+  if (n_t1) hipLaunchKernelGGL((k_t2_gemm<false, 2, kTK>), dim3((unsigned)n_t1), dim3(256), 0, st, d_g1, t1);
   mark(4);
-  if (n_t2) hipLaunchKernelGGL(k_t2_gemm<true>, dim3((unsigned)n_t2), dim3(256), 0, st, d_g2, (const int4 *)d_t2);
+  if (n_t2b) hipLaunchKernelGGL((k_t2_gemm<true, 4, kTK>), dim3((unsigned)n_t2b), dim3(256), 0, st, d_g2, t2 + n_t2);
+  if (n_t2) hipLaunchKernelGGL((k_t2_gemm<true, 2, kTK>), dim3((unsigned)n_t2), dim3(256), 0, st, d_g2, t2);
   mark(5);
   if (!diag.empty())
     hipLaunchKernelGGL(k_t2_diag, dim3((unsigned)std::min<size_t>(diag.size(), 1024)), dim3(128), 0, st, d_diag,

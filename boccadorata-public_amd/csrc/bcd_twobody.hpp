@@ -55,8 +55,10 @@ struct PandyaTask {
   double *K;                      // [npair x ld]
 };
 
-// C = A B^T (A [M x K], B [N x K], both K-contiguous).  With C != nullptr a
-// plain store; otherwise the epilogue: y = x3[c] * C(r, c), then y /= sqrt2
+// C = A B^T (A [M x K], B [N x K], both K-contiguous).  M, N are multiples
+// of the block tile and K of the K step; the operands are zero-padded to
+// them, so the loads need no bounds.  Mv, Nv bound what is written.  With
+// C != nullptr a plain store; otherwise the epilogue: y = x3[c] * C(r, c), then y /= sqrt2
 // for rs[r] and for cs[c], written as sgn * y at entry (Cidx, Ridx) =
 // (c0 + c, r0 + r), or (c0 + r, r0 + c) when transposed, of the output
 // matrix (stored as out[Cidx * ld + Ridx]); up to two writes per element.
@@ -65,7 +67,7 @@ struct PandyaTask {
 // r <= c is also written at the mirrored position (c, r).
 struct GemmTask {
   int M, N, K, lda, ldb, ldc;
-  int sym, _pad;
+  int Mv, Nv, sym, _pad;
   const double *A, *B;
   double *C;
   double *out;
@@ -134,12 +136,15 @@ struct T2Plan {
   std::vector<DiagTask> diag;
   std::vector<ZeroTask> zero;
   std::vector<int> sixj_J;         // J of each 6j-table slot
+  struct VCopy { double *dst; const double *src; int P, ld; };
+  std::vector<VCopy> vcopy;        // dense V blocks copied into zero-padded GEMM operands
   double *d_sixj = nullptr;
   // device copies of the task lists and their launch tables
   AmpTask *d_amp = nullptr; int *d_amp_rows = nullptr; long amp_rows = 0;
   PandyaTask *d_pan = nullptr; int *d_pan_rows = nullptr; long pan_rows = 0;
   GemmTask *d_g1 = nullptr, *d_g2 = nullptr;
-  int *d_t1 = nullptr, *d_t2 = nullptr; long n_t1 = 0, n_t2 = 0;
+  int *d_t1 = nullptr, *d_t2 = nullptr; long n_t1 = 0, n_t2 = 0;   // 64-tiles first, then n_t1b / n_t2b 128-tiles
+  long n_t1b = 0, n_t2b = 0;
   DiagTask *d_diag = nullptr;
   ZeroTask *d_zero = nullptr;
   int2 *d_zero_chunks = nullptr; long zero_chunks = 0;
