@@ -20,6 +20,7 @@ namespace bcd {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTK = 16, kTPad = 1;   // K step of the GEMMs
+constexpr int kNBuf = 1;             // LDS stages of the GEMM K loop
 constexpr int kBigTile = 128;   // block tile of tasks whose both dimensions reach it (else 64)
 constexpr bool kUseBigTiles = false;
 constexpr int kBin = 60;          // Pascal table rows (covers 2j <= 2*27)
@@ -138,16 +139,22 @@ __global__ void k_t2_pandya(const PandyaTask *tasks, const int2 *rows, InstDev I
 // steps of TK through a double-buffered LDS stage.  Operand layout of the
 // MFMA (measured on gfx950): lane l supplies A[l%16][l/16] and B[l/16][l%16]
 // and receives C[l/16 + 4v][l%16] in item v = 0..3.
-template <bool EPI, int TI, int TK>
-__global__ __launch_bounds__(256) void k_t2_gemm(const GemmTask *tasks, const int4 *tiles) {
+template <bool EPI, int TI, int TK, int NBUF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_t2_gemm(const GemmTask *tasks,
+                                                                                           const int4 *tiles) {
   constexpr int BT = 32 * TI;                 // block tile edge
   constexpr int LPT = BT * TK / 256;          // doubles per thread per operand per K step
   constexpr int TPR = TK / LPT;               // threads per tile row
+  constexpr int RP = TK + 2;                  // LDS row pitch (doubles): 16 lanes of a read hit distinct banks
+  static_assert(TK % 8 == 0 && LPT % 2 == 0, "K step in whole 16-byte runs per lane");
   const int4 tl = tiles[blockIdx.x];
   const GemmTask t = tasks[tl.x];
   const int m0 = tl.y * BT, n0 = tl.z * BT;
-  __shared__ double As[2][BT][TK + kTPad];
-  __shared__ double Bs[2][BT][TK + kTPad];
+  // Within a row, k = 4 kk + q is stored at q * (TK/4) + kk: the TK/4
+  // operands a lane feeds to the MFMAs of a K step (same q, every kk) are
+  // adjacent.
+  __shared__ double As[NBUF][BT][RP];
+  __shared__ double Bs[NBUF][BT][RP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 16 * TI, wn = (wave & 1) * 16 * TI;
   const int lr = tid / TPR, lk = (tid % TPR) * LPT;
@@ -163,11 +170,13 @@ __global__ __launch_bounds__(256) void k_t2_gemm(const GemmTask *tasks, const in
       rb[2 * q] = y.x; rb[2 * q + 1] = y.y;
     }
   };
+  // this thread holds k = lk + q, q < LPT
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < LPT; ++q) {
-      As[buf][lr][lk + q] = ra[q];
-      Bs[buf][lr][lk + q] = rb[q];
+      const int k = lk + q, pos = (k & 3) * (TK / 4) + (k >> 2);
+      As[buf][lr][pos] = ra[q];
+      Bs[buf][lr][pos] = rb[q];
     }
   };
   d4 acc[TI][TI];
@@ -177,25 +186,48 @@ __global__ __launch_bounds__(256) void k_t2_gemm(const GemmTask *tasks, const in
     for (int j = 0; j < TI; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   const int nk = (t.K + TK - 1) / TK;
   gload(0);
-  sstore(0);
-  __syncthreads();
+  if (NBUF == 2) {
+    sstore(0);
+    __syncthreads();
+  }
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
+    // two LDS stages: compute on one while the other is refilled (one barrier
+    // per K step); one stage: refill, barrier, compute, barrier
+    const int cur = NBUF == 2 ? (kt & 1) : 0;
+    if (NBUF == 1) {
+      sstore(0);
+      __syncthreads();
+    }
     if (kt + 1 < nk) gload((kt + 1) * TK);
+    constexpr int KK = TK / 4;
+    const int q4 = (lane >> 4) * KK;
+    double a[TI][KK], b[TI][KK];
 #pragma unroll
-    for (int kk = 0; kk < TK / 4; ++kk) {
-      const int kc = kk * 4 + (lane >> 4);
-      double a[TI], b[TI];
+    for (int i = 0; i < TI; ++i) {
+      const double2 *p = (const double2 *)&As[cur][wm + i * 16 + (lane & 15)][q4];
 #pragma unroll
-      for (int i = 0; i < TI; ++i) a[i] = As[cur][wm + i * 16 + (lane & 15)][kc];
+      for (int h = 0; h < KK / 2; ++h) {
+        const double2 x = p[h];
+        a[i][2 * h] = x.x; a[i][2 * h + 1] = x.y;
+      }
+    }
 #pragma unroll
-      for (int j = 0; j < TI; ++j) b[j] = Bs[cur][wn + j * 16 + (lane & 15)][kc];
+    for (int j = 0; j < TI; ++j) {
+      const double2 *p = (const double2 *)&Bs[cur][wn + j * 16 + (lane & 15)][q4];
+#pragma unroll
+      for (int h = 0; h < KK / 2; ++h) {
+        const double2 x = p[h];
+        b[j][2 * h] = x.x; b[j][2 * h + 1] = x.y;
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < TK / 4; ++kk)
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < TI; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-    if (kt + 1 < nk) sstore(cur ^ 1);
+        for (int j = 0; j < TI; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][kk], b[j][kk], acc[i][j], 0, 0, 0);
+    if (NBUF == 2 && kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
   }
 #pragma unroll
@@ -815,11 +847,11 @@ int T2Plan::launch(const T2System &S, const FragDev &frag, hipStream_t st, hipEv
                        S.I, d_sixj, S.d_jidx, S.nj, S.I.jmax + 1);
   mark(3);
   const int4 *t1 = (const int4 *)d_t1, *t2 = (const int4 *)d_t2;
-  if (n_t1b) hipLaunchKernelGGL((k_t2_gemm<false, 4, kTK>), dim3((unsigned)n_t1b), dim3(256), 0, st, d_g1, t1 + n_t1);
-  if (n_t1) hipLaunchKernelGGL((k_t2_gemm<false, 2, kTK>), dim3((unsigned)n_t1), dim3(256), 0, st, d_g1, t1);
+  if (n_t1b) hipLaunchKernelGGL((k_t2_gemm<false, 4, kTK, kNBuf>), dim3((unsigned)n_t1b), dim3(256), 0, st, d_g1, t1 + n_t1);
+  if (n_t1) hipLaunchKernelGGL((k_t2_gemm<false, 2, kTK, kNBuf>), dim3((unsigned)n_t1), dim3(256), 0, st, d_g1, t1);
   mark(4);
-  if (n_t2b) hipLaunchKernelGGL((k_t2_gemm<true, 4, kTK>), dim3((unsigned)n_t2b), dim3(256), 0, st, d_g2, t2 + n_t2);
-  if (n_t2) hipLaunchKernelGGL((k_t2_gemm<true, 2, kTK>), dim3((unsigned)n_t2), dim3(256), 0, st, d_g2, t2);
+  if (n_t2b) hipLaunchKernelGGL((k_t2_gemm<true, 4, kTK, kNBuf>), dim3((unsigned)n_t2b), dim3(256), 0, st, d_g2, t2 + n_t2);
+  if (n_t2) hipLaunchKernelGGL((k_t2_gemm<true, 2, kTK, kNBuf>), dim3((unsigned)n_t2), dim3(256), 0, st, d_g2, t2);
   mark(5);
   if (!diag.empty())
     hipLaunchKernelGGL(k_t2_diag, dim3((unsigned)std::min<size_t>(diag.size(), 1024)), dim3(128), 0, st, d_diag,
