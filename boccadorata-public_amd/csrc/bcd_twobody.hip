@@ -23,6 +23,8 @@ constexpr int kTK = 16, kTPad = 1;   // K step of the GEMMs
 constexpr int kNBuf = 1;             // LDS stages of the GEMM K loop
 constexpr int kBigTile = 128;   // block tile of tasks whose both dimensions reach it (else 64)
 constexpr bool kUseBigTiles = false;
+constexpr int kXcd = 8;          // XCDs of an MI355X
+constexpr int kPatch = 8;        // tile patch edge kept on one XCD
 constexpr int kBin = 60;          // Pascal table rows (covers 2j <= 2*27)
 constexpr long kZeroChunk = 1 << 16;
 
@@ -148,13 +150,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   constexpr int RP = TK + 2;                  // LDS row pitch (doubles): 16 lanes of a read hit distinct banks
   static_assert(TK % 8 == 0 && LPT % 2 == 0, "K step in whole 16-byte runs per lane");
   const int4 tl = tiles[blockIdx.x];
+  if (tl.x < 0) return;            // idle entry of the XCD-interleaved tile list
   const GemmTask t = tasks[tl.x];
   const int m0 = tl.y * BT, n0 = tl.z * BT;
   // Within a row, k = 4 kk + q is stored at q * (TK/4) + kk: the TK/4
   // operands a lane feeds to the MFMAs of a K step (same q, every kk) are
   // adjacent.
-  __shared__ double As[NBUF][BT][RP];
-  __shared__ double Bs[NBUF][BT][RP];
+  __shared__ double smem[2 * NBUF * BT * RP];
+  double(*As)[BT][RP] = reinterpret_cast<double(*)[BT][RP]>(smem);
+  double(*Bs)[BT][RP] = reinterpret_cast<double(*)[BT][RP]>(smem + NBUF * BT * RP);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 16 * TI, wn = (wave & 1) * 16 * TI;
   const int lr = tid / TPR, lk = (tid % TPR) * LPT;
@@ -229,6 +233,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][kk], b[j][kk], acc[i][j], 0, 0, 0);
     if (NBUF == 2 && kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
+  }
+  if constexpr (EPI && TI == 2) {
+    // Staged epilogue: half a tile (32 rows) at a time through LDS, then each
+    // placement written with consecutive lanes on consecutive addresses
+    // (along the tile rows or along the tile columns, whichever is
+    // contiguous in the output for that placement).
+    constexpr int HR = 32, YP = BT + 1;
+    static_assert(2 * NBUF * BT * RP >= HR * YP, "LDS stage too small for half a tile");
+    double *Y = smem;
+    for (int h = 0; h < 2; ++h) {
+      if (wm == h * HR) {
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+          for (int j = 0; j < TI; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              Y[(i * 16 + (lane >> 4) + 4 * v) * YP + wn + j * 16 + (lane & 15)] = acc[i][j][v];
+      }
+      __syncthreads();
+      const int rbase = m0 + h * HR;
+      for (int k = 0; k < t.nw; ++k) {
+        const bool tr = t.tr[k] != 0;
+        const long c0 = t.c0[k], r0 = t.r0[k];
+        const bool neg = t.sgn[k] < 0;
+        for (int mirror = 0; mirror < (t.sym ? 2 : 1); ++mirror) {
+          const bool rowfast = tr == (mirror != 0);
+          for (int it = 0; it < HR * BT / 256; ++it) {
+            const int e = it * 256 + tid;
+            const int rl = rowfast ? (e % HR) : (e / BT);
+            const int cl = rowfast ? (e / HR) : (e % BT);
+            const int row = rbase + rl, col = n0 + cl;
+            if (row >= t.Mv || col >= t.Nv) continue;
+            if (t.sym && (row > col || (mirror && row == col))) continue;
+            double y = Y[rl * YP + cl];
+            if (t.x3) y = t.x3[col] * y;
+            if (t.rs && t.rs[row]) y /= kSqrt2;
+            if (t.cs && t.cs[col]) y /= kSqrt2;
+            const bool swap = tr != (mirror != 0);      // entry (C, R) = (c0 + row, r0 + col)
+            const long C = swap ? c0 + row : c0 + col;
+            const long R = swap ? r0 + col : r0 + row;
+            t.out[C * t.ld + R] = neg ? -y : y;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    return;
   }
 #pragma unroll
   for (int i = 0; i < TI; ++i)
@@ -797,14 +849,36 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
     for (long c = 0; c * kZeroChunk < zero[k].n; ++c) zchunks.push_back(int2{(int)k, (int)c});
   // tiles of the 64 (small) and 128 (big) kernels; big tiles for tasks whose
   // both dimensions reach 128
+  // Blocks are handed to the XCDs round-robin by blockIdx.  Tiles are
+  // grouped into patches of kPatch x kPatch; each patch goes whole to one XCD
+  // (least loaded first), so the operand bands a patch reads stay in that
+  // XCD's L2.  The per-XCD queues are interleaved, padded with idle entries
+  // (task -1).
   auto tiles = [](const std::vector<GemmTask> &g, bool big) {
-    std::vector<int4> t;
+    std::vector<std::vector<int4>> q(kXcd);
+    std::vector<long> load(kXcd, 0);
+    std::vector<int4> patch;
     for (size_t k = 0; k < g.size(); ++k) {
       if ((kUseBigTiles && std::min(g[k].M, g[k].N) >= kBigTile) != big) continue;
       const int bt = big ? kBigTile : kBigTile / 2;
-      for (int m = 0; m * bt < g[k].M; ++m)
-        for (int n = g[k].sym ? m : 0; n * bt < g[k].N; ++n) t.push_back(int4{(int)k, m, n, 0});
+      const int mt = (g[k].M + bt - 1) / bt, nt = (g[k].N + bt - 1) / bt;
+      for (int pm = 0; pm < mt; pm += kPatch)
+        for (int pn = g[k].sym ? pm : 0; pn < nt; pn += kPatch) {
+          patch.clear();
+          for (int m = pm; m < std::min(mt, pm + kPatch); ++m)
+            for (int n = pn; n < std::min(nt, pn + kPatch); ++n)
+              if (!g[k].sym || n >= m) patch.push_back(int4{(int)k, m, n, 0});
+          if (patch.empty()) continue;
+          const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+          q[x].insert(q[x].end(), patch.begin(), patch.end());
+          load[x] += (long)patch.size();
+        }
     }
+    size_t len = 0;
+    for (const auto &v : q) len = std::max(len, v.size());
+    std::vector<int4> t(len * kXcd, int4{-1, 0, 0, 0});
+    for (int x = 0; x < kXcd; ++x)
+      for (size_t i = 0; i < q[x].size(); ++i) t[i * kXcd + x] = q[x][i];
     return t;
   };
   std::vector<int4> t1 = tiles(g1, false), t2 = tiles(g2, false), t1b = tiles(g1, true), t2b = tiles(g2, true);
