@@ -174,6 +174,9 @@ __device__ ArrowJob arrow_job(const ArrowSweep &a, int gc) {
   J.out_nqp = a.P.nqp[nxt] + gc;
   J.out_nqh = a.P.nqh[nxt] + gc;
   J.kolt = a.kolt + 3L * gc;
+  J.warm = a.warm ? a.warm + C.warm_off : nullptr;
+  J.warm_p = a.warm_p ? a.warm_p + (long)kArrowNI * gc : nullptr;
+  J.wld = C.cap;
   return J;
 }
 

@@ -62,6 +62,13 @@ struct ArrowJob {
   double *out_e, *out_x;           // records: qp (descending e) then qh (descending e)
   int *out_nqp, *out_nqh;
   double *kolt;                    // Calc_En partial sums
+  // Roots of the previous sweep (may be null): warm[k * wld + j] is root j
+  // of head k's problem and warm_p[k] that problem's pole count (-1: none).
+  // A root starts from its previous value when the count is unchanged and
+  // the value still lies in the root's interval; the roots are stored back.
+  double *warm;
+  int *warm_p;
+  int wld, _wpad;
 };
 
 __host__ __device__ inline size_t arrow_ws_doubles(int n, int nI) {
@@ -73,7 +80,10 @@ __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
 }
 
 // Root j (0..p) of the secular equation with sorted poles d[0..p-1] and
-// squared couplings zz (all non-zero).  The root is returned as an offset tau from the
+// squared couplings zz (all non-zero).  guess (NaN: none) is a starting
+// value, used when it lies strictly inside the root's interval: the origin is
+// then the nearer end and the first evaluation is at the guess.  The root is
+// returned as an offset tau from the
 // pole org it is closest to, so that x - d_i = tau - (d_i - d_org) keeps full
 // relative accuracy near the poles.
 //
@@ -115,21 +125,36 @@ __host__ __device__ inline void arrow_sums(const Blk &B, const double *d, const 
 
 template <class Blk>
 __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const double *d, const double *zz,
-                                          double alpha, double znorm, int *org_out, double *tau_out) {
+                                          double alpha, double znorm, double guess, int *org_out,
+                                          double *tau_out) {
   const double eps = 2.2204460492503131e-16;
-  int o, far;
-  double lo, hi, tau;
+  int o = 0, far = -1;
+  double lo = 0.0, hi = 0.0, tau = 0.0;
   // poles on the origin's side: below sp (psi_left) or from sp on; an extreme
   // root has them all on one side (sp = p)
   int sp = p;
-  bool psi_left = true;
+  bool psi_left = true, eval0 = false, ready = false;   // ready: origin and start chosen
   double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, mag = 0.0, f = 0.0;
   if (j == 0 || j == p) {
     o = j == 0 ? 0 : p - 1; far = -1;
     if (j == 0) { lo = ((fmin(alpha, d[0]) - d[0]) - znorm) * (1.0 + 1e-7); hi = 0.0; }
     else { lo = 0.0; hi = ((fmax(alpha, d[p - 1]) - d[p - 1]) + znorm) * (1.0 + 1e-7); }
     tau = 0.5 * (lo + hi);
-  } else {
+    const double tg = guess - d[o];
+    if (tg > lo && tg < hi) tau = tg;
+    ready = true;
+  } else if (guess > d[j - 1] && guess < d[j]) {
+    // start from the guess: origin at the nearer pole, the whole interval as
+    // bracket, first evaluation at the guess
+    const double a = d[j - 1], b = d[j];
+    if (guess - a <= b - guess) { o = j - 1; far = j; lo = 0.0; hi = b - a; tau = guess - a; }
+    else { o = j; far = j - 1; lo = a - b; hi = 0.0; tau = guess - b; psi_left = false; }
+    sp = j;
+    ready = tau > lo && tau < hi;
+    eval0 = ready;
+  }
+  if (!ready) {
+    o = 0; far = -1; sp = p; psi_left = true;
     const double a = d[j - 1], b = d[j], mid = 0.5 * (a + b);
     double sl, dsl, sr, dsr;
     arrow_sums(B, d, zz, 0, j, mid, 0.0, sl, dsl);
@@ -150,7 +175,7 @@ __host__ __device__ inline int arrow_root(const Blk &B, int j, int p, const doub
   const double dofs = d[o];
   int it = 0;
   for (; it < kArrowMaxIt; ++it) {
-    if (it > 0 || far < 0) {
+    if (it > 0 || far < 0 || eval0) {
       double sl, dsl, sr, dsr;
       arrow_sums(B, d, zz, 0, sp, dofs, tau, sl, dsl);
       arrow_sums(B, d, zz, sp, p, dofs, tau, sr, dsr);
@@ -339,14 +364,19 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i]]; zh[i] = zn[i] * zn[i]; }
     B.sync();
     tk = B.lap(1, tk);
-    // roots
+    // roots, started from the previous sweep's where the problem kept its size
+    const int pw = J.warm_p ? J.warm_p[k] : -1;
     for (int j = tid; j <= p; j += nt) {
       if (p == 0) { org[j] = -1; tau[j] = 0.0; continue; }
-      const int its = arrow_root(B, j, p, dn, zh, alpha, znorm, &org[j], &tau[j]);
+      double *wj = J.warm ? J.warm + (long)k * J.wld + j : nullptr;
+      const double guess = (wj && pw == p) ? *wj : __builtin_nan("");
+      const int its = arrow_root(B, j, p, dn, zh, alpha, znorm, guess, &org[j], &tau[j]);
+      if (wj) *wj = dn[org[j]] + tau[j];
       if (its >= kArrowMaxIt) cnt[2] = 1;
       flops += 9.0 * p * (its + 1);
     }
     B.sync();
+    if (tid == 0 && J.warm_p) J.warm_p[k] = p;
     tk = B.lap(2, tk);
     // couplings recomputed from the roots: zh_i^2 = -prod_j (d_i - x_j) /
     // prod_{l != i} (d_i - d_l), paired along the interlacing x_i < d_i < x_{i+1}

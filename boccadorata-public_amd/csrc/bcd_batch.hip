@@ -39,7 +39,7 @@ Batch::~Batch() {
 
 std::vector<DevArena *> Batch::arenas() {
   return {&a_tab_i, &a_tab_l, &a_tab_d, &a_inst, &a_chan, &a_blocks, &a_work, &a_prop, &a_rows,
-          &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof};
+          &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof, &a_warm};
 }
 
 #define HTRY(x)                                   \
@@ -99,7 +99,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   nImax = 1;
   const bool dyn = par.mode >= 1;
   long o_u1 = 0, o_h = 0, o_w = 0, o_tri = 0, o_mat = 0, o_piv = 0, o_lr = 0, o_e = 0, o_x = 0, o_blk = 0;
-  long o_pole = 0, o_aws = 0;
+  long o_pole = 0, o_aws = 0, o_warm = 0;
   std::vector<long> e_off, x_off;
   std::vector<int> cap;
   for (int i = 0; i < n_inst; ++i) {
@@ -141,6 +141,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
       C.lbk_off = o_lr; o_lr += pad8((long)C.lbk_cap * width);
       C.pole_off = o_pole; o_pole += pad8((long)(1 + nI) * (C.lfw_cap + C.lbk_cap));
       C.aws_off = o_aws; o_aws += pad8((long)((arrow_ws_bytes((int)cp, nI) + 7) / 8));
+      C.warm_off = o_warm; o_warm += pad8((long)nI * cp);
       e_off.push_back(o_e); o_e += pad8(cp);
       x_off.push_back(o_x); o_x += pad8(cp * nI);
       cap.push_back((int)cp);
@@ -151,6 +152,11 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   n_gc = (int)h_chans.size();
   n_blk = (long)h_blocks.size();
   mat_len = o_mat;
+
+  // ---- warm-start roots of the arrowhead sweeps
+  BTRY(a_warm.grow(sizeof(double) * pad8(std::max<long>(o_warm, 1)) + sizeof(int) * kArrowNI * (size_t)n_gc + 64));
+  d_warm = (double *)a_warm.ptr;
+  d_warm_p = (int *)(d_warm + pad8(std::max<long>(o_warm, 1)));
 
   // ---- one-body parts
   std::vector<double> u1(std::max<long>(o_u1, 1), 0.0);
@@ -322,6 +328,8 @@ int Batch::sweep(int src) {
     a.poles = d_poles; a.aws = aws_len ? (double *)a_aws.ptr : nullptr; a.kolt = d_kolt; a.status = d_status;
     a.work = d_work;
     a.prof = profile ? d_prof : nullptr;
+    a.warm = d_warm;
+    a.warm_p = d_warm_p;
     const int nmax = src == 0 ? nImax : (int)capmax;
     const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
     kt_begin(KT_ARROW);
@@ -374,6 +382,8 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   HTRY(hipStreamSynchronize(stream));
   for (InstState &s : h_is) { s.active = 1; s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0; }
   HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
+  // no arrowhead roots to start from in a stage's first sweep
+  HTRY(hipMemsetAsync(d_warm_p, 0xff, sizeof(int) * kArrowNI * (size_t)n_gc, stream));
   hipLaunchKernelGGL(k_finalize, dim3(n_inst), dim3(256), 0, stream, d_insts, d_chans, view(true), d_cs, d_kolt,
                      par.conv_kind, 1);
   HTRY(hipGetLastError());

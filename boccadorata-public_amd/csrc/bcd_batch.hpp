@@ -49,7 +49,9 @@ struct Batch {
   std::vector<long> n_fw_tot, n_bk_tot;
 
   DevArena a_tab_i, a_tab_l, a_tab_d, a_inst, a_chan, a_blocks, a_work, a_prop, a_rows, a_lws, a_mat, a_jobs, a_aws,
-      a_prof;
+      a_prof, a_warm;
+  double *d_warm = nullptr;                          // previous sweep's arrowhead roots (warm start)
+  int *d_warm_p = nullptr;
   InstDev *d_insts = nullptr;
   InstState *d_is = nullptr;
   ChanDev *d_chans = nullptr;

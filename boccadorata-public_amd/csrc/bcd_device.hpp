@@ -66,6 +66,7 @@ struct ChanDev {
   long pole_off;                 // diagonalised reduced sides: lam_fw[lfw_cap], Ct_fw[nI x lfw_cap],
                                  // lam_bk[lbk_cap], Ct_bk[nI x lbk_cap]
   long aws_off;                  // arrowhead workspace (doubles) for the global-memory path
+  long warm_off;                 // previous sweep's arrowhead roots, nI x cap doubles
 };
 
 // One (Jab, a <= b, g) block of the Sigma(2) build of channel gc; blocks of a

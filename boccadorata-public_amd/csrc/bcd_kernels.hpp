@@ -65,6 +65,8 @@ struct ArrowSweep {
   int *status;
   double *work;                  // accumulates the arrowhead FLOPs (may be null)
   unsigned long long *prof;      // 8 phase-time counters per channel (may be null)
+  double *warm;                  // previous sweep's roots per channel (ChanDev::warm_off; may be null)
+  int *warm_p;                   // their pole counts, kArrowNI per channel
 };
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow_sweep(ArrowSweep a);

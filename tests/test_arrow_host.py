@@ -34,26 +34,37 @@ def exe():
     return EXE
 
 
-def run(exe, cases):
+def run(exe, cases, with_flops=False):
+    """Runs every case; a case with "h2" runs a second time with static part
+    h2, warm-started from the first run's roots, and yields a pair of results.
+    Each result is (nqp, nqh, failed, e, x, kolt) plus, with with_flops, the
+    FLOPs the body counted (its iteration counts)."""
     buf = [struct.pack("<i", len(cases))]
     for c in cases:
         nI = c["h"].shape[0]
-        buf.append(struct.pack("<5id", nI, len(c["E0"]), len(c["E1"]), c["nt"], c["j2"], c["ef"]))
-        for k in ("h", "u1", "E0", "C0", "E1", "C1"):
+        warm = int("h2" in c)
+        buf.append(struct.pack("<6id", nI, len(c["E0"]), len(c["E1"]), c["nt"], c["j2"], warm, c["ef"]))
+        for k in ("h", "u1", "E0", "C0", "E1", "C1") + (("h2",) if warm else ()):
             buf.append(np.ascontiguousarray(c[k], np.float64).tobytes())
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([exe], input=b"".join(buf), capture_output=True, env=env, timeout=300)
     assert r.returncode == 0, r.stderr.decode()[-3000:]
     out, pos = [], 0
     raw = r.stdout
-    for c in cases:
-        nI = c["h"].shape[0]
-        n = nI + len(c["E0"]) + len(c["E1"])
+
+    def record(n, nI):
+        nonlocal pos
         nqp, nqh, failed = struct.unpack_from("<3i", raw, pos); pos += 12
         e = np.frombuffer(raw, np.float64, n, pos); pos += 8 * n
         x = np.frombuffer(raw, np.float64, n * nI, pos).reshape(n, nI); pos += 8 * n * nI
         k = np.frombuffer(raw, np.float64, 3, pos); pos += 24
-        out.append((nqp, nqh, failed, e, x, k))
+        fl = struct.unpack_from("<d", raw, pos)[0]; pos += 8
+        return (nqp, nqh, failed, e, x, k) + ((fl,) if with_flops else ())
+    for c in cases:
+        nI = c["h"].shape[0]
+        n = nI + len(c["E0"]) + len(c["E1"])
+        first = record(n, nI)
+        out.append((first, record(n, nI)) if "h2" in c else first)
     assert pos == len(raw)
     return out
 
@@ -168,3 +179,22 @@ def test_reduced_self_energies(exe):
                               C1=side[1][1], nt=128, j2=int(ms.j2[i]), ef=1.0))
     for c, r in zip(cases, run(exe, cases)):
         check(c, r, tol_e=1e-12, tol_z=1e-11, tol_x=1e-9)
+
+
+def test_warm_start_from_previous_roots(exe):
+    """A second solve with a slightly changed static part, started from the
+    first solve's roots (what the next sc0 sweep does), gives the dense
+    eigenpairs of the new matrix and needs fewer secular evaluations; a large
+    change, which moves roots out of their old intervals, still solves."""
+    rng = np.random.default_rng(11)
+    for nI, m0, m1, nt, rel, fewer in ((4, 60, 40, 128, 1e-7, True), (4, 200, 97, 256, 1e-9, True),
+                                       (2, 60, 40, 64, 1e-3, False), (4, 60, 40, 128, 0.3, False)):
+        c = case(rng, nI, m0, m1, nt=nt)
+        d = rng.standard_normal((nI, nI)) * rel * np.abs(c["h"]).max()
+        c["h2"] = c["h"] + d + d.T
+        (r1, r2), = run(exe, [c], with_flops=True)
+        check(c, r1[:6])
+        c2 = dict(c, h=c["h2"])
+        check(c2, r2[:6])
+        if fewer:
+            assert r2[6] < 0.75 * r1[6], (r1[6], r2[6])
