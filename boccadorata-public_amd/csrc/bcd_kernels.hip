@@ -201,10 +201,13 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
                                                const int *blk_fw_off, const int *blk_bk_off,
                                                const int *work_list, const int *work_count,
                                                const long *row_fw_base, const long *row_bk_base,
-                                               double *rows_fw, double *rows_bk) {
+                                               double *rows_fw, double *rows_bk, int vt_cap) {
 #pragma clang fp contract(off)
-  __shared__ double vt[kMaxVt];
-  __shared__ int pa_[kMaxPairs], pb_[kMaxPairs];
+  // dynamic LDS: the block's V table (vt_cap doubles, the batch's largest),
+  // then the (alpha, beta) pair lists
+  extern __shared__ double sig_lds[];
+  double *vt = sig_lds;
+  int *pa_ = (int *)(sig_lds + vt_cap), *pb_ = pa_ + kMaxPairs;
   const int lane = threadIdx.x;
   const int total = *work_count;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {

@@ -25,7 +25,9 @@ __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int 
 __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const BlockDev *blocks, PropView P,
                          const int *blk_fw_off, const int *blk_bk_off, const int *work_list,
                          const int *work_count, const long *row_fw_base, const long *row_bk_base,
-                         double *rows_fw, double *rows_bk);
+                         double *rows_fw, double *rows_bk, int vt_cap);
+// dynamic LDS bytes of k_sigma2 for a V table of vt_cap doubles
+inline size_t sigma2_lds(int vt_cap) { return sizeof(double) * (size_t)vt_cap + sizeof(int) * 2 * kMaxPairs; }
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P, ChanState *cs,
                                double *piv, int lanc_kind, int lanc_n, const long *row_fw_base,
                                const long *row_bk_base, double *rows_fw, double *rows_bk, double *lrows,

@@ -94,6 +94,7 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
   blocks.assign(nsubsh, {});
   mbound_fw.assign(nsubsh, 0);
   mbound_bk.assign(nsubsh, 0);
+  max_vt = 0;
   for (int I = 0; I < nsubsh; ++I) {
     for (int J = 0; J <= jmax; ++J)
       for (int a = 0; a < nsubsh; ++a)
@@ -109,6 +110,7 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
             for (int al = 0; al < no[a]; ++al) np += no[b] - ((a == b) ? al + nJab : 0);
             if (np < 0) np = 0;
             if (np >= kMaxPairs - 1 || (long)no[I] * np * no[g] > kMaxVt) return BCD_ERR_CAPACITY;
+            max_vt = std::max(max_vt, no[I] * np * no[g]);
             BlockDev B;
             B.gc = I; B.a = a; B.b = b; B.g = g; B.J = J; B.nJab = nJab; B.nJgd = nJgd; B._pad = 0;
             blocks[I].push_back(B);
