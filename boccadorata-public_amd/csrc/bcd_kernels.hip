@@ -176,10 +176,16 @@ __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int 
       const int tf = __shfl_up(sfw, o, 64), tb = __shfl_up(sbk, o, 64);
       if (lane >= o) { sfw += tf; sbk += tb; }
     }
+    // non-empty blocks join the work list: one atomic per wave, ranks by ballot
+    const bool take = b < C.blk_cnt && cfw + cbk > 0;
+    const unsigned long long mask = __ballot(take);
+    int wbase = 0;
+    if (lane == 0 && mask) wbase = atomicAdd(work_count, __popcll(mask));
+    wbase = __shfl(wbase, 0, 64);
     if (b < C.blk_cnt) {
       blk_fw_off[C.blk_off + b] = base_fw + sfw - cfw;
       blk_bk_off[C.blk_off + b] = base_bk + sbk - cbk;
-      if (cfw + cbk > 0) work_list[atomicAdd(work_count, 1)] = (int)(C.blk_off + b);
+      if (take) work_list[wbase + __popcll(mask & ((1ull << lane) - 1ull))] = (int)(C.blk_off + b);
     }
     base_fw += __shfl(sfw, 63, 64);
     base_bk += __shfl(sbk, 63, 64);
@@ -261,8 +267,10 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
       const int row_off = side ? blk_bk_off[bidx] : blk_fw_off[bidx];
       double *rows = side ? rows_bk + row_bk_base[B.gc] : rows_fw + row_fw_base[B.gc];
       const int w = 2 + nI;
-      for (int p = lane; p < npole; p += 64) {
-        int na, nb, ng = p % cG, q = p / cG;
+      // decode pole p into its fragments, in the reference's enumeration order
+      auto frag = [&](int p, int &na, int &nb, int &ng) {
+        int q = p / cG;
+        ng = p % cG;
         if (a == b) {
           na = 0;
           int len = cA - nJab;
@@ -271,32 +279,43 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
         } else {
           na = q / cB; nb = q % cB;
         }
+      };
+      // pass 1: the couplings, one (pole, orbit d) per lane; each sum runs
+      // over (alpha beta, gamma) in the reference's order
+      for (int q = lane; q < npole * nI; q += 64) {
+        const int p = q / nI, d = q % nI;
+        int na, nb, ng;
+        frag(p, na, nb, ng);
         const double *ya = xA + (long)na * noa, *yb = xB + (long)nb * nob, *yg = xG + (long)ng * nog;
-        double *row = rows + (long)(row_off + p) * w;
         double x4 = 1.0;
         if (a == b && na == nb) x4 /= kSqrt2;
+        const int od = C.orb0 + d;
+        int ie = d * npair * nog;
+        double x1 = 0.0;
+        for (int pi = 0; pi < npair; ++pi) {
+          const int al = pa_[pi], be = pb_[pi];
+          double x2 = ya[al] * yb[be];
+          if (a == b) {
+            x2 += xJab * ya[be] * yb[al];
+            if (al == be) x2 /= kSqrt2;
+          }
+          for (int gm = 0; gm < nog; ++gm, ++ie) {
+            if (I.orb0[g] + gm == od && nJgd) continue;
+            x1 += vt[ie] * x2 * yg[gm];
+          }
+        }
+        x1 *= x4;
+        rows[(long)(row_off + p) * w + 2 + d] = x1;
+      }
+      __syncthreads();
+      // pass 2: energy and sum of squares (over d in order) per pole
+      for (int p = lane; p < npole; p += 64) {
+        int na, nb, ng;
+        frag(p, na, nb, ng);
+        double *row = rows + (long)(row_off + p) * w;
         row[0] = eA[na] + eB[nb] - eG[ng];
         double sum2 = 0.0;
-        int ie = 0;
-        for (int d = 0; d < nI; ++d) {
-          const int od = C.orb0 + d;
-          double x1 = 0.0;
-          for (int pi = 0; pi < npair; ++pi) {
-            const int al = pa_[pi], be = pb_[pi];
-            double x2 = ya[al] * yb[be];
-            if (a == b) {
-              x2 += xJab * ya[be] * yb[al];
-              if (al == be) x2 /= kSqrt2;
-            }
-            for (int gm = 0; gm < nog; ++gm, ++ie) {
-              if (I.orb0[g] + gm == od && nJgd) continue;
-              x1 += vt[ie] * x2 * yg[gm];
-            }
-          }
-          x1 *= x4;
-          row[2 + d] = x1;
-          sum2 += x1 * x1;
-        }
+        for (int d = 0; d < nI; ++d) sum2 += row[2 + d] * row[2 + d];
         row[1] = sum2;
       }
     }
