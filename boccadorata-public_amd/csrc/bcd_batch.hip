@@ -407,7 +407,7 @@ int Batch::build_sigma2(int only_ish) {
   PropView P = view(true);
   HTRY(hipMemsetAsync(d_work_count, 0, sizeof(int), stream));
   kt_begin(KT_PLAN);
-  hipLaunchKernelGGL(k_sigma2_plan, dim3(n_gc), dim3(64), 0, stream, d_chans, d_blocks, 0, n_gc, P, d_insts, d_cs,
+  hipLaunchKernelGGL(k_sigma2_plan, dim3(n_gc), dim3(256), 0, stream, d_chans, d_blocks, 0, n_gc, P, d_insts, d_cs,
                      d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, only_ish);
   kt_end(KT_PLAN);
   HTRY(hipGetLastError());
@@ -477,8 +477,10 @@ int Batch::build_sigma2(int only_ish) {
     kt_begin(KT_SIGMA2);
     int vt_cap = 2;
     for (const HostSystem *S : sys) vt_cap = std::max(vt_cap, (S->max_vt + 1) & ~1);
-    hipLaunchKernelGGL(k_sigma2, dim3(std::min(nwork, 16384)), dim3(64), sigma2_lds(vt_cap), stream, d_insts,
-                       d_chans, d_blocks, P, d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base,
+    // precision 1: FP32 operands and products, FP64 sums
+    auto kern = par.precision == 1 ? k_sigma2<float, double> : k_sigma2<double, double>;
+    hipLaunchKernelGGL(kern, dim3(std::min(nwork, 16384)), dim3(64), sigma2_lds(vt_cap), stream, d_insts, d_chans,
+                       d_blocks, P, d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base,
                        d_row_bk_base, d_rows_fw, d_rows_bk, vt_cap);
     kt_end(KT_SIGMA2);
     HTRY(hipGetLastError());

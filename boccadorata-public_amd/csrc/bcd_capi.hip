@@ -225,7 +225,7 @@ int bcd_count_poles(bcd_ctx *ctx, int ish, const bcd_prop *g, int *nfw, int *nbk
   BTRY(stage_batch(ctx, g, 1, b));
   PropView P = b.view(true);
   HTRY(hipMemsetAsync(b.d_work_count, 0, sizeof(int), b.stream));
-  hipLaunchKernelGGL(k_sigma2_plan, dim3(b.n_gc), dim3(64), 0, b.stream, b.d_chans, b.d_blocks, 0, b.n_gc, P,
+  hipLaunchKernelGGL(k_sigma2_plan, dim3(b.n_gc), dim3(256), 0, b.stream, b.d_chans, b.d_blocks, 0, b.n_gc, P,
                      b.d_insts, b.d_cs, b.d_blk_fw_off, b.d_blk_bk_off, b.d_work_list, b.d_work_count, ish);
   HTRY(hipGetLastError());
   ChanState cs;
@@ -535,7 +535,9 @@ int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_v
                      const bcd_solve_params *par, bcd_batch **out) {
   if (!out || n_inst < 1 || !ms || !vpp || !par) return BCD_ERR_ARG;
   *out = nullptr;
-  if (par->mode < 0 || par->mode > 2 || par->dyson_solver < 0 || par->dyson_solver > 1) return BCD_ERR_ARG;
+  if (par->mode < 0 || par->mode > 2 || par->dyson_solver < 0 || par->dyson_solver > 1 || par->precision < 0 ||
+      par->precision > 1)
+    return BCD_ERR_ARG;
   BTRY(device_ok(device));
   std::unique_ptr<bcd_batch> B(new (std::nothrow) bcd_batch);
   if (!B) return BCD_ERR_NOMEM;

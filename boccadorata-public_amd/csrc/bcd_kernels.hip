@@ -144,9 +144,10 @@ __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_b
 // Pole counts of each (J,a,b,g) block from the fragment counts of the current
 // propagator (Count_2p1h_2h1p_poles, Self_energy_ext_Op_c.cpp:76-90), row
 // offsets in build order, and a compact list of the non-empty blocks.
-__global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
-                              PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
-                              int *blk_bk_off, int *work_list, int *work_count, int only_ish) {
+__global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin,
+                                                    int gc_end, PropView P, const InstDev *insts, ChanState *cs,
+                                                    int *blk_fw_off, int *blk_bk_off, int *work_list,
+                                                    int *work_count, int only_ish) {
   const int gc = gc_begin + blockIdx.x;
   if (gc >= gc_end) return;
   const ChanDev C = chans[gc];
@@ -154,11 +155,12 @@ __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int 
     if (threadIdx.x == 0) { cs[gc].mfw = 0; cs[gc].mbk = 0; }
     return;
   }
+  __shared__ int wsum[2][4];
   const InstDev I = insts[C.inst];
-  const int cur = prop_cur(P, C.inst), lane = threadIdx.x;
+  const int cur = prop_cur(P, C.inst), tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
   int base_fw = 0, base_bk = 0;
-  for (int b0 = 0; b0 < C.blk_cnt; b0 += 64) {
-    const int b = b0 + lane;
+  for (int b0 = 0; b0 < C.blk_cnt; b0 += nt) {
+    const int b = b0 + tid;
     int cfw = 0, cbk = 0;
     if (b < C.blk_cnt) {
       const BlockDev B = blocks[C.blk_off + b];
@@ -170,12 +172,20 @@ __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int 
       cbk = na * nb * ng;
       if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cbk = (na * nb / 2) * ng; }
     }
-    // inclusive wave scans
+    // inclusive scans: within each wave, then the waves before it
     int sfw = cfw, sbk = cbk;
     for (int o = 1; o < 64; o <<= 1) {
       const int tf = __shfl_up(sfw, o, 64), tb = __shfl_up(sbk, o, 64);
       if (lane >= o) { sfw += tf; sbk += tb; }
     }
+    if (lane == 63) { wsum[0][wv] = sfw; wsum[1][wv] = sbk; }
+    __syncthreads();
+    int pf = 0, pb = 0, tf = 0, tb = 0;
+    for (int w = 0; w < (nt >> 6); ++w) {
+      if (w < wv) { pf += wsum[0][w]; pb += wsum[1][w]; }
+      tf += wsum[0][w]; tb += wsum[1][w];
+    }
+    __syncthreads();
     // non-empty blocks join the work list: one atomic per wave, ranks by ballot
     const bool take = b < C.blk_cnt && cfw + cbk > 0;
     const unsigned long long mask = __ballot(take);
@@ -183,14 +193,14 @@ __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int 
     if (lane == 0 && mask) wbase = atomicAdd(work_count, __popcll(mask));
     wbase = __shfl(wbase, 0, 64);
     if (b < C.blk_cnt) {
-      blk_fw_off[C.blk_off + b] = base_fw + sfw - cfw;
-      blk_bk_off[C.blk_off + b] = base_bk + sbk - cbk;
+      blk_fw_off[C.blk_off + b] = base_fw + pf + sfw - cfw;
+      blk_bk_off[C.blk_off + b] = base_bk + pb + sbk - cbk;
       if (take) work_list[wbase + __popcll(mask & ((1ull << lane) - 1ull))] = (int)(C.blk_off + b);
     }
-    base_fw += __shfl(sfw, 63, 64);
-    base_bk += __shfl(sbk, 63, 64);
+    base_fw += tf;
+    base_bk += tb;
   }
-  if (lane == 0) {
+  if (tid == 0) {
     cs[gc].mfw = base_fw;
     cs[gc].mbk = base_bk;
   }
@@ -202,6 +212,10 @@ __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int 
 // the reference's loop and operation order (contraction off), so rows match
 // the reference to the last bit.  One wave per block, grid-stride over the
 // compact list of non-empty blocks.
+// T: arithmetic of the contraction -- double (the reference's, bit-identical
+// rows) or float (FP32 products and sums; the V table, amplitudes and sums in
+// FP32, the rows stored in FP64).
+template <class T, class A>
 __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanDev *chans,
                                                const BlockDev *blocks, PropView P,
                                                const int *blk_fw_off, const int *blk_bk_off,
@@ -209,10 +223,10 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
                                                const long *row_fw_base, const long *row_bk_base,
                                                double *rows_fw, double *rows_bk, int vt_cap) {
 #pragma clang fp contract(off)
-  // dynamic LDS: the block's V table (vt_cap doubles, the batch's largest),
+  // dynamic LDS: the block's V table (vt_cap entries, the batch's largest),
   // then the (alpha, beta) pair lists
   extern __shared__ double sig_lds[];
-  double *vt = sig_lds;
+  T *vt = (T *)sig_lds;
   int *pa_ = (int *)(sig_lds + vt_cap), *pb_ = pa_ + kMaxPairs;
   const int lane = threadIdx.x;
   const int total = *work_count;
@@ -244,10 +258,11 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
         val = x3 * vget(I, I.orb0[a] + pa_[pi], I.orb0[b] + pb_[pi], od, og, J);
         if (og == od) val *= kSqrt2;
       }
-      vt[e] = val;
+      vt[e] = (T)val;
     }
     __syncthreads();
-    const double xJab = (double)(1 - 2 * nJab);
+    const T xJab = (T)(1 - 2 * nJab);
+    const T kS2 = (T)kSqrt2;
     for (int side = 0; side < 2; ++side) {
       // side 0: 2p1h from qp(a), qp(b), qh(g); side 1: 2h1p from qh(a), qh(b), qp(g)
       const int cA = side ? P.nqh[cur][ga] : P.nqp[cur][ga];
@@ -287,25 +302,25 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
         int na, nb, ng;
         frag(p, na, nb, ng);
         const double *ya = xA + (long)na * noa, *yb = xB + (long)nb * nob, *yg = xG + (long)ng * nog;
-        double x4 = 1.0;
-        if (a == b && na == nb) x4 /= kSqrt2;
+        T x4 = (T)1.0;
+        if (a == b && na == nb) x4 /= kS2;
         const int od = C.orb0 + d;
         int ie = d * npair * nog;
-        double x1 = 0.0;
+        A x1 = (A)0.0;
         for (int pi = 0; pi < npair; ++pi) {
           const int al = pa_[pi], be = pb_[pi];
-          double x2 = ya[al] * yb[be];
+          T x2 = (T)ya[al] * (T)yb[be];
           if (a == b) {
-            x2 += xJab * ya[be] * yb[al];
-            if (al == be) x2 /= kSqrt2;
+            x2 += xJab * (T)ya[be] * (T)yb[al];
+            if (al == be) x2 /= kS2;
           }
           for (int gm = 0; gm < nog; ++gm, ++ie) {
             if (I.orb0[g] + gm == od && nJgd) continue;
-            x1 += vt[ie] * x2 * yg[gm];
+            x1 += (A)(vt[ie] * x2 * (T)yg[gm]);
           }
         }
-        x1 *= x4;
-        rows[(long)(row_off + p) * w + 2 + d] = x1;
+        x1 *= (A)x4;
+        rows[(long)(row_off + p) * w + 2 + d] = (double)x1;
       }
       __syncthreads();
       // pass 2: energy and sum of squares (over d in order) per pole
@@ -322,6 +337,14 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
     __syncthreads();
   }
 }
+
+#define BCD_SIGMA2_INST(T, A)                                                                               \
+  template __global__ void k_sigma2<T, A>(const InstDev *, const ChanDev *, const BlockDev *, PropView, const int *, \
+                                          const int *, const int *, const int *, const long *, const long *,       \
+                                          double *, double *, int);
+BCD_SIGMA2_INST(double, double)
+BCD_SIGMA2_INST(float, double)
+#undef BCD_SIGMA2_INST
 
 // ------------------------------------------------------------- Lanczos
 

@@ -22,6 +22,7 @@ __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_b
 __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
                               int *blk_bk_off, int *work_list, int *work_count, int only_ish);
+template <class T, class A>
 __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const BlockDev *blocks, PropView P,
                          const int *blk_fw_off, const int *blk_bk_off, const int *work_list,
                          const int *work_count, const long *row_fw_base, const long *row_bk_base,

@@ -146,6 +146,12 @@ typedef struct bcd_solve_params {
                         (each reduced side diagonalised once per solve);
                         1: dense Householder + QL of the whole Dyson matrix per
                         sweep.  Both give DysonMatrix's eigenpairs.              */
+  int precision;     /* 0: FP64 throughout (Sigma(2) rows bit-identical to the
+                        reference); 1: mixed -- the Sigma(2) coupling
+                        contraction with FP32 operands and products (sums
+                        in FP64); the Lanczos reduction, eigensolves and sc
+                        loop in FP64.  At Nmax = 7 its energies differ from
+                        the FP64 solve by up to ~1e-5 relative.            */
 } bcd_solve_params;
 
 /* g0 may be NULL (bare propagator, SpProp_t(ModSpace_t*, 1)); otherwise g0[i]
