@@ -43,7 +43,7 @@ class BcdSolveParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("hf_itr_max", C.c_int), ("sc_itr_max", C.c_int), ("lanczos", C.c_int),
                 ("lanc_kind", C.c_int), ("lanc_n", C.c_int), ("conv_kind", C.c_int), ("hf_conv", C.c_double),
                 ("sc_conv", C.c_double), ("t_fac", C.c_double), ("ext_u1", C.c_int), ("dyson_solver", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("accel", C.c_int)]
 
 
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
@@ -422,11 +422,13 @@ class Context(_TwoBodyMixin):
 
 
 def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1,
-                 hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1, dyson_solver=0, precision=0) -> BcdSolveParams:
+                 hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1, dyson_solver=0, precision=0,
+                 accel=0) -> BcdSolveParams:
     """dyson_solver 0: arrowhead updates (default); 1: dense Householder + QL per sweep.
-    precision 0: FP64 throughout; 1: mixed (FP32 Sigma(2) coupling contraction)."""
+    precision 0: FP64 throughout; 1: mixed (FP32 Sigma(2) coupling contraction).
+    accel 0: the reference's iteration; m = 1..8: Anderson mixing of the density over m iterates."""
     return BcdSolveParams(mode, hf_itr_max, sc_itr_max, int(lanczos), lanc_kind, lanc_n, conv_kind, hf_conv,
-                          sc_conv, t_fac, ext_u1, dyson_solver, precision)
+                          sc_conv, t_fac, ext_u1, dyson_solver, precision, accel)
 
 
 class Batch:

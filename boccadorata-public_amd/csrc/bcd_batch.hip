@@ -39,7 +39,7 @@ Batch::~Batch() {
 
 std::vector<DevArena *> Batch::arenas() {
   return {&a_tab_i, &a_tab_l, &a_tab_d, &a_inst, &a_chan, &a_blocks, &a_work, &a_prop, &a_rows,
-          &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof, &a_warm};
+          &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof, &a_warm, &a_mix};
 }
 
 #define HTRY(x)                                   \
@@ -205,6 +205,11 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_status = (int *)(W0 + l_end - 16);
   d_work_count = d_status + 2;
   d_work = W0 + l_end - 8;
+  rho_len = rho_tot;
+  if (par.accel > 0) {
+    BTRY(a_mix.grow(sizeof(double) * 2 * (size_t)(par.accel + 1) * std::max<long>(rho_tot, 1) + 64));
+    d_mix = (double *)a_mix.ptr;
+  }
   BTRY(a_prof.grow(sizeof(unsigned long long) * 8 * (size_t)n_gc));
   d_prof = (unsigned long long *)a_prof.ptr;
   HTRY(hipMemset(a_work.ptr, 0, sizeof(double) * l_end));
@@ -317,6 +322,8 @@ int Batch::sweep(int src) {
   const int gb = (n_gc + 127) / 128;
   kt_begin(KT_DENSITY);
   hipLaunchKernelGGL(k_density, dim3(n_gc), dim3(64), 0, stream, d_insts, d_chans, 0, n_gc, P, d_rho);
+  if (par.accel > 0)
+    hipLaunchKernelGGL(k_mix, dim3(n_inst), dim3(256), 0, stream, d_insts, d_is, d_rho, d_mix, rho_len, par.accel);
   kt_end(KT_DENSITY);
   kt_begin(KT_SIGINF);
   hipLaunchKernelGGL(k_sigma_inf, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_is, d_w, d_rho, d_h);
@@ -380,7 +387,10 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   // stage start: keep the current buffer, reset the sweep controls
   HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
   HTRY(hipStreamSynchronize(stream));
-  for (InstState &s : h_is) { s.active = 1; s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0; }
+  for (InstState &s : h_is) {
+    s.active = 1; s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0;
+    s.mix_k = 0; s.mix_s = 0; s.mix_fn = 0.0;
+  }
   HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
   // no arrowhead roots to start from in a stage's first sweep
   HTRY(hipMemsetAsync(d_warm_p, 0xff, sizeof(int) * kArrowNI * (size_t)n_gc, stream));

@@ -49,8 +49,10 @@ struct Batch {
   std::vector<long> n_fw_tot, n_bk_tot;
 
   DevArena a_tab_i, a_tab_l, a_tab_d, a_inst, a_chan, a_blocks, a_work, a_prop, a_rows, a_lws, a_mat, a_jobs, a_aws,
-      a_prof, a_warm;
+      a_prof, a_warm, a_mix;
   double *d_warm = nullptr;                          // previous sweep's arrowhead roots (warm start)
+  double *d_mix = nullptr;                           // density-mixing history (accel > 0)
+  long rho_len = 0;                                  // doubles of the density buffer
   int *d_warm_p = nullptr;
   InstDev *d_insts = nullptr;
   InstState *d_is = nullptr;

@@ -536,7 +536,7 @@ int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_v
   if (!out || n_inst < 1 || !ms || !vpp || !par) return BCD_ERR_ARG;
   *out = nullptr;
   if (par->mode < 0 || par->mode > 2 || par->dyson_solver < 0 || par->dyson_solver > 1 || par->precision < 0 ||
-      par->precision > 1)
+      par->precision > 1 || par->accel < 0 || par->accel > kMixMax)
     return BCD_ERR_ARG;
   BTRY(device_ok(device));
   std::unique_ptr<bcd_batch> B(new (std::nothrow) bcd_batch);

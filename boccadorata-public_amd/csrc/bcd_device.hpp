@@ -96,7 +96,10 @@ struct InstState {
   int itr_max;                   // ItrMax of the current stage (< 0: unbounded)
   int hf_its, sc_its;
   int err;
+  int mix_k;                     // density-mixing steps of the current stage
+  int mix_s;                     // first iterate of the current mixing history
   int _pad;
+  double mix_fn;                 // norm of the previous step's density residual
   double conv;
   double diff;
   double ef[2];                  // Fermi energy per charge for the next sweep

@@ -139,6 +139,130 @@ __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_b
   }
 }
 
+// Anderson mixing of the hole density (type II: H. F. Walker and P. Ni, SIAM J.
+// Numer. Anal. 49 (2011) 1715).  The sweep map is rho_in -> Sigma^inf -> Dyson
+// -> rho_out; the reference iterates it plainly (rho_in of the next sweep =
+// rho_out).  Here, with x = rho_in of the last sweep and f = rho_out - x,
+//   x_new = x + f - sum_i gamma_i (dX_i + dF_i),
+//   gamma = argmin | f - sum_i gamma_i dF_i |,
+// over the differences dX_i, dF_i of the last m iterates; the fixed point
+// rho_out(rho) = rho is the same.  One block per instance; hist holds the
+// iterates X and residuals F in rings of m + 1 slots, each slot laid out like
+// rho (rho_tot doubles).  Mixing starts once |f| <= kMixStart: the random
+// interactions here give several self-consistent solutions, and the plain
+// iteration decides which one a solve converges to; until then the steps are
+// plain.  A residual that grows tenfold restarts the history.
+__global__ __launch_bounds__(256) void k_mix(const InstDev *insts, InstState *is, double *rho, double *hist,
+                                             long rho_tot, int m) {
+  InstState *S = is + blockIdx.x;
+  if (!S->active) return;
+  const InstDev I = insts[blockIdx.x];
+  const int nq = I.nq, tid = threadIdx.x, mm = m + 1, k = S->mix_k;
+  double *r = rho + I.rho_off;
+  auto X = [&](int it) { return hist + (long)(it % mm) * rho_tot + I.rho_off; };
+  auto F = [&](int it) { return hist + (long)(mm + it % mm) * rho_tot + I.rho_off; };
+  if (k == 0) {
+    // first sweep of the stage: rho_in is the starting propagator's density
+    double *x0 = X(0);
+    for (int q = tid; q < nq; q += blockDim.x) x0[q] = r[q];
+    __syncthreads();
+    if (tid == 0) { S->mix_k = 1; S->mix_s = 0; S->mix_fn = 0.0; }
+    return;
+  }
+  constexpr int kNd = kMixMax * (kMixMax + 1) / 2 + kMixMax + 1;
+  __shared__ double red[4][kNd];
+  __shared__ double G[kNd];
+  __shared__ double A[kMixMax][kMixMax + 1];
+  __shared__ double gam[kMixMax];
+  __shared__ int n_sh;
+  const double *x = X(k - 1);
+  double *f = F(k - 1);
+  for (int q = tid; q < nq; q += blockDim.x) f[q] = r[q] - x[q];
+  int s = S->mix_s;
+  int n = min(m, k - 1 - s);
+  // dot products of the differences dF_i = F(k-i) - F(k-i-1), i = 1..n
+  double acc[kNd];
+#pragma unroll
+  for (int t = 0; t < kNd; ++t) acc[t] = 0.0;
+  for (int q = tid; q < nq; q += blockDim.x) {
+    double df[kMixMax];
+    const double fq = f[q];
+#pragma unroll
+    for (int i = 0; i < kMixMax; ++i) df[i] = (i < n) ? F(k - 1 - i)[q] - F(k - 2 - i)[q] : 0.0;
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < kMixMax; ++i) {
+#pragma unroll
+      for (int j = 0; j <= i; ++j) acc[t++] += df[i] * df[j];
+    }
+#pragma unroll
+    for (int i = 0; i < kMixMax; ++i) acc[t++] += df[i] * fq;
+    acc[t] += fq * fq;
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int t = 0; t < kNd; ++t) {
+    const double v = wave_sum(acc[t]);
+    if (lane == 0) red[wv][t] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int t = 0; t < kNd; ++t) {
+      G[t] = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) G[t] += red[w][t];
+    }
+    const double fn = sqrt(G[kNd - 1]);
+    // plain steps until the residual is small (the plain iteration picks the
+    // fixed point, mixing then converges onto it); restart on growth
+    if (fn > kMixStart || (n > 0 && fn > 10.0 * S->mix_fn)) { s = k - 1; n = 0; }
+    // normal equations (dF^T dF) gamma = dF^T f, Gaussian elimination with
+    // partial pivoting; a singular or non-finite system falls back to the
+    // plain step
+    int t = 0;
+    for (int i = 0; i < kMixMax; ++i)
+      for (int j = 0; j <= i; ++j, ++t)
+        if (i < n) { A[i][j] = G[t]; A[j][i] = G[t]; }
+    for (int i = 0; i < n; ++i) A[i][n] = G[kMixMax * (kMixMax + 1) / 2 + i];
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += A[i][i];
+    for (int i = 0; i < n; ++i) A[i][i] += 1e-13 * tr;
+    bool ok = n > 0 && tr > 0.0;
+    for (int c = 0; c < n && ok; ++c) {
+      int piv = c;
+      for (int i = c + 1; i < n; ++i)
+        if (fabs(A[i][c]) > fabs(A[piv][c])) piv = i;
+      if (!(fabs(A[piv][c]) > 1e-300)) { ok = false; break; }
+      if (piv != c)
+        for (int j = 0; j <= n; ++j) { const double tmp = A[c][j]; A[c][j] = A[piv][j]; A[piv][j] = tmp; }
+      for (int i = c + 1; i < n; ++i) {
+        const double l = A[i][c] / A[c][c];
+        for (int j = c; j <= n; ++j) A[i][j] -= l * A[c][j];
+      }
+    }
+    for (int i = n - 1; i >= 0 && ok; --i) {
+      double v = A[i][n];
+      for (int j = i + 1; j < n; ++j) v -= A[i][j] * gam[j];
+      gam[i] = v / A[i][i];
+      if (!isfinite(gam[i])) ok = false;
+    }
+    if (!ok) n = 0;
+    n_sh = n;
+    S->mix_k = k + 1;
+    S->mix_s = s;
+    S->mix_fn = fn;
+  }
+  __syncthreads();
+  n = n_sh;
+  double *xn = X(k);
+  for (int q = tid; q < nq; q += blockDim.x) {
+    double v = x[q] + f[q];
+    for (int i = 0; i < n; ++i)
+      v -= gam[i] * (X(k - 1 - i)[q] - X(k - 2 - i)[q] + F(k - 1 - i)[q] - F(k - 2 - i)[q]);
+    xn[q] = v;
+    r[q] = v;
+  }
+}
+
 // ------------------------------------------------------------- Sigma(2)
 
 // Pole counts of each (J,a,b,g) block from the fragment counts of the current
@@ -155,23 +279,46 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
     if (threadIdx.x == 0) { cs[gc].mfw = 0; cs[gc].mbk = 0; }
     return;
   }
-  __shared__ int wsum[2][4];
+  __shared__ int wsum[3][4];
+  __shared__ int list_base;
   const InstDev I = insts[C.inst];
   const int cur = prop_cur(P, C.inst), tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
-  int base_fw = 0, base_bk = 0;
+  auto counts = [&](int b, int &cfw, int &cbk) {
+    cfw = 0; cbk = 0;
+    if (b >= C.blk_cnt) return;
+    const BlockDev B = blocks[C.blk_off + b];
+    const int ga = I.gc0 + B.a, gb = I.gc0 + B.b, gg = I.gc0 + B.g;
+    int na = P.nqp[cur][ga], nb = P.nqp[cur][gb], ng = P.nqh[cur][gg];
+    cfw = na * nb * ng;
+    if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cfw = (na * nb / 2) * ng; }
+    na = P.nqh[cur][ga]; nb = P.nqh[cur][gb]; ng = P.nqp[cur][gg];
+    cbk = na * nb * ng;
+    if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cbk = (na * nb / 2) * ng; }
+  };
+  // first pass: the channel's non-empty blocks, then one atomic for its
+  // slice of the work list
+  int nz = 0;
+  for (int b0 = 0; b0 < C.blk_cnt; b0 += nt) {
+    int cfw, cbk;
+    counts(b0 + tid, cfw, cbk);
+    nz += cfw + cbk > 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o, 64);
+  if (lane == 0) wsum[2][wv] = nz;
+  __syncthreads();
+  if (tid == 0) {
+    int tot = 0;
+    for (int w = 0; w < (nt >> 6); ++w) tot += wsum[2][w];
+    list_base = tot ? atomicAdd(work_count, tot) : 0;
+  }
+  __syncthreads();
+  int base_fw = 0, base_bk = 0, base_nz = list_base;
   for (int b0 = 0; b0 < C.blk_cnt; b0 += nt) {
     const int b = b0 + tid;
-    int cfw = 0, cbk = 0;
-    if (b < C.blk_cnt) {
-      const BlockDev B = blocks[C.blk_off + b];
-      const int ga = I.gc0 + B.a, gb = I.gc0 + B.b, gg = I.gc0 + B.g;
-      int na = P.nqp[cur][ga], nb = P.nqp[cur][gb], ng = P.nqh[cur][gg];
-      cfw = na * nb * ng;
-      if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cfw = (na * nb / 2) * ng; }
-      na = P.nqh[cur][ga]; nb = P.nqh[cur][gb]; ng = P.nqp[cur][gg];
-      cbk = na * nb * ng;
-      if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cbk = (na * nb / 2) * ng; }
-    }
+    int cfw, cbk;
+    counts(b, cfw, cbk);
+    const bool take = cfw + cbk > 0;
+    const unsigned long long mask = __ballot(take);
     // inclusive scans: within each wave, then the waves before it
     int sfw = cfw, sbk = cbk;
     for (int o = 1; o < 64; o <<= 1) {
@@ -179,26 +326,22 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
       if (lane >= o) { sfw += tf; sbk += tb; }
     }
     if (lane == 63) { wsum[0][wv] = sfw; wsum[1][wv] = sbk; }
+    if (lane == 0) wsum[2][wv] = __popcll(mask);
     __syncthreads();
-    int pf = 0, pb = 0, tf = 0, tb = 0;
+    int pf = 0, pb = 0, pz = 0, tf = 0, tb = 0, tz = 0;
     for (int w = 0; w < (nt >> 6); ++w) {
-      if (w < wv) { pf += wsum[0][w]; pb += wsum[1][w]; }
-      tf += wsum[0][w]; tb += wsum[1][w];
+      if (w < wv) { pf += wsum[0][w]; pb += wsum[1][w]; pz += wsum[2][w]; }
+      tf += wsum[0][w]; tb += wsum[1][w]; tz += wsum[2][w];
     }
     __syncthreads();
-    // non-empty blocks join the work list: one atomic per wave, ranks by ballot
-    const bool take = b < C.blk_cnt && cfw + cbk > 0;
-    const unsigned long long mask = __ballot(take);
-    int wbase = 0;
-    if (lane == 0 && mask) wbase = atomicAdd(work_count, __popcll(mask));
-    wbase = __shfl(wbase, 0, 64);
     if (b < C.blk_cnt) {
       blk_fw_off[C.blk_off + b] = base_fw + pf + sfw - cfw;
       blk_bk_off[C.blk_off + b] = base_bk + pb + sbk - cbk;
-      if (take) work_list[wbase + __popcll(mask & ((1ull << lane) - 1ull))] = (int)(C.blk_off + b);
+      if (take) work_list[base_nz + pz + __popcll(mask & ((1ull << lane) - 1ull))] = (int)(C.blk_off + b);
     }
     base_fw += tf;
     base_bk += tb;
+    base_nz += tz;
   }
   if (tid == 0) {
     cs[gc].mfw = base_fw;
@@ -206,15 +349,66 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
   }
 }
 
+// Pass 1 of the coupling build for one block side: lane q owns the coupling
+// m(pole p, orbit d), q = p * nI + d, summed over (alpha beta, gamma) in the
+// reference's loop order (Self_energy_ext_Op_c.cpp:454-468) with contraction
+// off, so the value is bit-identical to the reference's.  The amplitudes
+// come from S-typed arrays: the block's fragments staged in LDS, or the
+// propagator in HBM when they do not fit the staging area.
+template <class T, class A, class S>
+__device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *xg, const T *vt, int noa, int nob,
+                                             int nog, int nI, int npole, int a_eq_b, int nJab, int nJgd, int cA,
+                                             int cB, int cG, int orb_g, int orb_d0, double *rows, int w) {
+#pragma clang fp contract(off)
+  const T xJab = (T)(1 - 2 * nJab);
+  const T kS2 = (T)kSqrt2;
+  for (int q = threadIdx.x; q < npole * nI; q += 64) {
+    const int p = q / nI, d = q % nI;
+    // fragments of pole p, in the reference's enumeration order
+    int na, nb, ng, r = p / cG;
+    ng = p % cG;
+    if (a_eq_b) {
+      na = 0;
+      int len = cA - nJab;
+      while (r >= len) { r -= len; ++na; --len; }
+      nb = na + nJab + r;
+    } else {
+      na = r / cB; nb = r % cB;
+    }
+    const S *ya = xa + na * noa, *yb = xb + nb * nob, *yg = xg + ng * nog;
+    const int od = orb_d0 + d;
+    T x4 = (T)1.0;
+    if (a_eq_b && na == nb) x4 /= kS2;
+    const T *v = vt + d * (a_eq_b ? (noa * (noa + 1) / 2 - nJab * noa) : noa * nob) * nog;
+    A x1 = (A)0.0;
+    for (int al = 0; al < noa; ++al) {
+      const T yaa = (T)ya[al], yba = (T)yb[al];
+      for (int be = a_eq_b ? al + nJab : 0; be < nob; ++be, v += nog) {
+        T x2 = yaa * (T)yb[be];
+        if (a_eq_b) {
+          x2 += xJab * (T)ya[be] * yba;
+          if (al == be) x2 /= kS2;
+        }
+        for (int gm = 0; gm < nog; ++gm) {
+          if (nJgd && orb_g + gm == od) continue;
+          x1 += (A)(v[gm] * x2 * (T)yg[gm]);
+        }
+      }
+    }
+    x1 *= (A)x4;
+    rows[(long)p * w + 2 + d] = (double)x1;
+  }
+}
+
 // The coupling build of Build_Dynamic_SlfEn_2ndOrder (Self_energy_ext_Op_c.cpp:
-// 380-539): per block the interaction part Vt(d; alpha,beta,gamma) is gathered
-// into LDS once, then every lane builds whole pole rows [E, sum m^2, m(d)] with
-// the reference's loop and operation order (contraction off), so rows match
-// the reference to the last bit.  One wave per block, grid-stride over the
-// compact list of non-empty blocks.
-// T: arithmetic of the contraction -- double (the reference's, bit-identical
-// rows) or float (FP32 products and sums; the V table, amplitudes and sums in
-// FP32, the rows stored in FP64).
+// 380-539), one wave per block, grid-stride over the compact list of
+// non-empty blocks.  Per block the interaction part Vt(d; alpha beta, gamma)
+// is gathered into LDS once; per side the fragment amplitudes the poles use
+// are staged next to it, then pass 1 builds the couplings (one lane per
+// (pole, d)) and pass 2 the energies and sums of squares (one lane per pole,
+// over d in order).  Rows match the reference to the last bit.
+// T: arithmetic of the contraction -- double (the reference's) or float
+// (FP32 operands and products); A: type of the sums.  Rows are stored in FP64.
 template <class T, class A>
 __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanDev *chans,
                                                const BlockDev *blocks, PropView P,
@@ -224,10 +418,10 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
                                                double *rows_fw, double *rows_bk, int vt_cap) {
 #pragma clang fp contract(off)
   // dynamic LDS: the block's V table (vt_cap entries, the batch's largest),
-  // then the (alpha, beta) pair lists
+  // then kSigStage staged amplitudes
   extern __shared__ double sig_lds[];
   T *vt = (T *)sig_lds;
-  int *pa_ = (int *)(sig_lds + vt_cap), *pb_ = pa_ + kMaxPairs;
+  T *stg = (T *)(sig_lds + vt_cap);
   const int lane = threadIdx.x;
   const int total = *work_count;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -239,30 +433,30 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
     const int nI = C.nI, a = B.a, b = B.b, g = B.g, J = B.J, nJab = B.nJab, nJgd = B.nJgd;
     const int noa = I.no[a], nob = I.no[b], nog = I.no[g];
     const int ga = I.gc0 + a, gb = I.gc0 + b, gg = I.gc0 + g;
+    const int oa0 = I.orb0[a], ob0 = I.orb0[b], og0 = I.orb0[g];
     // (alpha, beta) pairs in the reference order (beta >= alpha + nJab when a == b)
-    if (lane == 0) {
-      int np = 0;
-      for (int al = 0; al < noa; ++al)
-        for (int be = (a == b) ? al + nJab : 0; be < nob; ++be) { pa_[np] = al; pb_[np] = be; ++np; }
-      pa_[kMaxPairs - 1] = np;
-    }
-    __syncthreads();
-    const int npair = pa_[kMaxPairs - 1];
+    const int npair = (a == b) ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
     const int nvt = nI * npair * nog;
     const double x3 = sqrt((double)(J + J + 1) / (double)(1 + C.j2));
     for (int e = lane; e < nvt; e += 64) {
       const int ga_ = e % nog, pi = (e / nog) % npair, d = e / (nog * npair);
-      const int od = C.orb0 + d, og = I.orb0[g] + ga_;
+      int al, be;
+      if (a == b) {
+        int r = pi, len = noa - nJab;
+        al = 0;
+        while (r >= len) { r -= len; ++al; --len; }
+        be = al + nJab + r;
+      } else {
+        al = pi / nob; be = pi % nob;
+      }
+      const int od = C.orb0 + d, og = og0 + ga_;
       double val = 0.0;
       if (!(og == od && nJgd)) {
-        val = x3 * vget(I, I.orb0[a] + pa_[pi], I.orb0[b] + pb_[pi], od, og, J);
+        val = x3 * vget(I, oa0 + al, ob0 + be, od, og, J);
         if (og == od) val *= kSqrt2;
       }
       vt[e] = (T)val;
     }
-    __syncthreads();
-    const T xJab = (T)(1 - 2 * nJab);
-    const T kS2 = (T)kSqrt2;
     for (int side = 0; side < 2; ++side) {
       // side 0: 2p1h from qp(a), qp(b), qh(g); side 1: 2h1p from qh(a), qh(b), qp(g)
       const int cA = side ? P.nqh[cur][ga] : P.nqp[cur][ga];
@@ -280,61 +474,45 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
       const double *xB = P.x[cur] + P.x_off[gb] + (long)oB * nob;
       const double *xG = P.x[cur] + P.x_off[gg] + (long)oG * nog;
       const int row_off = side ? blk_bk_off[bidx] : blk_fw_off[bidx];
-      double *rows = side ? rows_bk + row_bk_base[B.gc] : rows_fw + row_fw_base[B.gc];
+      double *rows = (side ? rows_bk + row_bk_base[B.gc] : rows_fw + row_fw_base[B.gc]) + (long)row_off * (2 + nI);
       const int w = 2 + nI;
-      // decode pole p into its fragments, in the reference's enumeration order
-      auto frag = [&](int p, int &na, int &nb, int &ng) {
-        int q = p / cG;
+      // stage the amplitudes of the fragments this side uses (for a == b the
+      // two orbits share one fragment list)
+      const int nsa = cA * noa, nsb = (a == b) ? 0 : cB * nob, nsg = cG * nog;
+      const bool staged = nsa + nsb + nsg <= kSigStage;
+      if (staged) {
+        for (int k = lane; k < nsa; k += 64) stg[k] = (T)xA[k];
+        for (int k = lane; k < nsb; k += 64) stg[nsa + k] = (T)xB[k];
+        for (int k = lane; k < nsg; k += 64) stg[nsa + nsb + k] = (T)xG[k];
+      }
+      __syncthreads();
+      if (staged)
+        sigma2_pass1<T, A, T>(stg, (a == b) ? stg : stg + nsa, stg + nsa + nsb, vt, noa, nob, nog, nI, npole,
+                              a == b, nJab, nJgd, cA, cB, cG, og0, C.orb0, rows, w);
+      else
+        sigma2_pass1<T, A, double>(xA, xB, xG, vt, noa, nob, nog, nI, npole, a == b, nJab, nJgd, cA, cB, cG,
+                                   og0, C.orb0, rows, w);
+      __syncthreads();
+      // pass 2: energy and sum of squares (over d in order) per pole
+      for (int p = lane; p < npole; p += 64) {
+        int na, nb, ng, r = p / cG;
         ng = p % cG;
         if (a == b) {
           na = 0;
           int len = cA - nJab;
-          while (q >= len) { q -= len; ++na; --len; }
-          nb = na + nJab + q;
+          while (r >= len) { r -= len; ++na; --len; }
+          nb = na + nJab + r;
         } else {
-          na = q / cB; nb = q % cB;
+          na = r / cB; nb = r % cB;
         }
-      };
-      // pass 1: the couplings, one (pole, orbit d) per lane; each sum runs
-      // over (alpha beta, gamma) in the reference's order
-      for (int q = lane; q < npole * nI; q += 64) {
-        const int p = q / nI, d = q % nI;
-        int na, nb, ng;
-        frag(p, na, nb, ng);
-        const double *ya = xA + (long)na * noa, *yb = xB + (long)nb * nob, *yg = xG + (long)ng * nog;
-        T x4 = (T)1.0;
-        if (a == b && na == nb) x4 /= kS2;
-        const int od = C.orb0 + d;
-        int ie = d * npair * nog;
-        A x1 = (A)0.0;
-        for (int pi = 0; pi < npair; ++pi) {
-          const int al = pa_[pi], be = pb_[pi];
-          T x2 = (T)ya[al] * (T)yb[be];
-          if (a == b) {
-            x2 += xJab * (T)ya[be] * (T)yb[al];
-            if (al == be) x2 /= kS2;
-          }
-          for (int gm = 0; gm < nog; ++gm, ++ie) {
-            if (I.orb0[g] + gm == od && nJgd) continue;
-            x1 += (A)(vt[ie] * x2 * (T)yg[gm]);
-          }
-        }
-        x1 *= (A)x4;
-        rows[(long)(row_off + p) * w + 2 + d] = (double)x1;
-      }
-      __syncthreads();
-      // pass 2: energy and sum of squares (over d in order) per pole
-      for (int p = lane; p < npole; p += 64) {
-        int na, nb, ng;
-        frag(p, na, nb, ng);
-        double *row = rows + (long)(row_off + p) * w;
+        double *row = rows + (long)p * w;
         row[0] = eA[na] + eB[nb] - eG[ng];
         double sum2 = 0.0;
         for (int d = 0; d < nI; ++d) sum2 += row[2 + d] * row[2 + d];
         row[1] = sum2;
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 

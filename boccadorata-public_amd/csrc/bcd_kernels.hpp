@@ -8,17 +8,21 @@ namespace bcd {
 
 constexpr int kMaxVt = 2048;      // nI * (alpha,beta) pairs * gamma of one Sigma(2) block
 constexpr int kMaxPairs = 64;     // (alpha,beta) pairs of one block (+1 slot for the count)
+constexpr int kSigStage = 256;    // fragment amplitudes of one Sigma(2) block side staged in LDS
 constexpr int kMaxPiv = 8;        // Lanczos pivots per channel
 constexpr int kMaxNB = 16;        // coupling columns of a Lanczos input row
 constexpr int kLancRed = 2 + kMaxNB + kMaxPiv;
 constexpr int kMaxNI = 16;        // orbits per subshell
 constexpr int kMaxSub = 64;       // subshells per instance
+constexpr int kMixMax = 8;        // iterates of the Anderson density mixing
+constexpr double kMixStart = 1e-3;  // density residual norm below which the mixing starts
 
 __global__ void k_wbuild(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end, double *W);
 __global__ void k_density(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
                           PropView P, double *rho);
 __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
                             const InstState *is, const double *W, const double *rho, double *h);
+__global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *hist, long rho_tot, int m);
 __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
                               int *blk_bk_off, int *work_list, int *work_count, int only_ish);
@@ -28,7 +32,7 @@ __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const Block
                          const int *work_count, const long *row_fw_base, const long *row_bk_base,
                          double *rows_fw, double *rows_bk, int vt_cap);
 // dynamic LDS bytes of k_sigma2 for a V table of vt_cap doubles
-inline size_t sigma2_lds(int vt_cap) { return sizeof(double) * (size_t)vt_cap + sizeof(int) * 2 * kMaxPairs; }
+inline size_t sigma2_lds(int vt_cap) { return sizeof(double) * ((size_t)vt_cap + kSigStage); }
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P, ChanState *cs,
                                double *piv, int lanc_kind, int lanc_n, const long *row_fw_base,
                                const long *row_bk_base, double *rows_fw, double *rows_bk, double *lrows,

@@ -152,6 +152,12 @@ typedef struct bcd_solve_params {
                         in FP64); the Lanczos reduction, eigensolves and sc
                         loop in FP64.  At Nmax = 7 its energies differ from
                         the FP64 solve by up to ~1e-5 relative.            */
+  int accel;         /* 0: the reference's iteration (the hole density of each
+                        sweep's output feeds the next Sigma^inf); m = 1..8:
+                        Anderson mixing of that density over the last m
+                        iterates.  Same fixed point and stopping rule, fewer
+                        sweeps; results differ from the plain iteration
+                        within what the convergence tolerance leaves open. */
 } bcd_solve_params;
 
 /* g0 may be NULL (bare propagator, SpProp_t(ModSpace_t*, 1)); otherwise g0[i]

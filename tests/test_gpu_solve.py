@@ -153,3 +153,37 @@ def test_mixed_precision_error_vs_fp64(nmax):
     print(f"mixed vs fp64, nmax {nmax}: energy {we:.2e} Z {wz:.2e} moments {wm:.2e}; converged {conv}", flush=True)
     assert we < 3e-5 and wz < 1e-5 and wm < 1e-6
     assert we > 0.0   # the mixed path really ran
+
+
+@pytest.mark.parametrize("nmax", [5, 7])
+def test_anderson_mixing_same_fixed_point(nmax):
+    """Anderson mixing of the density (accel = 5) against the reference's plain
+    iteration on the same batch: both stop at max|de| < 1e-8, so they sit on
+    the same fixed point up to what that tolerance leaves open.  Solutions
+    above the Nmax Z cut agree to 1e-6 relative in energy and 1e-6 in Z, the
+    Z-weighted moments to 1e-7; the mixed iteration takes fewer sweeps."""
+    systems = [synth.make_system(nmax, s) for s in (0, 1, 2)]
+    out, its = {}, {}
+    for accel in (0, 5):
+        par = capi.solve_params(mode=2, hf_itr_max=100, sc_itr_max=60, lanczos=True, lanc_kind=1, lanc_n=50,
+                                conv_kind=1, hf_conv=1e-8, sc_conv=1e-8, accel=accel)
+        b = capi.Batch(systems, par)
+        b.solve()
+        assert all(b.converged(i) for i in range(len(systems))), accel
+        out[accel] = [b.result(i) for i in range(len(systems))]
+        its[accel] = sum(b.stats(i)["hf_its"] + b.stats(i)["sc_its"] for i in range(len(systems)))
+        b.close()
+    zsig = ZSIG_BY_NMAX[nmax]
+    we = wz = wm = 0.0
+    for g, ref in zip(out[5], out[0]):
+        assert np.array_equal(g.np_, ref.np_) and np.array_equal(g.nh, ref.nh)
+        for i in range(len(g.np_)):
+            ep, zp, _, eh, zh, _ = g.subshell(i)
+            rp, rz, _, rh, rzh, _ = ref.subshell(i)
+            for a, b_ in ((np.stack([ep, zp], 1), np.stack([rp, rz], 1)),
+                          (np.stack([eh, zh], 1), np.stack([rh, rzh], 1))):
+                e_err, z_err, m_err = spectral_parity(a, b_, 1e-6, zsig=zsig)
+                we, wz, wm = max(we, e_err), max(wz, z_err), max(wm, m_err)
+    print(f"anderson vs plain, nmax {nmax}: energy {we:.2e} Z {wz:.2e} moments {wm:.2e}; sweeps {its}", flush=True)
+    assert we < 1e-6 and wz < 1e-6 and wm < 1e-7
+    assert its[5] < its[0]

@@ -239,3 +239,35 @@ def test_syevd_batched():
         assert np.max(np.abs(w - wl)) < 1e-12 * max(1.0, np.abs(wl).max())
         assert np.max(np.abs(a @ v - v * w)) < 1e-11 * max(1.0, np.abs(wl).max())
         assert np.max(np.abs(v.T @ v - np.eye(len(w)))) < 1e-12
+
+
+def test_sigma2_rows_unstaged_fragments():
+    """Blocks whose fragment amplitudes exceed the LDS staging area (subshell 0
+    carries 130 quasiparticles) read them from HBM instead; the rows stay
+    bit-identical to the oracle's on both paths."""
+    from bcdor_amd import formats, synth
+    ms, vpp = synth.make_system(3, 0)
+    rng = np.random.default_rng(11)
+    ep, eh, fp, fh, npl, nhl = [], [], [], [], [], []
+    for i in range(ms.nsubsh):
+        nq, nh = (130, 1) if i == 0 else (2, 1)
+        for kind, cnt in ((0, nq), (1, nh)):
+            for _ in range(cnt):
+                x = np.zeros(ms.nspb)
+                x[:ms.no[i]] = rng.standard_normal(ms.no[i]) * 0.3
+                (fp if kind == 0 else fh).append(x)
+                (ep if kind == 0 else eh).append(rng.uniform(5, 90) if kind == 0 else rng.uniform(-50, -5))
+        npl.append(nq)
+        nhl.append(nh)
+    fp, fh = np.array(fp), np.array(fh)
+    g = formats.Propagator(np.array(npl, np.int32), np.array(nhl, np.int32), np.array(ep), (fp ** 2).sum(1), fp,
+                           np.array(eh), (fh ** 2).sum(1), fh)
+    S = po.System(ms, vpp)
+    P = po.Prop.from_propagator(S, g)
+    ctx = capi.Context(ms, vpp)
+    for i in range(ms.nsubsh):
+        fw, bk = ctx.sigma2_build(i, g)
+        wfw, wbk = po.build_sigma2(S, i, P)
+        assert fw.shape == wfw.shape and bk.shape == wbk.shape, i
+        assert np.array_equal(fw, wfw), (i, rel_err(fw, wfw))
+        assert np.array_equal(bk, wbk), (i, rel_err(bk, wbk))
