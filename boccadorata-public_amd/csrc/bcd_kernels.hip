@@ -653,6 +653,9 @@ __device__ __forceinline__ void buf_st(double x, __amdgpu_buffer_rsrc_t r, int v
 // earlier vectors in HBM, each thread touching only its own elements of
 // them; each Gram-Schmidt projection, latest vector first, is one block-wide
 // reduction of <v_i, v> and <v_i, v_i> followed by v -= (x1/x2) v_i.
+// Behind the nLanc vectors the workspace holds E and the NB coupling columns
+// of the input rows as contiguous vectors, and E * v_last of every finished
+// pivot chain, so the per-step sums read contiguous vectors only.
 template <int NT, int PER>
 __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, double *u, int *last, double *part) {
   const int tid = threadIdx.x, M = Jb.M, NB = Jb.NB, Tin = Jb.Tin, NA = Jb.NA_in;
@@ -660,8 +663,18 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
   const double *Min = Jb.Min;
   for (long i = tid; i < (long)Tout * nLanc; i += NT) Jb.Mout[i] = 0.0;
   // element p = tid + NT*t of a vector: byte offset 8*tid (lane) + 8*NT*t (scalar)
-  const __amdgpu_buffer_rsrc_t rV = buf_rsrc(Jb.V, 8L * nLanc * M), rM = buf_rsrc(Min, 8L * M * Tin);
+  const long nvec = lanczos_ws_vectors(nLanc, NB);
+  const __amdgpu_buffer_rsrc_t rV = buf_rsrc(Jb.V, 8L * nvec * M), rM = buf_rsrc(Min, 8L * M * Tin);
   const int vo = 8 * tid, mo = 8 * tid * Tin;
+  const int eo = 8 * nLanc * M, co = eo + 8 * M, wo = co + 8 * NB * M;   // E, couplings, E * v_last
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int p = tid + NT * t;
+    if (p < M) {
+      buf_st(buf_ld(rM, mo, 8 * NT * t * Tin), rV, vo, eo + 8 * NT * t);
+      for (int l = 0; l < NB; ++l) buf_st(buf_ld(rM, mo, 8 * (NT * t * Tin + NA + l)), rV, vo, co + 8 * (l * M + NT * t));
+    }
+  }
   double v[PER], vi[PER];
   int i_pvt = -1, n_pvt = -1, j_next = 0, ph = 0;
   int status = 0;
@@ -686,17 +699,17 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
         const int p = tid + NT * t;
         double acc = 0.0;
         if (p < M)
-          for (int l = 0; l < NB; ++l) acc += buf_ld(rM, mo, 8 * (NT * t * Tin + NA + l)) * u[l];
+          for (int l = 0; l < NB; ++l) acc += buf_ld(rV, vo, co + 8 * (l * M + NT * t)) * u[l];
         v[t] = acc;
       }
     } else {
 #pragma unroll
       for (int t = 0; t < PER; ++t) {
         const int p = tid + NT * t;
-        v[t] = p < M ? v[t] * buf_ld(rM, mo, 8 * NT * t * Tin) : 0.0;
+        v[t] = p < M ? v[t] * buf_ld(rV, vo, eo + 8 * NT * t) : 0.0;
       }
     }
-    // Gram-Schmidt against every earlier vector, latest first (Lanczos.cpp:164-173)
+  // Gram-Schmidt against every earlier vector, latest first (Lanczos.cpp:164-173)
     for (int i = j - 1; i >= 0; --i) {
       const int io = 8 * i * M;
       double s[2] = {0.0, 0.0};
@@ -723,20 +736,21 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
           const int p = tid + NT * t;
-          if (p < M) acc += (k == 0 ? buf_ld(rM, mo, 8 * NT * t * Tin) * v[t] : v[t]) * v[t];
+          if (p < M) acc += (k == 0 ? buf_ld(rV, vo, eo + 8 * NT * t) * v[t] : v[t]) * v[t];
         }
       } else if (k < 2 + NB) {
+        const int lo = co + 8 * (k - 2) * M;
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
           const int p = tid + NT * t;
-          if (p < M) acc += buf_ld(rM, mo, 8 * (NT * t * Tin + NA + k - 2)) * v[t];
+          if (p < M) acc += buf_ld(rV, vo, lo + 8 * NT * t) * v[t];
         }
       } else {
-        const int lo = 8 * last[k - 2 - NB] * M;
+        const int lo = wo + 8 * (k - 2 - NB) * M;
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
           const int p = tid + NT * t;
-          if (p < M) acc += buf_ld(rV, vo, lo + 8 * NT * t) * buf_ld(rM, mo, 8 * NT * t * Tin) * v[t];
+          if (p < M) acc += buf_ld(rV, vo, lo + 8 * NT * t) * v[t];
         }
       }
       acc = wave_sum(acc);
@@ -760,11 +774,16 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
       if (fresh) row[1] = row[1 + n_pvt];
     }
     if (n_pvt + 2 + NB > Tout) { status = -200; break; }
+    // a chain that ends here leaves E * v_last for the g_i of later rows
+    const bool chain_end = j == last[n_pvt] && j + 1 < nLanc;
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int p = tid + NT * t;
       v[t] /= xb;
-      if (p < M) buf_st(v[t], rV, vo, 8 * (j * M + NT * t));
+      if (p < M) {
+        buf_st(v[t], rV, vo, 8 * (j * M + NT * t));
+        if (chain_end) buf_st(v[t] * buf_ld(rV, vo, eo + 8 * NT * t), rV, vo, wo + 8 * (n_pvt * M + NT * t));
+      }
     }
   }
   if (tid == 0) {

@@ -14,6 +14,10 @@ constexpr int kMaxNB = 16;        // coupling columns of a Lanczos input row
 constexpr int kLancRed = 2 + kMaxNB + kMaxPiv;
 constexpr int kMaxNI = 16;        // orbits per subshell
 constexpr int kMaxSub = 64;       // subshells per instance
+// Lanczos workspace of one (channel, side), in vectors of M doubles: the
+// Krylov vectors, then E, the nb coupling columns and E * v_last of up to
+// kMaxPiv pivot chains
+__host__ __device__ inline long lanczos_ws_vectors(long nlanc, int nb) { return nlanc + 1 + nb + kMaxPiv; }
 constexpr int kMixMax = 8;        // iterates of the Anderson density mixing
 constexpr double kMixStart = 1e-3;  // density residual norm below which the mixing starts
 
