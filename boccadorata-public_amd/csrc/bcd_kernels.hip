@@ -990,6 +990,80 @@ __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs
 template __global__ void k_tridiag<true>(const DysonJob *, int);
 template __global__ void k_tridiag<false>(const DysonJob *, int);
 
+// sqrt(a^2 + b^2) without the library hypot's scaling where no over- or
+// underflow can occur (the arguments are uniform across the wave).
+__device__ __forceinline__ double ql_hyp(double a, double b) {
+  const double m = fmax(fabs(a), fabs(b));
+  if (m > 1e150 || (m < 1e-150 && m > 0.0)) return hypot(a, b);
+  return sqrt(a * a + b * b);
+}
+
+__device__ __forceinline__ double lane_get(double x, int l) {
+  const long long v = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The implicit QL iteration below for n <= 64: lane i holds d_i and e_i in
+// registers, the rotation chain reads them with readlane and each lane keeps
+// its own entry up to date; the first negligible e_m at or after lo comes
+// from one ballot.  The tracked rows stay in LDS (lane r owns row r).  On
+// return d holds the eigenvalues.  Returns 1 when an eigenvalue needs more
+// than 60 iterations.
+__device__ int ql_reg(double *d, double *e, double *Q, int n, int nI, int lane) {
+  const double eps = 2.2204460492503131e-16;
+  double dr = lane < n ? d[lane] : 0.0, er = lane < n ? e[lane] : 0.0;
+  int fail = 0;
+  for (int lo = 0; lo < n && !fail; ++lo) {
+    for (int it = 0;; ++it) {
+      const double dn = __shfl_down(dr, 1, 64);
+      const bool small = lane >= lo && lane < n && (lane == n - 1 || fabs(er) <= eps * (fabs(dr) + fabs(dn)));
+      const int m = __ffsll((long long)__ballot(small)) - 1;
+      if (m == lo) break;
+      if (it >= 60) { fail = 1; break; }
+      const double dlo = lane_get(dr, lo), elo = lane_get(er, lo);
+      double g = (lane_get(dr, lo + 1) - dlo) / (2.0 * elo);
+      double r = ql_hyp(g, 1.0);
+      double di1 = lane_get(dr, m);
+      g = di1 - dlo + elo / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      bool defl = false;
+      for (int i = m - 1; i >= lo; --i) {
+        const double ei = lane_get(er, i), di = lane_get(dr, i);
+        const double f = s * ei, b = c * ei;
+        r = ql_hyp(f, g);
+        if (lane == i + 1) er = r;
+        if (r == 0.0) {
+          if (lane == i + 1) dr = di1 - pp;
+          if (lane == m) er = 0.0;
+          defl = true;
+          break;
+        }
+        s = f / r; c = g / r;
+        g = di1 - pp;
+        r = (di - g) * s + 2.0 * c * b;
+        pp = s * r;
+        if (lane == i + 1) dr = g + pp;
+        g = c * r - b;
+        if (lane < nI) {
+          double *row = Q + (long)lane * n;
+          const double t = row[i + 1];
+          row[i + 1] = s * row[i] + c * t;
+          row[i] = c * row[i] - s * t;
+        }
+        di1 = di;
+      }
+      if (defl) continue;
+      if (lane == lo) { dr -= pp; er = g; }
+      if (lane == m) er = 0.0;
+    }
+  }
+  if (lane < n) d[lane] = dr;
+  __syncthreads();
+  return fail;
+}
+
 // Implicit QL with shifts on the tridiagonal (Bowdler, Martin, Reinsch and
 // Wilkinson 1968), rotations applied to the nI tracked rows; then the split
 // at the Fermi energy, the descending sort of quasiparticles and quasiholes
@@ -1009,6 +1083,9 @@ __global__ __launch_bounds__(64) void k_ql(const DysonJob *jobs, int njobs, int 
   __syncthreads();
   const double eps = 2.2204460492503131e-16;
   int fail = 0;
+  if (n <= 64) {
+    fail = ql_reg(d, e, Q, n, nI, lane);
+  } else
   for (int lo = 0; lo < n && !fail; ++lo) {
     for (int it = 0;; ++it) {
       int m = lo;
