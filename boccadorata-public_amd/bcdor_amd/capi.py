@@ -50,7 +50,8 @@ EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_creat
            "bcd_count_poles", "bcd_sigma2_build", "bcd_hf_pot", "bcd_reduce_mtx_lncz", "bcd_reduce_selfen_lncz",
            "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_destroy", "bcd_batch_solve",
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
-           "bcd_batch_kernel_times", "bcd_batch_work", "bcd_ladd_count", "bcd_ladd_basis", "bcd_ladd_fill",
+           "bcd_batch_kernel_times", "bcd_batch_work", "bcd_batch_autotune", "bcd_batch_lanczos_shapes",
+           "bcd_ladd_count", "bcd_ladd_basis", "bcd_ladd_fill",
            "bcd_ph_count", "bcd_ph_basis", "bcd_ph_fill", "bcd_t2_create", "bcd_t2_destroy", "bcd_t2_count",
            "bcd_t2_info", "bcd_t2_basis", "bcd_t2_build", "bcd_t2_matrix", "bcd_t2_set_profile",
            "bcd_t2_kernel_times", "bcd_t2_work"]
@@ -102,6 +103,8 @@ def lib():
         L.bcd_batch_solve.argtypes = [vp]
         L.bcd_batch_set_profile.argtypes = [vp, C.c_int]
         L.bcd_batch_timings.argtypes = [vp, DP, C.c_int]
+        L.bcd_batch_autotune.argtypes = [vp]
+        L.bcd_batch_lanczos_shapes.argtypes = [vp, IP, C.POINTER(C.c_float), C.c_int]
         L.bcd_batch_stats.argtypes = [vp, C.c_int, IP, IP, DP, DP, LP, LP]
         L.bcd_batch_converged.argtypes = [vp, C.c_int]
         L.bcd_batch_result.argtypes = [vp, C.c_int, C.c_int, IP, IP, IP, IP, DP, DP, DP, DP, DP, DP]
@@ -421,6 +424,14 @@ class Context(_TwoBodyMixin):
                 for k in range(len(mats))]
 
 
+# k_lanczos tile shapes (threads per job, elements per thread) and the upper
+# pole counts of the buckets the autotuner picks them for
+LANC_SHAPES = [(64, 4), (64, 8), (128, 8), (192, 8), (256, 8), (128, 16), (192, 16), (256, 16), (384, 8), (192, 24),
+               (256, 24), (384, 16), (384, 24), (768, 24), (1024, 24)]
+LANC_BUCKET_MAX = [256, 512, 1024, 2048, 4096, 8192, 16384, 24576]
+LANC_BUCKETS = len(LANC_BUCKET_MAX)
+
+
 def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1,
                  hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1, dyson_solver=0, precision=0,
                  accel=0) -> BcdSolveParams:
@@ -462,6 +473,18 @@ class Batch:
 
     def solve(self):
         check(lib().bcd_batch_solve(self.h), "bcd_batch_solve")
+
+    def autotune(self):
+        """Pick the Lanczos tile shape of every pole-count bucket during the next solve."""
+        check(lib().bcd_batch_autotune(self.h), "bcd_batch_autotune")
+
+    def lanczos_shapes(self):
+        """(shape index per bucket, timed ms per (bucket, shape), 0 where not timed)."""
+        sh = np.zeros(LANC_BUCKETS, np.int32)
+        ms = np.zeros((LANC_BUCKETS, len(LANC_SHAPES)), np.float32)
+        check(lib().bcd_batch_lanczos_shapes(self.h, ip(sh), ms.ctypes.data_as(C.POINTER(C.c_float)), LANC_BUCKETS),
+              "bcd_batch_lanczos_shapes")
+        return sh, ms
 
     def timings(self):
         t = np.zeros(6)

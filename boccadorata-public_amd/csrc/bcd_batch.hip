@@ -498,19 +498,46 @@ int Batch::build_sigma2(int only_ish) {
   return BCD_OK;
 }
 
+// Time every tile shape that holds the bucket's largest job (and is at most
+// four times wider than the bucket) on the bucket's own jobs, keep the
+// fastest.  The jobs are idempotent: each run rebuilds the same outputs.
+int Batch::tune_bucket(int bk, int mmax, int nj, const int *ord) {
+  hipEvent_t e0, e1;
+  HTRY(hipEventCreate(&e0));
+  HTRY(hipEventCreate(&e1));
+  int best = lanc_shape[bk];
+  float best_ms = 0.f;
+  for (int sh = 0; sh < kNLancShapes; ++sh) {
+    const long cap = (long)kLancShapes[sh].nt * kLancShapes[sh].per;
+    if (!lanczos_shape_fits(sh, mmax) || cap > 4L * kLancBucket[bk]) continue;
+    HTRY(hipEventRecord(e0, stream));
+    launch_lanczos(sh, nj, d_ljobs, ord, stream);
+    HTRY(hipEventRecord(e1, stream));
+    HTRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HTRY(hipEventElapsedTime(&ms, e0, e1));
+    lanc_tune_ms[bk][sh] = ms;
+    if (best_ms == 0.f || ms < best_ms) { best_ms = ms; best = sh; }
+  }
+  lanc_shape[bk] = best;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return BCD_OK;
+}
+
 int Batch::lanczos() {
   PropView P = view(true);
   hipLaunchKernelGGL(k_lanczos_prep, dim3((n_gc + 63) / 64), dim3(64), 0, stream, d_chans, 0, n_gc, P, d_cs, d_piv,
                      par.lanc_kind, par.lanc_n, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_lrows,
                      (double *)a_lws.ptr, d_ljobs);
   HTRY(hipGetLastError());
-  // jobs longest first, grouped by the block size their M needs; one launch
-  // per group, the largest blocks first
+  // jobs longest first, grouped into pole-count buckets; one launch per
+  // bucket with the bucket's tile shape, the largest jobs first
   std::vector<std::pair<int, int>> jm;
   for (int gc = 0; gc < n_gc; ++gc)
     for (int side = 0; side < 2; ++side) {
       const int M = side ? h_cs[gc].mbk : h_cs[gc].mfw;
-      if (M > 0 && !lanczos_threads(M)) return BCD_ERR_CAPACITY;
+      if (M > 0 && lanczos_bucket(M) < 0) return BCD_ERR_CAPACITY;
       jm.push_back({M, 2 * gc + side});
     }
   std::stable_sort(jm.begin(), jm.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
@@ -522,19 +549,15 @@ int Batch::lanczos() {
   kt_begin(KT_LANCZOS);
   size_t k0 = 0;
   while (k0 < jm.size()) {
-    const int nt = jm[k0].first > 0 ? lanczos_threads(jm[k0].first) : 192;
+    const int bk = jm[k0].first > 0 ? lanczos_bucket(jm[k0].first) : 0;
     size_t k1 = k0;
-    while (k1 < jm.size() && (jm[k1].first > 0 ? lanczos_threads(jm[k1].first) : 192) == nt) ++k1;
-    const int nj = (int)(k1 - k0);
-    const int *ord = d_lorder + k0;
-    switch (nt) {
-      case 192: hipLaunchKernelGGL(k_lanczos<192>, dim3(nj), dim3(192), 0, stream, d_ljobs, ord, nj); break;
-      case 384: hipLaunchKernelGGL(k_lanczos<384>, dim3(nj), dim3(384), 0, stream, d_ljobs, ord, nj); break;
-      case 768: hipLaunchKernelGGL(k_lanczos<768>, dim3(nj), dim3(768), 0, stream, d_ljobs, ord, nj); break;
-      default: hipLaunchKernelGGL(k_lanczos<1024>, dim3(nj), dim3(1024), 0, stream, d_ljobs, ord, nj); break;
-    }
+    while (k1 < jm.size() && (jm[k1].first > 0 ? lanczos_bucket(jm[k1].first) : 0) == bk) ++k1;
+    const int nj = (int)(k1 - k0), mmax = std::max(jm[k0].first, 1);
+    if (tune_lanczos) BTRY(tune_bucket(bk, mmax, nj, d_lorder + k0));
+    launch_lanczos(lanc_shape[bk], nj, d_ljobs, d_lorder + k0, stream);
     k0 = k1;
   }
+  tune_lanczos = false;
   kt_end(KT_LANCZOS);
   HTRY(hipGetLastError());
   HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));

@@ -98,6 +98,13 @@ struct Batch {
   int check(int &n_active);
   int stage(int src, int itr_max, double conv, bool hf);
   int build_sigma2(int only_ish);
+  int tune_bucket(int bk, int mmax, int nj, const int *ord);
+  // Lanczos tile shape per pole-count bucket; tune_lanczos: pick them by
+  // timing during the next solve
+  int lanc_shape[kNLancBuckets] = {kLancDefault[0], kLancDefault[1], kLancDefault[2], kLancDefault[3],
+                                   kLancDefault[4], kLancDefault[5], kLancDefault[6], kLancDefault[7]};
+  float lanc_tune_ms[kNLancBuckets][kNLancShapes] = {};
+  bool tune_lanczos = false;
   int prepare();
   int work(double *w, int n);
   double coupling_flops = -1.0;

@@ -143,13 +143,9 @@ int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout
   J.n_out = (int *)dn.p; J.V = (double *)dV.p; J.inst = -1;
   HTRY(hipMemcpy(djob.p, &J, sizeof J, hipMemcpyHostToDevice));
   const LancJob *jp = (const LancJob *)djob.p;
-  switch (lanczos_threads(NPLS_in)) {
-    case 192: hipLaunchKernelGGL(k_lanczos<192>, dim3(1), dim3(192), 0, ctx->stream, jp, nullptr, 1); break;
-    case 384: hipLaunchKernelGGL(k_lanczos<384>, dim3(1), dim3(384), 0, ctx->stream, jp, nullptr, 1); break;
-    case 768: hipLaunchKernelGGL(k_lanczos<768>, dim3(1), dim3(768), 0, ctx->stream, jp, nullptr, 1); break;
-    case 1024: hipLaunchKernelGGL(k_lanczos<1024>, dim3(1), dim3(1024), 0, ctx->stream, jp, nullptr, 1); break;
-    default: return BCD_ERR_CAPACITY;
-  }
+  const int bk = lanczos_bucket(NPLS_in);
+  if (bk < 0) return BCD_ERR_CAPACITY;
+  launch_lanczos(kLancDefault[bk], 1, jp, nullptr, ctx->stream);
   HTRY(hipGetLastError());
   HTRY(hipStreamSynchronize(ctx->stream));
   int nout = 0;
@@ -606,6 +602,22 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches) 
   if (!b || k < 0 || k >= KT_N || !avg_ms || !launches) return BCD_ERR_ARG;
   *launches = b->b.kcount[k];
   *avg_ms = b->b.kcount[k] ? b->b.ktime[k] / b->b.kcount[k] : 0.0;
+  return BCD_OK;
+}
+
+int bcd_batch_autotune(bcd_batch *b) {
+  if (!b) return BCD_ERR_ARG;
+  b->b.tune_lanczos = true;
+  return BCD_OK;
+}
+
+int bcd_batch_lanczos_shapes(bcd_batch *b, int *shapes, float *tune_ms, int n) {
+  if (!b || !shapes || n < kNLancBuckets) return BCD_ERR_ARG;
+  for (int k = 0; k < kNLancBuckets; ++k) {
+    shapes[k] = b->b.lanc_shape[k];
+    if (tune_ms)
+      for (int sh = 0; sh < kNLancShapes; ++sh) tune_ms[k * kNLancShapes + sh] = b->b.lanc_tune_ms[k][sh];
+  }
   return BCD_OK;
 }
 

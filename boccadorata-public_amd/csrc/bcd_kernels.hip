@@ -609,6 +609,12 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
 // compile-time count so that vals stays in registers.
 template <int NT, int NV>
 __device__ __forceinline__ void lanc_reduce(double *vals, double *red, int &ph) {
+  if constexpr (NT == 64) {
+    // one wave: the shuffle sums are the totals
+#pragma unroll
+    for (int k = 0; k < NV; ++k) vals[k] = wave_sum(vals[k]);
+    return;
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double *slot = red + ph * (NT / 64) * kLancRed;
 #pragma unroll
@@ -799,9 +805,8 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
 }
 
 // One block of NT threads per job, jobs taken in the given order (longest
-// first); the register tile per thread is 8 or 24 elements, whichever holds
-// the job's M (the host keeps M <= 24*NT).
-template <int NT>
+// first), PER elements of each vector per thread (the host keeps M <= NT*PER).
+template <int NT, int PER>
 __global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, const int *order, int njobs) {
   __shared__ double red[2 * (NT / 64) * kLancRed];
   __shared__ double u[kMaxNB];
@@ -809,15 +814,37 @@ __global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, const int *
   __shared__ int last[kMaxPiv + 1];
   if ((int)blockIdx.x >= njobs) return;
   const LancJob Jb = jobs[order ? order[blockIdx.x] : blockIdx.x];
-  if (Jb.nLanc <= 0 || Jb.M > kLancPer * NT) return;
-  if (Jb.M <= 8 * NT) lanczos_job<NT, 8>(Jb, red, u, last, part);
-  else lanczos_job<NT, kLancPer>(Jb, red, u, last, part);
+  if (Jb.nLanc <= 0 || Jb.M > PER * NT) return;
+  lanczos_job<NT, PER>(Jb, red, u, last, part);
 }
 
-template __global__ void k_lanczos<192>(const LancJob *, const int *, int);
-template __global__ void k_lanczos<384>(const LancJob *, const int *, int);
-template __global__ void k_lanczos<768>(const LancJob *, const int *, int);
-template __global__ void k_lanczos<1024>(const LancJob *, const int *, int);
+template <int I>
+static void launch_shape(int njobs, const LancJob *jobs, const int *order, hipStream_t stream) {
+  constexpr int NT = kLancShapes[I].nt, PER = kLancShapes[I].per;
+  hipLaunchKernelGGL((k_lanczos<NT, PER>), dim3(njobs), dim3(NT), 0, stream, jobs, order, njobs);
+}
+
+void launch_lanczos(int shape, int njobs, const LancJob *jobs, const int *order, hipStream_t stream) {
+  if (njobs <= 0) return;
+  switch (shape) {
+    case 0: launch_shape<0>(njobs, jobs, order, stream); break;
+    case 1: launch_shape<1>(njobs, jobs, order, stream); break;
+    case 2: launch_shape<2>(njobs, jobs, order, stream); break;
+    case 3: launch_shape<3>(njobs, jobs, order, stream); break;
+    case 4: launch_shape<4>(njobs, jobs, order, stream); break;
+    case 5: launch_shape<5>(njobs, jobs, order, stream); break;
+    case 6: launch_shape<6>(njobs, jobs, order, stream); break;
+    case 7: launch_shape<7>(njobs, jobs, order, stream); break;
+    case 8: launch_shape<8>(njobs, jobs, order, stream); break;
+    case 9: launch_shape<9>(njobs, jobs, order, stream); break;
+    case 10: launch_shape<10>(njobs, jobs, order, stream); break;
+    case 11: launch_shape<11>(njobs, jobs, order, stream); break;
+    case 12: launch_shape<12>(njobs, jobs, order, stream); break;
+    case 13: launch_shape<13>(njobs, jobs, order, stream); break;
+    default: launch_shape<14>(njobs, jobs, order, stream); break;
+  }
+}
+static_assert(kNLancShapes == 15, "launch_lanczos covers every shape");
 
 // ------------------------------------------------------------- Dyson
 

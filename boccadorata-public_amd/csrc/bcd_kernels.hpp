@@ -42,15 +42,32 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
                                const long *row_bk_base, double *rows_fw, double *rows_bk, double *lrows,
                                double *lws, LancJob *jobs);
 constexpr int kLancPer = 24;      // largest register tile (elements per thread) of k_lanczos
-template <int NT>
+template <int NT, int PER>
 __global__ void k_lanczos(const LancJob *jobs, const int *order, int njobs);
-// Threads per block that k_lanczos needs for a side of M poles (0: too many).
-inline int lanczos_threads(int M) {
-  const int nts[4] = {192, 384, 768, 1024};
-  for (int nt : nts)
-    if (M <= kLancPer * nt) return nt;
-  return 0;
+// Tile shapes of k_lanczos: NT threads per (channel, side) job, PER
+// elements of each vector per thread, so a job of M poles needs
+// M <= NT * PER.  Jobs are grouped by M into buckets (upper bounds
+// kLancBucket); every bucket has a default shape, and the batch autotuner can
+// pick another one by timing the bucket's own jobs.
+struct LancShape { int nt, per; };
+constexpr LancShape kLancShapes[] = {{64, 4},   {64, 8},   {128, 8},   {192, 8},   {256, 8},
+                                     {128, 16}, {192, 16}, {256, 16},  {384, 8},   {192, 24},
+                                     {256, 24}, {384, 16}, {384, 24},  {768, 24},  {1024, 24}};
+constexpr int kNLancShapes = sizeof(kLancShapes) / sizeof(kLancShapes[0]);
+constexpr int kLancBucket[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 24576};
+constexpr int kNLancBuckets = sizeof(kLancBucket) / sizeof(kLancBucket[0]);
+inline int lanczos_bucket(int M) {
+  for (int b = 0; b < kNLancBuckets; ++b)
+    if (M <= kLancBucket[b]) return b;
+  return -1;
 }
+// default shape of each bucket
+constexpr int kLancDefault[kNLancBuckets] = {0, 1, 2, 4, 9, 12, 13, 14};
+inline bool lanczos_shape_fits(int shape, int M) {
+  return shape >= 0 && shape < kNLancShapes && M <= kLancShapes[shape].nt * kLancShapes[shape].per;
+}
+// Launch shape `shape` of k_lanczos over njobs jobs (order may be NULL).
+void launch_lanczos(int shape, int njobs, const LancJob *jobs, const int *order, hipStream_t stream);
 __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, const ChanState *cs,
                              PropView P, int src, const double *h, const double *u1,
                              const long *row_fw_base, const long *row_bk_base, const double *rows_fw,

@@ -200,6 +200,18 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
  * block time (ms summed over blocks) per phase: load and sort, deflation,
  * roots, Loewner couplings, eigenvectors, reorder, split and output, norms. */
 int bcd_batch_work(bcd_batch *b, double *work, int n);
+/* Tile-shape autotuner of the Lanczos kernel (no counterpart in the
+ * reference, which runs one Lanczos at a time on the CPU).  Jobs are grouped
+ * by pole count into 8 buckets (M <= 256, 512, 1024, 2048, 4096, 8192, 16384,
+ * 24576); during the next bcd_batch_solve every shape that fits a bucket's
+ * largest job is timed on that bucket's own jobs and the fastest is kept for
+ * later solves.  Shapes differ only in how the block sums are split, so
+ * results agree to rounding. */
+int bcd_batch_autotune(bcd_batch *b);
+/* Shape index chosen per bucket (n >= 8) and, if tune_ms is not NULL, the
+ * timed ms of every (bucket, shape) pair as tune_ms[bucket * 15 + shape]
+ * (0 where not timed). */
+int bcd_batch_lanczos_shapes(bcd_batch *b, int *shapes, float *tune_ms, int n);
 
 /* ------------------------------------------ two-body propagator matrices
  * The pp/hh ladder of gII_prop_t (gII_b.cpp:62-563) and the ph ring of

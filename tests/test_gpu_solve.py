@@ -187,3 +187,44 @@ def test_anderson_mixing_same_fixed_point(nmax):
     print(f"anderson vs plain, nmax {nmax}: energy {we:.2e} Z {wz:.2e} moments {wm:.2e}; sweeps {its}", flush=True)
     assert we < 1e-6 and wz < 1e-6 and wm < 1e-7
     assert its[5] < its[0]
+
+
+def test_lanczos_autotune():
+    """The tile-shape autotuner times every shape that fits each pole-count
+    bucket on the batch's own Lanczos jobs during one solve and keeps the
+    fastest.  Shapes only regroup the block sums, so the tuned solve agrees
+    with the default-shape solve to rounding (spectral parity at 1e-10), and
+    later solves with the chosen shapes repeat it bit for bit."""
+    systems = [synth.make_system(5, s) for s in (0, 1, 2)]
+    par = capi.solve_params(mode=2, hf_itr_max=100, sc_itr_max=60, lanczos=True, lanc_kind=1, lanc_n=50,
+                            conv_kind=1, hf_conv=1e-8, sc_conv=1e-8)
+    b = capi.Batch(systems, par)
+    b.solve()
+    ref = [b.result(i) for i in range(len(systems))]
+    default, _ = b.lanczos_shapes()
+    b.autotune()
+    b.solve()
+    shapes, ms = b.lanczos_shapes()
+    tuned = [b.result(i) for i in range(len(systems))]
+    b.solve()
+    again = [b.result(i) for i in range(len(systems))]
+    b.close()
+    print("default", default.tolist(), "tuned", shapes.tolist(), flush=True)
+    timed = np.flatnonzero(ms.max(axis=1) > 0)
+    assert timed.size > 0
+    for bk in range(capi.LANC_BUCKETS):
+        nt, per = capi.LANC_SHAPES[shapes[bk]]
+        if bk in timed:
+            assert ms[bk, shapes[bk]] == ms[bk][ms[bk] > 0].min()
+        else:
+            assert shapes[bk] == default[bk]
+        assert nt * per >= (capi.LANC_BUCKET_MAX[bk - 1] + 1 if bk else 1)
+    zsig = ZSIG_BY_NMAX[5]
+    for g, r, a in zip(tuned, ref, again):
+        assert np.array_equal(g.ep, a.ep) and np.array_equal(g.fqp, a.fqp) and np.array_equal(g.eh, a.eh)
+        for i in range(len(r.np_)):
+            ep, zp, _, eh, zh, _ = g.subshell(i)
+            rp, rz, _, rh, rzh, _ = r.subshell(i)
+            for x, y in ((np.stack([ep, zp], 1), np.stack([rp, rz], 1)), (np.stack([eh, zh], 1), np.stack([rh, rzh], 1))):
+                e_err, z_err, m_err = spectral_parity(x, y, 1e-10, zsig=zsig)
+                assert e_err < 1e-10 and z_err < 1e-10 and m_err < 1e-10, (e_err, z_err, m_err)
