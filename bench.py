@@ -77,7 +77,11 @@ def screened_seeds(n):
 
 
 def timed(batch, n, steps, warmup, label):
-    """warmup untimed solves, then steps timed ones; returns (seconds, per-instance converged counts)."""
+    """warmup untimed solves (the first one also picks the Lanczos tile shapes
+    with the batch autotuner), then steps timed ones; returns (seconds,
+    per-instance converged counts)."""
+    if warmup > 0:
+        batch.autotune()
     for w in range(warmup):
         batch.solve()
         print(f"# {label} warmup {w} {batch.timings()}", file=sys.stderr, flush=True)
@@ -420,6 +424,9 @@ def main():
     print(f"# kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
     print(f"# arrowhead phases (block-ms) {json.dumps(work['arrow_phase_ms'])}", file=sys.stderr, flush=True)
     its = [batch.stats(i) for i in range(n)]
+    sh, tms = batch.lanczos_shapes()
+    shapes = {str(capi.LANC_BUCKET_MAX[k]): list(capi.LANC_SHAPES[sh[k]]) for k in range(capi.LANC_BUCKETS)
+              if tms[k].max() > 0}
     cpu = None if args.no_cpu else cpu_baseline(seeds, args.cpu_seconds)
     value = runs[head]["rate"]
     line = {
@@ -455,6 +462,9 @@ def main():
                                     "other solve: max rel. energy and abs. Z difference, and rel. error of the "
                                     "Z-weighted moments; against the FP64 plain-iteration GPU solve of the same "
                                     "instance",
+                   "lanczos_tile_shapes": shapes,
+                   "lanczos_tile_shapes_note": "autotuned per pole-count bucket (max M: [threads, elements per "
+                                               "thread]) during warmup",
                    "mean_hf_sweeps": float(np.mean([s["hf_its"] for s in its])),
                    "mean_sc0_sweeps": float(np.mean([s["sc_its"] for s in its]))},
         "roofline": rl,
