@@ -121,6 +121,12 @@ def spectral_error(g, ref, zsig=ZSIG):
     return worst
 
 
+# HBM bytes per launch of the dominant kernel family, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes of this workload (calibrated against a 1 GiB
+# copy): (kernel family, instances per step) -> (bytes, summary file)
+MEASURED_TRAFFIC = {("lanczos", 256): (307.1e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md")}
+
+
 def roofline(batch):
     """Per-kernel average launch time (hipEvents around every launch of one
     profiled solve) and the dominant kernel's achieved rate."""
@@ -166,6 +172,10 @@ def roofline(batch):
         entry["kernel"] = dom
         entry["avg_launch_ms"] = avg_ms
         entry["launches"] = launches
+        meas = MEASURED_TRAFFIC.get((dom, batch.n))
+        if meas:
+            entry["traffic"] = meas[0]
+            entry["traffic_note"] = f"HBM bytes per launch, rocprofv3 FETCH_SIZE/WRITE_SIZE, {meas[1]}"
     return entry, {k: {"avg_ms": v[0], "launches": v[1]} for k, v in kt.items()}, work
 
 
