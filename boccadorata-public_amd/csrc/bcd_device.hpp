@@ -72,7 +72,8 @@ struct ChanDev {
 // One (Jab, a <= b, g) block of the Sigma(2) build of channel gc; blocks of a
 // channel are stored in the reference's loop order.
 struct BlockDev {
-  int gc, a, b, g, J, nJab, nJgd, _pad;
+  int gc, a, b, g, J, nJab, nJgd;
+  int c2b;                       // two-body channel of the block's (alpha beta; J) pairs, -1: none
 };
 
 // Per-channel dynamic state (fragment counts live in separate arrays).

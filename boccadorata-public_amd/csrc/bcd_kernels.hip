@@ -358,7 +358,8 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
 template <class T, class A, class S>
 __device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *xg, const T *vt, int noa, int nob,
                                              int nog, int nI, int npole, int a_eq_b, int nJab, int nJgd, int cA,
-                                             int cB, int cG, int orb_g, int orb_d0, double *rows, int w) {
+                                             int cB, int cG, int orb_g, int orb_d0, double *rows, int w,
+                                             double *rb) {
 #pragma clang fp contract(off)
   const T xJab = (T)(1 - 2 * nJab);
   const T kS2 = (T)kSqrt2;
@@ -396,7 +397,8 @@ __device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *
       }
     }
     x1 *= (A)x4;
-    rows[(long)p * w + 2 + d] = (double)x1;
+    if (rb) rb[q] = (double)x1;
+    else rows[(long)p * w + 2 + d] = (double)x1;
   }
 }
 
@@ -415,13 +417,16 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
                                                const int *blk_fw_off, const int *blk_bk_off,
                                                const int *work_list, const int *work_count,
                                                const long *row_fw_base, const long *row_bk_base,
-                                               double *rows_fw, double *rows_bk, int vt_cap) {
+                                               double *rows_fw, double *rows_bk, int vt_cap, int c_cap) {
 #pragma clang fp contract(off)
   // dynamic LDS: the block's V table (vt_cap entries, the batch's largest),
-  // then kSigStage staged amplitudes
+  // kSigStage staged amplitudes, kSigRowBuf pass-1 couplings, then the pair
+  // rows of (alpha, beta) and (d, gamma) with the exchange signs of the latter
   extern __shared__ double sig_lds[];
   T *vt = (T *)sig_lds;
   T *stg = (T *)(sig_lds + vt_cap);
+  double *rbuf = sig_lds + vt_cap + kSigStage;
+  int *rowA = (int *)(rbuf + kSigRowBuf), *rowC = rowA + kMaxPairs, *sgnC = rowC + c_cap;
   const int lane = threadIdx.x;
   const int total = *work_count;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -436,10 +441,16 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
     const int oa0 = I.orb0[a], ob0 = I.orb0[b], og0 = I.orb0[g];
     // (alpha, beta) pairs in the reference order (beta >= alpha + nJab when a == b)
     const int npair = (a == b) ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
-    const int nvt = nI * npair * nog;
+    const int nvt = nI * npair * nog, nC = nI * nog;
     const double x3 = sqrt((double)(J + J + 1) / (double)(1 + C.j2));
-    for (int e = lane; e < nvt; e += 64) {
-      const int ga_ = e % nog, pi = (e / nog) % npair, d = e / (nog * npair);
+    // V table Vt(d; alpha beta, gamma) = <alpha beta; J|V|d gamma; J> as VppInt_t::get
+    // returns it (Vpp_potential.cpp:557-607): the pair rows of (alpha, beta) and of
+    // (d, gamma) first, then one load per entry.  (alpha, beta) never needs the
+    // exchange swap; (d, gamma) does when d > gamma, with the phase
+    // (-1)^((j_d + j_gamma)/2 - J + 1).
+    const long pbase = (long)J * I.norb;
+    const bool jok = J <= I.jmax;
+    for (int pi = lane; pi < npair; pi += 64) {
       int al, be;
       if (a == b) {
         int r = pi, len = noa - nJab;
@@ -449,10 +460,33 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
       } else {
         al = pi / nob; be = pi % nob;
       }
+      rowA[pi] = jok ? I.pidx[(pbase + oa0 + al) * I.norb + ob0 + be] : -1;
+    }
+    const int phC = abs((C.j2 + I.j2[g]) / 2 - J + 1) & 1;
+    for (int q = lane; q < nC; q += 64) {
+      const int od = C.orb0 + q / nog, og = og0 + q % nog;
+      const int lo = od < og ? od : og, hi = od < og ? og : od;
+      rowC[q] = jok ? I.pidx[(pbase + lo) * I.norb + hi] : -1;
+      sgnC[q] = od > og ? phC : 0;
+    }
+    const long voff_b = B.c2b >= 0 ? I.voff[B.c2b] : 0;
+    const int np_b = B.c2b >= 0 ? I.npair[B.c2b] : 0;
+    __syncthreads();
+    for (int e = lane; e < nvt; e += 64) {
+      const int ga_ = e % nog, pi = (e / nog) % npair, d = e / (nog * npair);
       const int od = C.orb0 + d, og = og0 + ga_;
       double val = 0.0;
       if (!(og == od && nJgd)) {
-        val = x3 * vget(I, oa0 + al, ob0 + be, od, og, J);
+        const int pa = rowA[pi], q = d * nog + ga_, pc = rowC[q];
+        double v = 0.0;
+        if (pa >= 0 && pc >= 0 && (pa >> 16) == (pc >> 16)) {
+          const int ca = pa >> 16;
+          const long vo = ca == B.c2b ? voff_b : I.voff[ca];
+          const int np = ca == B.c2b ? np_b : I.npair[ca];
+          v = I.V[vo + (long)(pa & 0xffff) * np + (pc & 0xffff)];
+          if (sgnC[q]) v = -v;
+        }
+        val = x3 * v;
         if (og == od) val *= kSqrt2;
       }
       vt[e] = (T)val;
@@ -486,12 +520,14 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
         for (int k = lane; k < nsg; k += 64) stg[nsa + nsb + k] = (T)xG[k];
       }
       __syncthreads();
+      // pass-1 results stay in LDS for pass 2 when they fit
+      double *rb = npole * nI <= kSigRowBuf ? rbuf : nullptr;
       if (staged)
         sigma2_pass1<T, A, T>(stg, (a == b) ? stg : stg + nsa, stg + nsa + nsb, vt, noa, nob, nog, nI, npole,
-                              a == b, nJab, nJgd, cA, cB, cG, og0, C.orb0, rows, w);
+                              a == b, nJab, nJgd, cA, cB, cG, og0, C.orb0, rows, w, rb);
       else
         sigma2_pass1<T, A, double>(xA, xB, xG, vt, noa, nob, nog, nI, npole, a == b, nJab, nJgd, cA, cB, cG,
-                                   og0, C.orb0, rows, w);
+                                   og0, C.orb0, rows, w, rb);
       __syncthreads();
       // pass 2: energy and sum of squares (over d in order) per pole
       for (int p = lane; p < npole; p += 64) {
@@ -508,7 +544,15 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
         double *row = rows + (long)p * w;
         row[0] = eA[na] + eB[nb] - eG[ng];
         double sum2 = 0.0;
-        for (int d = 0; d < nI; ++d) sum2 += row[2 + d] * row[2 + d];
+        if (rb) {
+          for (int d = 0; d < nI; ++d) {
+            const double m = rb[p * nI + d];
+            row[2 + d] = m;
+            sum2 += m * m;
+          }
+        } else {
+          for (int d = 0; d < nI; ++d) sum2 += row[2 + d] * row[2 + d];
+        }
         row[1] = sum2;
       }
       __syncthreads();
@@ -519,7 +563,7 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
 #define BCD_SIGMA2_INST(T, A)                                                                               \
   template __global__ void k_sigma2<T, A>(const InstDev *, const ChanDev *, const BlockDev *, PropView, const int *, \
                                           const int *, const int *, const int *, const long *, const long *,       \
-                                          double *, double *, int);
+                                          double *, double *, int, int);
 BCD_SIGMA2_INST(double, double)
 BCD_SIGMA2_INST(float, double)
 #undef BCD_SIGMA2_INST

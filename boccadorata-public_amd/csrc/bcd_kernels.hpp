@@ -8,7 +8,8 @@ namespace bcd {
 
 constexpr int kMaxVt = 2048;      // nI * (alpha,beta) pairs * gamma of one Sigma(2) block
 constexpr int kMaxPairs = 64;     // (alpha,beta) pairs of one block (+1 slot for the count)
-constexpr int kSigStage = 256;    // fragment amplitudes of one Sigma(2) block side staged in LDS
+constexpr int kSigStage = 128;    // fragment amplitudes of one Sigma(2) block side staged in LDS
+constexpr int kSigRowBuf = 128;   // (pole, orbit) couplings of one block side kept in LDS for pass 2
 constexpr int kMaxPiv = 8;        // Lanczos pivots per channel
 constexpr int kMaxNB = 16;        // coupling columns of a Lanczos input row
 constexpr int kLancRed = 2 + kMaxNB + kMaxPiv;
@@ -34,9 +35,13 @@ template <class T, class A>
 __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const BlockDev *blocks, PropView P,
                          const int *blk_fw_off, const int *blk_bk_off, const int *work_list,
                          const int *work_count, const long *row_fw_base, const long *row_bk_base,
-                         double *rows_fw, double *rows_bk, int vt_cap);
+                         double *rows_fw, double *rows_bk, int vt_cap, int c_cap);
 // dynamic LDS bytes of k_sigma2 for a V table of vt_cap doubles
-inline size_t sigma2_lds(int vt_cap) { return sizeof(double) * ((size_t)vt_cap + kSigStage); }
+// dynamic LDS bytes of k_sigma2: V table of vt_cap entries, the staged
+// amplitudes and couplings, and the pair rows (c_cap = the largest nI * n_g)
+inline size_t sigma2_lds(int vt_cap, int c_cap) {
+  return sizeof(double) * ((size_t)vt_cap + kSigStage + kSigRowBuf) + sizeof(int) * (kMaxPairs + 2 * (size_t)c_cap);
+}
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P, ChanState *cs,
                                double *piv, int lanc_kind, int lanc_n, const long *row_fw_base,
                                const long *row_bk_base, double *rows_fw, double *rows_bk, double *lrows,

@@ -95,6 +95,7 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
   mbound_fw.assign(nsubsh, 0);
   mbound_bk.assign(nsubsh, 0);
   max_vt = 0;
+  max_dg = 0;
   for (int I = 0; I < nsubsh; ++I) {
     for (int J = 0; J <= jmax; ++J)
       for (int a = 0; a < nsubsh; ++a)
@@ -111,8 +112,17 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
             if (np < 0) np = 0;
             if (np >= kMaxPairs - 1 || (long)no[I] * np * no[g] > kMaxVt) return BCD_ERR_CAPACITY;
             max_vt = std::max(max_vt, no[I] * np * no[g]);
+            max_dg = std::max(max_dg, no[I] * no[g]);
             BlockDev B;
-            B.gc = I; B.a = a; B.b = b; B.g = g; B.J = J; B.nJab = nJab; B.nJgd = nJgd; B._pad = 0;
+            B.gc = I; B.a = a; B.b = b; B.g = g; B.J = J; B.nJab = nJab; B.nJgd = nJgd;
+            // every (alpha beta) pair of the block lies in one two-body channel
+            // (same J, parity and charge); take it from the first stored pair
+            B.c2b = -1;
+            for (int al = 0; al < no[a] && B.c2b < 0; ++al)
+              for (int be = (a == b) ? al + nJab : 0; be < no[b] && B.c2b < 0; ++be) {
+                const int pa = pidx[((size_t)J * norb + orb0[a] + al) * norb + orb0[b] + be];
+                if (pa >= 0) B.c2b = pa >> 16;
+              }
             blocks[I].push_back(B);
             const long pairs = (a == b) ? (long)no[a] * (no[a] + 1) / 2 : (long)no[a] * no[b];
             mbound_fw[I] += pairs * no[g];

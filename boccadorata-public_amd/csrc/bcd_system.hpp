@@ -22,6 +22,7 @@ struct HostSystem {
   std::vector<std::vector<BlockDev>> blocks;   // [ish]
   std::vector<long> mbound_fw, mbound_bk;      // pole-count bounds for HF-like propagators
   int max_vt = 0;                              // largest nI * pairs * n_g of a Sigma(2) block
+  int max_dg = 0;                              // largest nI * n_g of a Sigma(2) block
   long dropped = 0;                            // interaction keys outside the pair tables
 
   int build(const bcd_modspace *ms, const bcd_vpp *vpp);
