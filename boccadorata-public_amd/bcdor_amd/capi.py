@@ -57,7 +57,7 @@ EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_creat
            "bcd_t2_kernel_times", "bcd_t2_work"]
 
 KERNEL_NAMES = ["sigma2", "lanczos", "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild", "sigma2_plan",
-                "arrow", "poles"]
+                "arrow", "poles", "merge"]
 
 _LIB = None
 

@@ -96,6 +96,9 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   h_insts.resize(n_inst);
   h_chans.clear();
   h_blocks.clear();
+  h_trip_start.clear();
+  h_trip_blk.clear();
+  merge_ok = true;
   nImax = 1;
   const bool dyn = par.mode >= 1;
   long o_u1 = 0, o_h = 0, o_w = 0, o_tri = 0, o_mat = 0, o_piv = 0, o_lr = 0, o_e = 0, o_x = 0, o_blk = 0;
@@ -128,6 +131,15 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
       C.cap = (int)cp;
       C.blk_off = o_blk; C.blk_cnt = (int)S.blocks[s].size();
       for (BlockDev B : S.blocks[s]) { B.gc = I.gc0 + s; h_blocks.push_back(B); }
+      C.trip_off = (int)h_trip_start.size();
+      C.ntrip = (int)S.trip_start[s].size() - 1;
+      {
+        const int tb0 = (int)h_trip_blk.size();
+        for (int v : S.trip_start[s]) h_trip_start.push_back(tb0 + v);
+        for (int v : S.trip_blk[s]) h_trip_blk.push_back((int)(o_blk + v));
+        for (size_t k = 0; k + 1 < S.trip_start[s].size(); ++k)
+          if (S.trip_start[s][k + 1] - S.trip_start[s][k] > kMergeMaxJ) merge_ok = false;
+      }
       o_blk += C.blk_cnt;
       C.u1_off = o_u1; o_u1 += pad8((long)nI * nI);
       C.h_off = o_h; o_h += pad8((long)nI * nI);
@@ -186,13 +198,21 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipMemcpy(d_x_off, x_off.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice));
   HTRY(hipMemcpy(d_cap, cap.data(), sizeof(int) * n_gc, hipMemcpyHostToDevice));
 
-  BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 3 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) + 64));
+  const long nts = std::max<long>((long)h_trip_start.size(), 1), ntb = std::max<long>((long)h_trip_blk.size(), 1);
+  BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 3 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) +
+                     pad8(sizeof(int) * nts) + pad8(sizeof(int) * ntb) + 64));
   cp8 = (char *)a_blocks.ptr;
   d_blocks = (BlockDev *)cp8; cp8 += pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1));
   d_blk_fw_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
   d_blk_bk_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
+  d_trip_start = (int *)cp8; cp8 += pad8(sizeof(int) * nts);
+  d_trip_blk = (int *)cp8; cp8 += pad8(sizeof(int) * ntb);
   d_work_list = (int *)cp8;
   if (n_blk) HTRY(hipMemcpy(d_blocks, h_blocks.data(), sizeof(BlockDev) * n_blk, hipMemcpyHostToDevice));
+  if (!h_trip_start.empty())
+    HTRY(hipMemcpy(d_trip_start, h_trip_start.data(), sizeof(int) * h_trip_start.size(), hipMemcpyHostToDevice));
+  if (!h_trip_blk.empty())
+    HTRY(hipMemcpy(d_trip_blk, h_trip_blk.data(), sizeof(int) * h_trip_blk.size(), hipMemcpyHostToDevice));
 
   // work doubles: u1, h, W, rho, tri, piv, lrows, kolt ; ints: status, work_count
   l_u1 = 0; l_h = l_u1 + pad8(o_u1); l_w = l_h + pad8(o_h); l_rho = l_w + pad8(o_w);
@@ -443,9 +463,14 @@ int Batch::build_sigma2(int only_ish) {
   }
   n_fw_tot.assign(n_inst, 0); n_bk_tot.assign(n_inst, 0);
   for (int gc = 0; gc < n_gc; ++gc) { n_fw_tot[h_chans[gc].inst] += h_cs[gc].mfw; n_bk_tot[h_chans[gc].inst] += h_cs[gc].mbk; }
-  BTRY(a_rows.grow(sizeof(double) * (tfw + tbk + 16)));
+  // the merged rows of the Lanczos path sit behind the rows, same layout
+  const long crow = par.lanczos ? tfw + tbk : 0;
+  BTRY(a_rows.grow(sizeof(double) * (tfw + tbk + crow + 16)));
   d_rows_fw = (double *)a_rows.ptr;
   d_rows_bk = d_rows_fw + tfw;
+  // (merging needs every (a, b, g) triple's J blocks in registers)
+  d_crows_fw = par.lanczos && merge_ok ? d_rows_bk + tbk : nullptr;
+  d_crows_bk = par.lanczos && merge_ok ? d_crows_fw + tfw : nullptr;
   if (par.lanczos) BTRY(a_lws.grow(sizeof(double) * (tws + 16)));
   HTRY(hipMemcpyAsync(d_row_fw_base, bfw.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice, stream));
   HTRY(hipMemcpyAsync(d_row_bk_base, bbk.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice, stream));
@@ -531,16 +556,25 @@ int Batch::tune_bucket(int bk, int mmax, int nj, const int *ord) {
 
 int Batch::lanczos() {
   PropView P = view(true);
-  hipLaunchKernelGGL(k_lanczos_prep, dim3((n_gc + 63) / 64), dim3(64), 0, stream, d_chans, 0, n_gc, P, d_cs, d_piv,
-                     par.lanc_kind, par.lanc_n, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_lrows,
-                     (double *)a_lws.ptr, d_ljobs);
+  // merge the J-degenerate poles, then the jobs pick merged or plain rows
+  kt_begin(KT_MERGE);
+  hipLaunchKernelGGL(k_merge_poles, dim3(2 * n_gc), dim3(256), 0, stream, d_chans, n_gc, d_cs, d_blocks, d_trip_start,
+                     d_trip_blk, P, d_insts, d_blk_fw_off, d_blk_bk_off, d_row_fw_base, d_row_bk_base, d_rows_fw,
+                     d_rows_bk, d_crows_fw, d_crows_bk);
+  kt_end(KT_MERGE);
   HTRY(hipGetLastError());
+  hipLaunchKernelGGL(k_lanczos_prep, dim3((n_gc + 63) / 64), dim3(64), 0, stream, d_chans, 0, n_gc, P, d_cs, d_piv,
+                     par.lanc_kind, par.lanc_n, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_crows_fw,
+                     d_crows_bk, d_lrows, (double *)a_lws.ptr, d_ljobs);
+  HTRY(hipGetLastError());
+  HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
   // jobs longest first, grouped into pole-count buckets; one launch per
   // bucket with the bucket's tile shape, the largest jobs first
   std::vector<std::pair<int, int>> jm;
   for (int gc = 0; gc < n_gc; ++gc)
     for (int side = 0; side < 2; ++side) {
-      const int M = side ? h_cs[gc].mbk : h_cs[gc].mfw;
+      const int M = side ? h_cs[gc].lmbk : h_cs[gc].lmfw;
       if (M > 0 && lanczos_bucket(M) < 0) return BCD_ERR_CAPACITY;
       jm.push_back({M, 2 * gc + side});
     }

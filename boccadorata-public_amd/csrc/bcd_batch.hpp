@@ -30,7 +30,7 @@ struct DevArena {
 
 enum KernelTimer {
   KT_SIGMA2, KT_LANCZOS, KT_TRIDIAG, KT_QL, KT_DENSITY, KT_SIGINF, KT_FINAL, KT_WBUILD, KT_PLAN, KT_ARROW, KT_POLES,
-  KT_N
+  KT_MERGE, KT_N
 };
 
 struct Batch {
@@ -43,6 +43,7 @@ struct Batch {
   std::vector<InstDev> h_insts;
   std::vector<ChanDev> h_chans;
   std::vector<BlockDev> h_blocks;
+  std::vector<int> h_trip_start, h_trip_blk;
   std::vector<ChanState> h_cs;
   std::vector<InstState> h_is;
   std::vector<int> hf_its, sc_its, hf_ok, sc_ok;
@@ -62,6 +63,9 @@ struct Batch {
   int *d_cap = nullptr;
   BlockDev *d_blocks = nullptr;
   int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_list = nullptr, *d_work_count = nullptr;
+  int *d_trip_start = nullptr, *d_trip_blk = nullptr;
+  double *d_crows_fw = nullptr, *d_crows_bk = nullptr;   // J-merged pole rows (Lanczos path)
+  bool merge_ok = true;                              // every triple has at most kMergeMaxJ J blocks
   int *d_status = nullptr;
   double *d_work = nullptr;                          // arrowhead FLOP counter of the current solve
   unsigned long long *d_prof = nullptr;              // arrowhead phase timers (profiling)

@@ -67,6 +67,7 @@ struct ChanDev {
                                  // lam_bk[lbk_cap], Ct_bk[nI x lbk_cap]
   long aws_off;                  // arrowhead workspace (doubles) for the global-memory path
   long warm_off;                 // previous sweep's arrowhead roots, nI x cap doubles
+  int trip_off, ntrip;           // (a, b, g) triples of the Sigma(2) blocks: starts in the triple table
 };
 
 // One (Jab, a <= b, g) block of the Sigma(2) build of channel gc; blocks of a
@@ -85,6 +86,8 @@ struct ChanState {
   int ipiv_fw[10], ipiv_bk[10];  // [n_pvt, 1, starts...]
   int itr_fw[8], itr_bk[8];      // iteration counts per pivot (after rescaling)
   int err;
+  int cfw, cbk;                  // pole counts after merging the J-degenerate groups
+  int lmfw, lmbk;                // pole counts the Lanczos jobs run on (merged or not)
   int _pad;
   double kolt[3];                // Calc_En partial sums of the last sweep
 };

@@ -576,10 +576,181 @@ BCD_SIGMA2_INST(float, double)
 // pivot (kind 0) or lanc_n vectors per side split over the pivots (kind 1),
 // rescaled as Reduce_SelfEn_Lncz does when they exceed the pole count
 // (Self_energy_ext_Op_e.cpp:147-220).  Writes one LancJob per side.
+// Poles of the same fragments (na, nb, ng) of one (a, b, g) triple in the
+// different J blocks share one energy eA + eB - eG: diag(E) is degenerate
+// on each such group.  Every Krylov vector restricted to a group stays in the
+// span of the group's coupling rows, so only the Gram matrix S = sum_p m_p m_p^T
+// of the group enters the Lanczos recursion (Lanczos.cpp:101-241: its inner
+// products, E-weighted products and coupling contractions).  A group of k >
+// nI rows is replaced by the nI rows of R = L^T, S = L L^T (Cholesky), which
+// gives the same reduced rows in exact arithmetic; groups of k <= nI rows are
+// copied.  One block per (channel, side), one thread per triple: count the
+// triple's output rows, scan, write them.  Rows come out triple by triple,
+// fragments in order, each group's rows in J order.
+__global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n_gc, ChanState *cs,
+                                                    const BlockDev *blocks, const int *trip_start,
+                                                    const int *trip_blk, PropView P, const InstDev *insts,
+                                                    const int *blk_fw_off, const int *blk_bk_off,
+                                                    const long *row_fw_base, const long *row_bk_base,
+                                                    const double *rows_fw, const double *rows_bk, double *crows_fw,
+                                                    double *crows_bk) {
+  // the chunk's non-empty triples: first block, J blocks, a == b, fragment
+  // counts, tuples (inclusive prefix), diagonal-group size and output base
+  __shared__ int q_s0[256], q_nb[256], q_eq[256], q_cA[256], q_cB[256], q_cG[256], q_tp[256], q_n0[256],
+      q_base[256];
+  __shared__ int wtot[3][4];
+  const int job = blockIdx.x;
+  if (job >= 2 * n_gc || !crows_fw) return;
+  const int gc = job >> 1, side = job & 1, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const ChanDev C = chans[gc];
+  const InstDev I = insts[C.inst];
+  const int cur = prop_cur(P, C.inst), nI = C.nI, w = 2 + nI;
+  const double *rows = side ? rows_bk + row_bk_base[gc] : rows_fw + row_fw_base[gc];
+  double *out = side ? crows_bk + row_bk_base[gc] : crows_fw + row_fw_base[gc];
+  const int *boff = side ? blk_bk_off : blk_fw_off;
+  auto rows_of = [&](int k) { return (k > nI && nI <= kMergeMaxNI) ? nI : k; };
+  int base = 0;
+  for (int t0 = 0; t0 < C.ntrip; t0 += 256) {
+    // pass 1, one thread per triple (na <= nb when a == b; each J block keeps
+    // the tuples with nb >= na + nJab)
+    int s0 = 0, nblk = 0, eq = 0, cA = 0, cB = 0, cG = 0, T = 0, n0 = 0, cnt = 0;
+    if (t0 + tid < C.ntrip) {
+      s0 = trip_start[C.trip_off + t0 + tid];
+      nblk = trip_start[C.trip_off + t0 + tid + 1] - s0;
+      const BlockDev B0 = blocks[trip_blk[s0]];
+      eq = B0.a == B0.b;
+      const int ga = I.gc0 + B0.a, gb = I.gc0 + B0.b, gg = I.gc0 + B0.g;
+      cA = side ? P.nqh[cur][ga] : P.nqp[cur][ga];
+      cB = side ? P.nqh[cur][gb] : P.nqp[cur][gb];
+      cG = side ? P.nqp[cur][gg] : P.nqh[cur][gg];
+      T = (eq ? cA * (cA + 1) / 2 : cA * cB) * cG;
+      if (T > 0) {
+        for (int x = 0; x < nblk; ++x) n0 += blocks[trip_blk[s0 + x]].nJab == 0;
+        cnt = eq ? cA * cG * rows_of(n0) + (T - cA * cG) * rows_of(nblk) : T * rows_of(nblk);
+      }
+    }
+    // compact the non-empty triples; scan their tuples and output rows
+    const bool live = T > 0;
+    const unsigned long long m = __ballot(live);
+    int iT = T, iC = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y1 = __shfl_up(iT, o, 64), y2 = __shfl_up(iC, o, 64);
+      if (lane >= o) { iT += y1; iC += y2; }
+    }
+    if (lane == 63) { wtot[0][wv] = iT; wtot[1][wv] = iC; wtot[2][wv] = __popcll(m); }
+    __syncthreads();
+    int pT = 0, pC = 0, pL = 0, nT = 0, nC = 0, nL = 0;
+    for (int v = 0; v < 4; ++v) {
+      if (v < wv) { pT += wtot[0][v]; pC += wtot[1][v]; pL += wtot[2][v]; }
+      nT += wtot[0][v]; nC += wtot[1][v]; nL += wtot[2][v];
+    }
+    if (live) {
+      const int k = pL + __popcll(m & ((1ull << lane) - 1ull));
+      q_s0[k] = s0; q_nb[k] = min(nblk, kMergeMaxJ); q_eq[k] = eq; q_cA[k] = cA; q_cB[k] = cB; q_cG[k] = cG;
+      q_tp[k] = pT + iT; q_n0[k] = n0; q_base[k] = base + pC + iC - cnt;
+    }
+    __syncthreads();
+    // pass 2, one thread per fragment tuple of the chunk's live triples
+    for (int f = tid; f < nT; f += 256) {
+      int lo = 0, hi = nL - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (q_tp[mid] > f) hi = mid; else lo = mid + 1;
+      }
+      const int k_ = lo, u = f - (q_tp[k_] - (q_eq[k_] ? q_cA[k_] * (q_cA[k_] + 1) / 2 : q_cA[k_] * q_cB[k_]) * q_cG[k_]);
+      const int a_eq_b = q_eq[k_], tA = q_cA[k_], tB = q_cB[k_], tG = q_cG[k_], nb_ = q_nb[k_], tn0 = q_n0[k_];
+      int q = u / tG, na = 0, nb;
+      const int ng = u % tG;
+      if (a_eq_b) {
+        int len = tA;
+        while (q >= len) { q -= len; ++na; --len; }
+        nb = na + q;
+      } else {
+        na = q / tB; nb = q % tB;
+      }
+      const bool diag = a_eq_b && na == nb;
+      const int kk = diag ? tn0 : nb_;
+      if (kk == 0) continue;
+      const int rn0 = rows_of(tn0), rnb = rows_of(nb_);
+      // rows of the tuples before this one: the pairs before (na, nb) times
+      // tG tuples each, plus the earlier ng of this pair
+      const int pq = u / tG, D = a_eq_b ? na + (nb > na ? 1 : 0) : 0;
+      int pos = q_base[k_] + (D * rn0 + (pq - D) * rnb) * tG + ng * (diag ? rn0 : rnb);
+      const int s0k = q_s0[k_];
+      auto row_of = [&](int x, int nJ) -> const double * {
+        const int bx = trip_blk[s0k + x];
+        const int pr = a_eq_b ? na * (tA - nJ) - na * (na - 1) / 2 + (nb - na - nJ) : na * tB + nb;
+        return rows + (long)(boff[bx] + pr * tG + ng) * w;
+      };
+      if (!(kk > nI && nI <= kMergeMaxNI)) {
+        for (int x = 0; x < nb_; ++x) {
+          const int nJ = blocks[trip_blk[s0k + x]].nJab;
+          if (a_eq_b && nb < na + nJ) continue;
+          const double *src = row_of(x, nJ);
+          for (int c = 0; c < w; ++c) out[(long)pos * w + c] = src[c];
+          ++pos;
+        }
+        continue;
+      }
+      double S_[kMergeMaxNI][kMergeMaxNI] = {}, L[kMergeMaxNI][kMergeMaxNI] = {};
+      double e = 0.0;
+      for (int x = 0; x < nb_; ++x) {
+        const int nJ = blocks[trip_blk[s0k + x]].nJab;
+        if (a_eq_b && nb < na + nJ) continue;
+        const double *src = row_of(x, nJ);
+        e = src[0];
+#pragma unroll
+        for (int ii = 0; ii < kMergeMaxNI; ++ii)
+#pragma unroll
+          for (int jj = 0; jj <= ii; ++jj)
+            if (ii < nI) S_[ii][jj] += src[2 + ii] * src[2 + jj];
+      }
+      double tr = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < kMergeMaxNI; ++ii) if (ii < nI) tr += S_[ii][ii];
+#pragma unroll
+      for (int jj = 0; jj < kMergeMaxNI; ++jj) {
+        if (jj >= nI) continue;
+        double d = S_[jj][jj];
+#pragma unroll
+        for (int mm = 0; mm < jj; ++mm) d -= L[jj][mm] * L[jj][mm];
+        L[jj][jj] = d > 1e-28 * tr ? sqrt(d) : 0.0;
+#pragma unroll
+        for (int ii = 0; ii < kMergeMaxNI; ++ii) {
+          if (ii <= jj || ii >= nI) continue;
+          double v = S_[ii][jj];
+#pragma unroll
+          for (int mm = 0; mm < jj; ++mm) v -= L[ii][mm] * L[jj][mm];
+          L[ii][jj] = L[jj][jj] > 0.0 ? v / L[jj][jj] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kMergeMaxNI; ++r) {
+        if (r >= nI) continue;
+        double *dst = out + (long)(pos + r) * w;
+        double s2 = 0.0;
+        dst[0] = e;
+#pragma unroll
+        for (int d = 0; d < kMergeMaxNI; ++d) {
+          if (d >= nI) continue;
+          const double mv = d >= r ? L[d][r] : 0.0;
+          dst[2 + d] = mv;
+          s2 += mv * mv;
+        }
+        dst[1] = s2;
+      }
+    }
+    base += nC;
+    __syncthreads();
+  }
+  if (tid == 0) (side ? cs[gc].cbk : cs[gc].cfw) = base;
+}
+
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P,
                                ChanState *cs, double *piv, int lanc_kind, int lanc_n,
                                const long *row_fw_base, const long *row_bk_base, double *rows_fw,
-                               double *rows_bk, double *lrows, double *lws, LancJob *jobs) {
+                               double *rows_bk, double *crows_fw, double *crows_bk, double *lrows, double *lws,
+                               LancJob *jobs) {
   const int gc = gc_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= gc_end) return;
   const ChanDev C = chans[gc];
@@ -618,9 +789,15 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
     int *ipv = side ? S.ipiv_bk : S.ipiv_fw;
     for (int k = 0; k < kMaxPiv + 2; ++k) ipv[k] = -100;
     for (int k = 0; k < npv; ++k) itr[k] = lanc_kind == 0 ? lanc_n : lanc_n / npv + (k < lanc_n % npv ? 1 : 0);
-    const int M = side ? S.mbk : S.mfw;
     int tot = 0;
     for (int k = 0; k < npv; ++k) if (itr[k] > 0) tot += itr[k];
+    // the J-merged rows span the same Krylov spaces; they are used when
+    // they still hold every requested iteration, so the reduction keeps the
+    // reference's shape (no saturation that the full rows would not reach)
+    const int Mfull = side ? S.mbk : S.mfw, Mc = crows_fw ? (side ? S.cbk : S.cfw) : 0;
+    const bool merged = Mc > 0 && Mc >= tot;
+    const int M = merged ? Mc : Mfull;
+    (side ? S.lmbk : S.lmfw) = 0;
     int nl = tot < M ? tot : M;
     LancJob &Jb = jobs[2 * gc + side];
     Jb.nLanc = 0;
@@ -637,7 +814,9 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
     if (nLanc > M) nLanc = M;
     const int cap = side ? C.lbk_cap : C.lfw_cap;
     if (nLanc > (nl > 1 ? nl : 1) || nLanc > cap) { S.err = 1; continue; }
-    Jb.Min = side ? rows_bk + row_bk_base[gc] : rows_fw + row_fw_base[gc];
+    Jb.Min = merged ? (side ? crows_bk + row_bk_base[gc] : crows_fw + row_fw_base[gc])
+                    : (side ? rows_bk + row_bk_base[gc] : rows_fw + row_fw_base[gc]);
+    (side ? S.lmbk : S.lmfw) = M;
     Jb.Tin = 2 + nI; Jb.NA_in = 2; Jb.NB = nI; Jb.M = M;
     Jb.piv = pv; Jb.nt_pvts = npv; Jb.nLanc = nLanc; Jb.itr = itr;
     Jb.Mout = lrows + (side ? C.lbk_off : C.lfw_off);

@@ -42,10 +42,17 @@ __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const Block
 inline size_t sigma2_lds(int vt_cap, int c_cap) {
   return sizeof(double) * ((size_t)vt_cap + kSigStage + kSigRowBuf) + sizeof(int) * (kMaxPairs + 2 * (size_t)c_cap);
 }
+constexpr int kMergeMaxNI = 4;    // J-degenerate pole groups are merged in channels up to this nI
+constexpr int kMergeMaxJ = 16;    // J blocks of one (a, b, g) triple (J <= j_a + j_b)
+__global__ void k_merge_poles(const ChanDev *chans, int n_gc, ChanState *cs, const BlockDev *blocks,
+                              const int *trip_start, const int *trip_blk, PropView P, const InstDev *insts,
+                              const int *blk_fw_off, const int *blk_bk_off, const long *row_fw_base,
+                              const long *row_bk_base, const double *rows_fw, const double *rows_bk,
+                              double *crows_fw, double *crows_bk);
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P, ChanState *cs,
                                double *piv, int lanc_kind, int lanc_n, const long *row_fw_base,
-                               const long *row_bk_base, double *rows_fw, double *rows_bk, double *lrows,
-                               double *lws, LancJob *jobs);
+                               const long *row_bk_base, double *rows_fw, double *rows_bk, double *crows_fw,
+                               double *crows_bk, double *lrows, double *lws, LancJob *jobs);
 constexpr int kLancPer = 24;      // largest register tile (elements per thread) of k_lanczos
 template <int NT, int PER>
 __global__ void k_lanczos(const LancJob *jobs, const int *order, int njobs);

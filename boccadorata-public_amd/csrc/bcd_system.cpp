@@ -1,6 +1,7 @@
 // bcd_system.cpp -- host-side packing of the model space and the interaction.
 #include "bcd_system.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <map>
@@ -129,6 +130,28 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
             mbound_bk[I] += pairs * no[g];
           }
         }
+  }
+  // blocks of one subshell grouped by (a, b, g): the same fragments of a, b, g
+  // give one pole energy in every J block of the triple
+  trip_start.assign(nsubsh, {});
+  trip_blk.assign(nsubsh, {});
+  for (int I = 0; I < nsubsh; ++I) {
+    std::vector<int> order(blocks[I].size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      const BlockDev &X = blocks[I][x], &Y = blocks[I][y];
+      if (X.a != Y.a) return X.a < Y.a;
+      if (X.b != Y.b) return X.b < Y.b;
+      return X.g < Y.g;
+    });
+    for (size_t k = 0; k < order.size(); ++k) {
+      const BlockDev &X = blocks[I][order[k]];
+      if (k == 0 || X.a != blocks[I][order[k - 1]].a || X.b != blocks[I][order[k - 1]].b ||
+          X.g != blocks[I][order[k - 1]].g)
+        trip_start[I].push_back((int)k);
+      trip_blk[I].push_back(order[k]);
+    }
+    trip_start[I].push_back((int)order.size());
   }
   return BCD_OK;
 }

@@ -20,6 +20,9 @@ struct HostSystem {
   std::vector<long> voff;
   std::vector<double> V;
   std::vector<std::vector<BlockDev>> blocks;   // [ish]
+  // per subshell: the blocks grouped by (a, b, g) triple, each group in J
+  // order; trip_start[ish] has one entry per triple plus the end
+  std::vector<std::vector<int>> trip_start, trip_blk;
   std::vector<long> mbound_fw, mbound_bk;      // pole-count bounds for HF-like propagators
   int max_vt = 0;                              // largest nI * pairs * n_g of a Sigma(2) block
   int max_dg = 0;                              // largest nI * n_g of a Sigma(2) block

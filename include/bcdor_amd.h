@@ -189,7 +189,8 @@ int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_
  * the last solve, for roofline accounting (hipEvents around every launch,
  * only while profiling is on): k = 0 "sigma2", 1 "lanczos", 2 "tridiag",
  * 3 "ql", 4 "density", 5 "sigma_inf", 6 "finalize", 7 "wbuild",
- * 8 "sigma2_plan", 9 "arrow", 10 "poles". */
+ * 8 "sigma2_plan", 9 "arrow", 10 "poles", 11 "merge" (merging of the
+ * J-degenerate pole groups before the Lanczos reduction). */
 int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
 /* Algorithmic work of the last solve: [0] coupling FLOPs (2*sum poles*K*n),
  * [1] Lanczos bytes (8*M*sum_j(2j+5+n+npiv)), [2] dense Dyson FLOPs (9 n^3
