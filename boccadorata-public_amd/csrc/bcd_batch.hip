@@ -623,8 +623,13 @@ int Batch::decompose_poles() {
   else
     hipLaunchKernelGGL(k_tridiag<false>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, false), stream, d_pjobs,
                        2 * n_gc);
-  hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * lmax * sizeof(double), stream, d_pjobs,
-                     2 * n_gc, d_status);
+  if (lmax <= 64 && nImax <= kQLGroupNI)
+    hipLaunchKernelGGL(k_ql_poles, dim3((2 * n_gc + 7) / 8), dim3(64),
+                       (size_t)8 * (2 + nImax) * lmax * sizeof(double), stream, d_pjobs, 2 * n_gc, d_status,
+                       lmax, nImax);
+  else
+    hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * lmax * sizeof(double), stream, d_pjobs,
+                       2 * n_gc, d_status);
   kt_end(KT_POLES);
   HTRY(hipGetLastError());
   return BCD_OK;

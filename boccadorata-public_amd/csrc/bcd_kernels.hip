@@ -1424,6 +1424,104 @@ __global__ __launch_bounds__(64) void k_ql(const DysonJob *jobs, int njobs, int 
   }
 }
 
+// The pole-block diagonalisation (poles mode) of k_ql for n <= 64 and
+// nI <= 8, eight matrices per wave: each group of eight lanes runs one
+// matrix's implicit QL (the same iteration as k_ql) with d, e and the
+// tracked rows in its own LDS slice; lane r < nI of a group rotates tracked
+// row r.  Every lane of a group computes the same scalar chain, so a wave
+// does the work of eight matrices in the instructions k_ql spends on one.
+// ld: LDS doubles per d or e array (the batch's largest side); qrows: tracked
+// rows per slice (the batch's largest nI).
+__global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, int njobs, int *status, int ld, int qrows) {
+  extern __shared__ double sm[];
+  const int lane = threadIdx.x, grp = lane >> 3, gl = lane & 7;
+  const int job = blockIdx.x * 8 + grp;
+  double *d = sm + (long)grp * (2 + qrows) * ld, *e = d + ld, *Q = e + ld;
+  const double eps = 2.2204460492503131e-16;
+  int n = 0, nI = 0;
+  const DysonJob *J = job < njobs ? jobs + job : nullptr;
+  if (J) { n = J->n; nI = J->nI; }
+  if (n > ld || nI > qrows || nI > kQLGroupNI) {
+    // does not fit this kernel's slices (the host never launches such jobs here)
+    if (gl == 0 && n > 0) atomicAdd(status, 1);
+    n = 0;
+  }
+  for (int i = gl; i < n; i += 8) { d[i] = J->tri[i]; e[i] = J->tri[n + i]; }
+  for (int i = gl; i < nI * n; i += 8) Q[i] = J->tri[2L * n + i];
+  __syncthreads();
+  // phase 0: find the block end m for lo; 1: one rotation of the chain; 2: done
+  int lo = 0, it = 0, m = 0, i = 0, phase = n > 0 ? 0 : 2, fail = 0;
+  double s = 1.0, c = 1.0, g = 0.0, pp = 0.0, di1 = 0.0;
+  while (__ballot(phase != 2)) {
+    if (phase == 0) {
+      int mm = n - 1;
+      for (int b0 = lo; b0 < n - 1; b0 += 8) {
+        const int p = b0 + gl;
+        const bool neg = p < n - 1 && fabs(e[p]) <= eps * (fabs(d[p]) + fabs(d[p + 1]));
+        const unsigned bits = (unsigned)((__ballot(neg) >> (8 * grp)) & 0xffull);
+        if (bits) { mm = b0 + __ffs(bits) - 1; break; }
+      }
+      if (mm == lo) {
+        ++lo;
+        it = 0;
+        if (lo >= n) phase = 2;
+      } else if (it >= 60) {
+        fail = 1;
+        phase = 2;
+      } else {
+        m = mm;
+        const double dlo = d[lo], elo = e[lo];
+        g = (d[lo + 1] - dlo) / (2.0 * elo);
+        const double r = ql_hyp(g, 1.0);
+        di1 = d[m];
+        g = di1 - dlo + elo / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+        s = 1.0; c = 1.0; pp = 0.0;
+        i = m - 1;
+        phase = 1;
+      }
+    } else if (phase == 1) {
+      const double ei = e[i], di = d[i];
+      const double f = s * ei, b = c * ei;
+      double r = ql_hyp(f, g);
+      e[i + 1] = r;
+      if (r == 0.0) {
+        d[i + 1] = di1 - pp;
+        e[m] = 0.0;
+        ++it;
+        phase = 0;
+      } else {
+        s = f / r; c = g / r;
+        g = di1 - pp;
+        r = (di - g) * s + 2.0 * c * b;
+        pp = s * r;
+        d[i + 1] = g + pp;
+        g = c * r - b;
+        if (gl < nI) {
+          double *row = Q + (long)gl * n;
+          const double t = row[i + 1];
+          row[i + 1] = s * row[i] + c * t;
+          row[i] = c * row[i] - s * t;
+        }
+        di1 = di;
+        if (--i < lo) {
+          d[lo] -= pp;
+          e[lo] = g;
+          e[m] = 0.0;
+          ++it;
+          phase = 0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (!J || n == 0) return;
+  if (fail) { if (gl == 0) atomicAdd(status, 1); return; }
+  for (int k = gl; k < n; k += 8) {
+    J->out_lam[k] = d[k];
+    for (int r = 0; r < nI; ++r) J->out_ct[(long)r * J->ct_ld + k] = Q[(long)r * n + k];
+  }
+}
+
 // Stable rank sort of pole rows by column col (ascending, or descending when
 // desc): row i goes to position #{j : key_j before key_i}, ties by index.
 __global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out) {

@@ -88,6 +88,8 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
 template <bool IN_LDS>
 __global__ void k_tridiag(const DysonJob *jobs, int njobs);
 __global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
+constexpr int kQLGroupNI = 8;     // tracked rows of k_ql_poles (one lane of an 8-lane group each)
+__global__ void k_ql_poles(const DysonJob *jobs, int njobs, int *status, int ld, int qrows);
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow(const ArrowJob *jobs, int njobs, int *status);
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
