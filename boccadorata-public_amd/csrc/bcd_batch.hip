@@ -617,12 +617,22 @@ int Batch::decompose_poles() {
   hipLaunchKernelGGL(k_poles_prep, dim3((2 * n_gc + 127) / 128), dim3(128), 0, stream, d_chans, n_gc, d_cs, d_lrows,
                      d_tri, in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs);
   kt_begin(KT_POLES);
-  if (in_lds)
+  const int ld = lmax | 1;
+  const size_t wlds = sizeof(double) * ((size_t)ld * ld + 128);
+  if (lmax <= 64 && nImax <= 16) {
+    if (nImax <= 4)
+      hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(2 * n_gc), dim3(64), wlds, stream, d_pjobs, 2 * n_gc, ld);
+    else if (nImax <= 8)
+      hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(2 * n_gc), dim3(64), wlds, stream, d_pjobs, 2 * n_gc, ld);
+    else
+      hipLaunchKernelGGL(k_tridiag_poles<16>, dim3(2 * n_gc), dim3(64), wlds, stream, d_pjobs, 2 * n_gc, ld);
+  } else if (in_lds) {
     hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, true), stream, d_pjobs,
                        2 * n_gc);
-  else
+  } else {
     hipLaunchKernelGGL(k_tridiag<false>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, false), stream, d_pjobs,
                        2 * n_gc);
+  }
   if (lmax <= 64 && nImax <= kQLGroupNI)
     hipLaunchKernelGGL(k_ql_poles, dim3((2 * n_gc + 7) / 8), dim3(64),
                        (size_t)8 * (2 + nImax) * lmax * sizeof(double), stream, d_pjobs, 2 * n_gc, d_status,

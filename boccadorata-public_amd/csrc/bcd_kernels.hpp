@@ -87,6 +87,8 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
                              double *kolt, DysonJob *jobs);
 template <bool IN_LDS>
 __global__ void k_tridiag(const DysonJob *jobs, int njobs);
+template <int NIMAX>
+__global__ void k_tridiag_poles(const DysonJob *jobs, int njobs, int ld);
 __global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
 constexpr int kQLGroupNI = 8;     // tracked rows of k_ql_poles (one lane of an 8-lane group each)
 __global__ void k_ql_poles(const DysonJob *jobs, int njobs, int *status, int ld, int qrows);
