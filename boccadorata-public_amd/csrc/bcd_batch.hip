@@ -199,15 +199,16 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipMemcpy(d_cap, cap.data(), sizeof(int) * n_gc, hipMemcpyHostToDevice));
 
   const long nts = std::max<long>((long)h_trip_start.size(), 1), ntb = std::max<long>((long)h_trip_blk.size(), 1);
-  BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 3 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) +
-                     pad8(sizeof(int) * nts) + pad8(sizeof(int) * ntb) + 64));
+  BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 2 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) +
+                     pad8(sizeof(Sig2Work) * std::max<long>(n_blk, 1)) + pad8(sizeof(int) * nts) +
+                     pad8(sizeof(int) * ntb) + 64));
   cp8 = (char *)a_blocks.ptr;
   d_blocks = (BlockDev *)cp8; cp8 += pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1));
   d_blk_fw_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
   d_blk_bk_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
   d_trip_start = (int *)cp8; cp8 += pad8(sizeof(int) * nts);
   d_trip_blk = (int *)cp8; cp8 += pad8(sizeof(int) * ntb);
-  d_work_list = (int *)cp8;
+  d_work_list = (Sig2Work *)cp8;
   if (n_blk) HTRY(hipMemcpy(d_blocks, h_blocks.data(), sizeof(BlockDev) * n_blk, hipMemcpyHostToDevice));
   if (!h_trip_start.empty())
     HTRY(hipMemcpy(d_trip_start, h_trip_start.data(), sizeof(int) * h_trip_start.size(), hipMemcpyHostToDevice));
@@ -445,6 +446,8 @@ int Batch::build_sigma2(int only_ish) {
   HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
   HTRY(hipStreamSynchronize(stream));
   // row buffers and Lanczos workspace sized by the actual pole counts
+  for (int gc = 0; gc < n_gc; ++gc)
+    if (h_cs[gc].err == 3) return BCD_ERR_CAPACITY;   // a fragment count above 65535
   std::vector<long> bfw(n_gc), bbk(n_gc);
   long tfw = 0, tbk = 0, tws = 0;
   bool relayout = false;

@@ -62,7 +62,8 @@ struct Batch {
   long *d_row_fw_base = nullptr, *d_row_bk_base = nullptr, *d_e_off = nullptr, *d_x_off = nullptr;
   int *d_cap = nullptr;
   BlockDev *d_blocks = nullptr;
-  int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_list = nullptr, *d_work_count = nullptr;
+  int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_count = nullptr;
+  Sig2Work *d_work_list = nullptr;
   int *d_trip_start = nullptr, *d_trip_blk = nullptr;
   double *d_crows_fw = nullptr, *d_crows_bk = nullptr;   // J-merged pole rows (Lanczos path)
   bool merge_ok = true;                              // every triple has at most kMergeMaxJ J blocks

@@ -77,6 +77,17 @@ struct BlockDev {
   int c2b;                       // two-body channel of the block's (alpha beta; J) pairs, -1: none
 };
 
+// One non-empty Sigma(2) block in the compact work list, with everything the
+// build needs before its first dependent load: the block, its channel and
+// instance, the fragment counts of both sides (2p1h: qp(a), qp(b), qh(g);
+// 2h1p: qh(a), qh(b), qp(g)) packed two per int, and the first row of each
+// side in the channel's rows.
+struct Sig2Work {
+  int bidx, gc, inst;
+  int n01, n23, n45;             // counts 0..5 as (lo | hi << 16)
+  int off_fw, off_bk;
+};
+
 // Per-channel dynamic state (fragment counts live in separate arrays).
 struct ChanState {
   int mfw, mbk;                  // unreduced pole counts (Sigma(2) build)

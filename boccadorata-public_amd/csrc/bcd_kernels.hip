@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_mix(const InstDev *insts, InstState *is
 // offsets in build order, and a compact list of the non-empty blocks.
 __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin,
                                                     int gc_end, PropView P, const InstDev *insts, ChanState *cs,
-                                                    int *blk_fw_off, int *blk_bk_off, int *work_list,
+                                                    int *blk_fw_off, int *blk_bk_off, Sig2Work *work_list,
                                                     int *work_count, int only_ish) {
   const int gc = gc_begin + blockIdx.x;
   if (gc >= gc_end) return;
@@ -281,17 +281,22 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
   }
   __shared__ int wsum[3][4];
   __shared__ int list_base;
+  if (threadIdx.x == 0) cs[gc].err = 0;
+  __syncthreads();
   const InstDev I = insts[C.inst];
   const int cur = prop_cur(P, C.inst), tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
-  auto counts = [&](int b, int &cfw, int &cbk) {
+  // pole counts of block b per side, and the six fragment counts
+  auto counts = [&](int b, int &cfw, int &cbk, int *f6) {
     cfw = 0; cbk = 0;
     if (b >= C.blk_cnt) return;
     const BlockDev B = blocks[C.blk_off + b];
     const int ga = I.gc0 + B.a, gb = I.gc0 + B.b, gg = I.gc0 + B.g;
     int na = P.nqp[cur][ga], nb = P.nqp[cur][gb], ng = P.nqh[cur][gg];
+    if (f6) { f6[0] = na; f6[1] = nb; f6[2] = ng; }
     cfw = na * nb * ng;
     if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cfw = (na * nb / 2) * ng; }
     na = P.nqh[cur][ga]; nb = P.nqh[cur][gb]; ng = P.nqp[cur][gg];
+    if (f6) { f6[3] = na; f6[4] = nb; f6[5] = ng; }
     cbk = na * nb * ng;
     if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cbk = (na * nb / 2) * ng; }
   };
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
   int nz = 0;
   for (int b0 = 0; b0 < C.blk_cnt; b0 += nt) {
     int cfw, cbk;
-    counts(b0 + tid, cfw, cbk);
+    counts(b0 + tid, cfw, cbk, nullptr);
     nz += cfw + cbk > 0;
   }
   for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o, 64);
@@ -315,8 +320,8 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
   int base_fw = 0, base_bk = 0, base_nz = list_base;
   for (int b0 = 0; b0 < C.blk_cnt; b0 += nt) {
     const int b = b0 + tid;
-    int cfw, cbk;
-    counts(b, cfw, cbk);
+    int cfw, cbk, f6[6] = {0, 0, 0, 0, 0, 0};
+    counts(b, cfw, cbk, f6);
     const bool take = cfw + cbk > 0;
     const unsigned long long mask = __ballot(take);
     // inclusive scans: within each wave, then the waves before it
@@ -335,9 +340,18 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
     }
     __syncthreads();
     if (b < C.blk_cnt) {
-      blk_fw_off[C.blk_off + b] = base_fw + pf + sfw - cfw;
-      blk_bk_off[C.blk_off + b] = base_bk + pb + sbk - cbk;
-      if (take) work_list[base_nz + pz + __popcll(mask & ((1ull << lane) - 1ull))] = (int)(C.blk_off + b);
+      const int ofw = base_fw + pf + sfw - cfw, obk = base_bk + pb + sbk - cbk;
+      blk_fw_off[C.blk_off + b] = ofw;
+      blk_bk_off[C.blk_off + b] = obk;
+      if (take) {
+        // the work item packs fragment counts into 16 bits
+        if ((f6[0] | f6[1] | f6[2] | f6[3] | f6[4] | f6[5]) >> 16) cs[gc].err = 3;
+        Sig2Work W;
+        W.bidx = (int)(C.blk_off + b); W.gc = gc; W.inst = C.inst;
+        W.n01 = f6[0] | (f6[1] << 16); W.n23 = f6[2] | (f6[3] << 16); W.n45 = f6[4] | (f6[5] << 16);
+        W.off_fw = ofw; W.off_bk = obk;
+        work_list[base_nz + pz + __popcll(mask & ((1ull << lane) - 1ull))] = W;
+      }
     }
     base_fw += tf;
     base_bk += tb;
@@ -415,7 +429,7 @@ template <class T, class A>
 __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanDev *chans,
                                                const BlockDev *blocks, PropView P,
                                                const int *blk_fw_off, const int *blk_bk_off,
-                                               const int *work_list, const int *work_count,
+                                               const Sig2Work *work_list, const int *work_count,
                                                const long *row_fw_base, const long *row_bk_base,
                                                double *rows_fw, double *rows_bk, int vt_cap, int c_cap) {
 #pragma clang fp contract(off)
@@ -430,11 +444,14 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
   const int lane = threadIdx.x;
   const int total = *work_count;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    const int bidx = work_list[t];
-    const BlockDev B = blocks[bidx];
-    const ChanDev C = chans[B.gc];
-    const InstDev I = insts[C.inst];
-    const int cur = prop_cur(P, C.inst);
+    const Sig2Work Wk = work_list[t];
+    const BlockDev B = blocks[Wk.bidx];
+    const ChanDev C = chans[Wk.gc];
+    const InstDev I = insts[Wk.inst];
+    const int cur = prop_cur(P, Wk.inst);
+    // fragment counts: qp(a), qp(b), qh(g), qh(a), qh(b), qp(g)
+    const int f0 = Wk.n01 & 0xffff, f1 = Wk.n01 >> 16, f2 = Wk.n23 & 0xffff, f3 = Wk.n23 >> 16;
+    const int f4 = Wk.n45 & 0xffff, f5 = Wk.n45 >> 16;
     const int nI = C.nI, a = B.a, b = B.b, g = B.g, J = B.J, nJab = B.nJab, nJgd = B.nJgd;
     const int noa = I.no[a], nob = I.no[b], nog = I.no[g];
     const int ga = I.gc0 + a, gb = I.gc0 + b, gg = I.gc0 + g;
@@ -493,11 +510,9 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
     }
     for (int side = 0; side < 2; ++side) {
       // side 0: 2p1h from qp(a), qp(b), qh(g); side 1: 2h1p from qh(a), qh(b), qp(g)
-      const int cA = side ? P.nqh[cur][ga] : P.nqp[cur][ga];
-      const int cB = side ? P.nqh[cur][gb] : P.nqp[cur][gb];
-      const int cG = side ? P.nqp[cur][gg] : P.nqh[cur][gg];
-      const int oA = side ? P.nqp[cur][ga] : 0, oB = side ? P.nqp[cur][gb] : 0;
-      const int oG = side ? 0 : P.nqp[cur][gg];
+      const int cA = side ? f3 : f0, cB = side ? f4 : f1, cG = side ? f5 : f2;
+      // the quasiholes of a subshell are stored after its quasiparticles
+      const int oA = side ? f0 : 0, oB = side ? f1 : 0, oG = side ? 0 : f5;
       int npole;
       if (a == b) { int nb2 = cA + 1; if (nJab == 1) nb2 -= 2; npole = (cA * nb2 / 2) * cG; }
       else npole = cA * cB * cG;
@@ -507,7 +522,7 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
       const double *xA = P.x[cur] + P.x_off[ga] + (long)oA * noa;
       const double *xB = P.x[cur] + P.x_off[gb] + (long)oB * nob;
       const double *xG = P.x[cur] + P.x_off[gg] + (long)oG * nog;
-      const int row_off = side ? blk_bk_off[bidx] : blk_fw_off[bidx];
+      const int row_off = side ? Wk.off_bk : Wk.off_fw;
       double *rows = (side ? rows_bk + row_bk_base[B.gc] : rows_fw + row_fw_base[B.gc]) + (long)row_off * (2 + nI);
       const int w = 2 + nI;
       // stage the amplitudes of the fragments this side uses (for a == b the
@@ -562,7 +577,7 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
 
 #define BCD_SIGMA2_INST(T, A)                                                                               \
   template __global__ void k_sigma2<T, A>(const InstDev *, const ChanDev *, const BlockDev *, PropView, const int *, \
-                                          const int *, const int *, const int *, const long *, const long *,       \
+                                          const int *, const Sig2Work *, const int *, const long *, const long *,  \
                                           double *, double *, int, int);
 BCD_SIGMA2_INST(double, double)
 BCD_SIGMA2_INST(float, double)

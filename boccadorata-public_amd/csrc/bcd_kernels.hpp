@@ -30,10 +30,10 @@ __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_b
 __global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *hist, long rho_tot, int m);
 __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
-                              int *blk_bk_off, int *work_list, int *work_count, int only_ish);
+                              int *blk_bk_off, Sig2Work *work_list, int *work_count, int only_ish);
 template <class T, class A>
 __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const BlockDev *blocks, PropView P,
-                         const int *blk_fw_off, const int *blk_bk_off, const int *work_list,
+                         const int *blk_fw_off, const int *blk_bk_off, const Sig2Work *work_list,
                          const int *work_count, const long *row_fw_base, const long *row_bk_base,
                          double *rows_fw, double *rows_bk, int vt_cap, int c_cap);
 // dynamic LDS bytes of k_sigma2 for a V table of vt_cap doubles
