@@ -526,8 +526,9 @@ class Batch:
         return out
 
     def work(self):
-        w = np.zeros(15)
-        check(lib().bcd_batch_work(self.h, dp(w), 15), "bcd_batch_work")
+        w = np.zeros(17)
+        check(lib().bcd_batch_work(self.h, dp(w), 17), "bcd_batch_work")
         phases = ["load_sort", "deflation", "roots", "loewner", "eigenvectors", "reorder", "output", "norms"]
         return dict(coupling_flops=w[0], lanczos_bytes=w[1], dyson_flops=w[2], n_eig=w[3], sum_n3=w[4],
-                    arrow_flops=w[5], poles_flops=w[6], arrow_phase_ms=dict(zip(phases, w[7:15])))
+                    arrow_flops=w[5], poles_flops=w[6], arrow_phase_ms=dict(zip(phases, w[7:15])),
+                    sigma2_poles=int(w[15]), lanczos_poles=int(w[16]))

@@ -688,6 +688,10 @@ int Batch::work(double *w, int n) {
     w[4] += hi * n0 * n0 * n0 + si * n1 * n1 * n1;
   }
   w[2] = 9.0 * w[4];
+  for (int gc = 0; n >= 17 && gc < (int)h_cs.size() && par.lanczos; ++gc) {
+    w[15] += std::max(h_cs[gc].mfw, 0) + std::max(h_cs[gc].mbk, 0);
+    w[16] += std::max(h_cs[gc].lmfw, 0) + std::max(h_cs[gc].lmbk, 0);
+  }
   return BCD_OK;
 }
 

@@ -623,7 +623,7 @@ int bcd_batch_lanczos_shapes(bcd_batch *b, int *shapes, float *tune_ms, int n) {
 
 int bcd_batch_work(bcd_batch *b, double *work, int n) {
   if (!b || !work || n < 7) return BCD_ERR_ARG;
-  return b->b.work(work, n >= 15 ? 15 : 7);
+  return b->b.work(work, n >= 17 ? 17 : n >= 15 ? 15 : 7);
 }
 
 }  // extern "C"

@@ -199,7 +199,10 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
  * once-per-solve diagonalisation of the reduced sides (9 L^3 per side).
  * n >= 7.  With n >= 15 and profiling on, [7..14] are the arrowhead kernel's
  * block time (ms summed over blocks) per phase: load and sort, deflation,
- * roots, Loewner couplings, eigenvectors, reorder, split and output, norms. */
+ * roots, Loewner couplings, eigenvectors, reorder, split and output, norms.
+ * With n >= 17 and the Lanczos on, [15] is the number of Sigma(2) poles over
+ * all channels and sides and [16] the number the Lanczos jobs ran on (fewer
+ * where J-degenerate pole groups were merged). */
 int bcd_batch_work(bcd_batch *b, double *work, int n);
 /* Tile-shape autotuner of the Lanczos kernel (no counterpart in the
  * reference, which runs one Lanczos at a time on the CPU).  Jobs are grouped

@@ -77,7 +77,11 @@ def test_config2_lanczos_sc0():
     b = capi.Batch([(ms, vpp)], par)
     b.solve()
     st = b.stats(0)
-    print("config2", st, b.timings(), flush=True)
+    w = b.work()
+    print("config2", st, b.timings(), w["sigma2_poles"], w["lanczos_poles"], flush=True)
+    # The Lanczos ran on J-merged pole groups, so this also holds the merge
+    # to the bar.
+    assert 0 < w["lanczos_poles"] < w["sigma2_poles"]
     ref, st_hf, st_sc = oracle_solve(ms, vpp, True, 1, 1e-8, 1e-8)
     assert st["hf_its"] == st_hf.iterations and st["sc_its"] == st_sc.iterations
     check_prop(b.result(0), ref, zsig=ZSIG_BY_NMAX[7])
