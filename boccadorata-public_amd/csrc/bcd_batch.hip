@@ -34,6 +34,11 @@ Batch::~Batch() {
   for (auto *a : arenas()) a->release();
   for (auto &e : ev) { if (e) hipEventDestroy(e); }
   for (auto &kv : kev) { if (kv.first) hipEventDestroy(kv.first); if (kv.second) hipEventDestroy(kv.second); }
+  if (lfork) hipEventDestroy(lfork);
+  for (int k = 0; k < kLancAux; ++k) {
+    if (ljoin[k]) hipEventDestroy(ljoin[k]);
+    if (lstream[k]) hipStreamDestroy(lstream[k]);
+  }
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -64,6 +69,11 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipSetDevice(device));
   HTRY(hipStreamCreate(&stream));
   for (auto &e : ev) HTRY(hipEventCreate(&e));
+  HTRY(hipEventCreateWithFlags(&lfork, hipEventDisableTiming));
+  for (int k = 0; k < kLancAux; ++k) {
+    HTRY(hipStreamCreate(&lstream[k]));
+    HTRY(hipEventCreateWithFlags(&ljoin[k], hipEventDisableTiming));
+  }
 
   // ---- static tables, concatenated per element type
   std::vector<int> ti;
@@ -588,16 +598,37 @@ int Batch::lanczos() {
   for (size_t k = 0; k < jm.size(); ++k) order[k] = jm[k].second;
   HTRY(hipMemcpyAsync(d_lorder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
   kt_begin(KT_LANCZOS);
+  // The bucket of the largest jobs runs on the batch's stream, the others
+  // round-robin on the extra streams, all after the fork and joined back
+  // before the stage ends.  Tuning runs the buckets one at a time so that
+  // each shape is timed alone.
+  const bool fork = !tune_lanczos;
+  bool used[kLancAux] = {};
+  if (fork) HTRY(hipEventRecord(lfork, stream));
   size_t k0 = 0;
+  int q = 0;
   while (k0 < jm.size()) {
     const int bk = jm[k0].first > 0 ? lanczos_bucket(jm[k0].first) : 0;
     size_t k1 = k0;
     while (k1 < jm.size() && (jm[k1].first > 0 ? lanczos_bucket(jm[k1].first) : 0) == bk) ++k1;
     const int nj = (int)(k1 - k0), mmax = std::max(jm[k0].first, 1);
     if (tune_lanczos) BTRY(tune_bucket(bk, mmax, nj, d_lorder + k0));
-    launch_lanczos(lanc_shape[bk], nj, d_ljobs, d_lorder + k0, stream);
+    hipStream_t s = stream;
+    if (fork && q > 0) {
+      const int a = (q - 1) % kLancAux;
+      if (!used[a]) HTRY(hipStreamWaitEvent(lstream[a], lfork, 0));
+      used[a] = true;
+      s = lstream[a];
+    }
+    launch_lanczos(lanc_shape[bk], nj, d_ljobs, d_lorder + k0, s);
+    ++q;
     k0 = k1;
   }
+  for (int a = 0; a < kLancAux; ++a)
+    if (used[a]) {
+      HTRY(hipEventRecord(ljoin[a], lstream[a]));
+      HTRY(hipStreamWaitEvent(stream, ljoin[a], 0));
+    }
   tune_lanczos = false;
   kt_end(KT_LANCZOS);
   HTRY(hipGetLastError());

@@ -33,9 +33,15 @@ enum KernelTimer {
   KT_MERGE, KT_N
 };
 
+// Extra streams the Lanczos buckets run on beside the batch's own stream, so
+// that the tail of one bucket's launch overlaps the others.
+constexpr int kLancAux = 3;
+
 struct Batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t lstream[kLancAux] = {};
+  hipEvent_t lfork = nullptr, ljoin[kLancAux] = {};
   bcd_solve_params par{};
   std::vector<const HostSystem *> sys;
   int n_inst = 0, n_gc = 0, nImax = 1;
