@@ -467,8 +467,8 @@ int Batch::build_sigma2(int only_ish) {
     bfw[gc] = tfw; tfw += pad8((long)h_cs[gc].mfw * w);
     bbk[gc] = tbk; tbk += pad8((long)h_cs[gc].mbk * w);
     if (par.lanczos) {
-      const long lws_fw = tws, lws_bk = tws + pad8(lanczos_ws_vectors(C.lfw_cap, C.nI) * h_cs[gc].mfw);
-      tws = lws_bk + pad8(lanczos_ws_vectors(C.lbk_cap, C.nI) * h_cs[gc].mbk);
+      const long lws_fw = tws, lws_bk = tws + pad8(lanczos_ws_vectors(C.lfw_cap, C.nI) * lanczos_stride(h_cs[gc].mfw));
+      tws = lws_bk + pad8(lanczos_ws_vectors(C.lbk_cap, C.nI) * lanczos_stride(h_cs[gc].mbk));
       if (C.lws_fw_off != lws_fw || C.lws_bk_off != lws_bk) relayout = true;
       C.lws_fw_off = lws_fw; C.lws_bk_off = lws_bk;
     }

@@ -131,7 +131,7 @@ int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout
   BTRY(dout.alloc(sizeof(double) * (size_t)Tout * nLanc));
   BTRY(dipv.alloc(sizeof(int) * (nt_pvts + 2)));
   BTRY(dn.alloc(sizeof(int)));
-  BTRY(dV.alloc(sizeof(double) * (size_t)lanczos_ws_vectors(nLanc, NB_in) * NPLS_in));
+  BTRY(dV.alloc(sizeof(double) * (size_t)lanczos_ws_vectors(nLanc, NB_in) * lanczos_stride(NPLS_in)));
   BTRY(djob.alloc(sizeof(LancJob)));
   HTRY(hipMemcpy(dpiv.p, pivots, sizeof(double) * nt_pvts * NB_in, hipMemcpyHostToDevice));
   HTRY(hipMemcpy(ditr.p, n_itr, sizeof(int) * nt_pvts, hipMemcpyHostToDevice));

@@ -19,6 +19,9 @@ constexpr int kMaxSub = 64;       // subshells per instance
 // Krylov vectors, then E, the nb coupling columns and E * v_last of up to
 // kMaxPiv pivot chains
 __host__ __device__ inline long lanczos_ws_vectors(long nlanc, int nb) { return nlanc + 1 + nb + kMaxPiv; }
+// Stride of the workspace vectors of a job with M poles: even, so that
+// element pairs are 16-byte aligned.
+__host__ __device__ inline int lanczos_stride(int M) { return M + (M & 1); }
 constexpr int kMixMax = 8;        // iterates of the Anderson density mixing
 constexpr double kMixStart = 1e-3;  // density residual norm below which the mixing starts
 
