@@ -199,17 +199,22 @@ __global__ __launch_bounds__(256) void k_mix(const InstDev *insts, InstState *is
     for (int i = 0; i < kMixMax; ++i) acc[t++] += df[i] * fq;
     acc[t] += fq * fq;
   }
+  // only the products of the n differences in use are reduced
   const int lane = tid & 63, wv = tid >> 6;
+  constexpr int kTri = kMixMax * (kMixMax + 1) / 2;
 #pragma unroll
   for (int t = 0; t < kNd; ++t) {
+    const bool used = t < kTri ? t < n * (n + 1) / 2 : t - kTri < n || t == kNd - 1;
+    if (!used) continue;
     const double v = wave_sum(acc[t]);
     if (lane == 0) red[wv][t] = v;
   }
   __syncthreads();
   if (tid == 0) {
     for (int t = 0; t < kNd; ++t) {
+      const bool used = t < kTri ? t < n * (n + 1) / 2 : t - kTri < n || t == kNd - 1;
       G[t] = 0.0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) G[t] += red[w][t];
+      for (int w = 0; used && w < (int)(blockDim.x >> 6); ++w) G[t] += red[w][t];
     }
     const double fn = sqrt(G[kNd - 1]);
     // plain steps until the residual is small (the plain iteration picks the
@@ -1709,19 +1714,26 @@ __global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc,
 __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const ChanDev *chans,
                                                   PropView P, const ChanState *cs, const double *kolt,
                                                   int conv_kind, int init) {
-  __shared__ int pq[2][kMaxSub + 1], ph[2][kMaxSub + 1];
+  // per subshell: fragment counts of both buffers (prefix sums), the start of
+  // its records and its charge, all in LDS so that the record loops below
+  // are flat over the instance's records
+  __shared__ int pq[2][kMaxSub + 1], ph[2][kMaxSub + 1], chg[kMaxSub];
+  __shared__ long eo[kMaxSub];
   __shared__ double red[4 * 4];
   const int inst = blockIdx.x, tid = threadIdx.x;
   const InstDev I = insts[inst];
   InstState &S = P.is[inst];
   if (!init && !S.active) return;
   const int ns = I.nsubsh, cur = S.cur, nxt = 1 - cur;
-  const int useb = init ? cur : nxt;
-  // fragment counts of both buffers into LDS in parallel, prefix sums from LDS
+  const int useb = init ? cur : nxt, bu = init ? 0 : 1;
   for (int t = tid; t < 2 * ns; t += blockDim.x) {
     const int b = t / ns, s = t % ns, buf = b == 0 ? cur : nxt;
     pq[b][s + 1] = P.nqp[buf][I.gc0 + s];
     ph[b][s + 1] = P.nqh[buf][I.gc0 + s];
+  }
+  for (int s = tid; s < ns; s += blockDim.x) {
+    eo[s] = P.e_off[I.gc0 + s];
+    chg[s] = I.ch[s] & 1;
   }
   __syncthreads();
   if (tid < 2) {
@@ -1730,15 +1742,30 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
     for (int s = 0; s < ns; ++s) { q[s + 1] += q[s]; h[s + 1] += h[s]; }
   }
   __syncthreads();
-  // Fermi energies from buffer useb
+  // subshell of the j-th record of a prefix-summed count list
+  auto owner = [&](const int *pre, int j) {
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (pre[mid + 1] > j) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+  };
+  // Fermi energies from buffer useb (counts in LDS slot bu)
   {
     double mn[2] = {1e300, 1e300}, mx[2] = {-1e300, -1e300};
-    for (int s = 0; s < ns; ++s) {
-      const int gc = I.gc0 + s, chs = I.ch[s] & 1;
-      const int np = P.nqp[useb][gc], nh = P.nqh[useb][gc];
-      const double *e = P.e[useb] + P.e_off[gc];
-      for (int k = tid; k < np; k += blockDim.x) mn[chs] = fmin(mn[chs], e[k]);
-      for (int k = tid; k < nh; k += blockDim.x) mx[chs] = fmax(mx[chs], e[np + k]);
+    const double *E = P.e[useb];
+    const int totp = pq[bu][ns], toth = ph[bu][ns];
+    for (int j = tid; j < totp + toth; j += blockDim.x) {
+      const bool hole = j >= totp;
+      const int jj = hole ? j - totp : j;
+      const int s = owner(hole ? ph[bu] : pq[bu], jj);
+      const double e = E[eo[s] + (hole ? pq[bu][s + 1] - pq[bu][s] + jj - ph[bu][s] : jj - pq[bu][s])];
+      if (hole) {
+        if (chg[s]) mx[1] = fmax(mx[1], e); else mx[0] = fmax(mx[0], e);
+      } else {
+        if (chg[s]) mn[1] = fmin(mn[1], e); else mn[0] = fmin(mn[0], e);
+      }
     }
     double v4[4] = {-mn[0], -mn[1], mx[0], mx[1]};
     for (int k = 0; k < 4; ++k) v4[k] = wave_max(v4[k]);
@@ -1752,15 +1779,11 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
       }
       // first stored quasiparticle / quasihole of the propagator as the start
       // values, as Seek_EFermi does
-      // counts of buffer useb are in LDS slot bu (prefix differences)
-      const int bu = useb == cur ? 0 : 1;
       double e0p = 0.0, e0h = 0.0;
       for (int s = 0; s < ns; ++s)
-        if (pq[bu][s + 1] > pq[bu][s]) { e0p = P.e[useb][P.e_off[I.gc0 + s]]; break; }
-      for (int s = 0; s < ns; ++s) {
-        const int gc = I.gc0 + s;
-        if (ph[bu][s + 1] > ph[bu][s]) { e0h = P.e[useb][P.e_off[gc] + (pq[bu][s + 1] - pq[bu][s])]; break; }
-      }
+        if (pq[bu][s + 1] > pq[bu][s]) { e0p = E[eo[s]]; break; }
+      for (int s = 0; s < ns; ++s)
+        if (ph[bu][s + 1] > ph[bu][s]) { e0h = E[eo[s] + (pq[bu][s + 1] - pq[bu][s])]; break; }
       for (int c = 0; c < 2; ++c) {
         const double x1 = fmin(e0p + 100., -red[c]), x2 = fmax(e0h - 100., red[2 + c]);
         S.ef[c] = 0.5 * (x1 + x2);
@@ -1778,18 +1801,14 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
     const int tot_old = side ? ph[0][ns] : pq[0][ns], tot_new = side ? ph[1][ns] : pq[1][ns];
     const int tmax = tot_old > tot_new ? tot_old : tot_new;
     for (int j = tid; j < tmax; j += blockDim.x) {
-      double eo = 0.0, en = 0.0;
+      double ev[2] = {0.0, 0.0};
       for (int b = 0; b < 2; ++b) {
         const int *pre = side ? ph[b] : pq[b];
-        const int tot = pre[ns];
-        if (j >= tot) continue;
-        int s = 0;
-        while (pre[s + 1] <= j) ++s;
-        const int gc = I.gc0 + s, buf = b == 0 ? cur : nxt;
-        const double val = P.e[buf][P.e_off[gc] + (side ? P.nqp[buf][gc] : 0) + (j - pre[s])];
-        if (b == 0) eo = val; else en = val;
+        if (j >= pre[ns]) continue;
+        const int s = owner(pre, j);
+        ev[b] = P.e[b == 0 ? cur : nxt][eo[s] + (side ? pq[b][s + 1] - pq[b][s] : 0) + (j - pre[s])];
       }
-      const double dlt = fabs(eo - en);
+      const double dlt = fabs(ev[0] - ev[1]);
       acc[0] += dlt;
       acc[1] = fmax(acc[1], dlt);
     }
