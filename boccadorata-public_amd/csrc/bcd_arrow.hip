@@ -192,6 +192,24 @@ __global__ __launch_bounds__(kArrowMaxThreads, 4) void k_arrow_sweep(ArrowSweep 
                     IN_LDS ? sm : J.ws);
 }
 
+// The HF sweep's Dyson problems: one thread per channel (static_dyson),
+// writing what k_arrow_sweep writes for a channel without poles, including
+// the pole count 0 of every head for the next sweep's warm start.
+__global__ __launch_bounds__(64) void k_dyson_static(ArrowSweep a) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= a.n_gc) return;
+  const ChanDev &C = a.chans[gc];
+  const InstState *S = a.P.is ? a.P.is + C.inst : nullptr;
+  if (S && !S->active) return;
+  const int nxt = 1 - (S ? S->cur : 0);
+  const double ef = S ? S->ef[C.ch & 1] : 0.0;
+  if (static_dyson<kStaticNI>(C.nI, C.j2, a.h + C.h_off, a.u1 + C.u1_off, ef, a.P.e[nxt] + a.P.e_off[gc],
+                              a.P.x[nxt] + a.P.x_off[gc], a.P.nqp[nxt] + gc, a.P.nqh[nxt] + gc, a.kolt + 3L * gc))
+    atomicAdd(a.status, 1);
+  if (a.warm_p)
+    for (int k = 0; k < C.nI && k < kArrowNI; ++k) a.warm_p[(long)kArrowNI * gc + k] = 0;
+}
+
 BCD_ARROW_INST(true, 4)
 BCD_ARROW_INST(true, 8)
 BCD_ARROW_INST(true, 16)

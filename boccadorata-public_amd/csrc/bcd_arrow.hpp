@@ -504,4 +504,138 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
   B.lap(6, tk);
 }
 
+
+// The Dyson problem of a channel without dynamic self-energy (the HF stage):
+// the Dyson matrix is the static part u1 + h alone (lower triangle read, as
+// arrow_body reads it).  One thread: cyclic Jacobi rotations (Golub and Van
+// Loan, Matrix Computations, 4th ed., 8.5.2) on the matrix held in registers
+// until the off-diagonal part is below 1e-17 of the matrix norm, then the
+// records, counts and Calc_En sums arrow_body writes for such a channel.  Each
+// eigenvector is signed so that its largest component (the first of equal
+// ones) is positive.  N is the compiled bound of nI.  Returns 0, or 1 when nI
+// lies outside [1, N] or the rotations do not converge.
+constexpr int kStaticNI = 4;
+constexpr int kStaticSweeps = 32;
+template <int N>
+__host__ __device__ inline int static_dyson(int nI, int j2, const double *h, const double *u1, double ef,
+                                            double *out_e, double *out_x, int *out_nqp, int *out_nqh,
+                                            double *kolt) {
+  if (nI < 1 || nI > N) return 1;
+  double A[N][N], V[N][N];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      A[r][c] = 0.0;
+      V[r][c] = r == c ? 1.0 : 0.0;
+    }
+  double nrm = 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c)
+      if (r < nI) {
+        const double v = u1[r * nI + c] + h[r * nI + c];
+        A[r][c] = v;
+        A[c][r] = v;
+        nrm += (r == c ? 1.0 : 2.0) * v * v;
+      }
+  bool done = false;
+  for (int sw = 0; sw < kStaticSweeps && !done; ++sw) {
+    double off = 0.0;
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) off += A[p][q] * A[p][q];
+    if (!(off > 1e-34 * nrm)) { done = true; break; }
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) {
+        const double apq = A[p][q];
+        if (q >= nI || apq == 0.0) continue;
+        // rotation J in the (p, q) plane with (J^T A J)_pq = 0
+        const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        A[p][q] = 0.0;
+        A[q][p] = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  if (!done) return 1;
+  double d[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    d[i] = A[i][i];
+    // sign: largest component positive
+    double big = 0.0, sg = 1.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+      if (r < nI && fabs(V[r][i]) > big) { big = fabs(V[r][i]); sg = V[r][i] < 0.0 ? -1.0 : 1.0; }
+#pragma unroll
+    for (int r = 0; r < N; ++r) V[r][i] *= sg;
+  }
+  int nqh = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) nqh += (i < nI && d[i] <= ef) ? 1 : 0;
+  const int nqp = nI - nqh;
+  double kb[9];
+#pragma unroll
+  for (int b = 0; b < 9; ++b) kb[b] = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i >= nI) continue;
+    // ascending rank, ties by position; both sides stored descending
+    int rk = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) rk += (k < nI && (d[k] < d[i] || (d[k] == d[i] && k < i))) ? 1 : 0;
+    const int slot = d[i] > ef ? (nI - 1 - rk) : nqp + (nqh - 1 - rk);
+    out_e[slot] = d[i];
+    double x1 = 0.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+      if (r < nI) {
+        out_x[slot * nI + r] = V[r][i];
+        x1 += V[r][i] * V[r][i];
+      }
+    if (d[i] <= ef) {
+      double x2 = 0.0;
+#pragma unroll
+      for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+          if (r < nI && c < nI) x2 += u1[r * nI + c] * V[r][i] * V[c][i];
+      const int bk = x1 > 0.01 ? 0 : (x1 > 1.e-5 ? 1 : 2);
+      kb[3 * bk] += x2;
+      kb[3 * bk + 1] += x1 * d[i];
+      kb[3 * bk + 2] += x1;
+    }
+  }
+  *out_nqp = nqp;
+  *out_nqh = nqh;
+  const double f = (double)j2 + 1.0;
+  kolt[0] = (kb[0] + kb[3] + kb[6]) * f;
+  kolt[1] = (kb[1] + kb[4] + kb[7]) * f;
+  kolt[2] = (kb[2] + kb[5] + kb[8]) * f;
+  return 0;
+}
+
 }  // namespace bcd

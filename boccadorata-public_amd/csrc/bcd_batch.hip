@@ -371,7 +371,10 @@ int Batch::sweep(int src) {
     const int nmax = src == 0 ? nImax : (int)capmax;
     const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
     kt_begin(KT_ARROW);
-    launch_arrow_sweep(a, nImax, !aws_len, nt, aws_len ? 0 : arrow_ws_bytes(nmax, nImax), stream);
+    if (src == 0 && nImax <= kStaticNI)
+      hipLaunchKernelGGL(k_dyson_static, dim3((n_gc + 63) / 64), dim3(64), 0, stream, a);
+    else
+      launch_arrow_sweep(a, nImax, !aws_len, nt, aws_len ? 0 : arrow_ws_bytes(nmax, nImax), stream);
     kt_end(KT_ARROW);
   } else {
     hipLaunchKernelGGL(k_dyson_prep, dim3(gb), dim3(128), 0, stream, d_chans, 0, n_gc, d_cs, P, src, d_h, d_u1,

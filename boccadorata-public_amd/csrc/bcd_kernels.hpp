@@ -117,6 +117,9 @@ struct ArrowSweep {
 };
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow_sweep(ArrowSweep a);
+// A sweep without dynamic self-energy when every nI <= kStaticNI: one thread
+// per channel solves its static Dyson matrix (static_dyson).
+__global__ void k_dyson_static(ArrowSweep a);
 // Launch one arrowhead kernel variant: the workspace in LDS or HBM, the
 // accumulator bound from nimax (arrow_ni_bound).
 void launch_arrow_sweep(const ArrowSweep &a, int nimax, bool in_lds, int nt, size_t lds, hipStream_t st);

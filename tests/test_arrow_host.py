@@ -198,3 +198,128 @@ def test_warm_start_from_previous_roots(exe):
         check(c2, r2[:6])
         if fewer:
             assert r2[6] < 0.75 * r1[6], (r1[6], r2[6])
+
+
+# ---------------------------------------------------------------- static_dyson
+
+STATIC_EXE = os.path.join(BUILD, "static_host")
+
+
+@pytest.fixture(scope="module")
+def static_exe():
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    os.makedirs(BUILD, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+           "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + CSRC,
+           os.path.join(HERE, "arrow_host", "static_host.cpp"), "-o", STATIC_EXE]
+    subprocess.run(cmd, check=True)
+    return STATIC_EXE
+
+
+def run_static(exe, cases):
+    buf = [struct.pack("<i", len(cases))]
+    for c in cases:
+        nI = c["h"].shape[0]
+        buf.append(struct.pack("<2id", nI, c["j2"], c["ef"]))
+        buf.append(np.ascontiguousarray(c["h"], np.float64).tobytes())
+        buf.append(np.ascontiguousarray(c["u1"], np.float64).tobytes())
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([exe], input=b"".join(buf), capture_output=True, env=env, timeout=120)
+    assert r.returncode == 0, r.stderr.decode()[-3000:]
+    raw, pos, out = r.stdout, 0, []
+    for c in cases:
+        nI = c["h"].shape[0]
+        rc, nqp, nqh = struct.unpack_from("<3i", raw, pos); pos += 12
+        e = np.frombuffer(raw, np.float64, nI, pos); pos += 8 * nI
+        x = np.frombuffer(raw, np.float64, nI * nI, pos).reshape(nI, nI); pos += 8 * nI * nI
+        k = np.frombuffer(raw, np.float64, 3, pos); pos += 24
+        out.append((rc, nqp, nqh, e, x, k))
+    assert pos == len(raw)
+    return out
+
+
+def static_reference(c):
+    """The records and Calc_En sums from numpy's dense eigensolver on the
+    static part (lower triangle of u1 + h), laid out as DysonMatrix stores them:
+    quasiparticles (e > E_F) descending, then quasiholes descending."""
+    a = np.tril(c["u1"] + c["h"])
+    a = a + np.tril(a, -1).T
+    w, v = np.linalg.eigh(a)
+    qp, qh = np.flatnonzero(w > c["ef"])[::-1], np.flatnonzero(w <= c["ef"])[::-1]
+    order = np.concatenate([qp, qh])
+    kb = np.zeros(9)
+    for i in qh:
+        x1 = float(v[:, i] @ v[:, i])
+        x2 = float(v[:, i] @ c["u1"] @ v[:, i])
+        bk = 0 if x1 > 0.01 else (1 if x1 > 1e-5 else 2)
+        kb[3 * bk:3 * bk + 3] += (x2, x1 * w[i], x1)
+    f = c["j2"] + 1.0
+    kolt = np.array([kb[0::3].sum(), kb[1::3].sum(), kb[2::3].sum()]) * f
+    return len(qp), len(qh), w[order], v[:, order].T, kolt
+
+
+def static_cases():
+    rng = np.random.default_rng(5)
+    cases = []
+    for nI in (1, 2, 3, 4):
+        for ef in (-1e3, 0.0, 1e3):
+            a = rng.normal(0.0, 10.0, (nI, nI))
+            u = rng.normal(0.0, 10.0, (nI, nI))
+            u = u + u.T
+            cases.append(dict(h=a, u1=u, ef=ef, j2=2 * nI - 1))
+    # only the lower triangle counts: garbage above the diagonal
+    h = np.tril(rng.normal(0.0, 5.0, (4, 4))) + np.triu(np.full((4, 4), 1e6), 1)
+    cases.append(dict(h=h, u1=np.zeros((4, 4)), ef=0.0, j2=3))
+    # wide spread of scales, tiny couplings, an exactly diagonal matrix
+    cases.append(dict(h=np.diag([1e5, -3.0, 2e-3, 40.0]) + 1e-9 * np.tril(np.ones((4, 4)), -1),
+                      u1=np.zeros((4, 4)), ef=1.0, j2=7))
+    cases.append(dict(h=np.diag([3.0, -1.0, 2.0]), u1=np.diag([1.0, 1.0, 1.0]), ef=2.5, j2=1))
+    return cases
+
+
+def test_static_dyson_against_eigh(static_exe):
+    """Distinct eigenvalues: energies to 1e-13 of the spectral radius,
+    amplitudes to 1e-11 with the largest component positive, the split at
+    E_F and the Calc_En sums."""
+    cases = static_cases()
+    for c, (rc, nqp, nqh, e, x, k) in zip(cases, run_static(static_exe, cases)):
+        assert rc == 0
+        rnqp, rnqh, re, rx, rk = static_reference(c)
+        assert (nqp, nqh) == (rnqp, rnqh)
+        scale = max(1.0, np.abs(re).max())
+        assert np.abs(e - re).max() < 1e-13 * scale
+        for i in range(len(e)):
+            big = np.argmax(np.abs(x[i]))
+            assert x[i, big] > 0 and np.argmax(np.abs(x[i]) > np.abs(x[i, big]) * (1 - 1e-12)) == big
+            s = np.sign(rx[i] @ x[i])
+            assert np.abs(x[i] - s * rx[i]).max() < 1e-11
+        assert np.abs(k - rk).max() < 1e-11 * max(1.0, np.abs(rk).max())
+
+
+def test_static_dyson_degenerate(static_exe):
+    """Repeated eigenvalues (rotated diagonal matrices, the zero matrix): the
+    records still hold orthonormal eigenvectors of the matrix, every energy
+    counted once."""
+    rng = np.random.default_rng(6)
+    cases = []
+    for lam in ([2.0, 2.0, -1.0, 5.0], [1.0, 1.0, 1.0, 1.0], [0.0, 0.0, 0.0, 0.0], [-3.0, 4.0, 4.0]):
+        n = len(lam)
+        q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        a = q @ np.diag(lam) @ q.T
+        cases.append(dict(h=np.tril(a), u1=np.zeros((n, n)), ef=1.5, j2=3))
+    for c, (rc, nqp, nqh, e, x, k) in zip(cases, run_static(static_exe, cases)):
+        assert rc == 0
+        a = np.tril(c["h"]) + np.tril(c["h"], -1).T
+        lam = np.sort(np.linalg.eigvalsh(a))
+        assert np.abs(np.sort(e) - lam).max() < 1e-13 * max(1.0, np.abs(lam).max())
+        assert nqh == int((e <= c["ef"]).sum()) and nqp == len(e) - nqh
+        assert np.all(e[:nqp] > c["ef"]) and np.all(np.diff(e[:nqp]) <= 0) and np.all(np.diff(e[nqp:]) <= 0)
+        assert np.abs(x @ x.T - np.eye(len(e))).max() < 1e-13
+        assert np.abs(a @ x.T - x.T * e).max() < 1e-12 * max(1.0, np.abs(lam).max())
+
+
+def test_static_dyson_rejects_large_ni(static_exe):
+    """nI above the compiled bound is reported, not solved."""
+    c = dict(h=np.eye(5), u1=np.zeros((5, 5)), ef=0.0, j2=1)
+    assert run_static(static_exe, [c])[0][0] == 1
