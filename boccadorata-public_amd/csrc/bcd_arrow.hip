@@ -104,11 +104,14 @@ __global__ __launch_bounds__(kArrowMaxThreads, 4) void k_arrow(const ArrowJob *j
   template __global__ void k_arrow_sweep<L, N>(ArrowSweep);
 
 
-// Jobs that diagonalise both reduced sides of every channel: job 2*gc + side.
+// Jobs that diagonalise both reduced sides of every channel: job 2*gc + side,
+// for the njobs job indices in order (all 2*n_gc when order is null).
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
-                             double *tri, double *mat, double *poles, DysonJob *jobs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * n_gc) return;
+                             double *tri, double *mat, double *poles, DysonJob *jobs, const int *order, int njobs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= njobs) return;
+  const int t = order ? order[k] : k;
+  if (t < 0 || t >= 2 * n_gc) return;
   const int gc = t >> 1, side = t & 1;
   const ChanDev C = chans[gc];
   const ChanState &Q = cs[gc];

@@ -608,6 +608,12 @@ int Batch::lanczos() {
   const bool fork = !tune_lanczos;
   bool used[kLancAux] = {};
   if (fork) HTRY(hipEventRecord(lfork, stream));
+  // outside tuning and profiling, each bucket's pole blocks are diagonalised
+  // on its stream right after its Lanczos jobs (they need nothing else), so
+  // that they overlap the other buckets' reductions
+  const PoleCfg pc = pole_cfg();
+  poles_done = fork && !profile && par.dyson_solver == 0 && pc.wave &&
+               tridiag_lds(pc.lmax, false) <= (size_t)kLdsBytes && (pc.in_lds || a_mat.ptr);
   size_t k0 = 0;
   int q = 0;
   while (k0 < jm.size()) {
@@ -624,6 +630,7 @@ int Batch::lanczos() {
       s = lstream[a];
     }
     launch_lanczos(lanc_shape[bk], nj, d_ljobs, d_lorder + k0, s);
+    if (poles_done) launch_poles(pc, d_lorder + k0, nj, s);
     ++q;
     k0 = k1;
   }
@@ -642,41 +649,59 @@ int Batch::lanczos() {
   return BCD_OK;
 }
 
+// Launch parameters of the pole-block kernels for this batch: the largest
+// reduced side, and whether the one-wave kernels (k_tridiag_poles,
+// k_ql_poles) take every side.
+Batch::PoleCfg Batch::pole_cfg() const {
+  PoleCfg c;
+  for (const ChanDev &C : h_chans) c.lmax = std::max(c.lmax, std::max(C.lfw_cap, C.lbk_cap));
+  c.ld = c.lmax | 1;
+  c.wlds = sizeof(double) * ((size_t)c.ld * c.ld + 128);
+  c.in_lds = tridiag_lds(c.lmax, true) <= (size_t)kLdsBytes;
+  c.wave = c.lmax <= 64 && nImax <= kQLGroupNI;
+  return c;
+}
+
+// The pole-block kernels of the njobs sides order[0..njobs) on stream s; the
+// one-wave kernels only (pole_cfg().wave).
+void Batch::launch_poles(const PoleCfg &c, const int *order, int njobs, hipStream_t s) {
+  hipLaunchKernelGGL(k_poles_prep, dim3((njobs + 127) / 128), dim3(128), 0, s, d_chans, n_gc, d_cs, d_lrows, d_tri,
+                     c.in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs, order, njobs);
+  if (nImax <= 4)
+    hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ld);
+  else
+    hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ld);
+  hipLaunchKernelGGL(k_ql_poles, dim3((njobs + 7) / 8), dim3(64), (size_t)8 * (2 + nImax) * c.lmax * sizeof(double),
+                     s, d_pjobs, order, njobs, d_status, c.lmax, nImax);
+}
+
 // Diagonalise every Lanczos-reduced side once (its fishbone block and
 // couplings are fixed for the whole sc0 stage): pole energies and rotated
-// couplings for the arrowhead sweeps.
+// couplings for the arrowhead sweeps.  Nothing to do when lanczos() already
+// ran them bucket by bucket.
 int Batch::decompose_poles() {
-  int lmax = 1;
-  for (const ChanDev &C : h_chans) lmax = std::max(lmax, std::max(C.lfw_cap, C.lbk_cap));
-  if (tridiag_lds(lmax, false) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
-  const bool in_lds = tridiag_lds(lmax, true) <= (size_t)kLdsBytes;
-  if (!in_lds && !a_mat.ptr) return BCD_ERR_CAPACITY;
-  hipLaunchKernelGGL(k_poles_prep, dim3((2 * n_gc + 127) / 128), dim3(128), 0, stream, d_chans, n_gc, d_cs, d_lrows,
-                     d_tri, in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs);
+  if (poles_done) return BCD_OK;
+  const PoleCfg c = pole_cfg();
+  if (tridiag_lds(c.lmax, false) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
+  if (!c.in_lds && !a_mat.ptr) return BCD_ERR_CAPACITY;
   kt_begin(KT_POLES);
-  const int ld = lmax | 1;
-  const size_t wlds = sizeof(double) * ((size_t)ld * ld + 128);
-  if (lmax <= 64 && nImax <= 16) {
-    if (nImax <= 4)
-      hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(2 * n_gc), dim3(64), wlds, stream, d_pjobs, 2 * n_gc, ld);
-    else if (nImax <= 8)
-      hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(2 * n_gc), dim3(64), wlds, stream, d_pjobs, 2 * n_gc, ld);
-    else
-      hipLaunchKernelGGL(k_tridiag_poles<16>, dim3(2 * n_gc), dim3(64), wlds, stream, d_pjobs, 2 * n_gc, ld);
-  } else if (in_lds) {
-    hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, true), stream, d_pjobs,
-                       2 * n_gc);
+  if (c.wave) {
+    launch_poles(c, nullptr, 2 * n_gc, stream);
   } else {
-    hipLaunchKernelGGL(k_tridiag<false>, dim3(2 * n_gc), dim3(256), tridiag_lds(lmax, false), stream, d_pjobs,
-                       2 * n_gc);
+    hipLaunchKernelGGL(k_poles_prep, dim3((2 * n_gc + 127) / 128), dim3(128), 0, stream, d_chans, n_gc, d_cs,
+                       d_lrows, d_tri, c.in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs, nullptr, 2 * n_gc);
+    if (c.lmax <= 64 && nImax <= 16)
+      hipLaunchKernelGGL(k_tridiag_poles<16>, dim3(2 * n_gc), dim3(64), c.wlds, stream, d_pjobs, nullptr, 2 * n_gc,
+                         c.ld);
+    else if (c.in_lds)
+      hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(c.lmax, true), stream, d_pjobs,
+                         2 * n_gc);
+    else
+      hipLaunchKernelGGL(k_tridiag<false>, dim3(2 * n_gc), dim3(256), tridiag_lds(c.lmax, false), stream, d_pjobs,
+                         2 * n_gc);
+    hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * c.lmax * sizeof(double), stream,
+                       d_pjobs, 2 * n_gc, d_status);
   }
-  if (lmax <= 64 && nImax <= kQLGroupNI)
-    hipLaunchKernelGGL(k_ql_poles, dim3((2 * n_gc + 7) / 8), dim3(64),
-                       (size_t)8 * (2 + nImax) * lmax * sizeof(double), stream, d_pjobs, 2 * n_gc, d_status,
-                       lmax, nImax);
-  else
-    hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * lmax * sizeof(double), stream, d_pjobs,
-                       2 * n_gc, d_status);
   kt_end(KT_POLES);
   HTRY(hipGetLastError());
   return BCD_OK;

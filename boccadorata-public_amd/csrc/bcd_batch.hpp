@@ -38,6 +38,11 @@ enum KernelTimer {
 constexpr int kLancAux = 3;
 
 struct Batch {
+  struct PoleCfg {
+    int lmax = 1, ld = 1;
+    size_t wlds = 0;
+    bool in_lds = false, wave = false;
+  };
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t lstream[kLancAux] = {};
@@ -116,11 +121,14 @@ struct Batch {
                                    kLancDefault[4], kLancDefault[5], kLancDefault[6], kLancDefault[7]};
   float lanc_tune_ms[kNLancBuckets][kNLancShapes] = {};
   bool tune_lanczos = false;
+  bool poles_done = false;       // lanczos() ran the pole-block kernels bucket by bucket
   int prepare();
   int work(double *w, int n);
   double coupling_flops = -1.0;
   int lanczos();
   int decompose_poles();
+  PoleCfg pole_cfg() const;
+  void launch_poles(const PoleCfg &c, const int *order, int njobs, hipStream_t s);
   int solve();
   int result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep, double *zp, double *fqp,
              double *eh, double *zh, double *fqh);

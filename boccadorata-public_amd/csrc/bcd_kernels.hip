@@ -1313,10 +1313,10 @@ __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs
 // as k_tridiag (Golub & Van Loan, Alg. 8.3.1).  ld: LDS row stride (odd,
 // >= n); dynamic LDS: ld * ld + 128 doubles.
 template <int NIMAX>
-__global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, int njobs, int ld) {
+__global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, const int *order, int njobs, int ld) {
   extern __shared__ double sm[];
   if ((int)blockIdx.x >= njobs) return;
-  const DysonJob J = jobs[blockIdx.x];
+  const DysonJob J = jobs[order ? order[blockIdx.x] : blockIdx.x];
   const int n = J.n, nI = J.nI, j = threadIdx.x;
   if (n <= 0 || n > 64 || n > ld || nI > NIMAX || !J.poles) return;
   double *A = sm, *v = A + (long)ld * ld, *wv = v + 64;
@@ -1399,9 +1399,9 @@ __global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, int 
   }
 }
 
-template __global__ void k_tridiag_poles<4>(const DysonJob *, int, int);
-template __global__ void k_tridiag_poles<8>(const DysonJob *, int, int);
-template __global__ void k_tridiag_poles<16>(const DysonJob *, int, int);
+template __global__ void k_tridiag_poles<4>(const DysonJob *, const int *, int, int);
+template __global__ void k_tridiag_poles<8>(const DysonJob *, const int *, int, int);
+template __global__ void k_tridiag_poles<16>(const DysonJob *, const int *, int, int);
 
 template __global__ void k_tridiag<true>(const DysonJob *, int);
 template __global__ void k_tridiag<false>(const DysonJob *, int);
@@ -1598,14 +1598,15 @@ __global__ __launch_bounds__(64) void k_ql(const DysonJob *jobs, int njobs, int 
 // does the work of eight matrices in the instructions k_ql spends on one.
 // ld: LDS doubles per d or e array (the batch's largest side); qrows: tracked
 // rows per slice (the batch's largest nI).
-__global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, int njobs, int *status, int ld, int qrows) {
+__global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, const int *order, int njobs, int *status, int ld,
+                                                  int qrows) {
   extern __shared__ double sm[];
   const int lane = threadIdx.x, grp = lane >> 3, gl = lane & 7;
   const int job = blockIdx.x * 8 + grp;
   double *d = sm + (long)grp * (2 + qrows) * ld, *e = d + ld, *Q = e + ld;
   const double eps = 2.2204460492503131e-16;
   int n = 0, nI = 0;
-  const DysonJob *J = job < njobs ? jobs + job : nullptr;
+  const DysonJob *J = job < njobs ? jobs + (order ? order[job] : job) : nullptr;
   if (J) { n = J->n; nI = J->nI; }
   if (n > ld || nI > qrows || nI > kQLGroupNI) {
     // does not fit this kernel's slices (the host never launches such jobs here)

@@ -90,15 +90,17 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
                              double *kolt, DysonJob *jobs);
 template <bool IN_LDS>
 __global__ void k_tridiag(const DysonJob *jobs, int njobs);
+// The pole-block kernels take njobs jobs, job k being jobs[order[k]] (or
+// jobs[k] when order is null).
 template <int NIMAX>
-__global__ void k_tridiag_poles(const DysonJob *jobs, int njobs, int ld);
+__global__ void k_tridiag_poles(const DysonJob *jobs, const int *order, int njobs, int ld);
 __global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
 constexpr int kQLGroupNI = 8;     // tracked rows of k_ql_poles (one lane of an 8-lane group each)
-__global__ void k_ql_poles(const DysonJob *jobs, int njobs, int *status, int ld, int qrows);
+__global__ void k_ql_poles(const DysonJob *jobs, const int *order, int njobs, int *status, int ld, int qrows);
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow(const ArrowJob *jobs, int njobs, int *status);
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
-                             double *tri, double *mat, double *poles, DysonJob *jobs);
+                             double *tri, double *mat, double *poles, DysonJob *jobs, const int *order, int njobs);
 // Everything one Dyson sweep over all channels reads (k_arrow_sweep).
 struct ArrowSweep {
   const ChanDev *chans;
