@@ -614,10 +614,11 @@ __global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n
                                                     const long *row_fw_base, const long *row_bk_base,
                                                     const double *rows_fw, const double *rows_bk, double *crows_fw,
                                                     double *crows_bk) {
-  // the chunk's non-empty triples: first block, J blocks, a == b, fragment
-  // counts, tuples (inclusive prefix), diagonal-group size and output base
-  __shared__ int q_s0[256], q_nb[256], q_eq[256], q_cA[256], q_cB[256], q_cG[256], q_tp[256], q_n0[256],
-      q_base[256];
+  // the chunk's non-empty triples: J blocks, a == b, fragment counts, tuples
+  // (inclusive prefix), diagonal-group size and output base, and per J block
+  // its first row and nJab as (row << 1) | nJab
+  __shared__ int q_nb[256], q_eq[256], q_cA[256], q_cB[256], q_cG[256], q_tp[256], q_n0[256], q_base[256];
+  __shared__ int q_jb[kMergeMaxJ][256];
   __shared__ int wtot[3][4];
   const int job = blockIdx.x;
   if (job >= 2 * n_gc || !crows_fw) return;
@@ -666,8 +667,12 @@ __global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n
     }
     if (live) {
       const int k = pL + __popcll(m & ((1ull << lane) - 1ull));
-      q_s0[k] = s0; q_nb[k] = min(nblk, kMergeMaxJ); q_eq[k] = eq; q_cA[k] = cA; q_cB[k] = cB; q_cG[k] = cG;
+      q_nb[k] = min(nblk, kMergeMaxJ); q_eq[k] = eq; q_cA[k] = cA; q_cB[k] = cB; q_cG[k] = cG;
       q_tp[k] = pT + iT; q_n0[k] = n0; q_base[k] = base + pC + iC - cnt;
+      for (int x = 0; x < nblk && x < kMergeMaxJ; ++x) {
+        const int bx = trip_blk[s0 + x];
+        q_jb[x][k] = (boff[bx] << 1) | (blocks[bx].nJab & 1);
+      }
     }
     __syncthreads();
     // pass 2, one thread per fragment tuple of the chunk's live triples
@@ -696,15 +701,13 @@ __global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n
       // tG tuples each, plus the earlier ng of this pair
       const int pq = u / tG, D = a_eq_b ? na + (nb > na ? 1 : 0) : 0;
       int pos = q_base[k_] + (D * rn0 + (pq - D) * rnb) * tG + ng * (diag ? rn0 : rnb);
-      const int s0k = q_s0[k_];
       auto row_of = [&](int x, int nJ) -> const double * {
-        const int bx = trip_blk[s0k + x];
         const int pr = a_eq_b ? na * (tA - nJ) - na * (na - 1) / 2 + (nb - na - nJ) : na * tB + nb;
-        return rows + (long)(boff[bx] + pr * tG + ng) * w;
+        return rows + (long)((q_jb[x][k_] >> 1) + pr * tG + ng) * w;
       };
       if (!(kk > nI && nI <= kMergeMaxNI)) {
         for (int x = 0; x < nb_; ++x) {
-          const int nJ = blocks[trip_blk[s0k + x]].nJab;
+          const int nJ = q_jb[x][k_] & 1;
           if (a_eq_b && nb < na + nJ) continue;
           const double *src = row_of(x, nJ);
           for (int c = 0; c < w; ++c) out[(long)pos * w + c] = src[c];
@@ -715,7 +718,7 @@ __global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n
       double S_[kMergeMaxNI][kMergeMaxNI] = {}, L[kMergeMaxNI][kMergeMaxNI] = {};
       double e = 0.0;
       for (int x = 0; x < nb_; ++x) {
-        const int nJ = blocks[trip_blk[s0k + x]].nJab;
+        const int nJ = q_jb[x][k_] & 1;
         if (a_eq_b && nb < na + nJ) continue;
         const double *src = row_of(x, nJ);
         e = src[0];
