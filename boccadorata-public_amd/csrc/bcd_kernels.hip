@@ -922,9 +922,10 @@ __device__ __forceinline__ void buf_st2(double a, double b, __amdgpu_buffer_rsrc
 // pair q = tid + NT*t being elements 2q and 2q+1, so that every access moves
 // 16 bytes per lane; vectors are stored with an even stride Ms, and the
 // padding element of an odd M is kept at zero.
-// Behind the nLanc vectors the workspace holds E and the NB coupling columns
-// of the input rows as contiguous vectors, and E * v_last of every finished
-// pivot chain, so the per-step sums read contiguous vectors only.
+// E stays in registers for the whole job (read once from the input rows);
+// behind the nLanc vectors the workspace holds the NB coupling columns of the
+// input rows as contiguous vectors, and E * v_last of every finished pivot
+// chain, so the per-step sums read contiguous vectors only.
 template <int NT, int PER>
 __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, double *u, int *last, double *part) {
   static_assert(PER % 2 == 0, "elements come in pairs");
@@ -937,15 +938,16 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
   const long nvec = lanczos_ws_vectors(nLanc, NB);
   const __amdgpu_buffer_rsrc_t rV = buf_rsrc(Jb.V, 8L * nvec * Ms), rM = buf_rsrc(Min, 8L * M * Tin);
   const int vo = 16 * tid, mo = 16 * tid * Tin;
-  const int eo = 8 * nLanc * Ms, co = eo + 8 * Ms, wo = co + 8 * NB * Ms;   // E, couplings, E * v_last
+  const int co = 8 * nLanc * Ms, wo = co + 8 * NB * Ms;   // couplings, E * v_last
+  double ev[PER];
 #pragma unroll
   for (int t = 0; t < PP; ++t) {
     const int p = 2 * (tid + NT * t), ms = 16 * NT * t * Tin;
+    ev[2 * t] = ev[2 * t + 1] = 0.0;
     if (p < M) {
       const bool h1 = p + 1 < M;
-      double e1 = 0.0;
-      if (h1) e1 = buf_ld(rM, mo, ms + 8 * Tin);
-      buf_st2(buf_ld(rM, mo, ms), e1, rV, vo, eo + 16 * NT * t);
+      if (h1) ev[2 * t + 1] = buf_ld(rM, mo, ms + 8 * Tin);
+      ev[2 * t] = buf_ld(rM, mo, ms);
       for (int l = 0; l < NB; ++l) {
         double c1 = 0.0;
         if (h1) c1 = buf_ld(rM, mo, ms + 8 * (Tin + NA + l));
@@ -987,16 +989,7 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
       }
     } else {
 #pragma unroll
-      for (int t = 0; t < PP; ++t) {
-        const int p = 2 * (tid + NT * t);
-        if (p < M) {
-          const double2 e = buf_ld2(rV, vo, eo + 16 * NT * t);
-          v[2 * t] *= e.x;
-          v[2 * t + 1] *= e.y;
-        } else {
-          v[2 * t] = v[2 * t + 1] = 0.0;
-        }
-      }
+      for (int t = 0; t < PER; ++t) v[t] *= ev[t];
     }
   // Gram-Schmidt against every earlier vector, latest first (Lanczos.cpp:164-173)
     for (int i = j - 1; i >= 0; --i) {
@@ -1027,23 +1020,21 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
     const int nred = 2 + NB + n_pvt, lane = tid & 63, wv = tid >> 6;
     for (int k = 0; k < nred; ++k) {
       double acc = 0.0;
-      if (k == 1) {
+      if (k == 0) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) acc += (ev[t] * v[t]) * v[t];
+      } else if (k == 1) {
 #pragma unroll
         for (int t = 0; t < PER; ++t) acc += v[t] * v[t];
       } else {
-        const int lo = k == 0 ? eo : (k < 2 + NB ? co + 8 * (k - 2) * Ms : wo + 8 * (k - 2 - NB) * Ms);
+        const int lo = k < 2 + NB ? co + 8 * (k - 2) * Ms : wo + 8 * (k - 2 - NB) * Ms;
 #pragma unroll
         for (int t = 0; t < PP; ++t) {
           const int p = 2 * (tid + NT * t);
           if (p < M) {
             const double2 x = buf_ld2(rV, vo, lo + 16 * NT * t);
-            if (k == 0) {
-              acc += (x.x * v[2 * t]) * v[2 * t];
-              acc += (x.y * v[2 * t + 1]) * v[2 * t + 1];
-            } else {
-              acc += x.x * v[2 * t];
-              acc += x.y * v[2 * t + 1];
-            }
+            acc += x.x * v[2 * t];
+            acc += x.y * v[2 * t + 1];
           }
         }
       }
@@ -1077,10 +1068,7 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
       v[2 * t + 1] /= xb;
       if (p < M) {
         buf_st2(v[2 * t], v[2 * t + 1], rV, vo, 8 * j * Ms + 16 * NT * t);
-        if (chain_end) {
-          const double2 e = buf_ld2(rV, vo, eo + 16 * NT * t);
-          buf_st2(v[2 * t] * e.x, v[2 * t + 1] * e.y, rV, vo, wo + 8 * n_pvt * Ms + 16 * NT * t);
-        }
+        if (chain_end) buf_st2(v[2 * t] * ev[2 * t], v[2 * t + 1] * ev[2 * t + 1], rV, vo, wo + 8 * n_pvt * Ms + 16 * NT * t);
       }
     }
   }

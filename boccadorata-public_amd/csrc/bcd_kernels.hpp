@@ -16,9 +16,9 @@ constexpr int kLancRed = 2 + kMaxNB + kMaxPiv;
 constexpr int kMaxNI = 16;        // orbits per subshell
 constexpr int kMaxSub = 64;       // subshells per instance
 // Lanczos workspace of one (channel, side), in vectors of M doubles: the
-// Krylov vectors, then E, the nb coupling columns and E * v_last of up to
+// Krylov vectors, then the nb coupling columns and E * v_last of up to
 // kMaxPiv pivot chains
-__host__ __device__ inline long lanczos_ws_vectors(long nlanc, int nb) { return nlanc + 1 + nb + kMaxPiv; }
+__host__ __device__ inline long lanczos_ws_vectors(long nlanc, int nb) { return nlanc + nb + kMaxPiv; }
 // Stride of the workspace vectors of a job with M poles: even, so that
 // element pairs are 16-byte aligned.
 __host__ __device__ inline int lanczos_stride(int M) { return M + (M & 1); }
