@@ -925,9 +925,14 @@ __device__ __forceinline__ void buf_st2(double a, double b, __amdgpu_buffer_rsrc
 // E stays in registers for the whole job (read once from the input rows);
 // behind the nLanc vectors the workspace holds the NB coupling columns of the
 // input rows as contiguous vectors, and E * v_last of every finished pivot
-// chain, so the per-step sums read contiguous vectors only.
+// chain, so the per-step sums read contiguous vectors only.  With a stash
+// (NT * PER doubles of LDS), each thread also keeps its own elements of the
+// latest vector there, and the first projection of every step reads them
+// from LDS instead of HBM; only the thread that wrote an element reads it, so
+// no barrier is involved.
 template <int NT, int PER>
-__device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, double *u, int *last, double *part) {
+__device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, double *u, int *last, double *part,
+                                            double2 *stash) {
   static_assert(PER % 2 == 0, "elements come in pairs");
   constexpr int PP = PER / 2;
   const int tid = threadIdx.x, M = Jb.M, NB = Jb.NB, Tin = Jb.Tin, NA = Jb.NA_in;
@@ -994,13 +999,15 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
   // Gram-Schmidt against every earlier vector, latest first (Lanczos.cpp:164-173)
     for (int i = j - 1; i >= 0; --i) {
       const int io = 8 * i * Ms;
+      const bool from_lds = stash && i == j - 1;
       double s[2] = {0.0, 0.0};
 #pragma unroll
       for (int t = 0; t < PP; ++t) {
         const int p = 2 * (tid + NT * t);
         double2 x;
         x.x = x.y = 0.0;
-        if (p < M) x = buf_ld2(rV, vo, io + 16 * NT * t);
+        if (from_lds) x = stash[tid + NT * t];
+        else if (p < M) x = buf_ld2(rV, vo, io + 16 * NT * t);
         vi[2 * t] = x.x;
         vi[2 * t + 1] = x.y;
         s[0] += x.x * v[2 * t];
@@ -1066,6 +1073,12 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
       const int p = 2 * (tid + NT * t);
       v[2 * t] /= xb;
       v[2 * t + 1] /= xb;
+      if (stash) {
+        double2 x;
+        x.x = v[2 * t];
+        x.y = v[2 * t + 1];
+        stash[tid + NT * t] = x;
+      }
       if (p < M) {
         buf_st2(v[2 * t], v[2 * t + 1], rV, vo, 8 * j * Ms + 16 * NT * t);
         if (chain_end) buf_st2(v[2 * t] * ev[2 * t], v[2 * t + 1] * ev[2 * t + 1], rV, vo, wo + 8 * n_pvt * Ms + 16 * NT * t);
@@ -1094,16 +1107,22 @@ __global__ __launch_bounds__(NT) void k_lanczos(const LancJob *jobs, const int *
   __shared__ double u[kMaxNB];
   __shared__ double part[kLancRed * (NT / 64)];
   __shared__ int last[kMaxPiv + 1];
+  extern __shared__ double2 lanc_stash[];
   if ((int)blockIdx.x >= njobs) return;
   const LancJob Jb = jobs[order ? order[blockIdx.x] : blockIdx.x];
   if (Jb.nLanc <= 0 || Jb.M > PER * NT) return;
-  lanczos_job<NT, PER>(Jb, red, u, last, part);
+  lanczos_job<NT, PER>(Jb, red, u, last, part, lanczos_stash_bytes(NT, PER) ? lanc_stash : nullptr);
 }
 
 template <int I>
 static void launch_shape(int njobs, const LancJob *jobs, const int *order, hipStream_t stream) {
   constexpr int NT = kLancShapes[I].nt, PER = kLancShapes[I].per;
-  hipLaunchKernelGGL((k_lanczos<NT, PER>), dim3(njobs), dim3(NT), 0, stream, jobs, order, njobs);
+  constexpr int lds = lanczos_stash_bytes(NT, PER);
+  static bool attr = false;
+  if (lds > 65536 && !attr)
+    attr = hipFuncSetAttribute((const void *)k_lanczos<NT, PER>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) ==
+           hipSuccess;
+  hipLaunchKernelGGL((k_lanczos<NT, PER>), dim3(njobs), dim3(NT), lds, stream, jobs, order, njobs);
 }
 
 void launch_lanczos(int shape, int njobs, const LancJob *jobs, const int *order, hipStream_t stream) {

@@ -19,6 +19,10 @@ constexpr int kMaxSub = 64;       // subshells per instance
 // Krylov vectors, then the nb coupling columns and E * v_last of up to
 // kMaxPiv pivot chains
 __host__ __device__ inline long lanczos_ws_vectors(long nlanc, int nb) { return nlanc + nb + kMaxPiv; }
+// Dynamic LDS of a Lanczos tile of NT threads x PER elements: each thread's
+// elements of the latest Krylov vector, or 0 (no stash) where that exceeds
+// 128 KiB.
+constexpr int lanczos_stash_bytes(int nt, int per) { return 8 * nt * per <= 128 * 1024 ? 8 * nt * per : 0; }
 // Stride of the workspace vectors of a job with M poles: even, so that
 // element pairs are 16-byte aligned.
 __host__ __device__ inline int lanczos_stride(int M) { return M + (M & 1); }
