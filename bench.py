@@ -124,7 +124,7 @@ def spectral_error(g, ref, zsig=ZSIG):
 # HBM bytes per launch of the dominant kernel family, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes of this workload (calibrated against a 1 GiB
 # copy): (kernel family, instances per step) -> (bytes, summary file)
-MEASURED_TRAFFIC = {("lanczos", 256): (230.2e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md")}
+MEASURED_TRAFFIC = {("lanczos", 256): (225.7e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md")}
 
 
 def roofline(batch):
