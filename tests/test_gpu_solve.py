@@ -232,3 +232,26 @@ def test_lanczos_autotune():
             for x, y in ((np.stack([ep, zp], 1), np.stack([rp, rz], 1)), (np.stack([eh, zh], 1), np.stack([rh, rzh], 1))):
                 e_err, z_err, m_err = spectral_parity(x, y, 1e-10, zsig=zsig)
                 assert e_err < 1e-10 and z_err < 1e-10 and m_err < 1e-10, (e_err, z_err, m_err)
+
+
+def test_profiled_solve_equals_timed_solve():
+    """The timed solve runs each pole-count bucket's Lanczos jobs and then its
+    pole blocks on one of four streams; a profiled solve runs the pole blocks
+    after every bucket, one kernel group at a time.  Each side's work is the
+    same either way, so both give the same propagators bit for bit."""
+    systems = [synth.make_system(nm, s) for nm, s in ((7, 0), (5, 1), (5, 2))]
+    par = capi.solve_params(mode=2, hf_itr_max=100, sc_itr_max=60, lanczos=True, lanc_kind=1, lanc_n=50,
+                            conv_kind=1, hf_conv=1e-8, sc_conv=1e-8, accel=5)
+    b = capi.Batch(systems, par)
+    b.solve()
+    timed = [b.result(i) for i in range(len(systems))]
+    b.set_profile(True)
+    b.solve()
+    prof = [b.result(i) for i in range(len(systems))]
+    kt = b.kernel_times()
+    b.close()
+    assert kt["poles"][1] > 0 and kt["merge"][1] > 0
+    for g, r in zip(prof, timed):
+        assert np.array_equal(g.np_, r.np_) and np.array_equal(g.nh, r.nh)
+        assert np.array_equal(g.ep, r.ep) and np.array_equal(g.eh, r.eh)
+        assert np.array_equal(g.fqp, r.fqp) and np.array_equal(g.fqh, r.fqh)
