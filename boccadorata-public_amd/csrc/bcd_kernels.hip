@@ -374,7 +374,41 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
 // off, so the value is bit-identical to the reference's.  The amplitudes
 // come from S-typed arrays: the block's fragments staged in LDS, or the
 // propagator in HBM when they do not fit the staging area.
-template <class T, class A, class S>
+// The (alpha beta, gamma) sum of one (pole, d) for a compile-time gamma
+// count NOG: the pole's gamma amplitudes stay in registers and the NOG table
+// entries of each (alpha, beta) are loaded together; the terms are added in
+// the reference's order.  The entries of gamma == d when nJgd = 1 are exact
+// zeros in the table, and adding them leaves the sum's bits unchanged (it
+// starts at +0 and never becomes -0), so they need no test.
+template <class T, class A, class S, int NOG>
+__device__ __forceinline__ A sigma2_term_sum(const S *ya, const S *yb, const S *yg, const T *v, int noa, int nob,
+                                             int a_eq_b, int nJab, T xJab, T kS2) {
+#pragma clang fp contract(off)
+  T yq[NOG];
+#pragma unroll
+  for (int gm = 0; gm < NOG; ++gm) yq[gm] = (T)yg[gm];
+  A x1 = (A)0.0;
+#pragma unroll 1
+  for (int al = 0; al < noa; ++al) {
+    const T yaa = (T)ya[al], yba = (T)yb[al];
+#pragma unroll 1
+    for (int be = a_eq_b ? al + nJab : 0; be < nob; ++be, v += NOG) {
+      T vv[NOG];
+#pragma unroll
+      for (int gm = 0; gm < NOG; ++gm) vv[gm] = v[gm];
+      T x2 = yaa * (T)yb[be];
+      if (a_eq_b) {
+        x2 += xJab * (T)ya[be] * yba;
+        if (al == be) x2 /= kS2;
+      }
+#pragma unroll
+      for (int gm = 0; gm < NOG; ++gm) x1 += (A)(vv[gm] * x2 * yq[gm]);
+    }
+  }
+  return x1;
+}
+
+template <class T, class A, class S, bool STAGED>
 __device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *xg, const T *vt, int noa, int nob,
                                              int nog, int nI, int npole, int a_eq_b, int nJab, int nJgd, int cA,
                                              int cB, int cG, int orb_g, int orb_d0, double *rows, int w,
@@ -401,6 +435,14 @@ __device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *
     if (a_eq_b && na == nb) x4 /= kS2;
     const T *v = vt + d * (a_eq_b ? (noa * (noa + 1) / 2 - nJab * noa) : noa * nob) * nog;
     A x1 = (A)0.0;
+    // staged amplitudes (S == T): gamma count specialised; the rare path
+    // that reads the propagator in HBM keeps the general loop
+    switch (STAGED ? nog : 0) {
+      case 1: x1 = sigma2_term_sum<T, A, S, 1>(ya, yb, yg, v, noa, nob, a_eq_b, nJab, xJab, kS2); break;
+      case 2: x1 = sigma2_term_sum<T, A, S, 2>(ya, yb, yg, v, noa, nob, a_eq_b, nJab, xJab, kS2); break;
+      case 3: x1 = sigma2_term_sum<T, A, S, 3>(ya, yb, yg, v, noa, nob, a_eq_b, nJab, xJab, kS2); break;
+      case 4: x1 = sigma2_term_sum<T, A, S, 4>(ya, yb, yg, v, noa, nob, a_eq_b, nJab, xJab, kS2); break;
+      default:
     for (int al = 0; al < noa; ++al) {
       const T yaa = (T)ya[al], yba = (T)yb[al];
       for (int be = a_eq_b ? al + nJab : 0; be < nob; ++be, v += nog) {
@@ -414,6 +456,7 @@ __device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *
           x1 += (A)(v[gm] * x2 * (T)yg[gm]);
         }
       }
+    }
     }
     x1 *= (A)x4;
     if (rb) rb[q] = (double)x1;
@@ -543,10 +586,10 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
       // pass-1 results stay in LDS for pass 2 when they fit
       double *rb = npole * nI <= kSigRowBuf ? rbuf : nullptr;
       if (staged)
-        sigma2_pass1<T, A, T>(stg, (a == b) ? stg : stg + nsa, stg + nsa + nsb, vt, noa, nob, nog, nI, npole,
+        sigma2_pass1<T, A, T, true>(stg, (a == b) ? stg : stg + nsa, stg + nsa + nsb, vt, noa, nob, nog, nI, npole,
                               a == b, nJab, nJgd, cA, cB, cG, og0, C.orb0, rows, w, rb);
       else
-        sigma2_pass1<T, A, double>(xA, xB, xG, vt, noa, nob, nog, nI, npole, a == b, nJab, nJgd, cA, cB, cG,
+        sigma2_pass1<T, A, double, false>(xA, xB, xG, vt, noa, nob, nog, nI, npole, a == b, nJab, nJgd, cA, cB, cG,
                                    og0, C.orb0, rows, w, rb);
       __syncthreads();
       // pass 2: energy and sum of squares (over d in order) per pole
