@@ -283,11 +283,13 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
 
   // jobs
   BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) +
-                   pad8(sizeof(int) * 2 * n_gc) + 64));
+                   pad8(sizeof(int) * 2 * n_gc) + pad8(sizeof(int) * kCsPack * n_gc) + 64));
   d_djobs = (DysonJob *)a_jobs.ptr;
   d_pjobs = d_djobs + n_gc;
   d_ljobs = (LancJob *)((char *)a_jobs.ptr + pad8(sizeof(DysonJob) * 3 * n_gc));
   d_lorder = (int *)((char *)d_ljobs + pad8(sizeof(LancJob) * 2 * n_gc));
+  d_cspack = (int *)((char *)d_lorder + pad8(sizeof(int) * 2 * n_gc));
+  h_cspack.resize((size_t)kCsPack * n_gc);
   HTRY(hipMemset(a_jobs.ptr, 0, a_jobs.cap));
 
   // dense matrices in HBM when the largest one does not fit in LDS: the
@@ -447,6 +449,30 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   return BCD_OK;
 }
 
+// The per-channel counts the host needs between stages (pole counts, error
+// flag, reduced sizes), gathered on the device and copied back compactly;
+// the rest of h_cs is refreshed only by pull_cs().
+int Batch::pull_counts() {
+  hipLaunchKernelGGL(k_cs_pack, dim3((n_gc + 255) / 256), dim3(256), 0, stream, d_cs, n_gc, d_cspack);
+  HTRY(hipGetLastError());
+  HTRY(hipMemcpyAsync(h_cspack.data(), d_cspack, sizeof(int) * kCsPack * n_gc, hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
+  h_cs.resize(n_gc);
+  for (int gc = 0; gc < n_gc; ++gc) {
+    const int *q = &h_cspack[(size_t)kCsPack * gc];
+    ChanState &S = h_cs[gc];
+    S.mfw = q[0]; S.mbk = q[1]; S.lfw = q[2]; S.lbk = q[3]; S.err = q[4]; S.lmfw = q[5]; S.lmbk = q[6];
+  }
+  return BCD_OK;
+}
+
+int Batch::pull_cs() {
+  h_cs.resize(n_gc);
+  HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
+  HTRY(hipStreamSynchronize(stream));
+  return BCD_OK;
+}
+
 int Batch::build_sigma2(int only_ish) {
   PropView P = view(true);
   HTRY(hipMemsetAsync(d_work_count, 0, sizeof(int), stream));
@@ -455,9 +481,7 @@ int Batch::build_sigma2(int only_ish) {
                      d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, only_ish);
   kt_end(KT_PLAN);
   HTRY(hipGetLastError());
-  h_cs.resize(n_gc);
-  HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
-  HTRY(hipStreamSynchronize(stream));
+  BTRY(pull_counts());
   // row buffers and Lanczos workspace sized by the actual pole counts
   for (int gc = 0; gc < n_gc; ++gc)
     if (h_cs[gc].err == 3) return BCD_ERR_CAPACITY;   // a fragment count above 65535
@@ -583,8 +607,7 @@ int Batch::lanczos() {
                      par.lanc_kind, par.lanc_n, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_crows_fw,
                      d_crows_bk, d_lrows, (double *)a_lws.ptr, d_ljobs);
   HTRY(hipGetLastError());
-  HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
-  HTRY(hipStreamSynchronize(stream));
+  BTRY(pull_counts());
   // jobs longest first, grouped into pole-count buckets; one launch per
   // bucket with the bucket's tile shape, the largest jobs first
   std::vector<std::pair<int, int>> jm;
@@ -642,8 +665,7 @@ int Batch::lanczos() {
   tune_lanczos = false;
   kt_end(KT_LANCZOS);
   HTRY(hipGetLastError());
-  HTRY(hipMemcpyAsync(h_cs.data(), d_cs, sizeof(ChanState) * n_gc, hipMemcpyDeviceToHost, stream));
-  HTRY(hipStreamSynchronize(stream));
+  BTRY(pull_counts());
   for (int gc = 0; gc < n_gc; ++gc)
     if (h_cs[gc].err || h_cs[gc].lfw < 0 || h_cs[gc].lbk < 0) return BCD_ERR_LANCZOS;
   return BCD_OK;
@@ -710,6 +732,7 @@ int Batch::decompose_poles() {
 // Algorithmic work of the last solve (see bcd_batch_work in bcdor_amd.h).
 int Batch::work(double *w, int n) {
   for (int k = 0; k < n; ++k) w[k] = 0.0;
+  if (!h_cs.empty()) BTRY(pull_cs());
   if (n >= 15) {
     std::vector<unsigned long long> t(8 * (size_t)n_gc);
     int khz = 0;

@@ -90,6 +90,8 @@ struct Batch {
   DysonJob *d_djobs = nullptr, *d_pjobs = nullptr;   // per-sweep dense jobs; side diagonalisation jobs
   LancJob *d_ljobs = nullptr;
   int *d_lorder = nullptr;                           // Lanczos jobs, longest first
+  int *d_cspack = nullptr;                           // kCsPack counts per channel (pull_counts)
+  std::vector<int> h_cspack;
   long aws_len = 0;                                  // global arrowhead workspace (doubles), 0: LDS
 
   hipEvent_t ev[6] = {};
@@ -127,6 +129,8 @@ struct Batch {
   double coupling_flops = -1.0;
   int lanczos();
   int decompose_poles();
+  int pull_counts();
+  int pull_cs();
   PoleCfg pole_cfg() const;
   void launch_poles(const PoleCfg &c, const int *order, int njobs, hipStream_t s);
   int solve();

@@ -270,6 +270,14 @@ __global__ __launch_bounds__(256) void k_mix(const InstDev *insts, InstState *is
 
 // ------------------------------------------------------------- Sigma(2)
 
+__global__ void k_cs_pack(const ChanState *cs, int n_gc, int *out) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= n_gc) return;
+  const ChanState &S = cs[gc];
+  int *q = out + (long)kCsPack * gc;
+  q[0] = S.mfw; q[1] = S.mbk; q[2] = S.lfw; q[3] = S.lbk; q[4] = S.err; q[5] = S.lmfw; q[6] = S.lmbk; q[7] = 0;
+}
+
 // Pole counts of each (J,a,b,g) block from the fragment counts of the current
 // propagator (Count_2p1h_2h1p_poles, Self_energy_ext_Op_c.cpp:76-90), row
 // offsets in build order, and a compact list of the non-empty blocks.
@@ -1161,10 +1169,10 @@ template <int I>
 static void launch_shape(int njobs, const LancJob *jobs, const int *order, hipStream_t stream) {
   constexpr int NT = kLancShapes[I].nt, PER = kLancShapes[I].per;
   constexpr int lds = lanczos_stash_bytes(NT, PER);
-  static bool attr = false;
-  if (lds > 65536 && !attr)
-    attr = hipFuncSetAttribute((const void *)k_lanczos<NT, PER>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) ==
-           hipSuccess;
+  // above 64 KiB the kernel needs the larger dynamic-LDS limit on the current
+  // device (a failure shows up as a launch error)
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void *)k_lanczos<NT, PER>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   hipLaunchKernelGGL((k_lanczos<NT, PER>), dim3(njobs), dim3(NT), lds, stream, jobs, order, njobs);
 }
 

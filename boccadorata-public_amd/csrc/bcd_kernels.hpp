@@ -103,6 +103,9 @@ constexpr int kQLGroupNI = 8;     // tracked rows of k_ql_poles (one lane of an 
 __global__ void k_ql_poles(const DysonJob *jobs, const int *order, int njobs, int *status, int ld, int qrows);
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow(const ArrowJob *jobs, int njobs, int *status);
+// mfw, mbk, lfw, lbk, err, lmfw, lmbk (and one spare) of every channel
+constexpr int kCsPack = 8;
+__global__ void k_cs_pack(const ChanState *cs, int n_gc, int *out);
 __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs, const double *lrows,
                              double *tri, double *mat, double *poles, DysonJob *jobs, const int *order, int njobs);
 // Everything one Dyson sweep over all channels reads (k_arrow_sweep).
