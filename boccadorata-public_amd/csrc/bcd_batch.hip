@@ -545,25 +545,22 @@ int Batch::build_sigma2(int only_ish) {
     }
     coupling_flops = f;
   }
-  int nwork = 0;
-  HTRY(hipMemcpyAsync(&nwork, d_work_count, sizeof(int), hipMemcpyDeviceToHost, stream));
-  HTRY(hipStreamSynchronize(stream));
-  if (nwork > 0) {
-    kt_begin(KT_SIGMA2);
-    int vt_cap = 2;
-    int c_cap = 2;
-    for (const HostSystem *S : sys) {
-      vt_cap = std::max(vt_cap, (S->max_vt + 1) & ~1);
-      c_cap = std::max(c_cap, (S->max_dg + 1) & ~1);
-    }
-    // precision 1: FP32 operands and products, FP64 sums
-    auto kern = par.precision == 1 ? k_sigma2<float, double> : k_sigma2<double, double>;
-    hipLaunchKernelGGL(kern, dim3(std::min(nwork, 16384)), dim3(64), sigma2_lds(vt_cap, c_cap), stream, d_insts, d_chans,
-                       d_blocks, P, d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base,
-                       d_row_bk_base, d_rows_fw, d_rows_bk, vt_cap, c_cap);
-    kt_end(KT_SIGMA2);
-    HTRY(hipGetLastError());
+  // the kernel reads the work-list length from the device and strides over
+  // it, so a fixed grid needs no copy of the count back to the host
+  kt_begin(KT_SIGMA2);
+  int vt_cap = 2;
+  int c_cap = 2;
+  for (const HostSystem *S : sys) {
+    vt_cap = std::max(vt_cap, (S->max_vt + 1) & ~1);
+    c_cap = std::max(c_cap, (S->max_dg + 1) & ~1);
   }
+  // precision 1: FP32 operands and products, FP64 sums
+  auto kern = par.precision == 1 ? k_sigma2<float, double> : k_sigma2<double, double>;
+  hipLaunchKernelGGL(kern, dim3(kSigGrid), dim3(64), sigma2_lds(vt_cap, c_cap), stream, d_insts, d_chans, d_blocks, P,
+                     d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base, d_row_bk_base, d_rows_fw,
+                     d_rows_bk, vt_cap, c_cap);
+  kt_end(KT_SIGMA2);
+  HTRY(hipGetLastError());
   return BCD_OK;
 }
 

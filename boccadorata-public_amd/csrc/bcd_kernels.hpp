@@ -8,6 +8,7 @@ namespace bcd {
 
 constexpr int kMaxVt = 2048;      // nI * (alpha,beta) pairs * gamma of one Sigma(2) block
 constexpr int kMaxPairs = 64;     // (alpha,beta) pairs of one block (+1 slot for the count)
+constexpr int kSigGrid = 16384;    // waves of the Sigma(2) build, striding over the work list
 constexpr int kSigStage = 128;    // fragment amplitudes of one Sigma(2) block side staged in LDS
 constexpr int kSigRowBuf = 128;   // (pole, orbit) couplings of one block side kept in LDS for pass 2
 constexpr int kMaxPiv = 8;        // Lanczos pivots per channel
