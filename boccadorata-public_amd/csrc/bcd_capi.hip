@@ -106,7 +106,7 @@ int stage_batch(bcd_ctx *ctx, const bcd_prop *g, int mode, Batch &b) {
 
 struct DevBuf {
   void *p = nullptr;
-  ~DevBuf() { if (p) hipFree(p); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
   int alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 8) == hipSuccess ? BCD_OK : BCD_ERR_NOMEM; }
 };
 
@@ -211,7 +211,7 @@ void bcd_ctx_destroy(bcd_ctx *ctx) {
   if (!ctx) return;
   ctx->t2.reset();
   ctx->scratch.release();
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
