@@ -440,8 +440,11 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   const int hard_cap = itr_max >= 0 ? itr_max + 1 : kMaxSweeps;
   int n_active = n_inst, done = 0;
   while (done < hard_cap && n_active > 0) {
-    // sweeps are queued kSweepChunk at a time between reads of the flags
-    const int k = std::min(kSweepChunk, hard_cap - done);
+    // sweeps are queued kSweepChunk at a time between reads of the flags,
+    // twice as many in the HF stage of the arrowhead path, whose sweeps cost
+    // about as much as one read
+    const int chunk = (hf && par.dyson_solver == 0) ? 2 * kSweepChunk : kSweepChunk;
+    const int k = std::min(chunk, hard_cap - done);
     for (int q = 0; q < k; ++q) BTRY(sweep(src));
     done += k;
     BTRY(check(n_active));
