@@ -380,7 +380,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; shard instances across GPUs "
-                                   "(one process per GPU) to scale, see DESIGN.md"}))
+                                   "(one process per GPU) to scale"}))
         sys.exit(2)
     if capi.lib().bcd_device_count() < 1:
         raise SystemExit("no GPU visible: the HIP path has no CPU fallback")
