@@ -167,38 +167,87 @@ def read_propagator(path: str, ms: ModelSpace) -> Propagator:
 
 
 # -------------------------------------------------------------------- pivots
-def write_pivots(path: str, ms: ModelSpace, pivots: dict) -> None:
-    """Pivot blocks as SpProp_t::DysPivots writes and Make_LancPivots reads
-    (Sp_Propagator.cpp:1225-1289, Miscellanea.cpp:334-390).
-    pivots[ish] = (vectors [npv, no], itr_fw [npv], itr_bk [npv])."""
+_PIV_HEAD = "#   Template pivots file\n#\n# number of (ilj\\pi) subshells, max n. of radial orbitals:\n"
+
+
+def dys_pivots(path: str, ms: ModelSpace, g: Propagator, n_fw: int, n_bk: int, z_cut: float = 60.0) -> None:
+    """The pivots file SpProp_t::DysPivots writes from g (Sp_Propagator.cpp:
+    1225-1289), byte for byte: per subshell the header line with npvts = the
+    subshell's orbit count, then its quasiholes and then its quasiparticles
+    with 100*Z >= z_cut, each side by ascending energy (a fragment whose
+    energy equals the one picked before it is skipped), every pivot with
+    iteration counts n_fw n_bk, its components as %18.6le and its energy as a
+    comment."""
     with open(path, "w") as f:
-        f.write("#   Template pivots file\n#\n# number of (ilj\\pi) subshells, max n. of radial orbitals:\n")
+        f.write(_PIV_HEAD)
+        f.write("# %7i %7i\n\n" % (ms.nsubsh, g.nspb))
+        for i in range(ms.nsubsh):
+            f.write("\n\nshell,npvts:   %i   %i  # %s \n" % (i, ms.no[i], ms.names[i]))
+            ep, zp, fqp, eh, zh, fqh = g.subshell(i)
+            for e, z, x, tag in ((eh, zh, fqh, "e-"), (ep, zp, fqp, "e+")):
+                x2 = -1.0e10
+                while True:
+                    x1, pick = 1.0e10, -1
+                    for j in range(len(e)):
+                        if e[j] < x1 and e[j] > x2 and 100.0 * z[j] >= z_cut:
+                            x1, pick = e[j], j
+                    if pick < 0:
+                        break
+                    x2 = x1
+                    f.write(" %3i  %3i  " % (n_fw, n_bk))
+                    f.write("".join(" %18.6e " % v for v in x[pick, :ms.no[i]]))
+                    f.write("   #    %s = %13.6f\n" % (tag, e[pick]))
+
+
+def write_pivots(path: str, ms: ModelSpace, pivots: dict, full_precision: bool = True) -> None:
+    """A pivots file in the layout Make_LancPivots reads (Miscellanea.cpp:
+    334-390) with explicit per-pivot iteration counts.
+    pivots[ish] = (vectors [npv, no], itr_fw [npv], itr_bk [npv]); the header
+    gives npv.  Components are written with %.17g so that the reader gets
+    every bit back, or with DysPivots' own %18.6le (full_precision=False)."""
+    fx = (lambda x: f" {x:.17g} ") if full_precision else (lambda x: " %18.6e " % x)
+    with open(path, "w") as f:
+        f.write(_PIV_HEAD)
         f.write(f"# {ms.nsubsh:7d} {ms.nspb:7d}\n\n")
         for ish in range(ms.nsubsh):
             vec, ifw, ibk = pivots[ish]
             f.write(f"\n\nshell,npvts:   {ish}   {len(ifw)}  # {ms.names[ish]} \n")
             for k in range(len(ifw)):
-                f.write(f" {int(ifw[k]):3d}  {int(ibk[k]):3d}  " + "".join(f" {x:.17g} " for x in vec[k]) + "\n")
+                f.write(f" {int(ifw[k]):3d}  {int(ibk[k]):3d}  " + "".join(fx(x) for x in vec[k]) + "\n")
 
 
 def read_pivots(path: str, ms: ModelSpace) -> dict:
+    """Make_LancPivots (Miscellanea.cpp:334-390): after the 'shell,npvts: ish n'
+    line of a subshell it reads n lines, keeps those that start with two
+    integers (the iteration counts fw and bk) and takes the subshell's orbit
+    count of numbers after them.  Returns ish -> (vectors, itr_fw, itr_bk)."""
     out = {}
     with open(path) as f:
         lines = f.read().split("\n")
-    p = 0
-    while p < len(lines):
-        t = lines[p]
-        if t.startswith("shell,npvts:"):
-            ish, n = (int(x) for x in t.split(":")[1].split()[:2])
-            vec, ifw, ibk = [], [], []
-            for k in range(n):
-                s = lines[p + 1 + k].split("#")[0].split()
-                ifw.append(int(s[0])); ibk.append(int(s[1]))
-                vec.append([float(x) for x in s[2:2 + ms.no[ish]]])
-            out[ish] = (np.asarray(vec, np.float64).reshape(n, ms.no[ish]), np.asarray(ifw, np.int32),
-                        np.asarray(ibk, np.int32))
-            p += n
-        p += 1
+    # each subshell's block is read on its own from its header on (the
+    # reference reopens the file per subshell), so a block that announces
+    # more lines than it has reads on into the next one
+    for h, t in enumerate(lines):
+        if not t.startswith("shell,npvts:"):
+            continue
+        ish, n = (int(x) for x in t.split(":")[1].split()[:2])
+        if ish in out:
+            continue
+        vec, ifw, ibk = [], [], []
+        p = h + 1
+        for _ in range(n):
+            if p >= len(lines):
+                break
+            s = lines[p].split()
+            p += 1
+            try:
+                a, b = int(s[0]), int(s[1])
+            except (ValueError, IndexError):
+                continue
+            ifw.append(a); ibk.append(b)
+            vec.append([float(x) for x in s[2:2 + ms.no[ish]]])
+        out[ish] = (np.asarray(vec, np.float64).reshape(len(ifw), ms.no[ish]), np.asarray(ifw, np.int32),
+                    np.asarray(ibk, np.int32))
     return out
 
 

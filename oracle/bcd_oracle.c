@@ -929,9 +929,18 @@ typedef struct {
   int *ipiv_fw, *ipiv_bk;
 } chan_sigma;
 
-/* Pivots for subshell ish as DysPivots (Sp_Propagator.cpp:1225-1289) writes
- * them: quasiholes then quasiparticles, each by ascending energy, Z >= z_cut. */
-static int make_pivots(const orc_sys *s, const orc_prop *g, int ish, double *vec) {
+/* "%18.6le" then %lf: the value DysPivots prints (Sp_Propagator.cpp:1252)
+ * and Make_LancPivots reads back (Miscellanea.cpp:366-370). */
+static double print_round(double x) {
+  char buf[64];
+  snprintf(buf, sizeof buf, "%18.6le", x);
+  return strtod(buf, NULL);
+}
+
+/* DysPivots (Sp_Propagator.cpp:1225-1289): quasiholes then quasiparticles,
+ * each side by ascending energy (a fragment whose energy equals the last one
+ * picked is skipped), keeping those with 100*Z >= z_cut. */
+int orc_make_pivots(const orc_sys *s, const orc_prop *g, int ish, double z_cut, int round_print, double *vec) {
   int npv = 0, side, j, k, nI = s->no[ish];
   for (side = 0; side < 2; ++side) {
     int cnt = side ? g->np[ish] : g->nh[ish], off = side ? g->op[ish] : g->oh[ish];
@@ -942,15 +951,25 @@ static int make_pivots(const orc_sys *s, const orc_prop *g, int ish, double *vec
       double x1 = 1.0e10;
       pick = -100;
       for (j = 0; j < cnt; ++j)
-        if (e[off + j] < x1 && e[off + j] > x2 && 100. * z[off + j] >= 60.0) { x1 = e[off + j]; pick = j; }
+        if (e[off + j] < x1 && e[off + j] > x2 && 100. * z[off + j] >= z_cut) { x1 = e[off + j]; pick = j; }
       if (pick >= 0) {
         x2 = x1;
-        for (k = 0; k < nI; ++k) vec[npv * nI + k] = f[(size_t)(off + pick) * g->nspb + k];
+        for (k = 0; k < nI; ++k) {
+          const double v = f[(size_t)(off + pick) * g->nspb + k];
+          vec[npv * nI + k] = round_print ? print_round(v) : v;
+        }
         ++npv;
       }
     }
   }
   return npv;
+}
+
+double orc_fermi(const orc_sys *s, const orc_prop *g, int ish, const orc_params *par) {
+  int k;
+  for (k = 0; par && k < par->n_ef; ++k)
+    if (par->ef_ish[k] == ish) return par->ef_val[k];
+  return orc_seek_efermi(s, g, ish);
 }
 
 static void split_itr(int total, int npiv, int *itr) {
@@ -1012,7 +1031,7 @@ int orc_run(const orc_sys *s, const orc_prop *gin, const orc_params *par,
             for (k = 0; k < npv; ++k) { ifw[k] = itr_fw[piv_ioff[ish] + k]; ibk[k] = itr_bk[piv_ioff[ish] + k]; }
           } else {
             pv = (double *)malloc(sizeof(double) * (size_t)(g1->np[ish] + g1->nh[ish] + 1) * nI);
-            npv = make_pivots(s, g1, ish, pv);
+            npv = orc_make_pivots(s, g1, ish, 60.0, par->piv_round, pv);
             ifw = (int *)malloc(sizeof(int) * (npv > 0 ? npv : 1));
             ibk = (int *)malloc(sizeof(int) * (npv > 0 ? npv : 1));
             if (par->lanc_kind == 0) { for (k = 0; k < npv; ++k) ifw[k] = ibk[k] = par->lanc_n; }
@@ -1046,12 +1065,13 @@ int orc_run(const orc_sys *s, const orc_prop *gin, const orc_params *par,
       int nI = s->no[ish], nqp, nqh;
       chan_sigma *c = &cs[ish];
       double k3[3];
-      double ef = orc_seek_efermi(s, g1, ish);
+      double ef = orc_fermi(s, g1, ish, par);
       memset(u, 0, sizeof(double) * nI * nI);
       orc_hf_pot(s, ish, g1, u, nI);
       memset(u1, 0, sizeof(double) * nI * nI);
       if (par->t_fac != 0.0) {
-        orc_kin_ho(s, ish, 1.0, tk, nI);
+        /* Get_Tab_kin_ho then *= U1body_fac (Main.cpp:266-270) */
+        orc_kin_ho(s, ish, par->hw, tk, nI);
         for (i = 0; i < nI * nI; ++i) u1[i] += tk[i] * par->t_fac;
       }
       if (par->ext_u1 == 1) for (i = 0; i < nI; ++i) u1[i * nI + i] += s->orb_e[s->orb0[ish] + i];

@@ -119,10 +119,26 @@ typedef struct orc_params {
   int lanc_n;
   int conv_kind;   /* 0: sum|de| < conv (quick_n_dirty_diff); 1: max|de| < conv       */
   double conv;
-  double t_fac;    /* U1body_fac*htom multiplying T_HO (0 for Trel=7)                 */
+  double t_fac;    /* U1body_fac multiplying htom*T_HO (0 for Trel=7)                 */
   int ext_u1;      /* 1: add diag(model-space spe) (ExtU1=1)                          */
   int n_pivot_sets;/* >0: explicit pivots supplied per subshell (see orc_run)          */
+  double hw;       /* htom of the kinetic table (Get_Tab_kin_ho, Utilities.cpp:106-124) */
+  int piv_round;   /* 1: derived pivots rounded to the %18.6le text DysPivots writes   */
+  int n_ef;        /* forced Fermi energies (Set_external_EFermi, DysonMatrix.cpp:601-617) */
+  const int *ef_ish;
+  const double *ef_val;
 } orc_params;
+
+/* Pivots of subshell ish as DysPivots (Sp_Propagator.cpp:1225-1289) writes
+ * them from g: quasiholes then quasiparticles, each by ascending energy with
+ * 100*Z >= z_cut; round_print: each component rounded as "%18.6le" prints it
+ * and Make_LancPivots (Miscellanea.cpp:366-370) reads it back.  Returns the
+ * number of pivots; vec receives them row-major [npv x no(ish)]. */
+int orc_make_pivots(const orc_sys *s, const orc_prop *g, int ish, double z_cut, int round_print, double *vec);
+
+/* Seek_EFermi with the forced values of par (DysonMatrix.cpp:619-635): the
+ * first forced entry for ish wins, else the midpoint of g. */
+double orc_fermi(const orc_sys *s, const orc_prop *g, int ish, const orc_params *par);
 
 typedef struct orc_stats {
   int iterations;           /* number of Dyson sweeps performed                    */

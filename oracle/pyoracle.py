@@ -24,7 +24,8 @@ F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 class Params(C.Structure):
     _fields_ = [("mode", C.c_int), ("itr_max", C.c_int), ("lanczos", C.c_int), ("lanc_kind", C.c_int),
                 ("lanc_n", C.c_int), ("conv_kind", C.c_int), ("conv", C.c_double), ("t_fac", C.c_double),
-                ("ext_u1", C.c_int), ("n_pivot_sets", C.c_int)]
+                ("ext_u1", C.c_int), ("n_pivot_sets", C.c_int), ("hw", C.c_double), ("piv_round", C.c_int),
+                ("n_ef", C.c_int), ("ef_ish", C.c_void_p), ("ef_val", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -76,6 +77,8 @@ def lib():
                                 C.c_int, C.c_int, F64P, I32P, C.c_int,
                                 C.c_int, C.c_int, F64P, I32P, C.c_int,
                                 C.c_double, F64P, C.POINTER(C.c_int), F64P, C.POINTER(C.c_int), F64P]
+        L.orc_make_pivots.restype = C.c_int
+        L.orc_make_pivots.argtypes = [vp, vp, C.c_int, C.c_double, C.c_int, F64P]
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [vp, vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, C.POINTER(Stats)]
         L.orc_s6j.restype = C.c_double; L.orc_s6j.argtypes = [C.c_int] * 6
@@ -241,11 +244,24 @@ def seek_efermi(sysm: System, g: Prop, ish: int) -> float:
 MODES = {"hf": 0, "2nd": 1, "sc0": 2}
 
 
+def make_pivots(sysm: System, g: Prop, ish: int, z_cut=60.0, round_print=True) -> np.ndarray:
+    """DysPivots' pivots of subshell ish from g ([npv, no]), optionally rounded
+    to the %18.6le text the reference writes and reads back."""
+    nI = int(sysm.ms.no[ish])
+    vec = np.zeros((sysm.ms.norb + 1) * nI)
+    n = lib().orc_make_pivots(sysm.h, g.h, ish, z_cut, int(round_print), vec)
+    return vec[:n * nI].reshape(n, nI)
+
+
 def run(sysm: System, g: Prop, mode="hf", itr_max=-1, lanczos=False, lanc_kind=1, lanc_n=50,
-        conv_kind=1, conv=1e-8, t_fac=0.0, ext_u1=1, pivots=None):
-    """Run_itrs restated; returns (Prop, Stats).  pivots: dict ish -> (vec, itr_fw, itr_bk)."""
+        conv_kind=1, conv=1e-8, t_fac=0.0, ext_u1=1, pivots=None, hw=0.0, piv_round=False, ef_force=None):
+    """Run_itrs restated; returns (Prop, Stats).  pivots: dict ish -> (vec, itr_fw, itr_bk).
+    t_fac = U1body_fac and hw = htom of the kinetic term; ef_force: dict ish -> forced E_F."""
     L = lib()
-    par = Params(MODES[mode], itr_max, int(lanczos), lanc_kind, lanc_n, conv_kind, conv, t_fac, ext_u1, 0)
+    ef_ish = np.array(sorted(ef_force) if ef_force else [0], np.int32)
+    ef_val = np.array([ef_force[i] for i in sorted(ef_force)] if ef_force else [0.0], np.float64)
+    par = Params(MODES[mode], itr_max, int(lanczos), lanc_kind, lanc_n, conv_kind, conv, t_fac, ext_u1, 0,
+                 hw, int(piv_round), len(ef_force) if ef_force else 0, ef_ish.ctypes.data, ef_val.ctypes.data)
     st = Stats()
     out = Prop(L.orc_prop_create(sysm.h, 1, 1))
     args = [None] * 6

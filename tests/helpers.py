@@ -8,6 +8,16 @@ from bcdor_amd import formats, synth
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+def v_digest(vpp) -> str:
+    """SHA-256 of an interaction's keys and values (tests/golden/make_golden.py)."""
+    import hashlib
+    h = hashlib.sha256()
+    for a in (vpp.a, vpp.b, vpp.c, vpp.d, vpp.J):
+        h.update(np.ascontiguousarray(a, np.int32).tobytes())
+    h.update(np.ascontiguousarray(vpp.v, np.float64).tobytes())
+    return h.hexdigest()
+
+
 def load_golden(name):
     """Return (ModelSpace, Interaction, Propagator, npz) of a golden case; the
     interaction is regenerated from its seed and checked against the digest."""
@@ -16,12 +26,7 @@ def load_golden(name):
     if "v_v" in z.files:
         assert np.array_equal(z["v_v"], vpp.v) and np.array_equal(z["v_a"], vpp.a), \
             "regenerated interaction differs from the stored one"
-    import hashlib
-    h = hashlib.sha256()
-    for a in (vpp.a, vpp.b, vpp.c, vpp.d, vpp.J):
-        h.update(np.ascontiguousarray(a, np.int32).tobytes())
-    h.update(np.ascontiguousarray(vpp.v, np.float64).tobytes())
-    assert h.hexdigest() == str(z["v_digest"]), "interaction generator drifted from the golden digest"
+    assert v_digest(vpp) == str(z["v_digest"]), "interaction generator drifted from the golden digest"
     g = formats.Propagator(z["prop_np"], z["prop_nh"], z["prop_ep"], (z["prop_fqp"] ** 2).sum(1), z["prop_fqp"],
                            z["prop_eh"], (z["prop_fqh"] ** 2).sum(1), z["prop_fqh"])
     return ms, vpp, g, z
