@@ -43,12 +43,13 @@ class BcdSolveParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("hf_itr_max", C.c_int), ("sc_itr_max", C.c_int), ("lanczos", C.c_int),
                 ("lanc_kind", C.c_int), ("lanc_n", C.c_int), ("conv_kind", C.c_int), ("hf_conv", C.c_double),
                 ("sc_conv", C.c_double), ("t_fac", C.c_double), ("ext_u1", C.c_int), ("dyson_solver", C.c_int),
-                ("precision", C.c_int), ("accel", C.c_int)]
+                ("precision", C.c_int), ("accel", C.c_int), ("hw", C.c_double), ("piv_round", C.c_int)]
 
 
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
            "bcd_count_poles", "bcd_sigma2_build", "bcd_hf_pot", "bcd_reduce_mtx_lncz", "bcd_reduce_selfen_lncz",
-           "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_destroy", "bcd_batch_solve",
+           "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_create_ex", "bcd_batch_set_efermi",
+           "bcd_pivot_round", "bcd_batch_destroy", "bcd_batch_solve",
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
            "bcd_batch_kernel_times", "bcd_batch_work", "bcd_batch_autotune", "bcd_batch_lanczos_shapes",
            "bcd_ladd_count", "bcd_ladd_basis", "bcd_ladd_fill",
@@ -98,6 +99,10 @@ def lib():
         L.bcd_syevd_batched.argtypes = [vp, C.c_int, IP, LP, DP, LP, DP]
         L.bcd_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(BcdModspace), C.POINTER(BcdVpp),
                                        C.POINTER(BcdProp), C.POINTER(BcdSolveParams), C.POINTER(vp)]
+        L.bcd_batch_create_ex.argtypes = [C.c_int, C.c_int, C.POINTER(BcdModspace), C.POINTER(BcdVpp),
+                                          C.POINTER(BcdProp), C.POINTER(BcdSolveParams), IP, C.POINTER(vp)]
+        L.bcd_batch_set_efermi.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.bcd_pivot_round.argtypes = [C.c_int, DP, DP, C.c_long]
         L.bcd_batch_destroy.argtypes = [vp]
         L.bcd_batch_destroy.restype = None
         L.bcd_batch_solve.argtypes = [vp]
@@ -372,11 +377,13 @@ class Context(_TwoBodyMixin):
         Min = _d(Min)
         piv = _d(pivots).copy().reshape(-1)
         itr = _i(n_itr).copy()
-        out = np.zeros(NPLS_out * (NA_out + NB_out))
+        out = np.zeros(max(NPLS_out, 1) * (NA_out + NB_out))
         ipv = np.zeros(len(itr) + 2, np.int32)
         r = lib().bcd_reduce_mtx_lncz(self.h, NA_out, NB_out, NPLS_out, dp(out), ip(ipv), NA_in, NB_in,
-                                      Min.shape[0], dp(Min.reshape(-1)), len(itr), dp(piv), ip(itr))
-        return r, out.reshape(NPLS_out, NA_out + NB_out), ipv, piv.reshape(len(itr), -1)
+                                      Min.shape[0], dp(Min.reshape(-1) if Min.size else np.zeros(1)), len(itr),
+                                      dp(piv), ip(itr))
+        return r, out[:max(NPLS_out, 0) * (NA_out + NB_out)].reshape(-1, NA_out + NB_out), ipv, \
+            piv.reshape(len(itr), -1)
 
     def reduce_selfen_lncz(self, n_orb, rows_fw, rows_bk, pivots, itr_fw, itr_bk):
         fw = _d(rows_fw).copy(); bk = _d(rows_bk).copy()
@@ -434,18 +441,29 @@ LANC_BUCKETS = len(LANC_BUCKET_MAX)
 
 def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1,
                  hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1, dyson_solver=0, precision=0,
-                 accel=0) -> BcdSolveParams:
+                 accel=0, hw=0.0, piv_round=0) -> BcdSolveParams:
     """dyson_solver 0: arrowhead updates (default); 1: dense Householder + QL per sweep.
     precision 0: FP64 throughout; 1: mixed (FP32 Sigma(2) coupling contraction).
-    accel 0: the reference's iteration; m = 1..8: Anderson mixing of the density over m iterates."""
+    accel 0: the reference's iteration; m = 1..8: Anderson mixing of the density over m iterates.
+    t_fac = U1body_fac and hw = htom of the kinetic term (hw * T_HO) * t_fac.
+    piv_round 1: pivots rounded as the pivots file carries them (%18.6le)."""
     return BcdSolveParams(mode, hf_itr_max, sc_itr_max, int(lanczos), lanc_kind, lanc_n, conv_kind, hf_conv,
-                          sc_conv, t_fac, ext_u1, dyson_solver, precision, accel)
+                          sc_conv, t_fac, ext_u1, dyson_solver, precision, accel, hw, int(piv_round))
+
+
+def pivot_round(x, device: int = 0) -> np.ndarray:
+    """The pivots-file rounding (%18.6le read back) computed on the GPU."""
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    out = np.zeros_like(x)
+    check(lib().bcd_pivot_round(device, dp(x), dp(out), x.size), "bcd_pivot_round")
+    return out
 
 
 class Batch:
     """A device-resident batch of independent instances (bcd_batch)."""
 
-    def __init__(self, systems, params: BcdSolveParams, g0=None, device: int = 0):
+    def __init__(self, systems, params: BcdSolveParams, g0=None, device: int = 0, lanc_n=None):
+        """lanc_n: optional Lanczos length per instance (bcd_batch_create_ex)."""
         L = lib()
         n = len(systems)
         keep = _Keep()
@@ -455,8 +473,10 @@ class Batch:
         gpa = None
         if g0 is not None:
             gpa = (BcdProp * n)(*[prop_struct(g, ms.nspb, keep) for g, (ms, _) in zip(g0, systems)])
+        ln = None if lanc_n is None else ip(keep.i(np.asarray(lanc_n, np.int32)))
         h = C.c_void_p()
-        check(L.bcd_batch_create(device, n, msa, vpa, gpa, C.byref(params), C.byref(h)), "bcd_batch_create")
+        check(L.bcd_batch_create_ex(device, n, msa, vpa, gpa, C.byref(params), ln, C.byref(h)),
+              "bcd_batch_create_ex")
         self.h = h
         self.n = n
 
@@ -470,6 +490,10 @@ class Batch:
 
     def set_profile(self, on: bool):
         check(lib().bcd_batch_set_profile(self.h, int(on)), "bcd_batch_set_profile")
+
+    def set_efermi(self, inst: int, ish: int, ef: float):
+        """Force the Fermi energy of one subshell (Set_external_EFermi)."""
+        check(lib().bcd_batch_set_efermi(self.h, inst, ish, ef), "bcd_batch_set_efermi")
 
     def solve(self):
         check(lib().bcd_batch_solve(self.h), "bcd_batch_solve")

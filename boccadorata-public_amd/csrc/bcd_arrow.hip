@@ -170,7 +170,7 @@ __device__ ArrowJob arrow_job(const ArrowSweep &a, int gc) {
     if (a.P.is && threadIdx.x == 0) a.P.is[C.inst].err = 2;
     return J;
   }
-  J.efermi = S ? S->ef[C.ch & 1] : 0.0;
+  J.efermi = chan_efermi(C, S);
   J.ws = a.aws ? a.aws + C.aws_off : nullptr;
   J.out_e = a.P.e[nxt] + a.P.e_off[gc];
   J.out_x = a.P.x[nxt] + a.P.x_off[gc];
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(64) void k_dyson_static(ArrowSweep a) {
   const InstState *S = a.P.is ? a.P.is + C.inst : nullptr;
   if (S && !S->active) return;
   const int nxt = 1 - (S ? S->cur : 0);
-  const double ef = S ? S->ef[C.ch & 1] : 0.0;
+  const double ef = chan_efermi(C, S);
   if (static_dyson<kStaticNI>(C.nI, C.j2, a.h + C.h_off, a.u1 + C.u1_off, ef, a.P.e[nxt] + a.P.e_off[gc],
                               a.P.x[nxt] + a.P.x_off[gc], a.P.nqp[nxt] + gc, a.P.nqh[nxt] + gc, a.kolt + 3L * gc))
     atomicAdd(a.status, 1);

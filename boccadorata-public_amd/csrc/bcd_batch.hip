@@ -136,7 +136,11 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
       nImax = std::max(nImax, nI);
       C.inst = i; C.ish = s; C.nI = nI; C.j2 = S.j2[s]; C.ch = S.ch[s]; C.orb0 = S.orb0[s];
       // Lanczos vectors per side and the resulting Dyson dimension bound
-      long lside = par.lanc_kind == 0 ? (long)par.lanc_n * std::min(nI, kMaxPiv) : par.lanc_n;
+      const int ln = inst_lanc_n.empty() || inst_lanc_n[i] <= 0 ? par.lanc_n : inst_lanc_n[i];
+      C.lanc_n = ln;
+      C.ef_set = 0;
+      C.ef_force = 0.0;
+      long lside = par.lanc_kind == 0 ? (long)ln * std::min(nI, kMaxPiv) : ln;
       long lfw = std::min<long>(lside, S.mbound_fw[s]), lbk = std::min<long>(lside, S.mbound_bk[s]);
       long cp = nI;
       if (dyn && !sigma_only) cp += par.lanczos ? lfw + lbk : S.mbound_fw[s] + S.mbound_bk[s];
@@ -185,7 +189,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
 
   // ---- one-body parts
   std::vector<double> u1(std::max<long>(o_u1, 1), 0.0);
-  for (const ChanDev &C : h_chans) sys[C.inst]->u1(C.ish, par.t_fac, par.ext_u1, u1.data() + C.u1_off);
+  for (const ChanDev &C : h_chans) sys[C.inst]->u1(C.ish, par.hw, par.t_fac, par.ext_u1, u1.data() + C.u1_off);
 
   // ---- device buffers (fixed size)
   BTRY(a_inst.grow(pad8(sizeof(InstDev) * n_inst) + pad8(sizeof(InstState) * n_inst) + 64));
@@ -315,6 +319,19 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   if (arrow_set_lds_limit(kLdsBytes)) return BCD_ERR_DEVICE;
   h_is.assign(n_inst, InstState());
+  return BCD_OK;
+}
+
+// Set_external_EFermi (DysonMatrix.cpp:601-617): the first value given for
+// a subshell is the one Seek_EFermi returns for it from then on.
+int Batch::set_efermi(int inst, int ish, double ef) {
+  if (inst < 0 || inst >= n_inst || ish < 0 || ish >= h_insts[inst].nsubsh) return BCD_ERR_ARG;
+  ChanDev &C = h_chans[h_insts[inst].gc0 + ish];
+  if (C.ef_set) return BCD_OK;
+  C.ef_set = 1;
+  C.ef_force = ef;
+  HTRY(hipSetDevice(device));
+  HTRY(hipMemcpy(d_chans + h_insts[inst].gc0 + ish, &C, sizeof(ChanDev), hipMemcpyHostToDevice));
   return BCD_OK;
 }
 
@@ -608,7 +625,7 @@ int Batch::lanczos() {
   kt_end(KT_MERGE);
   HTRY(hipGetLastError());
   hipLaunchKernelGGL(k_lanczos_prep, dim3((n_gc + 63) / 64), dim3(64), 0, stream, d_chans, 0, n_gc, P, d_cs, d_piv,
-                     par.lanc_kind, par.lanc_n, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_crows_fw,
+                     par.lanc_kind, par.piv_round, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_crows_fw,
                      d_crows_bk, d_lrows, (double *)a_lws.ptr, d_ljobs);
   HTRY(hipGetLastError());
   BTRY(pull_counts());

@@ -49,6 +49,8 @@ struct Batch {
   hipEvent_t lfork = nullptr, ljoin[kLancAux] = {};
   bcd_solve_params par{};
   std::vector<const HostSystem *> sys;
+  std::vector<int> inst_lanc_n;      // Lanczos length per instance (empty: par.lanc_n for all); set before setup
+  int set_efermi(int inst, int ish, double ef);
   int n_inst = 0, n_gc = 0, nImax = 1;
   long n_blk = 0, capmax = 0, mat_len = 0;
   std::vector<InstDev> h_insts;

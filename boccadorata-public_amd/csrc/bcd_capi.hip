@@ -297,22 +297,32 @@ int bcd_reduce_selfen_lncz(bcd_ctx *ctx, int n_orb, int nfw, double *rows_fw, in
   for (int k = 0; k < dim; ++k) { i_piv_fw[k] = -100; i_piv_bk[k] = -100; }
   *ntot_den = nden;
   *nfw_out = *nbk_out = 0;
-  // sort each side on the device (fw ascending, bk descending, as the
-  // reference does before the reduction, Self_energy_ext_Op_e.cpp:119-130)
-  DevBuf dfw_in, dfw, dbk_in, dbk;
+  // sort each side on the device (fw ascending, bk descending) with the
+  // reference's heapsort, so rows of equal energy keep the reference's order
+  // (Self_energy_ext_Op_e.cpp:119-130, Utilities.cpp:25-94)
+  DevBuf dfw_in, dfw, dbk_in, dbk, dkey, didx;
   BTRY(dfw_in.alloc(sizeof(double) * (size_t)std::max(nfw, 1) * T));
   BTRY(dfw.alloc(sizeof(double) * (size_t)std::max(nfw, 1) * T));
   BTRY(dbk_in.alloc(sizeof(double) * (size_t)std::max(nbk, 1) * T));
   BTRY(dbk.alloc(sizeof(double) * (size_t)std::max(nbk, 1) * T));
+  const int mx = std::max(nfw, nbk);
+  if (mx > kHeapLdsRows) {
+    BTRY(dkey.alloc(sizeof(double) * (size_t)mx));
+    BTRY(didx.alloc(sizeof(int) * (size_t)mx));
+  }
+  auto heap = [&](const DevBuf &in, int M, int desc, const DevBuf &out) {
+    const bool lds = M <= kHeapLdsRows;
+    hipLaunchKernelGGL(k_heap_rows, dim3(1), dim3(256), lds ? (size_t)M * 12 + 16 : 0, ctx->stream,
+                       (const double *)in.p, M, T, 0, desc, (double *)dkey.p, (int *)didx.p, lds ? 1 : 0,
+                       (double *)out.p);
+  };
   if (nfw) {
     HTRY(hipMemcpy(dfw_in.p, rows_fw, sizeof(double) * (size_t)nfw * T, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_rank_sort, dim3((nfw + 255) / 256), dim3(256), 0, ctx->stream, (const double *)dfw_in.p,
-                       nfw, T, 0, 0, (double *)dfw.p);
+    heap(dfw_in, nfw, 0, dfw);
   }
   if (nbk) {
     HTRY(hipMemcpy(dbk_in.p, rows_bk, sizeof(double) * (size_t)nbk * T, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_rank_sort, dim3((nbk + 255) / 256), dim3(256), 0, ctx->stream, (const double *)dbk_in.p,
-                       nbk, T, 0, 1, (double *)dbk.p);
+    heap(dbk_in, nbk, 1, dbk);
   }
   HTRY(hipGetLastError());
   HTRY(hipStreamSynchronize(ctx->stream));
@@ -529,14 +539,21 @@ int bcd_syevd_batched(bcd_ctx *ctx, int nmat, const int *n, const long *off, dou
 
 int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_vpp *vpp, const bcd_prop *g0,
                      const bcd_solve_params *par, bcd_batch **out) {
+  return bcd_batch_create_ex(device, n_inst, ms, vpp, g0, par, nullptr, out);
+}
+
+int bcd_batch_create_ex(int device, int n_inst, const bcd_modspace *ms, const bcd_vpp *vpp, const bcd_prop *g0,
+                        const bcd_solve_params *par, const int *lanc_n, bcd_batch **out) {
   if (!out || n_inst < 1 || !ms || !vpp || !par) return BCD_ERR_ARG;
   *out = nullptr;
   if (par->mode < 0 || par->mode > 2 || par->dyson_solver < 0 || par->dyson_solver > 1 || par->precision < 0 ||
-      par->precision > 1 || par->accel < 0 || par->accel > kMixMax)
+      par->precision > 1 || par->accel < 0 || par->accel > kMixMax || par->hw < 0.0 ||
+      (par->t_fac != 0.0 && par->hw <= 0.0))
     return BCD_ERR_ARG;
   BTRY(device_ok(device));
   std::unique_ptr<bcd_batch> B(new (std::nothrow) bcd_batch);
   if (!B) return BCD_ERR_NOMEM;
+  if (lanc_n) B->b.inst_lanc_n.assign(lanc_n, lanc_n + n_inst);
   std::vector<const HostSystem *> sp;
   std::vector<HostProp> gp;
   for (int i = 0; i < n_inst; ++i) {
@@ -556,6 +573,26 @@ int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_v
 }
 
 void bcd_batch_destroy(bcd_batch *b) { delete b; }
+
+int bcd_batch_set_efermi(bcd_batch *b, int inst, int ish, double ef) {
+  if (!b) return BCD_ERR_ARG;
+  return b->b.set_efermi(inst, ish, ef);
+}
+
+int bcd_pivot_round(int device, const double *x, double *out, long n) {
+  if (!x || !out || n < 0) return BCD_ERR_ARG;
+  BTRY(device_ok(device));
+  if (n == 0) return BCD_OK;
+  HTRY(hipSetDevice(device));
+  DevBuf d;
+  BTRY(d.alloc(sizeof(double) * (size_t)n));
+  HTRY(hipMemcpy(d.p, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_pivot_round, dim3((unsigned)std::min<long>((n + 255) / 256, 4096)), dim3(256), 0, nullptr,
+                     (const double *)d.p, (double *)d.p, n);
+  HTRY(hipGetLastError());
+  HTRY(hipMemcpy(out, d.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  return BCD_OK;
+}
 
 int bcd_batch_solve(bcd_batch *b) {
   if (!b) return BCD_ERR_ARG;

@@ -68,6 +68,9 @@ struct ChanDev {
   long aws_off;                  // arrowhead workspace (doubles) for the global-memory path
   long warm_off;                 // previous sweep's arrowhead roots, nI x cap doubles
   int trip_off, ntrip;           // (a, b, g) triples of the Sigma(2) blocks: starts in the triple table
+  int lanc_n;                    // Lanczos length of the channel's instance (read with lanc_kind)
+  int ef_set;                    // 1: the Fermi energy of this subshell is forced to ef_force
+  double ef_force;               // (Set_external_EFermi, DysonMatrix.cpp:601-617)
 };
 
 // One (Jab, a <= b, g) block of the Sigma(2) build of channel gc; blocks of a
@@ -103,6 +106,8 @@ struct ChanState {
   double kolt[3];                // Calc_En partial sums of the last sweep
 };
 
+struct InstState;
+
 // Per-instance dynamic state.
 struct InstState {
   int cur;                       // propagator buffer holding the current iterate
@@ -120,6 +125,13 @@ struct InstState {
   double ef[2];                  // Fermi energy per charge for the next sweep
   double e_kolt, u1_tot, a_tot;
 };
+
+// Fermi energy a Dyson sweep of channel C splits at: the forced value, or
+// the midpoint k_finalize took from the sweep's input propagator
+// (Seek_EFermi, DysonMatrix.cpp:619-635).
+__device__ inline double chan_efermi(const ChanDev &C, const InstState *S) {
+  return C.ef_set ? C.ef_force : (S ? S->ef[C.ch & 1] : 0.0);
+}
 
 }  // namespace bcd
 

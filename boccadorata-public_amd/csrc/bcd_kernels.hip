@@ -8,6 +8,7 @@
 //   k_finalize                        sweep control Main.cpp:403-451, Sp_Propagator.cpp:1189-1213,
 //                                                   DysonMatrix.cpp:619-635
 #include "bcd_kernels.hpp"
+#include "bcd_order.hpp"
 
 namespace bcd {
 
@@ -643,8 +644,10 @@ BCD_SIGMA2_INST(float, double)
 
 // Pivots and iteration counts of every channel from the current (HF)
 // propagator: quasiholes then quasiparticles by ascending energy with Z >= 60%
-// (SpProp_t::DysPivots, Sp_Propagator.cpp:1225-1289), lanc_n iterations per
-// pivot (kind 0) or lanc_n vectors per side split over the pivots (kind 1),
+// (SpProp_t::DysPivots, Sp_Propagator.cpp:1225-1289), rounded as the pivots
+// file carries them when piv_round is set, the instance's lanc_n (ChanDev)
+// iterations per pivot (kind 0) or lanc_n vectors per side split over the
+// pivots (kind 1),
 // rescaled as Reduce_SelfEn_Lncz does when they exceed the pole count
 // (Self_energy_ext_Op_e.cpp:147-220).  Writes one LancJob per side.
 // Poles of the same fragments (na, nb, ng) of one (a, b, g) triple in the
@@ -821,7 +824,7 @@ __global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n
 }
 
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P,
-                               ChanState *cs, double *piv, int lanc_kind, int lanc_n,
+                               ChanState *cs, double *piv, int lanc_kind, int piv_round,
                                const long *row_fw_base, const long *row_bk_base, double *rows_fw,
                                double *rows_bk, double *crows_fw, double *crows_bk, double *lrows, double *lws,
                                LancJob *jobs) {
@@ -829,7 +832,7 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
   if (gc >= gc_end) return;
   const ChanDev C = chans[gc];
   ChanState &S = cs[gc];
-  const int cur = prop_cur(P, C.inst), nI = C.nI;
+  const int cur = prop_cur(P, C.inst), nI = C.nI, lanc_n = C.lanc_n;
   const int nqp = P.nqp[cur][gc], nqh = P.nqh[cur][gc];
   const double *e = P.e[cur] + P.e_off[gc];
   const double *x = P.x[cur] + P.x_off[gc];
@@ -850,7 +853,10 @@ __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, P
       }
       if (pick >= 0) {
         x2 = x1;
-        for (int k = 0; k < nI; ++k) pv[npv * nI + k] = x[(long)(off + pick) * nI + k];
+        for (int k = 0; k < nI; ++k) {
+          const double xk = x[(long)(off + pick) * nI + k];
+          pv[npv * nI + k] = piv_round ? dys_round(xk) : xk;
+        }
         ++npv;
       }
     }
@@ -1234,7 +1240,7 @@ __global__ void k_dyson_prep(const ChanDev *chans, int gc_begin, int gc_end, con
   }
   J.n = nI + J.nfw + J.nbk;
   if (J.n > C.cap) { J.n = 0; if (P.is) P.is[C.inst].err = 2; return; }
-  J.efermi = S ? S->ef[C.ch & 1] : 0.0;
+  J.efermi = chan_efermi(C, S);
   J.tri = tri + C.tri_off;
   J.mat = mat ? mat + C.mat_off : nullptr;
   J.out_e = P.e[nxt] + P.e_off[gc];
@@ -1750,18 +1756,33 @@ __global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, const int
   }
 }
 
-// Stable rank sort of pole rows by column col (ascending, or descending when
-// desc): row i goes to position #{j : key_j before key_i}, ties by index.
-__global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  const double ki = rows[(long)i * T + col];
-  int rank = 0;
-  for (int j = 0; j < M; ++j) {
-    const double kj = rows[(long)j * T + col];
-    rank += desc ? (kj > ki || (kj == ki && j < i)) : (kj < ki || (kj == ki && j < i));
+// The reference's heapsort of pole rows by column col (Sort_u_double2dim /
+// Sort_d_double2dim, Utilities.cpp:25-94), one block: thread 0 sorts (key,
+// row index) pairs with the reference's sift order (heap_perm) -- in LDS when
+// lds_keys is set, else in the global scratch keys/idx -- then every thread
+// gathers rows into out.  Equal keys end up where the reference puts them.
+__global__ __launch_bounds__(256) void k_heap_rows(const double *rows, int M, int T, int col, int desc,
+                                                   double *keys, int *idx, int lds_keys, double *out) {
+  extern __shared__ double hs[];
+  double *K = lds_keys ? hs : keys;
+  int *X = lds_keys ? (int *)(hs + M) : idx;
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    K[i] = rows[(long)i * T + col];
+    X[i] = i;
   }
-  for (int c = 0; c < T; ++c) out[(long)rank * T + c] = rows[(long)i * T + c];
+  __syncthreads();
+  if (threadIdx.x == 0) heap_perm(K, X, M, desc);
+  __syncthreads();
+  for (long e = threadIdx.x; e < (long)M * T; e += blockDim.x) {
+    const long r = e / T, c = e % T;
+    out[e] = rows[(long)X[r] * T + c];
+  }
+}
+
+// x as the pivots file carries it (dys_round), element-wise.
+__global__ void k_pivot_round(const double *x, double *out, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = dys_round(x[i]);
 }
 
 // ------------------------------------------------------------- sweep control

@@ -58,7 +58,7 @@ __global__ void k_merge_poles(const ChanDev *chans, int n_gc, ChanState *cs, con
                               const long *row_bk_base, const double *rows_fw, const double *rows_bk,
                               double *crows_fw, double *crows_bk);
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P, ChanState *cs,
-                               double *piv, int lanc_kind, int lanc_n, const long *row_fw_base,
+                               double *piv, int lanc_kind, int piv_round, const long *row_fw_base,
                                const long *row_bk_base, double *rows_fw, double *rows_bk, double *crows_fw,
                                double *crows_bk, double *lrows, double *lws, LancJob *jobs);
 constexpr int kLancPer = 24;      // largest register tile (elements per thread) of k_lanczos
@@ -136,7 +136,11 @@ void launch_arrow_sweep(const ArrowSweep &a, int nimax, bool in_lds, int nt, siz
 void launch_arrow(const ArrowJob *jobs, int njobs, int *status, int nimax, bool in_lds, int nt, size_t lds,
                   hipStream_t st);
 int arrow_set_lds_limit(int bytes);
-__global__ void k_rank_sort(const double *rows, int M, int T, int col, int desc, double *out);
+__global__ void k_heap_rows(const double *rows, int M, int T, int col, int desc, double *keys, int *idx,
+                            int lds_keys, double *out);
+// rows whose heapsort keys and indices fit the dynamic LDS of k_heap_rows
+constexpr int kHeapLdsRows = (64 * 1024) / 12;
+__global__ void k_pivot_round(const double *x, double *out, long n);
 __global__ void k_finalize(const InstDev *insts, const ChanDev *chans, PropView P, const ChanState *cs,
                            const double *kolt, int conv_kind, int init);
 

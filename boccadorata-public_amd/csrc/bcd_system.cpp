@@ -156,7 +156,11 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
   return BCD_OK;
 }
 
-void HostSystem::u1(int ish, double t_fac, int ext_u1, double *out) const {
+// Sig_1b of Run_itrs: Get_Tab_kin_ho fills htom * fab_tkin (Utilities.cpp:
+// 106-143), the table is scaled by U1body_fac and added to zero (Main.cpp:
+// 266-270); with ExtU1=1 the model-space energies are added on the diagonal
+// (Main.cpp:279-285).  Trel=7 (t_fac == 0) adds no kinetic table at all.
+void HostSystem::u1(int ish, double hw, double t_fac, int ext_u1, double *out) const {
   const int n = no[ish], ll = l[ish];
   for (int r = 0; r < n; ++r)
     for (int c = 0; c < n; ++c) {
@@ -164,7 +168,8 @@ void HostSystem::u1(int ish, double t_fac, int ext_u1, double *out) const {
       double t = 0.0;
       if (na == nb) t = ((double)(na + nb + ll) + 1.5) / 2.0;
       if (std::abs(na - nb) == 1) t = std::sqrt((double)(na + nb + 1) * (double)(na + nb + ll + ll + 2) / 4.0) / 2.0;
-      double v = t_fac * t;
+      double v = 0.0;
+      if (t_fac != 0.0) v += (hw * t) * t_fac;
       if (ext_u1 == 1 && r == c) v += orb_e[orb0[ish] + r];
       out[r * n + c] = v;
     }

@@ -29,8 +29,9 @@ struct HostSystem {
   long dropped = 0;                            // interaction keys outside the pair tables
 
   int build(const bcd_modspace *ms, const bcd_vpp *vpp);
-  // one-body part U1 of subshell ish: t_fac * T_HO + ext_u1 * diag(spe)
-  void u1(int ish, double t_fac, int ext_u1, double *out) const;
+  // one-body part U1 of subshell ish: (hw * T_HO) * t_fac, then + diag(spe)
+  // when ext_u1 == 1, in the order the reference adds them (Main.cpp:266-285)
+  void u1(int ish, double hw, double t_fac, int ext_u1, double *out) const;
 };
 
 }  // namespace bcd

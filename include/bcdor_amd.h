@@ -140,7 +140,7 @@ typedef struct bcd_solve_params {
   int conv_kind;     /* 0: sum_i |de_i| (quick_n_dirty_diff); 1: max_i |de_i|         */
   double hf_conv;    /* HF stops when the measure drops below this                   */
   double sc_conv;    /* sc0 stops when the measure drops below this                  */
-  double t_fac;      /* U1body_fac * hbar*omega multiplying T_HO (0: Trel=7)        */
+  double t_fac;      /* U1body_fac (Initialize.cpp:65-74; 0: no kinetic term, Trel=7) */
   int ext_u1;        /* 1: add diag(model-space spe) to U1 (ExtU1=1)                */
   int dyson_solver;  /* 0: arrowhead updates over the diagonalised pole blocks
                         (each reduced side diagonalised once per solve);
@@ -158,12 +158,40 @@ typedef struct bcd_solve_params {
                         iterates.  Same fixed point and stopping rule, fewer
                         sweeps; results differ from the plain iteration
                         within what the convergence tolerance leaves open. */
+  double hw;         /* htom (hbar*omega) of the kinetic term: U1 = (hw * T_HO) *
+                        t_fac, the order of Get_Tab_kin_ho(...) *= U1body_fac
+                        (Utilities.cpp:106-124, Main.cpp:266-270)              */
+  int piv_round;     /* 1: the Lanczos pivots derived from the HF propagator are
+                        rounded as the pivots file carries them (DysPivots
+                        prints %18.6le, Make_LancPivots reads it back;
+                        Sp_Propagator.cpp:1252, Miscellanea.cpp:366-370), so
+                        the chain starts where the reference's MakePivots ->
+                        LanDysPiv workflow starts; 0: full precision         */
 } bcd_solve_params;
 
 /* g0 may be NULL (bare propagator, SpProp_t(ModSpace_t*, 1)); otherwise g0[i]
  * is the starting propagator of instance i. */
 int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_vpp *vpp,
                      const bcd_prop *g0, const bcd_solve_params *par, bcd_batch **out);
+/* As bcd_batch_create, with a Lanczos length per instance: lanc_n[i] (> 0)
+ * replaces par->lanc_n for instance i, read with par->lanc_kind (iterations
+ * per pivot, or Krylov vectors per side split over the pivots -- a pivots
+ * file's per-pivot n_itr columns, Miscellanea.cpp:348-380).  lanc_n may be
+ * NULL. */
+int bcd_batch_create_ex(int device, int n_inst, const bcd_modspace *ms, const bcd_vpp *vpp,
+                        const bcd_prop *g0, const bcd_solve_params *par, const int *lanc_n,
+                        bcd_batch **out);
+/* Set_external_EFermi (DysonMatrix.cpp:601-617; the SetEf= flag,
+ * Command_line.cpp:347-350): every later Dyson sweep of subshell ish of
+ * instance inst splits quasiparticles from quasiholes at ef instead of the
+ * midpoint Seek_EFermi takes from the input propagator (DysonMatrix.cpp:
+ * 619-635).  The first value set for a subshell stays, as in the reference. */
+int bcd_batch_set_efermi(bcd_batch *b, int inst, int ish, double ef);
+/* The rounding of a pivots file round trip (DysPivots' "%18.6le" read back
+ * with %lf) on device: out[i] = x[i] with 7 significant decimal digits, as
+ * the batched solve applies it when piv_round = 1.  Exact for |x| >= 1e-16;
+ * below, within an ulp or two. */
+int bcd_pivot_round(int device, const double *x, double *out, long n);
 void bcd_batch_destroy(bcd_batch *b);
 /* Run the full solve of every instance; inputs stay resident, so repeated
  * calls redo the same work. */
