@@ -1,21 +1,27 @@
 """Benchmark: converged SCGF solves per second on one MI355X.
 
-Workload: synthetic closed-shell systems, Nmax = 7 (8 shells, 36 orbits per
-species, 30 channels), hw = 20 MeV, sigma = 2, spe noise 0.5, one seed per
-instance.  One solve = HF from the bare propagator, Sigma(2) from the HF
-propagator, Lanczos reduction to 50 Krylov vectors per side with full FP64
-reorthogonalisation, then sc0 sweeps of Sigma^inf + Dyson until max|de| < 1e-8
-(HF to the same criterion).  A step is one bcd_batch_solve over a batch of
---batch independent instances already resident in HBM.
+Default workload (--workload batched): 256 independent synthetic closed-shell
+systems in one batch, hw uniform in [10, 30] MeV, interaction scale sigma
+uniform in [1, 3], Nmax uniform over {5, 7, 9, 11} (22 to 46 channels, up to
+~0.6 GB of interaction per instance) and a Lanczos length uniform in [30, 80]
+Krylov vectors per side, one seed per instance (bcdor_amd.synth.batched_specs).
+One solve = HF from the bare propagator, Sigma(2) from the HF propagator,
+Lanczos reduction with full FP64 reorthogonalisation, then sc0 sweeps of
+Sigma^inf + Dyson until max|de| < 1e-8 (HF to the same criterion).  A step is
+one bcd_batch_solve over the whole batch, already resident in HBM.
 
 The metric is batched (and approximate where that stays within 1e-6) against
-plain FP64 on the GPU.  Three batched modes are timed on the same instances:
-FP64 with the reference's plain iteration; FP64 with Anderson mixing of the
-density (same fixed point, fewer sweeps); and the latter with FP32 operands in
-the Sigma(2) coupling contraction.  value = converged solves/s of the fastest
-mode, counting only solves within 1e-6 of the plain FP64 solve of the same
-instance.  Beside it: every mode's rate, error and sweeps, and the plain FP64
-rate of one instance per solve.
+plain FP64 on the GPU.  The batched modes are timed on the same instances
+(bcd_batch_set_params switches between them): FP64 with the reference's
+plain iteration; FP64 with Anderson mixing of the density (same fixed point,
+fewer sweeps); and mixed precision in the Sigma(2) coupling contraction.
+value = converged solves/s of the fastest mode, counting only solves within
+1e-6 of the plain FP64 solve of the same instance.  Beside it: every mode's
+rate, error and sweeps, and the plain FP64 rate of one instance per solve.
+
+--workload nmax7 keeps the single-model-space workload (Nmax 7, hw 20,
+sigma 2, 50 vectors per side); --workload twobody the ladder and ring
+matrices.
 """
 import argparse
 import json
@@ -124,10 +130,10 @@ def spectral_error(g, ref, zsig=ZSIG):
 # HBM bytes per launch of the dominant kernel family, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes of this workload (calibrated against a 1 GiB
 # copy): (kernel family, instances per step) -> (bytes, summary file)
-MEASURED_TRAFFIC = {("lanczos", 256): (225.7e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md")}
+MEASURED_TRAFFIC = {("nmax7", "lanczos", 256): (225.7e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md")}
 
 
-def roofline(batch):
+def roofline(batch, workload):
     """Per-kernel average launch time (hipEvents around every launch of one
     profiled solve) and the dominant kernel's achieved rate."""
     batch.set_profile(True)
@@ -172,7 +178,7 @@ def roofline(batch):
         entry["kernel"] = dom
         entry["avg_launch_ms"] = avg_ms
         entry["launches"] = launches
-        meas = MEASURED_TRAFFIC.get((dom, batch.n))
+        meas = MEASURED_TRAFFIC.get((workload, dom, batch.n))
         if meas:
             entry["traffic"] = meas[0]
             entry["traffic_note"] = f"HBM bytes per launch, rocprofv3 FETCH_SIZE/WRITE_SIZE, {meas[1]}"
@@ -367,26 +373,8 @@ def main_twobody(args):
     print(json.dumps(line), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("scgf", "twobody"), default="scgf",
-                    help="scgf: the converged-solve benchmark (default); twobody: ladder + ring matrices")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; shard instances across GPUs "
-                                   "(one process per GPU) to scale"}))
-        sys.exit(2)
-    if capi.lib().bcd_device_count() < 1:
-        raise SystemExit("no GPU visible: the HIP path has no CPU fallback")
-    if args.workload == "twobody":
-        return main_twobody(args)
-
+def main_nmax7(args):
+    """The single-model-space workload (Nmax 7, hw 20, sigma 2, 50 vectors per side)."""
     t0 = time.perf_counter()
     seeds, systems, dropped = screened_seeds(args.batch)
     n = args.batch
@@ -430,7 +418,7 @@ def main():
               f"mean sweeps {r['sweeps']:.1f}", file=sys.stderr, flush=True)
     head = max(runs, key=lambda k: runs[k]["rate"])
     batch = batches[head]
-    rl, kernels, work = roofline(batch)
+    rl, kernels, work = roofline(batch, "nmax7")
     print(f"# kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
     print(f"# arrowhead phases (block-ms) {json.dumps(work['arrow_phase_ms'])}", file=sys.stderr, flush=True)
     its = [batch.stats(i) for i in range(n)]
@@ -481,6 +469,220 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+# ----------------------------------------------------- heterogeneous batch
+BATCH_SEED = 4                # batched_specs(n, BATCH_SEED)
+ZSIG_BY_NMAX = {5: 1e-5, 7: 1e-6, 9: 1e-5, 11: 1e-5}   # rounding-determined Z below these (Nmax 9, 11: 1e-5)
+SINGLE_PER_NMAX = 2           # instances per Nmax solved one per batch for the plain single-instance rate
+
+
+def build_batched(specs, par, threads):
+    return capi.Batch.streamed(len(specs), lambda i: specs[i].make(), par, lanc_n=[s.lanc_n for s in specs],
+                               threads=threads)
+
+
+def screened_batch(n, threads):
+    """The batch of the first n instance specs, with every instance whose solve
+    does not converge within the caps given a new interaction seed (its hw,
+    sigma, Nmax and Lanczos length are kept) and the batch rebuilt, at most
+    twice: random interactions of these scales can blur the shell gap so
+    that HF oscillates at the Fermi surface (the reference algorithm does the
+    same on them).  Screening is one untimed solve per build."""
+    import dataclasses
+    specs = synth.batched_specs(n, BATCH_SEED)
+    reseeded = 0
+    for attempt in range(3):
+        t0 = time.perf_counter()
+        b = build_batched(specs, params(), threads)
+        t1 = time.perf_counter()
+        b.solve()
+        bad = [i for i in range(n) if not b.converged(i)]
+        print(f"# build {attempt}: {t1 - t0:.1f} s to generate and upload, screening solve "
+              f"{time.perf_counter() - t1:.1f} s, {len(bad)} not converged", file=sys.stderr, flush=True)
+        if not bad or attempt == 2:
+            return specs, b, reseeded
+        for i in bad:
+            specs[i] = dataclasses.replace(specs[i], seed=specs[i].seed + 1000003 * (attempt + 1))
+        reseeded += len(bad)
+        b.close()
+
+
+def cpu_baseline_batched(specs, seconds, threads=None):
+    """The oracle (FP64 C restatement of the reference algorithm) on the
+    batched workload's instances, one per host thread, for a bounded time:
+    threads are split evenly over the Nmax values and each works through that
+    Nmax's instances in order.  The mean time per solve of each Nmax gives
+    the throughput of the whole mix (uniform over Nmax) on these threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = threads or min(16, os.cpu_count() or 1)
+    by_nmax = {}
+    for s in specs:
+        by_nmax.setdefault(s.nmax, []).append(s)
+    keys = sorted(by_nmax)
+    times = {k: [] for k in keys}
+    lock = threading.Lock()
+    t_end = time.perf_counter() + seconds
+
+    def worker(k):
+        nm = keys[k % len(keys)]
+        lst = by_nmax[nm]
+        j = k // len(keys)
+        while time.perf_counter() < t_end and j < len(lst):
+            sp = lst[j]
+            ms, vpp = sp.make()
+            S = po.System(ms, vpp)
+            t0 = time.perf_counter()
+            ghf, st_hf = po.run(S, po.Prop.bare(S), "hf", itr_max=HF_CAP, conv_kind=1, conv=CONV)
+            g, st = po.run(S, ghf, "sc0", itr_max=SC_CAP, lanczos=True, lanc_kind=1, lanc_n=sp.lanc_n,
+                           conv_kind=1, conv=CONV)
+            dt = time.perf_counter() - t0
+            with lock:
+                times[nm].append(dt)
+            j += max(1, threads // len(keys))
+
+    t0 = time.perf_counter()
+    pool = [threading.Thread(target=worker, args=(k,)) for k in range(threads)]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    wall = time.perf_counter() - t0
+    if any(not v for v in times.values()):
+        return None
+    frac = {k: sum(1 for s in specs if s.nmax == k) / len(specs) for k in keys}
+    mean_t = sum(frac[k] * float(np.mean(times[k])) for k in keys)
+    done = sum(len(v) for v in times.values())
+    per = ", ".join(f"Nmax {k}: {len(times[k])} in {np.mean(times[k]):.2f} s mean" for k in keys)
+    return {"value": threads / mean_t, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{done} solves of the batched workload's instances ({per}) on {threads} threads in "
+                      f"{wall:.1f} s wall; value = threads / mean time per solve of the workload's Nmax mix"}
+
+
+def main_batched(args):
+    n = args.batch
+    specs, plain, reseeded = screened_batch(n, args.threads)
+    by_nmax = {k: sum(1 for s in specs if s.nmax == k) for k in sorted({s.nmax for s in specs})}
+    modes = {"fp64": params(0, 0), "fp64_anderson": params(0, 5), "mixed_anderson": params(1, 5)}
+    runs, results = {}, {}
+    for label, par in modes.items():
+        plain.set_params(par)
+        dt, conv = timed(plain, n, args.steps, args.warmup, label)
+        runs[label] = {"dt": dt, "conv": conv, "stages": plain.timings(),
+                       "sweeps": float(np.mean([plain.stats(i)["hf_its"] + plain.stats(i)["sc_its"]
+                                                for i in range(n)]))}
+        results[label] = [plain.result(i) for i in range(n)]
+        print(f"# {label}: timed {args.steps} steps in {dt:.3f} s; last stages {runs[label]['stages']}",
+              file=sys.stderr, flush=True)
+    ref = results["fp64"]
+    zs = [ZSIG_BY_NMAX[s.nmax] for s in specs]
+    for label, r in runs.items():
+        errs = np.zeros(n) if label == "fp64" else \
+            np.array([spectral_error(results[label][i], ref[i], zs[i]) for i in range(n)])
+        r["ok"] = errs <= ERR_BOUND
+        r["rate"] = float(np.sum(r["conv"] * r["ok"])) / r["dt"]
+        r["max_err"] = float(errs.max())
+        print(f"# {label}: {r['rate']:.1f} solves/s within {ERR_BOUND}; max error vs fp64 {r['max_err']:.2e}; "
+              f"mean sweeps {r['sweeps']:.1f}", file=sys.stderr, flush=True)
+    head = max(runs, key=lambda k: runs[k]["rate"])
+    plain.set_params(modes[head])
+    rl, kernels, work = roofline(plain, "batched")
+    print(f"# kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
+    its = [plain.stats(i) for i in range(n)]
+    sh, tms = plain.lanczos_shapes()
+    shapes = {str(capi.LANC_BUCKET_MAX[k]): list(capi.LANC_SHAPES[sh[k]]) for k in range(capi.LANC_BUCKETS)
+              if tms[k].max() > 0}
+    plain.close()
+    # plain FP64 on the GPU, one instance per solve: a sample of each Nmax
+    sample = []
+    for k in by_nmax:
+        sample += [i for i in range(n) if specs[i].nmax == k][:SINGLE_PER_NMAX]
+    t_single, err_single = {}, 0.0
+    for i in sample:
+        b = capi.Batch([specs[i].make()], params(0, 0), lanc_n=[specs[i].lanc_n])
+        b.solve()
+        t1 = time.perf_counter()
+        b.solve()
+        t_single.setdefault(specs[i].nmax, []).append(time.perf_counter() - t1)
+        err_single = max(err_single, spectral_error(b.result(0), ref[i], zs[i]))
+        b.close()
+    frac = {k: v / n for k, v in by_nmax.items()}
+    single_rate = 1.0 / sum(frac[k] * float(np.mean(t_single[k])) for k in frac)
+    print(f"# single-instance fp64: {json.dumps({k: float(np.mean(v)) for k, v in t_single.items()})} s per solve "
+          f"by Nmax; {single_rate:.2f} solves/s for the mix; max error vs batched {err_single:.2e}",
+          file=sys.stderr, flush=True)
+    # an Nmax 11 solve takes ~50 s on one core: the window holds at least one
+    cpu = None if args.no_cpu else cpu_baseline_batched(specs, max(args.cpu_seconds, 70.0))
+    value = runs[head]["rate"]
+    line = {
+        "metric": "converged SCGF solves/sec: mixed+batched vs plain FP64 GPU at <=1e-6 rel. err",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * runs[head]["dt"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if modes[head].precision == 0 else "f64 (f32 Sigma(2) coupling operands)",
+        "data": "synthetic (seeded generator: HO j-scheme, hw U[10,30], sigma U[1,3], spe noise 0.5, "
+                "Nmax U{5,7,9,11}, Lanczos U[30,80] per instance)",
+        "config": {"workload": f"batched_{n}_heterogeneous_nmax5to11", "instances_per_step": n,
+                   "instances_by_nmax": by_nmax, "lanczos_vectors_per_side": "per instance, U[30,80]",
+                   "conv_max_de": CONV, "itr_max_hf": HF_CAP, "itr_max_sc0": SC_CAP,
+                   "instances_reseeded_in_screening": reseeded, "headline_mode": head,
+                   "headline_rule": "the fastest batched mode, counting only converged solves within 1e-6 of the "
+                                    "FP64 solve of the same instance (plain iteration)",
+                   "modes": {k: {"precision": "FP32 Sigma(2) coupling operands" if modes[k].precision else "FP64",
+                                 "density_mixing": f"Anderson, {modes[k].accel} iterates" if modes[k].accel
+                                 else "none",
+                                 "solves_per_s": r["rate"], "ms_per_step": 1e3 * r["dt"] / args.steps,
+                                 "max_error_vs_fp64": r["max_err"], "instances_within_bound": int(r["ok"].sum()),
+                                 "converged": int(r["conv"].min()), "mean_sweeps": r["sweeps"],
+                                 "stages_ms": r["stages"]} for k, r in runs.items()},
+                   "plain_fp64_single_instance_solves_per_s": single_rate,
+                   "plain_fp64_single_instance_sample": {str(k): len(v) for k, v in t_single.items()},
+                   "single_vs_batched_max_error": err_single,
+                   "batched_over_plain_single": value / single_rate,
+                   "error_measure": "per subshell, every solution with Z above the Nmax's rounding floor "
+                                    f"({ZSIG_BY_NMAX}) matched to the nearest in energy of the other solve: max "
+                                    "rel. energy and abs. Z difference, and rel. error of the Z-weighted moments; "
+                                    "against the FP64 plain-iteration GPU solve of the same instance",
+                   "lanczos_tile_shapes": shapes,
+                   "mean_hf_sweeps": float(np.mean([s["hf_its"] for s in its])),
+                   "mean_sc0_sweeps": float(np.mean([s["sc_its"] for s in its]))},
+        "roofline": rl,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--threads", type=int, default=16, help="host threads generating the instances")
+    ap.add_argument("--workload", choices=("batched", "nmax7", "twobody"), default="batched",
+                    help="batched: the heterogeneous converged-solve benchmark (default); nmax7: one model "
+                         "space; twobody: ladder + ring matrices")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; shard instances across GPUs "
+                                   "(one process per GPU) to scale"}))
+        sys.exit(2)
+    if capi.lib().bcd_device_count() < 1:
+        raise SystemExit("no GPU visible: the HIP path has no CPU fallback")
+    if args.workload == "twobody":
+        return main_twobody(args)
+    if args.workload == "nmax7":
+        return main_nmax7(args)
+    return main_batched(args)
 
 
 if __name__ == "__main__":

@@ -49,7 +49,8 @@ class BcdSolveParams(C.Structure):
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
            "bcd_count_poles", "bcd_sigma2_build", "bcd_hf_pot", "bcd_reduce_mtx_lncz", "bcd_reduce_selfen_lncz",
            "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_create_ex", "bcd_batch_set_efermi",
-           "bcd_pivot_round", "bcd_batch_destroy", "bcd_batch_solve",
+           "bcd_pivot_round", "bcd_batch_set_params", "bcd_batch_begin", "bcd_batch_add", "bcd_batch_end", "bcd_batch_destroy",
+           "bcd_batch_solve",
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
            "bcd_batch_kernel_times", "bcd_batch_work", "bcd_batch_autotune", "bcd_batch_lanczos_shapes",
            "bcd_ladd_count", "bcd_ladd_basis", "bcd_ladd_fill",
@@ -102,6 +103,11 @@ def lib():
         L.bcd_batch_create_ex.argtypes = [C.c_int, C.c_int, C.POINTER(BcdModspace), C.POINTER(BcdVpp),
                                           C.POINTER(BcdProp), C.POINTER(BcdSolveParams), IP, C.POINTER(vp)]
         L.bcd_batch_set_efermi.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.bcd_batch_set_params.argtypes = [vp, C.POINTER(BcdSolveParams)]
+        L.bcd_batch_begin.argtypes = [C.c_int, C.c_int, C.POINTER(BcdSolveParams), C.POINTER(vp)]
+        L.bcd_batch_add.argtypes = [vp, C.c_int, C.POINTER(BcdModspace), C.POINTER(BcdVpp), C.POINTER(BcdProp),
+                                    C.c_int]
+        L.bcd_batch_end.argtypes = [vp]
         L.bcd_pivot_round.argtypes = [C.c_int, DP, DP, C.c_long]
         L.bcd_batch_destroy.argtypes = [vp]
         L.bcd_batch_destroy.restype = None
@@ -434,8 +440,8 @@ class Context(_TwoBodyMixin):
 # k_lanczos tile shapes (threads per job, elements per thread) and the upper
 # pole counts of the buckets the autotuner picks them for
 LANC_SHAPES = [(64, 4), (64, 8), (128, 8), (192, 8), (256, 8), (128, 16), (192, 16), (256, 16), (384, 8), (192, 24),
-               (256, 24), (384, 16), (384, 24), (768, 24), (1024, 24)]
-LANC_BUCKET_MAX = [256, 512, 1024, 2048, 4096, 8192, 16384, 24576]
+               (256, 24), (384, 16), (384, 24), (768, 24), (1024, 24), (1024, 32), (1024, 40)]
+LANC_BUCKET_MAX = [256, 512, 1024, 2048, 4096, 8192, 16384, 24576, 32768, 40960]
 LANC_BUCKETS = len(LANC_BUCKET_MAX)
 
 
@@ -480,6 +486,38 @@ class Batch:
         self.h = h
         self.n = n
 
+    @classmethod
+    def streamed(cls, n: int, make, params: BcdSolveParams, lanc_n=None, threads: int = 8, device: int = 0):
+        """A batch built one instance at a time (bcd_batch_begin / _add / _end):
+        make(i) returns the (ModelSpace, Interaction) of instance i; up to
+        `threads` instances are generated and packed at once, and each one's
+        arrays are dropped once its interaction is in HBM, so batches whose
+        interactions together outgrow host memory can be built.  Only the
+        model spaces are kept (for result())."""
+        from concurrent.futures import ThreadPoolExecutor
+        L = lib()
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        check(L.bcd_batch_begin(device, n, C.byref(params), C.byref(h)), "bcd_batch_begin")
+        self.h = h
+        self.n = n
+        spaces = [None] * n
+
+        def add(i):
+            ms, vpp = make(i)
+            keep = _Keep()
+            rc = L.bcd_batch_add(h, i, C.byref(modspace_struct(ms, keep)), C.byref(vpp_struct(vpp, keep)), None,
+                                 0 if lanc_n is None else int(lanc_n[i]))
+            spaces[i] = ms
+            return rc
+
+        with ThreadPoolExecutor(max(1, threads)) as ex:
+            for i, rc in enumerate(ex.map(add, range(n))):
+                check(rc, f"bcd_batch_add({i})")
+        check(L.bcd_batch_end(h), "bcd_batch_end")
+        self.systems = [(ms, None) for ms in spaces]
+        return self
+
     def close(self):
         if getattr(self, "h", None):
             lib().bcd_batch_destroy(self.h)
@@ -490,6 +528,10 @@ class Batch:
 
     def set_profile(self, on: bool):
         check(lib().bcd_batch_set_profile(self.h, int(on)), "bcd_batch_set_profile")
+
+    def set_params(self, params: BcdSolveParams):
+        """Precision, density mixing, ItrMax and stopping rule of later solves."""
+        check(lib().bcd_batch_set_params(self.h, C.byref(params)), "bcd_batch_set_params")
 
     def set_efermi(self, inst: int, ish: int, ef: float):
         """Force the Fermi energy of one subshell (Set_external_EFermi)."""

@@ -150,3 +150,33 @@ def make_system(nmax: int, seed: int, hw: float = 20.0, sigma: float = 2.0, spe_
     ms = make_model_space(nmax, hw, rng, spe_noise)
     vpp = make_interaction(ms, rng, sigma)
     return ms, vpp
+
+
+@dataclasses.dataclass
+class InstanceSpec:
+    """One instance of a heterogeneous batch: its generator inputs and its
+    Lanczos length (Krylov vectors per side)."""
+    nmax: int
+    seed: int
+    hw: float
+    sigma: float
+    lanc_n: int
+
+    def make(self):
+        return make_system(self.nmax, self.seed, hw=self.hw, sigma=self.sigma)
+
+
+def batched_specs(n: int = 256, seed: int = 4, nmax_set=(5, 7, 9, 11), hw_range=(10.0, 30.0),
+                  sigma_range=(1.0, 3.0), lanc_range=(30, 80)) -> list:
+    """The heterogeneous batched workload: n independent instances with hw
+    uniform in hw_range, interaction scale sigma uniform in sigma_range, Nmax
+    uniform over nmax_set and Lanczos lengths uniform over lanc_range
+    (inclusive).  One generator draws the four arrays in that order from
+    default_rng(seed); instance i gets interaction seed 100000 * seed + i."""
+    rng = np.random.default_rng(seed)
+    hw = rng.uniform(*hw_range, n)
+    sigma = rng.uniform(*sigma_range, n)
+    nmax = rng.choice(np.asarray(nmax_set), n)
+    lanc = rng.integers(lanc_range[0], lanc_range[1] + 1, n)
+    return [InstanceSpec(int(nmax[i]), 100000 * seed + i, float(hw[i]), float(sigma[i]), int(lanc[i]))
+            for i in range(n)]

@@ -93,7 +93,10 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
     o.orb0 = put_i(S.orb0); o.orb_sh = put_i(S.orb_sh); o.qoff = put_i(S.qoff); o.pidx = put_i(S.pidx);
     o.npair = put_i(S.npair);
     o.voff = (long)tl.size(); tl.insert(tl.end(), S.voff.begin(), S.voff.end()); tl.resize(pad8(tl.size()));
-    o.V = (long)td.size(); td.insert(td.end(), S.V.begin(), S.V.end()); td.resize(pad8(td.size()));
+    // an interaction already on the device (uploaded when the instance was
+    // added) stays where it is
+    o.V = -1;
+    if (!S.dV) { o.V = (long)td.size(); td.insert(td.end(), S.V.begin(), S.V.end()); td.resize(pad8(td.size())); }
   }
   BTRY(a_tab_i.grow(sizeof(int) * std::max<size_t>(ti.size(), 1)));
   BTRY(a_tab_l.grow(sizeof(long) * std::max<size_t>(tl.size(), 1)));
@@ -126,7 +129,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
     I.gc0 = (int)h_chans.size(); I.nq = S.nq; I._pad = 0;
     I.l = Ti + o.l; I.j2 = Ti + o.j2; I.ip = Ti + o.ip; I.ch = Ti + o.ch; I.no = Ti + o.no;
     I.orb0 = Ti + o.orb0; I.orb_sh = Ti + o.orb_sh; I.qoff = Ti + o.qoff; I.pidx = Ti + o.pidx;
-    I.voff = Tl + o.voff; I.npair = Ti + o.npair; I.V = Td + o.V;
+    I.voff = Tl + o.voff; I.npair = Ti + o.npair; I.V = S.dV ? S.dV : Td + o.V;
     I.rho_off = rho_tot;
     rho_tot += pad8(S.nq);
     for (int s = 0; s < S.nsubsh; ++s) {
@@ -319,6 +322,26 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   if (arrow_set_lds_limit(kLdsBytes)) return BCD_ERR_DEVICE;
   h_is.assign(n_inst, InstState());
+  return BCD_OK;
+}
+
+// New iteration settings for later solves.  The fields that size the device
+// state (mode, Lanczos settings, Dyson solver, pivot rounding, kinetic term)
+// must stay; precision, density mixing, ItrMax and the stopping rule may
+// change.
+int Batch::set_params(const bcd_solve_params &p) {
+  if (p.mode != par.mode || p.lanczos != par.lanczos || p.lanc_kind != par.lanc_kind || p.lanc_n != par.lanc_n ||
+      p.dyson_solver != par.dyson_solver || p.piv_round != par.piv_round || p.t_fac != par.t_fac ||
+      p.hw != par.hw || p.ext_u1 != par.ext_u1)
+    return BCD_ERR_ARG;
+  if (p.precision < 0 || p.precision > 1 || p.accel < 0 || p.accel > kMixMax) return BCD_ERR_ARG;
+  HTRY(hipSetDevice(device));
+  if (p.accel > 0) {
+    BTRY(a_mix.grow(sizeof(double) * 2 * (size_t)(p.accel + 1) * std::max<long>(rho_len, 1) + 64));
+    d_mix = (double *)a_mix.ptr;
+  }
+  par = p;
+  coupling_flops = -1.0;
   return BCD_OK;
 }
 
@@ -771,12 +794,36 @@ int Batch::work(double *w, int n) {
     const double lf = std::max(h_cs[gc].lfw, 0), lb = std::max(h_cs[gc].lbk, 0);
     w[6] += 9.0 * (lf * lf * lf + lb * lb * lb);
   }
+  // Lanczos bytes as k_lanczos moves them (lanczos_job): the rows the job
+  // runs on (merged or not, M of them) once -- E and the nI couplings --, the
+  // coupling columns written once; per step j the nI columns again when a
+  // pivot chain starts, each earlier Krylov vector but the latest once (the
+  // latest is on chip), the nI columns and the E*v_last vector of every
+  // finished chain for the sums, the new vector written, and E*v_last
+  // written at the end of a chain.
   for (int gc = 0; gc < (int)h_cs.size(); ++gc) {
     const ChanDev &C = h_chans[gc];
     const ChanState &Q = h_cs[gc];
     for (int side = 0; side < 2 && par.lanczos; ++side) {
-      const double M = side ? Q.mbk : Q.mfw, L = side ? Q.lbk : Q.lfw;
-      if (L > 0) w[1] += 8.0 * M * (L * (L - 1.0) + L * (5.0 + C.nI + Q.npiv_tot));
+      const double M = side ? Q.lmbk : Q.lmfw;
+      const int L = side ? Q.lbk : Q.lfw;
+      if (L <= 0 || M <= 0) continue;
+      const int *itr = side ? Q.itr_bk : Q.itr_fw;
+      const int nb = C.nI;
+      double vec = 1.0 + 2.0 * nb;          // E and couplings read, couplings written
+      int j = 0, done_chains = 0;
+      for (int k = 0; k < Q.npiv_tot && k < kMaxPiv && j < L; ++k) {
+        if (itr[k] < 1) continue;
+        for (int t = 0; t < itr[k] && j < L; ++t, ++j) {
+          if (t == 0) vec += nb;                       // start vector M u
+          vec += j > 1 ? j - 1 : 0;                   // Gram-Schmidt reads
+          vec += nb + done_chains;                    // couplings and g_i sums
+          vec += 1.0;                                 // the new vector
+        }
+        if (j < L) vec += 1.0;                        // E * v_last of the finished chain
+        ++done_chains;
+      }
+      w[1] += 8.0 * M * vec;
     }
   }
   for (int gc = 0; gc < n_gc; ++gc) {

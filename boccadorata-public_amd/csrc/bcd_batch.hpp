@@ -51,6 +51,7 @@ struct Batch {
   std::vector<const HostSystem *> sys;
   std::vector<int> inst_lanc_n;      // Lanczos length per instance (empty: par.lanc_n for all); set before setup
   int set_efermi(int inst, int ish, double ef);
+  int set_params(const bcd_solve_params &p);
   int n_inst = 0, n_gc = 0, nImax = 1;
   long n_blk = 0, capmax = 0, mat_len = 0;
   std::vector<InstDev> h_insts;
@@ -122,7 +123,9 @@ struct Batch {
   // Lanczos tile shape per pole-count bucket; tune_lanczos: pick them by
   // timing during the next solve
   int lanc_shape[kNLancBuckets] = {kLancDefault[0], kLancDefault[1], kLancDefault[2], kLancDefault[3],
-                                   kLancDefault[4], kLancDefault[5], kLancDefault[6], kLancDefault[7]};
+                                   kLancDefault[4], kLancDefault[5], kLancDefault[6], kLancDefault[7],
+                                   kLancDefault[8], kLancDefault[9]};
+  static_assert(kNLancBuckets == 10, "every bucket has a default shape");
   float lanc_tune_ms[kNLancBuckets][kNLancShapes] = {};
   bool tune_lanczos = false;
   bool poles_done = false;       // lanczos() ran the pole-block kernels bucket by bucket

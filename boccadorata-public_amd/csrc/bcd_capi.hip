@@ -19,8 +19,20 @@
 using namespace bcd;
 
 struct bcd_batch {
+  // instances added one at a time (bcd_batch_begin / _add / _end): each one's
+  // interaction goes to HBM as it is added and its host copy is freed, so
+  // the host never holds more than the instances being added
+  std::vector<DevArena> vbufs;
   std::vector<std::unique_ptr<HostSystem>> systems;
+  std::vector<HostProp> g0;
+  std::vector<int> lanc_n;
+  bcd_solve_params par{};
+  int device = 0;
+  bool ready = false;
   Batch b;
+  ~bcd_batch() {
+    for (DevArena &a : vbufs) a.release();
+  }
 };
 
 namespace {
@@ -546,6 +558,32 @@ int bcd_batch_create_ex(int device, int n_inst, const bcd_modspace *ms, const bc
                         const bcd_solve_params *par, const int *lanc_n, bcd_batch **out) {
   if (!out || n_inst < 1 || !ms || !vpp || !par) return BCD_ERR_ARG;
   *out = nullptr;
+  bcd_batch *B = nullptr;
+  BTRY(bcd_batch_begin(device, n_inst, par, &B));
+  std::unique_ptr<bcd_batch> hold(B);
+  for (int i = 0; i < n_inst; ++i) {
+    // the instances of one call keep their interaction in the batch's own
+    // arena (one upload in bcd_batch_end), as small batches prefer
+    std::unique_ptr<HostSystem> S(new (std::nothrow) HostSystem);
+    if (!S) return BCD_ERR_NOMEM;
+    BTRY(S->build(ms + i, vpp + i));
+    if (g0) {
+      if (!prop_ok(*S, g0 + i)) return BCD_ERR_ARG;
+      B->g0[i] = to_host_prop(*S, g0 + i);
+    } else {
+      B->g0[i] = bare_prop(*S);
+    }
+    B->lanc_n[i] = lanc_n ? lanc_n[i] : 0;
+    B->systems[i] = std::move(S);
+  }
+  BTRY(bcd_batch_end(B));
+  *out = hold.release();
+  return BCD_OK;
+}
+
+int bcd_batch_begin(int device, int n_inst, const bcd_solve_params *par, bcd_batch **out) {
+  if (!out || n_inst < 1 || !par) return BCD_ERR_ARG;
+  *out = nullptr;
   if (par->mode < 0 || par->mode > 2 || par->dyson_solver < 0 || par->dyson_solver > 1 || par->precision < 0 ||
       par->precision > 1 || par->accel < 0 || par->accel > kMixMax || par->hw < 0.0 ||
       (par->t_fac != 0.0 && par->hw <= 0.0))
@@ -553,26 +591,57 @@ int bcd_batch_create_ex(int device, int n_inst, const bcd_modspace *ms, const bc
   BTRY(device_ok(device));
   std::unique_ptr<bcd_batch> B(new (std::nothrow) bcd_batch);
   if (!B) return BCD_ERR_NOMEM;
-  if (lanc_n) B->b.inst_lanc_n.assign(lanc_n, lanc_n + n_inst);
-  std::vector<const HostSystem *> sp;
-  std::vector<HostProp> gp;
-  for (int i = 0; i < n_inst; ++i) {
-    B->systems.emplace_back(new HostSystem);
-    BTRY(B->systems.back()->build(ms + i, vpp + i));
-    sp.push_back(B->systems.back().get());
-    if (g0) {
-      if (!prop_ok(*sp.back(), g0 + i)) return BCD_ERR_ARG;
-      gp.push_back(to_host_prop(*sp.back(), g0 + i));
-    } else {
-      gp.push_back(bare_prop(*sp.back()));
-    }
-  }
-  BTRY(B->b.setup(device, sp, *par, gp));
+  B->device = device;
+  B->par = *par;
+  B->systems.resize(n_inst);
+  B->g0.resize(n_inst);
+  B->lanc_n.assign(n_inst, 0);
+  B->vbufs.resize(n_inst);
   *out = B.release();
   return BCD_OK;
 }
 
+int bcd_batch_add(bcd_batch *b, int inst, const bcd_modspace *ms, const bcd_vpp *vpp, const bcd_prop *g0,
+                  int lanc_n) {
+  if (!b || b->ready || inst < 0 || inst >= (int)b->systems.size() || b->systems[inst] || !ms || !vpp)
+    return BCD_ERR_ARG;
+  std::unique_ptr<HostSystem> S(new (std::nothrow) HostSystem);
+  if (!S) return BCD_ERR_NOMEM;
+  BTRY(S->build(ms, vpp));
+  if (g0 && !prop_ok(*S, g0)) return BCD_ERR_ARG;
+  HostProp G = g0 ? to_host_prop(*S, g0) : bare_prop(*S);
+  HTRY(hipSetDevice(b->device));
+  DevArena &A = b->vbufs[inst];
+  BTRY(A.grow(sizeof(double) * (size_t)std::max<long>(S->v_len, 1)));
+  if (S->v_len) HTRY(hipMemcpy(A.ptr, S->V.data(), sizeof(double) * S->v_len, hipMemcpyHostToDevice));
+  S->dV = (const double *)A.ptr;
+  std::vector<double>().swap(S->V);
+  b->g0[inst] = std::move(G);
+  b->lanc_n[inst] = lanc_n;
+  b->systems[inst] = std::move(S);
+  return BCD_OK;
+}
+
+int bcd_batch_end(bcd_batch *b) {
+  if (!b || b->ready) return BCD_ERR_ARG;
+  std::vector<const HostSystem *> sp;
+  for (auto &S : b->systems) {
+    if (!S) return BCD_ERR_ARG;   // an instance was never added
+    sp.push_back(S.get());
+  }
+  b->b.inst_lanc_n = b->lanc_n;
+  BTRY(b->b.setup(b->device, sp, b->par, b->g0));
+  std::vector<HostProp>().swap(b->g0);
+  b->ready = true;
+  return BCD_OK;
+}
+
 void bcd_batch_destroy(bcd_batch *b) { delete b; }
+
+int bcd_batch_set_params(bcd_batch *b, const bcd_solve_params *par) {
+  if (!b || !b->ready || !par) return BCD_ERR_ARG;
+  return b->b.set_params(*par);
+}
 
 int bcd_batch_set_efermi(bcd_batch *b, int inst, int ish, double ef) {
   if (!b) return BCD_ERR_ARG;
@@ -595,7 +664,7 @@ int bcd_pivot_round(int device, const double *x, double *out, long n) {
 }
 
 int bcd_batch_solve(bcd_batch *b) {
-  if (!b) return BCD_ERR_ARG;
+  if (!b || !b->ready) return BCD_ERR_ARG;
   return b->b.solve();
 }
 

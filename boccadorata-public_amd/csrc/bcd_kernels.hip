@@ -1199,10 +1199,12 @@ void launch_lanczos(int shape, int njobs, const LancJob *jobs, const int *order,
     case 11: launch_shape<11>(njobs, jobs, order, stream); break;
     case 12: launch_shape<12>(njobs, jobs, order, stream); break;
     case 13: launch_shape<13>(njobs, jobs, order, stream); break;
-    default: launch_shape<14>(njobs, jobs, order, stream); break;
+    case 14: launch_shape<14>(njobs, jobs, order, stream); break;
+    case 15: launch_shape<15>(njobs, jobs, order, stream); break;
+    default: launch_shape<16>(njobs, jobs, order, stream); break;
   }
 }
-static_assert(kNLancShapes == 15, "launch_lanczos covers every shape");
+static_assert(kNLancShapes == 17, "launch_lanczos covers every shape");
 
 // ------------------------------------------------------------- Dyson
 

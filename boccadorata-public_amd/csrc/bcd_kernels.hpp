@@ -72,9 +72,12 @@ __global__ void k_lanczos(const LancJob *jobs, const int *order, int njobs);
 struct LancShape { int nt, per; };
 constexpr LancShape kLancShapes[] = {{64, 4},   {64, 8},   {128, 8},   {192, 8},   {256, 8},
                                      {128, 16}, {192, 16}, {256, 16},  {384, 8},   {192, 24},
-                                     {256, 24}, {384, 16}, {384, 24},  {768, 24},  {1024, 24}};
+                                     {256, 24}, {384, 16}, {384, 24},  {768, 24},  {1024, 24},
+                                     {1024, 32}, {1024, 40}};
 constexpr int kNLancShapes = sizeof(kLancShapes) / sizeof(kLancShapes[0]);
-constexpr int kLancBucket[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 24576};
+// The last two buckets take channels whose HF occupations moved (more
+// quasiholes than the bare propagator gives), found at Nmax 11.
+constexpr int kLancBucket[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 24576, 32768, 40960};
 constexpr int kNLancBuckets = sizeof(kLancBucket) / sizeof(kLancBucket[0]);
 inline int lanczos_bucket(int M) {
   for (int b = 0; b < kNLancBuckets; ++b)
@@ -82,7 +85,7 @@ inline int lanczos_bucket(int M) {
   return -1;
 }
 // default shape of each bucket
-constexpr int kLancDefault[kNLancBuckets] = {0, 1, 2, 4, 9, 12, 13, 14};
+constexpr int kLancDefault[kNLancBuckets] = {0, 1, 2, 4, 9, 12, 13, 14, 15, 16};
 inline bool lanczos_shape_fits(int shape, int M) {
   return shape >= 0 && shape < kNLancShapes && M <= kLancShapes[shape].nt * kLancShapes[shape].per;
 }

@@ -72,6 +72,7 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
   long tot = 0;
   for (int c = 0; c < nchan2b; ++c) { voff[c] = tot; tot += (long)npair[c] * npair[c]; }
   V.assign(tot, 0.0);
+  v_len = tot;
   dropped = 0;
   if (vpp && vpp->n_me > 0) {
     for (long q = 0; q < vpp->n_me; ++q) {

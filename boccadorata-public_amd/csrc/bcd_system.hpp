@@ -18,7 +18,9 @@ struct HostSystem {
   std::vector<double> orb_e;
   std::vector<int> pidx, npair;
   std::vector<long> voff;
-  std::vector<double> V;
+  std::vector<double> V;                       // dense blocks (emptied once uploaded, see dV)
+  const double *dV = nullptr;                  // the dense blocks in HBM when uploaded at add time
+  long v_len = 0;                              // doubles of the dense blocks
   std::vector<std::vector<BlockDev>> blocks;   // [ish]
   // per subshell: the blocks grouped by (a, b, g) triple, each group in J
   // order; trip_start[ish] has one entry per triple plus the end

@@ -181,6 +181,22 @@ int bcd_batch_create(int device, int n_inst, const bcd_modspace *ms, const bcd_v
 int bcd_batch_create_ex(int device, int n_inst, const bcd_modspace *ms, const bcd_vpp *vpp,
                         const bcd_prop *g0, const bcd_solve_params *par, const int *lanc_n,
                         bcd_batch **out);
+/* The same batch built one instance at a time, for batches whose
+ * interactions together outgrow host memory: begin sizes it, add packs one
+ * instance (model space, interaction, optional start g0, Lanczos length
+ * lanc_n or 0 for par->lanc_n), moves its interaction to HBM and frees the
+ * host copy; end lays out the device state.  add may be called from several
+ * host threads at once for different inst.  Every instance must be added
+ * before end; solve and the queries need end. */
+int bcd_batch_begin(int device, int n_inst, const bcd_solve_params *par, bcd_batch **out);
+int bcd_batch_add(bcd_batch *b, int inst, const bcd_modspace *ms, const bcd_vpp *vpp, const bcd_prop *g0,
+                  int lanc_n);
+int bcd_batch_end(bcd_batch *b);
+/* Change the iteration settings of a built batch for its later solves:
+ * precision, accel, hf/sc ItrMax, conv_kind and the tolerances.  The other
+ * fields must equal the batch's (they size its device state); otherwise
+ * BCD_ERR_ARG. */
+int bcd_batch_set_params(bcd_batch *b, const bcd_solve_params *par);
 /* Set_external_EFermi (DysonMatrix.cpp:601-617; the SetEf= flag,
  * Command_line.cpp:347-350): every later Dyson sweep of subshell ish of
  * instance inst splits quasiparticles from quasiholes at ef instead of the
@@ -221,7 +237,10 @@ int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_
  * J-degenerate pole groups before the Lanczos reduction). */
 int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
 /* Algorithmic work of the last solve: [0] coupling FLOPs (2*sum poles*K*n),
- * [1] Lanczos bytes (8*M*sum_j(2j+5+n+npiv)), [2] dense Dyson FLOPs (9 n^3
+ * [1] Lanczos bytes as the reduction moves them (8*M per vector: the pole rows
+ * the jobs run on once, each earlier Krylov vector but the latest once per
+ * step, the n coupling columns and E*v_last of finished chains per step, the
+ * writes), [2] dense Dyson FLOPs (9 n^3
  * per eigensolve), [3] number of Dyson eigensolves, [4] sum of Dyson n^3,
  * [5] arrowhead-solver FLOPs as counted by the kernels, [6] FLOPs of the
  * once-per-solve diagonalisation of the reduced sides (9 L^3 per side).
@@ -234,14 +253,14 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
 int bcd_batch_work(bcd_batch *b, double *work, int n);
 /* Tile-shape autotuner of the Lanczos kernel (no counterpart in the
  * reference, which runs one Lanczos at a time on the CPU).  Jobs are grouped
- * by pole count into 8 buckets (M <= 256, 512, 1024, 2048, 4096, 8192, 16384,
- * 24576); during the next bcd_batch_solve every shape that fits a bucket's
+ * by pole count into 10 buckets (M <= 256, 512, 1024, 2048, 4096, 8192, 16384,
+ * 24576, 32768, 40960); during the next bcd_batch_solve every shape that fits a bucket's
  * largest job is timed on that bucket's own jobs and the fastest is kept for
  * later solves.  Shapes differ only in how the block sums are split, so
  * results agree to rounding. */
 int bcd_batch_autotune(bcd_batch *b);
-/* Shape index chosen per bucket (n >= 8) and, if tune_ms is not NULL, the
- * timed ms of every (bucket, shape) pair as tune_ms[bucket * 15 + shape]
+/* Shape index chosen per bucket (n >= 10) and, if tune_ms is not NULL, the
+ * timed ms of every (bucket, shape) pair as tune_ms[bucket * 17 + shape]
  * (0 where not timed). */
 int bcd_batch_lanczos_shapes(bcd_batch *b, int *shapes, float *tune_ms, int n);
 
