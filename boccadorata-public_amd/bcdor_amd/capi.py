@@ -49,7 +49,7 @@ class BcdSolveParams(C.Structure):
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
            "bcd_count_poles", "bcd_sigma2_build", "bcd_hf_pot", "bcd_reduce_mtx_lncz", "bcd_reduce_selfen_lncz",
            "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_create_ex", "bcd_batch_set_efermi",
-           "bcd_pivot_round", "bcd_batch_set_params", "bcd_batch_begin", "bcd_batch_add", "bcd_batch_end", "bcd_batch_destroy",
+           "bcd_pivot_round", "bcd_batch_set_params", "bcd_batch_error_detail", "bcd_batch_begin", "bcd_batch_add", "bcd_batch_end", "bcd_batch_destroy",
            "bcd_batch_solve",
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
            "bcd_batch_kernel_times", "bcd_batch_work", "bcd_batch_autotune", "bcd_batch_lanczos_shapes",
@@ -104,6 +104,7 @@ def lib():
                                           C.POINTER(BcdProp), C.POINTER(BcdSolveParams), IP, C.POINTER(vp)]
         L.bcd_batch_set_efermi.argtypes = [vp, C.c_int, C.c_int, C.c_double]
         L.bcd_batch_set_params.argtypes = [vp, C.POINTER(BcdSolveParams)]
+        L.bcd_batch_error_detail.argtypes = [vp, C.c_char_p, C.c_int]
         L.bcd_batch_begin.argtypes = [C.c_int, C.c_int, C.POINTER(BcdSolveParams), C.POINTER(vp)]
         L.bcd_batch_add.argtypes = [vp, C.c_int, C.POINTER(BcdModspace), C.POINTER(BcdVpp), C.POINTER(BcdProp),
                                     C.c_int]
@@ -538,7 +539,15 @@ class Batch:
         check(lib().bcd_batch_set_efermi(self.h, inst, ish, ef), "bcd_batch_set_efermi")
 
     def solve(self):
-        check(lib().bcd_batch_solve(self.h), "bcd_batch_solve")
+        rc = lib().bcd_batch_solve(self.h)
+        if rc < 0:
+            raise BcdError(f"bcd_batch_solve failed: {rc} ({ERRORS.get(rc, 'reference error code')}): "
+                           f"{self.error_detail()}")
+
+    def error_detail(self) -> str:
+        buf = C.create_string_buffer(4096)
+        lib().bcd_batch_error_detail(self.h, buf, 4096)
+        return buf.value.decode()
 
     def autotune(self):
         """Pick the Lanczos tile shape of every pole-count bucket during the next solve."""

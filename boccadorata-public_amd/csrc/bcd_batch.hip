@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 namespace bcd {
 
@@ -453,8 +454,12 @@ int Batch::check(int &n_active) {
   HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
   HTRY(hipStreamSynchronize(stream));
   n_active = 0;
-  for (const InstState &s : h_is) {
-    if (s.err) return BCD_ERR_CAPACITY;
+  for (int i = 0; i < n_inst; ++i) {
+    const InstState &s = h_is[i];
+    if (s.err) {
+      note_error("Dyson sweep: a channel of instance " + std::to_string(i) + " exceeds its propagator slots");
+      return BCD_ERR_CAPACITY;
+    }
     n_active += s.active;
   }
   int st = 0;
@@ -468,7 +473,7 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
   HTRY(hipStreamSynchronize(stream));
   for (InstState &s : h_is) {
-    s.active = 1; s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0;
+    s.active = s.failed ? 0 : 1; s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0;
     s.mix_k = 0; s.mix_s = 0; s.mix_fn = 0.0;
   }
   HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
@@ -491,7 +496,7 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   }
   for (int i = 0; i < n_inst; ++i) {
     (hf ? hf_its : sc_its)[i] = h_is[i].its;
-    (hf ? hf_ok : sc_ok)[i] = h_is[i].diff < conv;
+    (hf ? hf_ok : sc_ok)[i] = h_is[i].diff < conv && !h_is[i].failed;
   }
   return BCD_OK;
 }
@@ -531,7 +536,10 @@ int Batch::build_sigma2(int only_ish) {
   BTRY(pull_counts());
   // row buffers and Lanczos workspace sized by the actual pole counts
   for (int gc = 0; gc < n_gc; ++gc)
-    if (h_cs[gc].err == 3) return BCD_ERR_CAPACITY;   // a fragment count above 65535
+    if (h_cs[gc].err == 3) {   // a fragment count above 65535
+      note_error("Sigma(2): a fragment count of channel " + std::to_string(gc) + " exceeds 65535");
+      return BCD_ERR_CAPACITY;
+    }
   std::vector<long> bfw(n_gc), bbk(n_gc);
   long tfw = 0, tbk = 0, tws = 0;
   bool relayout = false;
@@ -546,7 +554,10 @@ int Batch::build_sigma2(int only_ish) {
       if (C.lws_fw_off != lws_fw || C.lws_bk_off != lws_bk) relayout = true;
       C.lws_fw_off = lws_fw; C.lws_bk_off = lws_bk;
     }
-    if (!sigma_only && !par.lanczos && h_cs[gc].mfw + h_cs[gc].mbk + C.nI > C.cap) return BCD_ERR_CAPACITY;
+    if (!sigma_only && !par.lanczos && h_cs[gc].mfw + h_cs[gc].mbk + C.nI > C.cap) {
+      note_error("Dyson: channel " + std::to_string(gc) + " has more poles than its propagator slots");
+      return BCD_ERR_CAPACITY;
+    }
   }
   n_fw_tot.assign(n_inst, 0); n_bk_tot.assign(n_inst, 0);
   for (int gc = 0; gc < n_gc; ++gc) { n_fw_tot[h_chans[gc].inst] += h_cs[gc].mfw; n_bk_tot[h_chans[gc].inst] += h_cs[gc].mbk; }
@@ -654,12 +665,31 @@ int Batch::lanczos() {
   BTRY(pull_counts());
   // jobs longest first, grouped into pole-count buckets; one launch per
   // bucket with the bucket's tile shape, the largest jobs first
+  // a channel with more poles than the largest Lanczos tile holds drops its
+  // instance from this solve (reported as not converged); the others go on
+  std::vector<char> drop(n_inst, 0);
+  for (int gc = 0; gc < n_gc; ++gc)
+    for (int side = 0; side < 2; ++side) {
+      const int M = side ? h_cs[gc].lmbk : h_cs[gc].lmfw;
+      if (M > 0 && lanczos_bucket(M) < 0) {
+        drop[h_chans[gc].inst] = 1;
+        note_error("Lanczos: instance " + std::to_string(h_chans[gc].inst) + " subshell " +
+                   std::to_string(h_chans[gc].ish) + " has " + std::to_string(M) + " poles on one side, above " +
+                   std::to_string(kLancBucket[kNLancBuckets - 1]) + "; the instance is dropped from the solve");
+      }
+    }
+  if (std::count(drop.begin(), drop.end(), 1)) {
+    HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
+    HTRY(hipStreamSynchronize(stream));
+    for (int i = 0; i < n_inst; ++i)
+      if (drop[i]) { h_is[i].failed = 1; h_is[i].active = 0; }
+    HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
+  }
   std::vector<std::pair<int, int>> jm;
   for (int gc = 0; gc < n_gc; ++gc)
     for (int side = 0; side < 2; ++side) {
       const int M = side ? h_cs[gc].lmbk : h_cs[gc].lmfw;
-      if (M > 0 && lanczos_bucket(M) < 0) return BCD_ERR_CAPACITY;
-      jm.push_back({M, 2 * gc + side});
+      jm.push_back({drop[h_chans[gc].inst] ? 0 : M, 2 * gc + side});
     }
   std::stable_sort(jm.begin(), jm.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
     return a.first > b.first;
@@ -711,7 +741,11 @@ int Batch::lanczos() {
   HTRY(hipGetLastError());
   BTRY(pull_counts());
   for (int gc = 0; gc < n_gc; ++gc)
-    if (h_cs[gc].err || h_cs[gc].lfw < 0 || h_cs[gc].lbk < 0) return BCD_ERR_LANCZOS;
+    if (!drop[h_chans[gc].inst] && (h_cs[gc].err || h_cs[gc].lfw < 0 || h_cs[gc].lbk < 0)) {
+      note_error("Lanczos: channel " + std::to_string(gc) + " failed (code " +
+                 std::to_string(std::min(h_cs[gc].lfw, h_cs[gc].lbk)) + ")");
+      return BCD_ERR_LANCZOS;
+    }
   return BCD_OK;
 }
 
@@ -748,8 +782,10 @@ void Batch::launch_poles(const PoleCfg &c, const int *order, int njobs, hipStrea
 int Batch::decompose_poles() {
   if (poles_done) return BCD_OK;
   const PoleCfg c = pole_cfg();
-  if (tridiag_lds(c.lmax, false) > (size_t)kLdsBytes) return BCD_ERR_CAPACITY;
-  if (!c.in_lds && !a_mat.ptr) return BCD_ERR_CAPACITY;
+  if (tridiag_lds(c.lmax, false) > (size_t)kLdsBytes || (!c.in_lds && !a_mat.ptr)) {
+    note_error("pole blocks: a reduced side of " + std::to_string(c.lmax) + " rows does not fit the kernels");
+    return BCD_ERR_CAPACITY;
+  }
   kt_begin(KT_POLES);
   if (c.wave) {
     launch_poles(c, nullptr, 2 * n_gc, stream);
@@ -855,8 +891,14 @@ int Batch::prepare() {
   return BCD_OK;
 }
 
+void Batch::note_error(const std::string &what) {
+  if (!err_detail.empty()) err_detail += "; ";
+  err_detail += what;
+}
+
 int Batch::solve() {
   HTRY(hipSetDevice(device));
+  err_detail.clear();
   for (int k = 0; k < KT_N; ++k) { ktime[k] = 0.0; kcount[k] = 0; }
   hf_its.assign(n_inst, 0);
   sc_its.assign(n_inst, 0);

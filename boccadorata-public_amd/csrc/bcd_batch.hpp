@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -52,6 +53,8 @@ struct Batch {
   std::vector<int> inst_lanc_n;      // Lanczos length per instance (empty: par.lanc_n for all); set before setup
   int set_efermi(int inst, int ish, double ef);
   int set_params(const bcd_solve_params &p);
+  std::string err_detail;            // what the last solve reported (limits hit, instances dropped)
+  void note_error(const std::string &what);
   int n_inst = 0, n_gc = 0, nImax = 1;
   long n_blk = 0, capmax = 0, mat_len = 0;
   std::vector<InstDev> h_insts;

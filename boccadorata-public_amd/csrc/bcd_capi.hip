@@ -638,6 +638,15 @@ int bcd_batch_end(bcd_batch *b) {
 
 void bcd_batch_destroy(bcd_batch *b) { delete b; }
 
+int bcd_batch_error_detail(const bcd_batch *b, char *buf, int n) {
+  if (!b || !buf || n < 1) return BCD_ERR_ARG;
+  const std::string &s = b->b.err_detail;
+  const int k = std::min((int)s.size(), n - 1);
+  std::memcpy(buf, s.data(), k);
+  buf[k] = 0;
+  return (int)s.size();
+}
+
 int bcd_batch_set_params(bcd_batch *b, const bcd_solve_params *par) {
   if (!b || !b->ready || !par) return BCD_ERR_ARG;
   return b->b.set_params(*par);

@@ -118,7 +118,7 @@ struct InstState {
   int err;
   int mix_k;                     // density-mixing steps of the current stage
   int mix_s;                     // first iterate of the current mixing history
-  int _pad;
+  int failed;                    // 1: dropped from this solve (a channel exceeds a kernel limit)
   double mix_fn;                 // norm of the previous step's density residual
   double conv;
   double diff;

@@ -1877,7 +1877,7 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
     __syncthreads();
   }
   if (init) {
-    if (tid == 0) { S.its = 0; S.active = 1; S.diff = 0.0; }
+    if (tid == 0) { S.its = 0; S.active = S.failed ? 0 : 1; S.diff = 0.0; }
     return;
   }
   // storage-index difference between buffers cur (old) and nxt (new)

@@ -192,6 +192,12 @@ int bcd_batch_begin(int device, int n_inst, const bcd_solve_params *par, bcd_bat
 int bcd_batch_add(bcd_batch *b, int inst, const bcd_modspace *ms, const bcd_vpp *vpp, const bcd_prop *g0,
                   int lanc_n);
 int bcd_batch_end(bcd_batch *b);
+/* What the last solve reported beyond its return code: the limit a failing
+ * call hit, and instances dropped from the solve (a channel with more poles
+ * than the largest Lanczos tile, 40960; bcd_batch_converged then gives 0 for
+ * them while the others are solved).  Writes at most n-1 characters and a
+ * terminating 0 to buf; returns the full length. */
+int bcd_batch_error_detail(const bcd_batch *b, char *buf, int n);
 /* Change the iteration settings of a built batch for its later solves:
  * precision, accel, hf/sc ItrMax, conv_kind and the tolerances.  The other
  * fields must equal the batch's (they size its device state); otherwise
