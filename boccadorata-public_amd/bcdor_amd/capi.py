@@ -52,7 +52,7 @@ EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_creat
            "bcd_pivot_round", "bcd_batch_set_params", "bcd_batch_error_detail", "bcd_batch_begin", "bcd_batch_add", "bcd_batch_end", "bcd_batch_destroy",
            "bcd_batch_solve",
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
-           "bcd_batch_kernel_times", "bcd_batch_work", "bcd_batch_autotune", "bcd_batch_lanczos_shapes",
+           "bcd_batch_kernel_times", "bcd_batch_work", "bcd_batch_set_lanczos_shapes", "bcd_batch_autotune", "bcd_batch_lanczos_shapes",
            "bcd_ladd_count", "bcd_ladd_basis", "bcd_ladd_fill",
            "bcd_ph_count", "bcd_ph_basis", "bcd_ph_fill", "bcd_t2_create", "bcd_t2_destroy", "bcd_t2_count",
            "bcd_t2_info", "bcd_t2_basis", "bcd_t2_build", "bcd_t2_matrix", "bcd_t2_set_profile",
@@ -122,6 +122,7 @@ def lib():
         L.bcd_batch_result.argtypes = [vp, C.c_int, C.c_int, IP, IP, IP, IP, DP, DP, DP, DP, DP, DP]
         L.bcd_batch_kernel_times.argtypes = [vp, C.c_int, DP, LP]
         L.bcd_batch_work.argtypes = [vp, DP, C.c_int]
+        L.bcd_batch_set_lanczos_shapes.argtypes = [vp, IP, C.c_int]
         PP = C.POINTER(BcdProp)
         for f in (L.bcd_ladd_count, L.bcd_ph_count):
             f.argtypes = [vp, PP] + [C.c_int] * 5 + [IP, IP, IP, IP]
@@ -381,6 +382,7 @@ class Context(_TwoBodyMixin):
         return u.reshape(n, n)
 
     def reduce_mtx_lncz(self, NA_out, NB_out, NPLS_out, Min, NA_in, NB_in, pivots, n_itr):
+        """Reduce_Mtx_Lncz: (return value, Mout rows, i_piv, normalised pivots)."""
         Min = _d(Min)
         piv = _d(pivots).copy().reshape(-1)
         itr = _i(n_itr).copy()
@@ -444,6 +446,9 @@ LANC_SHAPES = [(64, 4), (64, 8), (128, 8), (192, 8), (256, 8), (128, 16), (192, 
                (256, 24), (384, 16), (384, 24), (768, 24), (1024, 24), (1024, 32), (1024, 40)]
 LANC_BUCKET_MAX = [256, 512, 1024, 2048, 4096, 8192, 16384, 24576, 32768, 40960]
 LANC_BUCKETS = len(LANC_BUCKET_MAX)
+# a second valid shape per bucket, different from the default one (its block
+# sums are split differently): the rounding probe of a solve
+LANC_ALT_SHAPES = [1, 2, 3, 7, 11, 13, 14, 15, 16, 16]
 
 
 def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1,
@@ -560,6 +565,10 @@ class Batch:
         check(lib().bcd_batch_lanczos_shapes(self.h, ip(sh), ms.ctypes.data_as(C.POINTER(C.c_float)), LANC_BUCKETS),
               "bcd_batch_lanczos_shapes")
         return sh, ms
+
+    def set_lanczos_shapes(self, shapes):
+        sh = np.asarray(shapes, np.int32)
+        check(lib().bcd_batch_set_lanczos_shapes(self.h, ip(sh), len(sh)), "bcd_batch_set_lanczos_shapes")
 
     def timings(self):
         t = np.zeros(6)

@@ -117,7 +117,7 @@ __global__ void k_poles_prep(const ChanDev *chans, int n_gc, const ChanState *cs
   const ChanState &Q = cs[gc];
   DysonJob &J = jobs[t];
   const int L = side ? Q.lbk : Q.lfw, capS = side ? C.lbk_cap : C.lfw_cap;
-  J.nI = C.nI; J.j2 = C.j2; J.ch = C.ch; J.inst = C.inst; J._pad = 0;
+  J.nI = C.nI; J.j2 = C.j2; J.ch = C.ch; J.inst = C.inst; J.path = pole_path(capS, C.nI);
   J.n = L > 0 ? L : 0;
   J.nfw = J.n; J.nbk = 0;
   J.den_fw = J.den_bk = Q.den;
@@ -189,6 +189,8 @@ template <bool IN_LDS, int NIMAX>
 __global__ __launch_bounds__(kArrowMaxThreads, 4) void k_arrow_sweep(ArrowSweep a) {
   extern __shared__ double sm[];
   if ((int)blockIdx.x >= a.n_gc) return;
+  if (a.chans[blockIdx.x].nI <= a.skip_ni) return;
+  if (a.nt_class && arrow_threads(a.chans[blockIdx.x].cap) != a.nt_class) return;
   const ArrowJob J = arrow_job(a, blockIdx.x);
   if (J.n <= 0 || J.nI > NIMAX) return;
   arrow_body<NIMAX>(DevBlock{a.status, a.work, a.prof ? a.prof + 8L * blockIdx.x : nullptr}, J,
@@ -202,6 +204,7 @@ __global__ __launch_bounds__(64) void k_dyson_static(ArrowSweep a) {
   const int gc = blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= a.n_gc) return;
   const ChanDev &C = a.chans[gc];
+  if (C.nI > kStaticNI) return;
   const InstState *S = a.P.is ? a.P.is + C.inst : nullptr;
   if (S && !S->active) return;
   const int nxt = 1 - (S ? S->cur : 0);

@@ -115,7 +115,6 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   h_blocks.clear();
   h_trip_start.clear();
   h_trip_blk.clear();
-  merge_ok = true;
   nImax = 1;
   const bool dyn = par.mode >= 1;
   long o_u1 = 0, o_h = 0, o_w = 0, o_tri = 0, o_mat = 0, o_piv = 0, o_lr = 0, o_e = 0, o_x = 0, o_blk = 0;
@@ -158,8 +157,9 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
         const int tb0 = (int)h_trip_blk.size();
         for (int v : S.trip_start[s]) h_trip_start.push_back(tb0 + v);
         for (int v : S.trip_blk[s]) h_trip_blk.push_back((int)(o_blk + v));
+        C.merge = 1;
         for (size_t k = 0; k + 1 < S.trip_start[s].size(); ++k)
-          if (S.trip_start[s][k + 1] - S.trip_start[s][k] > kMergeMaxJ) merge_ok = false;
+          if (S.trip_start[s][k + 1] - S.trip_start[s][k] > kMergeMaxJ) C.merge = 0;
       }
       o_blk += C.blk_cnt;
       C.u1_off = o_u1; o_u1 += pad8((long)nI * nI);
@@ -185,6 +185,12 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   n_gc = (int)h_chans.size();
   n_blk = (long)h_blocks.size();
   mat_len = o_mat;
+  arrow_classes.clear();
+  for (const ChanDev &C : h_chans) {
+    const int cls = arrow_threads(C.cap);
+    if (std::find(arrow_classes.begin(), arrow_classes.end(), cls) == arrow_classes.end()) arrow_classes.push_back(cls);
+  }
+  std::sort(arrow_classes.begin(), arrow_classes.end());
 
   // ---- warm-start roots of the arrowhead sweeps
   BTRY(a_warm.grow(sizeof(double) * pad8(std::max<long>(o_warm, 1)) + sizeof(int) * kArrowNI * (size_t)n_gc + 64));
@@ -417,11 +423,26 @@ int Batch::sweep(int src) {
     a.warm_p = d_warm_p;
     const int nmax = src == 0 ? nImax : (int)capmax;
     const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
+    a.skip_ni = 0;
+    a.nt_class = 0;
     kt_begin(KT_ARROW);
-    if (src == 0 && nImax <= kStaticNI)
+    if (src == 0) {
+      // HF sweeps: channels up to kStaticNI orbits by one thread each, the
+      // larger ones (if any) by the arrowhead kernel
       hipLaunchKernelGGL(k_dyson_static, dim3((n_gc + 63) / 64), dim3(64), 0, stream, a);
-    else
-      launch_arrow_sweep(a, nImax, !aws_len, nt, aws_len ? 0 : arrow_ws_bytes(nmax, nImax), stream);
+      a.skip_ni = kStaticNI;
+      if (nImax > kStaticNI)
+        launch_arrow_sweep(a, nImax, !aws_len, nt, aws_len ? 0 : arrow_ws_bytes(nmax, nImax), stream);
+    } else {
+      // one launch per block size, each channel in the launch of its own
+      // size class (arrow_threads), so that its reductions do not depend on
+      // the other channels of the batch
+      for (int cls : arrow_classes) {
+        a.nt_class = cls;
+        const int cmax = cls == kArrowMaxThreads ? (int)capmax : std::min<int>(cls, (int)capmax);
+        launch_arrow_sweep(a, nImax, !aws_len, cls, aws_len ? 0 : arrow_ws_bytes(cmax, nImax), stream);
+      }
+    }
     kt_end(KT_ARROW);
   } else {
     hipLaunchKernelGGL(k_dyson_prep, dim3(gb), dim3(128), 0, stream, d_chans, 0, n_gc, d_cs, P, src, d_h, d_u1,
@@ -566,9 +587,10 @@ int Batch::build_sigma2(int only_ish) {
   BTRY(a_rows.grow(sizeof(double) * (tfw + tbk + crow + 16)));
   d_rows_fw = (double *)a_rows.ptr;
   d_rows_bk = d_rows_fw + tfw;
-  // (merging needs every (a, b, g) triple's J blocks in registers)
-  d_crows_fw = par.lanczos && merge_ok ? d_rows_bk + tbk : nullptr;
-  d_crows_bk = par.lanczos && merge_ok ? d_crows_fw + tfw : nullptr;
+  // (merging needs every (a, b, g) triple's J blocks in LDS; channels with
+  // longer triples keep their rows, ChanDev::merge)
+  d_crows_fw = par.lanczos ? d_rows_bk + tbk : nullptr;
+  d_crows_bk = par.lanczos ? d_crows_fw + tfw : nullptr;
   if (par.lanczos) BTRY(a_lws.grow(sizeof(double) * (tws + 16)));
   HTRY(hipMemcpyAsync(d_row_fw_base, bfw.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice, stream));
   HTRY(hipMemcpyAsync(d_row_bk_base, bbk.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice, stream));
@@ -709,8 +731,7 @@ int Batch::lanczos() {
   // on its stream right after its Lanczos jobs (they need nothing else), so
   // that they overlap the other buckets' reductions
   const PoleCfg pc = pole_cfg();
-  poles_done = fork && !profile && par.dyson_solver == 0 && pc.wave &&
-               tridiag_lds(pc.lmax, false) <= (size_t)kLdsBytes && (pc.in_lds || a_mat.ptr);
+  poles_done = fork && !profile && par.dyson_solver == 0;
   size_t k0 = 0;
   int q = 0;
   while (k0 < jm.size()) {
@@ -749,61 +770,66 @@ int Batch::lanczos() {
   return BCD_OK;
 }
 
-// Launch parameters of the pole-block kernels for this batch: the largest
-// reduced side, and whether the one-wave kernels (k_tridiag_poles,
-// k_ql_poles) take every side.
+// Launch parameters of the pole-block kernels for this batch.  Each reduced
+// side takes the one-wave path or the general one by its own size
+// (pole_path), so both may run in one batch.
 Batch::PoleCfg Batch::pole_cfg() const {
   PoleCfg c;
-  for (const ChanDev &C : h_chans) c.lmax = std::max(c.lmax, std::max(C.lfw_cap, C.lbk_cap));
-  c.ld = c.lmax | 1;
-  c.wlds = sizeof(double) * ((size_t)c.ld * c.ld + 128);
+  for (const ChanDev &C : h_chans)
+    for (int side = 0; side < 2; ++side) {
+      const int cap = side ? C.lbk_cap : C.lfw_cap;
+      if (cap <= 0) continue;
+      c.lmax = std::max(c.lmax, cap);
+      if (pole_path(cap, C.nI) == 0) { c.any_wave = true; c.lw = std::max(c.lw, cap); }
+      else c.any_general = true;
+    }
+  c.ldw = c.lw | 1;
+  c.wlds = sizeof(double) * ((size_t)c.ldw * c.ldw + 128);
   c.in_lds = tridiag_lds(c.lmax, true) <= (size_t)kLdsBytes;
-  c.wave = c.lmax <= 64 && nImax <= kQLGroupNI;
   return c;
 }
 
-// The pole-block kernels of the njobs sides order[0..njobs) on stream s; the
-// one-wave kernels only (pole_cfg().wave).
+// Prepares the pole-block jobs of the njobs sides order[0..njobs) and runs
+// the one-wave path on those it takes (k_tridiag_poles, k_ql_poles) on
+// stream s; the general path runs in general_poles().
 void Batch::launch_poles(const PoleCfg &c, const int *order, int njobs, hipStream_t s) {
   hipLaunchKernelGGL(k_poles_prep, dim3((njobs + 127) / 128), dim3(128), 0, s, d_chans, n_gc, d_cs, d_lrows, d_tri,
                      c.in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs, order, njobs);
+  if (!c.any_wave) return;
   if (nImax <= 4)
-    hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ld);
+    hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
   else
-    hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ld);
-  hipLaunchKernelGGL(k_ql_poles, dim3((njobs + 7) / 8), dim3(64), (size_t)8 * (2 + nImax) * c.lmax * sizeof(double),
-                     s, d_pjobs, order, njobs, d_status, c.lmax, nImax);
+    hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
+  hipLaunchKernelGGL(k_ql_poles, dim3((njobs + 7) / 8), dim3(64),
+                     (size_t)8 * (2 + std::min(nImax, kQLGroupNI)) * c.lw * sizeof(double), s, d_pjobs, order, njobs,
+                     d_status, c.lw, std::min(nImax, kQLGroupNI));
+}
+
+// The general path (k_tridiag + k_ql) over every prepared side it takes.
+void Batch::general_poles(const PoleCfg &c, hipStream_t s) {
+  if (!c.any_general) return;
+  if (c.in_lds)
+    hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(c.lmax, true), s, d_pjobs, 2 * n_gc);
+  else
+    hipLaunchKernelGGL(k_tridiag<false>, dim3(2 * n_gc), dim3(256), tridiag_lds(c.lmax, false), s, d_pjobs,
+                       2 * n_gc);
+  hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * c.lmax * sizeof(double), s, d_pjobs,
+                     2 * n_gc, d_status);
 }
 
 // Diagonalise every Lanczos-reduced side once (its fishbone block and
 // couplings are fixed for the whole sc0 stage): pole energies and rotated
-// couplings for the arrowhead sweeps.  Nothing to do when lanczos() already
-// ran them bucket by bucket.
+// couplings for the arrowhead sweeps.  The one-wave sides are done already
+// when lanczos() ran them bucket by bucket.
 int Batch::decompose_poles() {
-  if (poles_done) return BCD_OK;
   const PoleCfg c = pole_cfg();
-  if (tridiag_lds(c.lmax, false) > (size_t)kLdsBytes || (!c.in_lds && !a_mat.ptr)) {
+  if (tridiag_lds(c.lmax, false) > (size_t)kLdsBytes || (!c.in_lds && c.any_general && !a_mat.ptr)) {
     note_error("pole blocks: a reduced side of " + std::to_string(c.lmax) + " rows does not fit the kernels");
     return BCD_ERR_CAPACITY;
   }
   kt_begin(KT_POLES);
-  if (c.wave) {
-    launch_poles(c, nullptr, 2 * n_gc, stream);
-  } else {
-    hipLaunchKernelGGL(k_poles_prep, dim3((2 * n_gc + 127) / 128), dim3(128), 0, stream, d_chans, n_gc, d_cs,
-                       d_lrows, d_tri, c.in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs, nullptr, 2 * n_gc);
-    if (c.lmax <= 64 && nImax <= 16)
-      hipLaunchKernelGGL(k_tridiag_poles<16>, dim3(2 * n_gc), dim3(64), c.wlds, stream, d_pjobs, nullptr, 2 * n_gc,
-                         c.ld);
-    else if (c.in_lds)
-      hipLaunchKernelGGL(k_tridiag<true>, dim3(2 * n_gc), dim3(256), tridiag_lds(c.lmax, true), stream, d_pjobs,
-                         2 * n_gc);
-    else
-      hipLaunchKernelGGL(k_tridiag<false>, dim3(2 * n_gc), dim3(256), tridiag_lds(c.lmax, false), stream, d_pjobs,
-                         2 * n_gc);
-    hipLaunchKernelGGL(k_ql, dim3(2 * n_gc), dim3(64), (size_t)(2 + nImax) * c.lmax * sizeof(double), stream,
-                       d_pjobs, 2 * n_gc, d_status);
-  }
+  if (!poles_done) launch_poles(c, nullptr, 2 * n_gc, stream);
+  general_poles(c, stream);
   kt_end(KT_POLES);
   HTRY(hipGetLastError());
   return BCD_OK;

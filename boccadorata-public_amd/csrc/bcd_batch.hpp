@@ -40,10 +40,13 @@ constexpr int kLancAux = 3;
 
 struct Batch {
   struct PoleCfg {
-    int lmax = 1, ld = 1;
-    size_t wlds = 0;
-    bool in_lds = false, wave = false;
+    int lmax = 1;                 // largest reduced side (rows)
+    int lw = 1, ldw = 1;          // largest side of the one-wave path and its LDS leading dimension
+    size_t wlds = 0;              // LDS of k_tridiag_poles
+    bool in_lds = false;          // the general path's largest side fits LDS
+    bool any_wave = false, any_general = false;
   };
+  std::vector<int> arrow_classes;   // block sizes (arrow_threads) present among the channels
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t lstream[kLancAux] = {};
@@ -83,7 +86,6 @@ struct Batch {
   Sig2Work *d_work_list = nullptr;
   int *d_trip_start = nullptr, *d_trip_blk = nullptr;
   double *d_crows_fw = nullptr, *d_crows_bk = nullptr;   // J-merged pole rows (Lanczos path)
-  bool merge_ok = true;                              // every triple has at most kMergeMaxJ J blocks
   int *d_status = nullptr;
   double *d_work = nullptr;                          // arrowhead FLOP counter of the current solve
   unsigned long long *d_prof = nullptr;              // arrowhead phase timers (profiling)
@@ -141,6 +143,7 @@ struct Batch {
   int pull_cs();
   PoleCfg pole_cfg() const;
   void launch_poles(const PoleCfg &c, const int *order, int njobs, hipStream_t s);
+  void general_poles(const PoleCfg &c, hipStream_t s);
   int solve();
   int result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep, double *zp, double *fqp,
              double *eh, double *zh, double *fqh);

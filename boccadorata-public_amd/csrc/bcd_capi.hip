@@ -122,10 +122,10 @@ struct DevBuf {
   int alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 8) == hipSuccess ? BCD_OK : BCD_ERR_NOMEM; }
 };
 
-// One Lanczos job on device data.  Host-side checks follow Reduce_Mtx_Lncz
-// (Lanczos.cpp:50-82) and return its error codes instead of exiting.
-int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout, int *i_piv_out, int NA_in,
-                int NB_in, int NPLS_in, const double *d_Min, int nt_pvts, double *pivots, int *n_itr) {
+// Argument checks of Reduce_Mtx_Lncz (Lanczos.cpp:50-82), returning its
+// error codes instead of exiting; *nLanc is the number of vectors to build.
+int lanczos_checks(int NA_out, int NB_out, int NPLS_out, int NB_in, int NPLS_in, int nt_pvts, const int *n_itr,
+                   int *nLanc_out) {
   if (NPLS_in < 1 || NPLS_out < 1) return -1;
   if (NB_out != NB_in) return -100;
   if (NB_in > kMaxNB || NB_in < 1 || nt_pvts > kMaxPiv || nt_pvts < 1) return BCD_ERR_ARG;
@@ -136,37 +136,66 @@ int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout
   if (nLanc > NPLS_in) nLanc = NPLS_in;
   if (nLanc > NPLS_out) return -101;
   if (nact + 1 > NA_out && nLanc < NPLS_in) return -102;
-  const int Tout = NA_out + NB_out;
-  DevBuf dpiv, ditr, dout, dipv, dn, dV, djob;
-  BTRY(dpiv.alloc(sizeof(double) * nt_pvts * NB_in));
-  BTRY(ditr.alloc(sizeof(int) * nt_pvts));
-  BTRY(dout.alloc(sizeof(double) * (size_t)Tout * nLanc));
-  BTRY(dipv.alloc(sizeof(int) * (nt_pvts + 2)));
-  BTRY(dn.alloc(sizeof(int)));
-  BTRY(dV.alloc(sizeof(double) * (size_t)lanczos_ws_vectors(nLanc, NB_in) * lanczos_stride(NPLS_in)));
-  BTRY(djob.alloc(sizeof(LancJob)));
-  HTRY(hipMemcpy(dpiv.p, pivots, sizeof(double) * nt_pvts * NB_in, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(ditr.p, n_itr, sizeof(int) * nt_pvts, hipMemcpyHostToDevice));
+  if (lanczos_bucket(NPLS_in) < 0) return BCD_ERR_CAPACITY;
+  *nLanc_out = nLanc;
+  return BCD_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Device layout of one Reduce_Mtx_Lncz call inside one region: the input rows
+// (when they come from the host), pivots, iteration counts, output rows,
+// i_piv, the row count, the job and the Krylov workspace.
+struct LancLayout {
+  size_t min, piv, itr, out, ipv, n, job, V, total;
+};
+
+LancLayout lanczos_layout(bool with_min, int NPLS_in, int Tin, int nt_pvts, int NB_in, int Tout, int nLanc) {
+  LancLayout L;
+  size_t o = 0;
+  L.min = o; o += with_min ? align256(sizeof(double) * (size_t)NPLS_in * Tin) : 0;
+  L.piv = o; o += align256(sizeof(double) * (size_t)nt_pvts * NB_in);
+  L.itr = o; o += align256(sizeof(int) * (size_t)nt_pvts);
+  L.out = o; o += align256(sizeof(double) * (size_t)Tout * nLanc);
+  L.ipv = o; o += align256(sizeof(int) * (size_t)(nt_pvts + 2));
+  L.n = o; o += align256(sizeof(int));
+  L.job = o; o += align256(sizeof(LancJob));
+  L.V = o; o += align256(sizeof(double) * (size_t)lanczos_ws_vectors(nLanc, NB_in) * lanczos_stride(NPLS_in));
+  L.total = o;
+  return L;
+}
+
+// One Lanczos job with every device buffer in the region at base (laid out
+// by lanczos_layout).  The input rows are d_Min on the device, or h_Min
+// copied into the region.  Allocates nothing.
+int lanczos_in_region(bcd_ctx *ctx, char *base, const LancLayout &L, int nLanc, int NA_out, int NB_out, double *Mout,
+                      int *i_piv_out, int NA_in, int NB_in, int NPLS_in, const double *h_Min, const double *d_Min,
+                      int nt_pvts, double *pivots, int *n_itr) {
+  const int Tout = NA_out + NB_out, Tin = NA_in + NB_in;
+  if (h_Min) {
+    HTRY(hipMemcpyAsync(base + L.min, h_Min, sizeof(double) * (size_t)NPLS_in * Tin, hipMemcpyHostToDevice,
+                        ctx->stream));
+    d_Min = (const double *)(base + L.min);
+  }
+  HTRY(hipMemcpyAsync(base + L.piv, pivots, sizeof(double) * nt_pvts * NB_in, hipMemcpyHostToDevice, ctx->stream));
+  HTRY(hipMemcpyAsync(base + L.itr, n_itr, sizeof(int) * nt_pvts, hipMemcpyHostToDevice, ctx->stream));
   LancJob J;
   std::memset(&J, 0, sizeof J);
-  J.Min = d_Min; J.Tin = NA_in + NB_in; J.NA_in = NA_in; J.NB = NB_in; J.M = NPLS_in;
-  J.piv = (const double *)dpiv.p; J.nt_pvts = nt_pvts; J.nLanc = nLanc; J.itr = (const int *)ditr.p;
-  J.Mout = (double *)dout.p; J.Tout = Tout; J.NA_out = NA_out; J.i_piv_out = (int *)dipv.p;
-  J.n_out = (int *)dn.p; J.V = (double *)dV.p; J.inst = -1;
-  HTRY(hipMemcpy(djob.p, &J, sizeof J, hipMemcpyHostToDevice));
-  const LancJob *jp = (const LancJob *)djob.p;
-  const int bk = lanczos_bucket(NPLS_in);
-  if (bk < 0) return BCD_ERR_CAPACITY;
-  launch_lanczos(kLancDefault[bk], 1, jp, nullptr, ctx->stream);
+  J.Min = d_Min; J.Tin = Tin; J.NA_in = NA_in; J.NB = NB_in; J.M = NPLS_in;
+  J.piv = (const double *)(base + L.piv); J.nt_pvts = nt_pvts; J.nLanc = nLanc; J.itr = (const int *)(base + L.itr);
+  J.Mout = (double *)(base + L.out); J.Tout = Tout; J.NA_out = NA_out; J.i_piv_out = (int *)(base + L.ipv);
+  J.n_out = (int *)(base + L.n); J.V = (double *)(base + L.V); J.inst = -1;
+  HTRY(hipMemcpyAsync(base + L.job, &J, sizeof J, hipMemcpyHostToDevice, ctx->stream));
+  launch_lanczos(kLancDefault[lanczos_bucket(NPLS_in)], 1, (const LancJob *)(base + L.job), nullptr, ctx->stream);
   HTRY(hipGetLastError());
-  HTRY(hipStreamSynchronize(ctx->stream));
   int nout = 0;
-  HTRY(hipMemcpy(&nout, dn.p, sizeof(int), hipMemcpyDeviceToHost));
+  HTRY(hipMemcpyAsync(&nout, base + L.n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HTRY(hipStreamSynchronize(ctx->stream));
   if (nout < 0) return nout;
-  HTRY(hipMemcpy(Mout, dout.p, sizeof(double) * (size_t)Tout * nLanc, hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(Mout, base + L.out, sizeof(double) * (size_t)Tout * nLanc, hipMemcpyDeviceToHost));
   int npv = 0;
-  HTRY(hipMemcpy(&npv, dipv.p, sizeof(int), hipMemcpyDeviceToHost));
-  HTRY(hipMemcpy(i_piv_out, dipv.p, sizeof(int) * (npv + 2), hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(&npv, base + L.ipv, sizeof(int), hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(i_piv_out, base + L.ipv, sizeof(int) * (npv + 2), hipMemcpyDeviceToHost));
   // the reference normalises every pivot it starts from in place
   int ip = -1;
   for (int k = 0; k < npv; ++k) {
@@ -178,6 +207,20 @@ int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout
     for (int l = 0; l < NB_in; ++l) pivots[ip * NB_in + l] /= x1;
   }
   return nout;
+}
+
+// One Lanczos job on device data (the rows in d_Min), buffers allocated for
+// the call.
+int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout, int *i_piv_out, int NA_in,
+                int NB_in, int NPLS_in, const double *d_Min, int nt_pvts, double *pivots, int *n_itr) {
+  int nLanc = 0;
+  const int rc = lanczos_checks(NA_out, NB_out, NPLS_out, NB_in, NPLS_in, nt_pvts, n_itr, &nLanc);
+  if (rc != BCD_OK) return rc;
+  const LancLayout L = lanczos_layout(false, NPLS_in, NA_in + NB_in, nt_pvts, NB_in, NA_out + NB_out, nLanc);
+  DevBuf region;
+  BTRY(region.alloc(L.total));
+  return lanczos_in_region(ctx, (char *)region.p, L, nLanc, NA_out, NB_out, Mout, i_piv_out, NA_in, NB_in, NPLS_in,
+                           nullptr, d_Min, nt_pvts, pivots, n_itr);
 }
 
 }  // namespace
@@ -426,7 +469,7 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
     if (npv[s] && ns[s] > 0) {
       DysonJob &J = P[njob];
       std::memset(&J, 0, sizeof J);
-      J.nI = n_orb; J.j2 = j2; J.n = ns[s]; J.inst = -1; J.poles = 1;
+      J.nI = n_orb; J.j2 = j2; J.n = ns[s]; J.inst = -1; J.poles = 1; J.path = 1;   // the general kernels below
       J.nfw = ns[s]; J.den_fw = J.den_bk = dens[s]; J.npiv_fw = npv[s];
       J.rfw = rows[s]; J.ipiv_fw = ipv[s];
       J.tri = (double *)dtri.p + (size_t)njob * lside * (2 + n_orb);
@@ -733,6 +776,14 @@ int bcd_batch_lanczos_shapes(bcd_batch *b, int *shapes, float *tune_ms, int n) {
     if (tune_ms)
       for (int sh = 0; sh < kNLancShapes; ++sh) tune_ms[k * kNLancShapes + sh] = b->b.lanc_tune_ms[k][sh];
   }
+  return BCD_OK;
+}
+
+int bcd_batch_set_lanczos_shapes(bcd_batch *b, const int *shapes, int n) {
+  if (!b || !shapes || n < kNLancBuckets) return BCD_ERR_ARG;
+  for (int k = 0; k < kNLancBuckets; ++k)
+    if (!lanczos_shape_fits(shapes[k], kLancBucket[k])) return BCD_ERR_ARG;
+  for (int k = 0; k < kNLancBuckets; ++k) b->b.lanc_shape[k] = shapes[k];
   return BCD_OK;
 }
 

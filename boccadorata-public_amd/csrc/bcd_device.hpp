@@ -70,6 +70,8 @@ struct ChanDev {
   int trip_off, ntrip;           // (a, b, g) triples of the Sigma(2) blocks: starts in the triple table
   int lanc_n;                    // Lanczos length of the channel's instance (read with lanc_kind)
   int ef_set;                    // 1: the Fermi energy of this subshell is forced to ef_force
+  int merge;                     // 1: every (a, b, g) triple has at most kMergeMaxJ J blocks (k_merge_poles)
+  int _pad2;
   double ef_force;               // (Set_external_EFermi, DysonMatrix.cpp:601-617)
 };
 
@@ -157,7 +159,8 @@ struct LancJob {
 struct DysonJob {
   int nI, j2, nfw, nbk, den_fw, den_bk, npiv_fw, npiv_bk;
   int n, inst;                   // dimension; gating instance (-1: none)
-  int ch, _pad;
+  int ch;
+  int path;                      // pole blocks: 0 one-wave kernels (L <= 64, nI <= 8), 1 the general ones
   double efermi;                 // used when inst < 0
   const double *h, *u1;          // static part = u1 + h, both nI x nI
   const double *rfw, *rbk;       // pole rows

@@ -678,6 +678,12 @@ __global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n
   if (job >= 2 * n_gc || !crows_fw) return;
   const int gc = job >> 1, side = job & 1, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const ChanDev C = chans[gc];
+  if (!C.merge) {
+    // a triple with more J blocks than the LDS tables hold: the rows stay
+    // as built (no merged count, so the Lanczos job reads the built rows)
+    if (tid == 0) (side ? cs[gc].cbk : cs[gc].cfw) = 0;
+    return;
+  }
   const InstDev I = insts[C.inst];
   const int cur = prop_cur(P, C.inst), nI = C.nI, w = 2 + nI;
   const double *rows = side ? rows_bk + row_bk_base[gc] : rows_fw + row_fw_base[gc];
@@ -1264,7 +1270,7 @@ __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs
   if ((int)blockIdx.x >= njobs) return;
   const DysonJob J = jobs[blockIdx.x];
   const int n = J.n;
-  if (n <= 0) return;
+  if (n <= 0 || (J.poles && J.path == 0)) return;
   const int nI = J.nI, tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
   double *A = IN_LDS ? sm : J.mat;
   double *aux = IN_LDS ? sm + (long)n * n : sm;
@@ -1387,7 +1393,7 @@ __global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, cons
   if ((int)blockIdx.x >= njobs) return;
   const DysonJob J = jobs[order ? order[blockIdx.x] : blockIdx.x];
   const int n = J.n, nI = J.nI, j = threadIdx.x;
-  if (n <= 0 || n > 64 || n > ld || nI > NIMAX || !J.poles) return;
+  if (n <= 0 || n > 64 || n > ld || nI > NIMAX || !J.poles || J.path != 0) return;
   double *A = sm, *v = A + (long)ld * ld, *wv = v + 64;
   double q[NIMAX];
 #pragma unroll
@@ -1560,7 +1566,7 @@ __global__ __launch_bounds__(64) void k_ql(const DysonJob *jobs, int njobs, int 
   if ((int)blockIdx.x >= njobs) return;
   const DysonJob J = jobs[blockIdx.x];
   const int n = J.n;
-  if (n <= 0) return;
+  if (n <= 0 || (J.poles && J.path == 0)) return;
   const int nI = J.nI, lane = threadIdx.x;
   double *d = sm, *e = d + n, *Q = e + n;
   for (int i = lane; i < n; i += 64) { d[i] = J.tri[i]; e[i] = J.tri[n + i]; }
@@ -1676,7 +1682,7 @@ __global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, const int
   const double eps = 2.2204460492503131e-16;
   int n = 0, nI = 0;
   const DysonJob *J = job < njobs ? jobs + (order ? order[job] : job) : nullptr;
-  if (J) { n = J->n; nI = J->nI; }
+  if (J) { n = J->path == 0 ? J->n : 0; nI = J->nI; }
   if (n > ld || nI > qrows || nI > kQLGroupNI) {
     // does not fit this kernel's slices (the host never launches such jobs here)
     if (gl == 0 && n > 0) atomicAdd(status, 1);

@@ -127,11 +127,24 @@ struct ArrowSweep {
   unsigned long long *prof;      // 8 phase-time counters per channel (may be null)
   double *warm;                  // previous sweep's roots per channel (ChanDev::warm_off; may be null)
   int *warm_p;                   // their pole counts, kArrowNI per channel
+  int skip_ni;                   // channels with nI <= skip_ni are left to k_dyson_static (0: none)
+  int nt_class;                  // > 0: only channels whose arrow_threads(cap) equals it (one launch per class)
 };
+// Threads of the arrowhead block of a channel with cap propagator slots.
+// The block size sets the order of the kernel's reductions, so it depends on
+// the channel alone: a channel gives the same bits in any batch.
+__host__ __device__ inline int arrow_threads(int cap) { return cap <= 64 ? 64 : (cap <= 128 ? 128 : kArrowMaxThreads); }
+// Pole block of a reduced side with room for cap rows: the one-wave kernels
+// (k_tridiag_poles, k_ql_poles) when it has at most 64 rows and the channel
+// at most kQLGroupNI orbits, the general ones otherwise -- chosen per side,
+// not per batch, for the same reason.
+__host__ __device__ inline int pole_path(int cap, int nI) { return cap <= 64 && nI <= 8 ? 0 : 1; }
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow_sweep(ArrowSweep a);
-// A sweep without dynamic self-energy when every nI <= kStaticNI: one thread
-// per channel solves its static Dyson matrix (static_dyson).
+// A sweep without dynamic self-energy: one thread per channel with nI <=
+// kStaticNI solves its static Dyson matrix (static_dyson); the arrowhead
+// kernel takes the larger channels of the same sweep, so every channel's
+// path depends on its own size only, not on the rest of the batch.
 __global__ void k_dyson_static(ArrowSweep a);
 // Launch one arrowhead kernel variant: the workspace in LDS or HBM, the
 // accumulator bound from nimax (arrow_ni_bound).

@@ -257,6 +257,11 @@ int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches);
  * all channels and sides and [16] the number the Lanczos jobs ran on (fewer
  * where J-degenerate pole groups were merged). */
 int bcd_batch_work(bcd_batch *b, double *work, int n);
+/* Set the Lanczos tile shape of every bucket (n >= 10 shape indices, each
+ * holding its bucket's largest pole count).  Shapes split the block sums
+ * differently, so a second solve with other shapes shows how far rounding
+ * alone moves each solution. */
+int bcd_batch_set_lanczos_shapes(bcd_batch *b, const int *shapes, int n);
 /* Tile-shape autotuner of the Lanczos kernel (no counterpart in the
  * reference, which runs one Lanczos at a time on the CPU).  Jobs are grouped
  * by pole count into 10 buckets (M <= 256, 512, 1024, 2048, 4096, 8192, 16384,
