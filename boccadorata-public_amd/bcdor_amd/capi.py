@@ -49,7 +49,7 @@ class BcdSolveParams(C.Structure):
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
            "bcd_count_poles", "bcd_sigma2_build", "bcd_hf_pot", "bcd_reduce_mtx_lncz", "bcd_reduce_selfen_lncz",
            "bcd_dyson", "bcd_syevd_batched", "bcd_batch_create", "bcd_batch_create_ex", "bcd_batch_set_efermi",
-           "bcd_pivot_round", "bcd_batch_set_params", "bcd_batch_error_detail", "bcd_batch_begin", "bcd_batch_add", "bcd_batch_end", "bcd_batch_destroy",
+           "bcd_pivot_round", "bcd_reduce_mtx_lncz_noalloc", "bcd_noalloc_scratch_bytes", "bcd_batch_set_params", "bcd_batch_error_detail", "bcd_batch_begin", "bcd_batch_add", "bcd_batch_end", "bcd_batch_destroy",
            "bcd_batch_solve",
            "bcd_batch_set_profile", "bcd_batch_timings", "bcd_batch_stats", "bcd_batch_converged", "bcd_batch_result",
            "bcd_batch_kernel_times", "bcd_batch_work", "bcd_batch_set_lanczos_shapes", "bcd_batch_autotune", "bcd_batch_lanczos_shapes",
@@ -93,6 +93,9 @@ def lib():
         L.bcd_hf_pot.argtypes = [vp, C.c_int, C.POINTER(BcdProp), DP, C.c_int]
         L.bcd_reduce_mtx_lncz.argtypes = [vp, C.c_int, C.c_int, C.c_int, DP, IP, C.c_int, C.c_int, C.c_int, DP,
                                           C.c_int, DP, IP]
+        L.bcd_reduce_mtx_lncz_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, DP, IP, C.c_int, C.c_int, C.c_int,
+                                                  DP, C.c_int, DP, IP]
+        L.bcd_noalloc_scratch_bytes.restype = C.c_long
         L.bcd_reduce_selfen_lncz.argtypes = [vp, C.c_int, C.c_int, DP, C.c_int, DP, C.c_int, DP, IP, IP, DP, IP,
                                              IP, DP, IP, IP, IP]
         L.bcd_dyson.argtypes = [vp, C.c_int, C.c_int, DP, DP, C.c_int, C.c_int, DP, IP, C.c_int, C.c_int,
@@ -381,14 +384,16 @@ class Context(_TwoBodyMixin):
         check(lib().bcd_hf_pot(self.h, ish, C.byref(prop_struct(g, self.ms.nspb, keep)), dp(u), n), "bcd_hf_pot")
         return u.reshape(n, n)
 
-    def reduce_mtx_lncz(self, NA_out, NB_out, NPLS_out, Min, NA_in, NB_in, pivots, n_itr):
-        """Reduce_Mtx_Lncz: (return value, Mout rows, i_piv, normalised pivots)."""
+    def reduce_mtx_lncz(self, NA_out, NB_out, NPLS_out, Min, NA_in, NB_in, pivots, n_itr, noalloc=False):
+        """Reduce_Mtx_Lncz: (return value, Mout rows, i_piv, normalised pivots).
+        noalloc=True calls bcd_reduce_mtx_lncz_noalloc (no device allocation)."""
         Min = _d(Min)
         piv = _d(pivots).copy().reshape(-1)
         itr = _i(n_itr).copy()
         out = np.zeros(max(NPLS_out, 1) * (NA_out + NB_out))
         ipv = np.zeros(len(itr) + 2, np.int32)
-        r = lib().bcd_reduce_mtx_lncz(self.h, NA_out, NB_out, NPLS_out, dp(out), ip(ipv), NA_in, NB_in,
+        fn = lib().bcd_reduce_mtx_lncz_noalloc if noalloc else lib().bcd_reduce_mtx_lncz
+        r = fn(self.h, NA_out, NB_out, NPLS_out, dp(out), ip(ipv), NA_in, NB_in,
                                       Min.shape[0], dp(Min.reshape(-1) if Min.size else np.zeros(1)), len(itr),
                                       dp(piv), ip(itr))
         return r, out[:max(NPLS_out, 0) * (NA_out + NB_out)].reshape(-1, NA_out + NB_out), ipv, \

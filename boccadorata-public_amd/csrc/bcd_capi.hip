@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -223,6 +224,32 @@ int run_lanczos(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout
                            nullptr, d_Min, nt_pvts, pivots, n_itr);
 }
 
+// Scratch of bcd_reduce_mtx_lncz_noalloc: every device buffer of one call
+// lives here (input rows, pivots, counts, output rows, i_piv, the job and the
+// Krylov workspace), so that entry point allocates nothing.  Each device the
+// library runs on holds one copy until the process ends; calls on one device
+// share it.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of this device's scratch, looked up once per device
+// with the device current (the array's name is only a host-side stand-in).
+int noalloc_scratch(int device, char **out) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return BCD_ERR_DEVICE;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BCD_ERR_DEVICE;
+    g_scratch_ptr[device] = (char *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return BCD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -339,6 +366,24 @@ int bcd_reduce_mtx_lncz(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, doub
   return run_lanczos(ctx, NA_out, NB_out, NPLS_out, Mout, i_piv_out, NA_in, NB_in, NPLS_in,
                      (const double *)dmin.p, nt_pvts, pivots, n_itr);
 }
+
+int bcd_reduce_mtx_lncz_noalloc(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout, int *i_piv_out,
+                                int NA_in, int NB_in, int NPLS_in, const double *Min, int nt_pvts, double *pivots,
+                                int *n_itr) {
+  if (!ctx || !Mout || !i_piv_out || !Min || !pivots || !n_itr) return BCD_ERR_ARG;
+  int nLanc = 0;
+  const int rc = lanczos_checks(NA_out, NB_out, NPLS_out, NB_in, NPLS_in, nt_pvts, n_itr, &nLanc);
+  if (rc != BCD_OK) return rc;
+  const LancLayout L = lanczos_layout(true, NPLS_in, NA_in + NB_in, nt_pvts, NB_in, NA_out + NB_out, nLanc);
+  if (L.total > kNoAllocScratchBytes) return BCD_ERR_CAPACITY;
+  HTRY(hipSetDevice(ctx->device));
+  char *base = nullptr;
+  BTRY(noalloc_scratch(ctx->device, &base));
+  return lanczos_in_region(ctx, base, L, nLanc, NA_out, NB_out, Mout, i_piv_out, NA_in, NB_in, NPLS_in, Min, nullptr,
+                           nt_pvts, pivots, n_itr);
+}
+
+long bcd_noalloc_scratch_bytes(void) { return (long)kNoAllocScratchBytes; }
 
 int bcd_reduce_selfen_lncz(bcd_ctx *ctx, int n_orb, int nfw, double *rows_fw, int nbk, double *rows_bk,
                            int nt_pvts, double *pivots, int *n_itr_fw, int *n_itr_bk, double *out_fw,

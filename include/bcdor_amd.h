@@ -95,6 +95,24 @@ int bcd_reduce_mtx_lncz(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, doub
                         int *i_piv_out, int NA_in, int NB_in, int NPLS_in, const double *Min,
                         int nt_pvts, double *pivots, int *n_itr);
 
+/* bcd_reduce_mtx_lncz without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and return codes as
+ * bcd_reduce_mtx_lncz; every device buffer of the call lives in a fixed
+ * scratch region compiled into the library (bcd_noalloc_scratch_bytes(),
+ * 64 MiB), one per device, held until the process ends.  A call whose
+ * buffers exceed it returns BCD_ERR_CAPACITY before any GPU work (the
+ * Krylov workspace takes 8*(n_itr total + NB_in + 8)*NPLS_in bytes, rounded
+ * up to an even NPLS_in; the input rows 8*NPLS_in*(NA_in+NB_in)).  Calls that
+ * overlap on one device share that region and race: issue them from one
+ * host thread per device, one after another. */
+int bcd_reduce_mtx_lncz_noalloc(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, double *Mout,
+                                int *i_piv_out, int NA_in, int NB_in, int NPLS_in, const double *Min,
+                                int nt_pvts, double *pivots, int *n_itr);
+/* Size in bytes of the per-device scratch of bcd_reduce_mtx_lncz_noalloc. */
+long bcd_noalloc_scratch_bytes(void);
+
 /* Wide_self_energy_t::Reduce_SelfEn_Lncz (Self_energy_ext_Op_e.cpp:55-232):
  * sorts the unreduced rows in place (fw ascending, bk descending), rescales
  * n_itr in place when it exceeds the pole count, reduces both sides.  Output
