@@ -191,6 +191,7 @@ __global__ __launch_bounds__(kArrowMaxThreads, 4) void k_arrow_sweep(ArrowSweep 
   if ((int)blockIdx.x >= a.n_gc) return;
   if (a.chans[blockIdx.x].nI <= a.skip_ni) return;
   if (a.nt_class && arrow_threads(a.chans[blockIdx.x].cap) != a.nt_class) return;
+  if (a.nb_class && arrow_ni_bound(a.chans[blockIdx.x].nI) != a.nb_class) return;
   const ArrowJob J = arrow_job(a, blockIdx.x);
   if (J.n <= 0 || J.nI > NIMAX) return;
   arrow_body<NIMAX>(DevBlock{a.status, a.work, a.prof ? a.prof + 8L * blockIdx.x : nullptr}, J,

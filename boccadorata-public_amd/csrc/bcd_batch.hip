@@ -186,11 +186,15 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   n_blk = (long)h_blocks.size();
   mat_len = o_mat;
   arrow_classes.clear();
+  hf_classes.clear();
   for (const ChanDev &C : h_chans) {
-    const int cls = arrow_threads(C.cap);
+    const std::pair<int, int> cls(arrow_threads(C.cap), arrow_ni_bound(C.nI));
     if (std::find(arrow_classes.begin(), arrow_classes.end(), cls) == arrow_classes.end()) arrow_classes.push_back(cls);
+    if (C.nI > kStaticNI && std::find(hf_classes.begin(), hf_classes.end(), cls.second) == hf_classes.end())
+      hf_classes.push_back(cls.second);
   }
   std::sort(arrow_classes.begin(), arrow_classes.end());
+  std::sort(hf_classes.begin(), hf_classes.end());
 
   // ---- warm-start roots of the arrowhead sweeps
   BTRY(a_warm.grow(sizeof(double) * pad8(std::max<long>(o_warm, 1)) + sizeof(int) * kArrowNI * (size_t)n_gc + 64));
@@ -421,26 +425,30 @@ int Batch::sweep(int src) {
     a.prof = profile ? d_prof : nullptr;
     a.warm = d_warm;
     a.warm_p = d_warm_p;
-    const int nmax = src == 0 ? nImax : (int)capmax;
-    const int nt = nmax <= 64 ? 64 : (nmax <= 128 ? 128 : kArrowMaxThreads);
     a.skip_ni = 0;
     a.nt_class = 0;
+    a.nb_class = 0;
     kt_begin(KT_ARROW);
     if (src == 0) {
       // HF sweeps: channels up to kStaticNI orbits by one thread each, the
-      // larger ones (if any) by the arrowhead kernel
+      // larger ones (if any) by the arrowhead kernel, one launch per
+      // accumulator bound
       hipLaunchKernelGGL(k_dyson_static, dim3((n_gc + 63) / 64), dim3(64), 0, stream, a);
       a.skip_ni = kStaticNI;
-      if (nImax > kStaticNI)
-        launch_arrow_sweep(a, nImax, !aws_len, nt, aws_len ? 0 : arrow_ws_bytes(nmax, nImax), stream);
+      for (int nb : hf_classes) {
+        a.nb_class = nb;
+        launch_arrow_sweep(a, nb, !aws_len, 64, aws_len ? 0 : arrow_ws_bytes(nImax, std::min(nb, nImax)), stream);
+      }
     } else {
-      // one launch per block size, each channel in the launch of its own
-      // size class (arrow_threads), so that its reductions do not depend on
-      // the other channels of the batch
-      for (int cls : arrow_classes) {
-        a.nt_class = cls;
-        const int cmax = cls == kArrowMaxThreads ? (int)capmax : std::min<int>(cls, (int)capmax);
-        launch_arrow_sweep(a, nImax, !aws_len, cls, aws_len ? 0 : arrow_ws_bytes(cmax, nImax), stream);
+      // one launch per (block size, accumulator bound) class: each channel
+      // runs in the launch of its own class, so its reductions and its
+      // unrolled code do not depend on the other channels of the batch
+      for (const auto &cls : arrow_classes) {
+        a.nt_class = cls.first;
+        a.nb_class = cls.second;
+        const int cmax = cls.first == kArrowMaxThreads ? (int)capmax : std::min<int>(cls.first, (int)capmax);
+        launch_arrow_sweep(a, cls.second, !aws_len, cls.first, aws_len ? 0 : arrow_ws_bytes(cmax, std::min(cls.second, nImax)),
+                           stream);
       }
     }
     kt_end(KT_ARROW);
@@ -796,9 +804,8 @@ void Batch::launch_poles(const PoleCfg &c, const int *order, int njobs, hipStrea
   hipLaunchKernelGGL(k_poles_prep, dim3((njobs + 127) / 128), dim3(128), 0, s, d_chans, n_gc, d_cs, d_lrows, d_tri,
                      c.in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs, order, njobs);
   if (!c.any_wave) return;
-  if (nImax <= 4)
-    hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
-  else
+  hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
+  if (nImax > 4)
     hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
   hipLaunchKernelGGL(k_ql_poles, dim3((njobs + 7) / 8), dim3(64),
                      (size_t)8 * (2 + std::min(nImax, kQLGroupNI)) * c.lw * sizeof(double), s, d_pjobs, order, njobs,

@@ -46,7 +46,8 @@ struct Batch {
     bool in_lds = false;          // the general path's largest side fits LDS
     bool any_wave = false, any_general = false;
   };
-  std::vector<int> arrow_classes;   // block sizes (arrow_threads) present among the channels
+  std::vector<std::pair<int, int>> arrow_classes;   // (arrow_threads, arrow_ni_bound) present among the channels
+  std::vector<int> hf_classes;      // arrow_ni_bound of the channels above kStaticNI orbits
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t lstream[kLancAux] = {};

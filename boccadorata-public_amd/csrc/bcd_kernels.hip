@@ -1393,7 +1393,9 @@ __global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, cons
   if ((int)blockIdx.x >= njobs) return;
   const DysonJob J = jobs[order ? order[blockIdx.x] : blockIdx.x];
   const int n = J.n, nI = J.nI, j = threadIdx.x;
-  if (n <= 0 || n > 64 || n > ld || nI > NIMAX || !J.poles || J.path != 0) return;
+  // each job in the launch of its own accumulator bound (4 or 8), so that
+  // the unrolled code that runs it does not depend on the batch
+  if (n <= 0 || n > 64 || n > ld || nI > NIMAX || (NIMAX > 4 && nI <= 4) || !J.poles || J.path != 0) return;
   double *A = sm, *v = A + (long)ld * ld, *wv = v + 64;
   double q[NIMAX];
 #pragma unroll

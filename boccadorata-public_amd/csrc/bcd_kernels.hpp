@@ -129,6 +129,7 @@ struct ArrowSweep {
   int *warm_p;                   // their pole counts, kArrowNI per channel
   int skip_ni;                   // channels with nI <= skip_ni are left to k_dyson_static (0: none)
   int nt_class;                  // > 0: only channels whose arrow_threads(cap) equals it (one launch per class)
+  int nb_class;                  // > 0: only channels whose arrow_ni_bound(nI) equals it
 };
 // Threads of the arrowhead block of a channel with cap propagator slots.
 // The block size sets the order of the kernel's reductions, so it depends on
