@@ -43,7 +43,8 @@ class BcdSolveParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("hf_itr_max", C.c_int), ("sc_itr_max", C.c_int), ("lanczos", C.c_int),
                 ("lanc_kind", C.c_int), ("lanc_n", C.c_int), ("conv_kind", C.c_int), ("hf_conv", C.c_double),
                 ("sc_conv", C.c_double), ("t_fac", C.c_double), ("ext_u1", C.c_int), ("dyson_solver", C.c_int),
-                ("precision", C.c_int), ("accel", C.c_int), ("hw", C.c_double), ("piv_round", C.c_int)]
+                ("precision", C.c_int), ("accel", C.c_int), ("hw", C.c_double), ("mix_start", C.c_double),
+                ("piv_round", C.c_int)]
 
 
 EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_create", "bcd_ctx_destroy",
@@ -56,7 +57,14 @@ EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_creat
            "bcd_ladd_count", "bcd_ladd_basis", "bcd_ladd_fill",
            "bcd_ph_count", "bcd_ph_basis", "bcd_ph_fill", "bcd_t2_create", "bcd_t2_destroy", "bcd_t2_count",
            "bcd_t2_info", "bcd_t2_basis", "bcd_t2_build", "bcd_t2_matrix", "bcd_t2_set_profile",
-           "bcd_t2_kernel_times", "bcd_t2_work"]
+           "bcd_t2_kernel_times", "bcd_t2_work", "bcd_mlp_create", "bcd_mlp_destroy", "bcd_mlp_num_params",
+           "bcd_mlp_get_params", "bcd_mlp_set_params", "bcd_mlp_forward", "bcd_mlp_loss_grad", "bcd_mlp_fit",
+           "bcd_batch_set_learned_start", "bcd_batch_set_fallback_tol", "bcd_batch_record_learning",
+           "bcd_batch_num_channels", "bcd_batch_learning_data", "bcd_batch_start_info"]
+
+LEARN_IN, LEARN_OUT, LEARN_NI = 80, 32, 6
+LEARN_D1, LEARN_D2, LEARN_TRI = 53, 24, 21
+LEARN_SCALE = 10.0
 
 KERNEL_NAMES = ["sigma2", "lanczos", "tridiag", "ql", "density", "sigma_inf", "finalize", "wbuild", "sigma2_plan",
                 "arrow", "poles", "merge"]
@@ -134,6 +142,23 @@ def lib():
         for f in (L.bcd_ladd_fill, L.bcd_ph_fill):
             f.argtypes = [vp, PP, C.c_int, C.c_int, C.c_int, C.c_int, IP, IP, DP, C.c_int, IP, IP, DP, C.c_int, DP,
                           C.c_int, DP, C.c_int]
+        FP = C.POINTER(C.c_float)
+        L.bcd_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
+        L.bcd_mlp_destroy.argtypes = [vp]
+        L.bcd_mlp_destroy.restype = None
+        L.bcd_mlp_num_params.argtypes = [vp]
+        L.bcd_mlp_num_params.restype = C.c_long
+        L.bcd_mlp_get_params.argtypes = [vp, FP]
+        L.bcd_mlp_set_params.argtypes = [vp, FP]
+        L.bcd_mlp_forward.argtypes = [vp, C.c_int, FP, FP]
+        L.bcd_mlp_loss_grad.argtypes = [vp, C.c_int, FP, FP, FP, FP]
+        L.bcd_mlp_fit.argtypes = [vp, C.c_int, FP, FP, C.c_int, C.c_int, C.c_float, C.c_ulonglong, FP]
+        L.bcd_batch_set_learned_start.argtypes = [vp, vp]
+        L.bcd_batch_set_fallback_tol.argtypes = [vp, C.c_double]
+        L.bcd_batch_record_learning.argtypes = [vp, C.c_int]
+        L.bcd_batch_num_channels.argtypes = [vp]
+        L.bcd_batch_learning_data.argtypes = [vp, FP, FP, IP, C.c_int]
+        L.bcd_batch_start_info.argtypes = [vp, C.c_int, IP, IP, IP, DP]
         L.bcd_t2_create.argtypes = [vp, PP, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
         L.bcd_t2_destroy.argtypes = [vp]
         L.bcd_t2_destroy.restype = None
@@ -458,14 +483,15 @@ LANC_ALT_SHAPES = [1, 2, 3, 7, 11, 13, 14, 15, 16, 16]
 
 def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1,
                  hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1, dyson_solver=0, precision=0,
-                 accel=0, hw=0.0, piv_round=0) -> BcdSolveParams:
+                 accel=0, hw=0.0, piv_round=0, mix_start=0.0) -> BcdSolveParams:
     """dyson_solver 0: arrowhead updates (default); 1: dense Householder + QL per sweep.
     precision 0: FP64 throughout; 1: mixed (FP32 Sigma(2) coupling contraction).
-    accel 0: the reference's iteration; m = 1..8: Anderson mixing of the density over m iterates.
+    accel 0: the reference's iteration; m = 1..8: Anderson mixing of the density over m iterates,
+    starting once the density residual norm is below mix_start (<= 0: 1e-3).
     t_fac = U1body_fac and hw = htom of the kinetic term (hw * T_HO) * t_fac.
     piv_round 1: pivots rounded as the pivots file carries them (%18.6le)."""
     return BcdSolveParams(mode, hf_itr_max, sc_itr_max, int(lanczos), lanc_kind, lanc_n, conv_kind, hf_conv,
-                          sc_conv, t_fac, ext_u1, dyson_solver, precision, accel, hw, int(piv_round))
+                          sc_conv, t_fac, ext_u1, dyson_solver, precision, accel, hw, mix_start, int(piv_round))
 
 
 def pivot_round(x, device: int = 0) -> np.ndarray:
@@ -474,6 +500,64 @@ def pivot_round(x, device: int = 0) -> np.ndarray:
     out = np.zeros_like(x)
     check(lib().bcd_pivot_round(device, dp(x), dp(out), x.size), "bcd_pivot_round")
     return out
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The learned start's network (bcd_mlp): n_in -> n_hidden -> n_hidden -> n_out
+    with ReLU, BF16 weights on the matrix cores, FP32 accumulation, trained on
+    the device."""
+
+    def __init__(self, n_in=LEARN_IN, n_hidden=256, n_out=LEARN_OUT, seed=0, device: int = 0):
+        h = C.c_void_p()
+        check(lib().bcd_mlp_create(device, n_in, n_hidden, n_out, seed, C.byref(h)), "bcd_mlp_create")
+        self.h = h
+        self.n_in, self.n_hidden, self.n_out = n_in, n_hidden, n_out
+        self.n_params = lib().bcd_mlp_num_params(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bcd_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        check(lib().bcd_mlp_get_params(self.h, _fp(p)), "bcd_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        check(lib().bcd_mlp_set_params(self.h, _fp(p)), "bcd_mlp_set_params")
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        check(lib().bcd_mlp_forward(self.h, x.shape[0], _fp(x), _fp(y)), "bcd_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        check(lib().bcd_mlp_loss_grad(self.h, x.shape[0], _fp(x), _fp(t), _fp(loss), _fp(g)), "bcd_mlp_loss_grad")
+        return float(loss[0]), g
+
+    def fit(self, x, t, epochs=100, batch=1024, lr=1e-3, seed=0) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        hist = np.zeros(max(epochs, 1), np.float32)
+        check(lib().bcd_mlp_fit(self.h, x.shape[0], _fp(x), _fp(t), epochs, batch, lr, seed, _fp(hist)),
+              "bcd_mlp_fit")
+        return hist[:epochs]
 
 
 class Batch:
@@ -605,6 +689,34 @@ class Batch:
                                      dp(zh), dp(fqh)), "bcd_batch_result")
         return Propagator(npa, nha, ep[:nq], zp[:nq], fqp[:nq * ns].reshape(nq, ns), eh[:nh], zh[:nh],
                           fqh[:nh * ns].reshape(nh, ns))
+
+    def set_learned_start(self, net):
+        """Seed the sc0 iteration with the network's predicted densities (None: off)."""
+        self._net = net
+        check(lib().bcd_batch_set_learned_start(self.h, net.h if net is not None else None),
+              "bcd_batch_set_learned_start")
+
+    def set_fallback_tol(self, tol: float):
+        check(lib().bcd_batch_set_fallback_tol(self.h, float(tol)), "bcd_batch_set_fallback_tol")
+
+    def record_learning(self, on: bool = True):
+        check(lib().bcd_batch_record_learning(self.h, int(on)), "bcd_batch_record_learning")
+
+    def learning_data(self):
+        """(inputs [n_chan x 80], targets [n_chan x 32], instance of each channel) of the last solve."""
+        n = check(lib().bcd_batch_num_channels(self.h), "bcd_batch_num_channels")
+        f = np.zeros((n, LEARN_IN), np.float32)
+        t = np.zeros((n, LEARN_OUT), np.float32)
+        ci = np.zeros(n, np.int32)
+        check(lib().bcd_batch_learning_data(self.h, _fp(f), _fp(t), ip(ci), n), "bcd_batch_learning_data")
+        return f, t, ci
+
+    def start_info(self, i):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        d = C.c_double()
+        check(lib().bcd_batch_start_info(self.h, i, C.byref(a), C.byref(b), C.byref(c), C.byref(d)),
+              "bcd_batch_start_info")
+        return dict(seeded=bool(a.value), fell_back=bool(b.value), first_its=c.value, residual=d.value)
 
     def kernel_times(self):
         out = {}

@@ -5,6 +5,8 @@
 // kernel launches over all channels of all instances on one stream; the host
 // only checks the per-instance convergence flags between sweeps.
 #include "bcd_batch.hpp"
+#include "bcd_learn.hpp"
+#include "bcd_mlp.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -48,7 +50,7 @@ Batch::~Batch() {
 
 std::vector<DevArena *> Batch::arenas() {
   return {&a_tab_i, &a_tab_l, &a_tab_d, &a_inst, &a_chan, &a_blocks, &a_work, &a_prop, &a_rows,
-          &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof, &a_warm, &a_mix};
+          &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof, &a_warm, &a_mix, &a_learn};
 }
 
 #define HTRY(x)                                   \
@@ -70,6 +72,13 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   par = p;
   sys = systems;
   n_inst = (int)sys.size();
+  h_inst_feat.assign((size_t)kLearnInstFeat * n_inst, 0.f);
+  for (int i = 0; i < n_inst; ++i) {
+    h_inst_feat[(size_t)kLearnInstFeat * i + 0] = (float)sys[i]->hw_est;
+    h_inst_feat[(size_t)kLearnInstFeat * i + 1] = (float)sys[i]->v_rms;
+    h_inst_feat[(size_t)kLearnInstFeat * i + 2] = (float)sys[i]->nmax_est;
+  }
+  learn_valid = false;
   HTRY(hipSetDevice(device));
   HTRY(hipStreamCreate(&stream));
   for (auto &e : ev) HTRY(hipEventCreate(&e));
@@ -410,8 +419,18 @@ int Batch::sweep(int src) {
   const int gb = (n_gc + 127) / 128;
   kt_begin(KT_DENSITY);
   hipLaunchKernelGGL(k_density, dim3(n_gc), dim3(64), 0, stream, d_insts, d_chans, 0, n_gc, P, d_rho);
-  if (par.accel > 0)
-    hipLaunchKernelGGL(k_mix, dim3(n_inst), dim3(256), 0, stream, d_insts, d_is, d_rho, d_mix, rho_len, par.accel);
+  if (seed_pending) {
+    // second sc0 sweep of a learned start: the predicted densities replace
+    // those of the first sweep's output
+    hipLaunchKernelGGL(k_learn_seed, dim3((n_gc + 63) / 64), dim3(64), 0, stream, d_insts, d_chans, n_gc, d_seed_on,
+                       d_hf_e, d_hf_x, d_hf_n, d_feat, d_pred, d_rho);
+    seed_pending = false;
+  }
+  // density mixing in the sc0 stage only: the HF propagator, and with it
+  // Sigma(2) and its Lanczos reduction, stay those of the plain iteration
+  if (par.accel > 0 && src != 0)
+    hipLaunchKernelGGL(k_mix, dim3(n_inst), dim3(256), 0, stream, d_insts, d_is, d_rho, d_mix, rho_len, par.accel,
+                       par.mix_start > 0.0 ? par.mix_start : kMixStart);
   kt_end(KT_DENSITY);
   kt_begin(KT_SIGINF);
   hipLaunchKernelGGL(k_sigma_inf, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_is, d_w, d_rho, d_h);
@@ -501,8 +520,10 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   // stage start: keep the current buffer, reset the sweep controls
   HTRY(hipMemcpyAsync(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost, stream));
   HTRY(hipStreamSynchronize(stream));
-  for (InstState &s : h_is) {
-    s.active = s.failed ? 0 : 1; s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    InstState &s = h_is[i];
+    s.active = s.failed ? 0 : (stage_mask.empty() || stage_mask[i] ? 1 : 0);
+    s.its = 0; s.itr_max = itr_max; s.conv = conv; s.err = 0;
     s.mix_k = 0; s.mix_s = 0; s.mix_fn = 0.0;
   }
   HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
@@ -519,7 +540,10 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
     // about as much as one read
     const int chunk = (hf && par.dyson_solver == 0) ? 2 * kSweepChunk : kSweepChunk;
     const int k = std::min(chunk, hard_cap - done);
-    for (int q = 0; q < k; ++q) BTRY(sweep(src));
+    for (int q = 0; q < k; ++q) {
+      BTRY(sweep(src));
+      if (learn_hook && done + q == 0) BTRY(after_first_sweep());
+    }
     done += k;
     BTRY(check(n_active));
   }
@@ -937,6 +961,13 @@ int Batch::solve() {
   sc_its.assign(n_inst, 0);
   hf_ok.assign(n_inst, 0);
   sc_ok.assign(n_inst, par.mode == 0 ? 1 : 0);
+  seeded.assign(n_inst, 0);
+  fell_back.assign(n_inst, 0);
+  first_its.assign(n_inst, 0);
+  resid.assign(n_inst, -1.0);
+  learn_valid = false;
+  seed_pending = false;
+  stage_mask.clear();
   HTRY(hipMemsetAsync(d_status, 0, sizeof(int), stream));
   HTRY(hipMemsetAsync(d_work, 0, sizeof(double), stream));
   HTRY(hipMemsetAsync(d_prof, 0, 8 * sizeof(unsigned long long) * n_gc, stream));
@@ -955,11 +986,35 @@ int Batch::solve() {
   HTRY(hipEventRecord(ev[2], stream));
   if (par.mode >= 1) {
     BTRY(build_sigma2(-1));
+    // the learned start's inputs and the HF snapshot its fallback restarts
+    // from, taken while the propagator is the HF one and the unreduced rows
+    // are in place
+    const bool checked = par.mode == 2 && (mlp || par.accel > 0) && fallback_tol > 0.0;
+    const bool learned = mlp && par.mode == 2;
+    if (learn_record || checked || learned) BTRY(learn_features());
+    if (learned) {
+      BTRY(mlp->reserve(n_gc));
+      for (int i = 0; i < n_inst; ++i) seeded[i] = h_is[i].failed ? 0 : 1;
+      HTRY(hipMemcpyAsync(d_seed_on, seeded.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice, stream));
+    }
     HTRY(hipEventRecord(ev[3], stream));
     if (par.lanczos) BTRY(lanczos());
     if (par.lanczos && par.dyson_solver == 0) BTRY(decompose_poles());
     HTRY(hipEventRecord(ev[4], stream));
-    BTRY(stage(par.lanczos ? 2 : 1, par.mode == 1 ? 0 : par.sc_itr_max, par.sc_conv, false));
+    const int src = par.lanczos ? 2 : 1;
+    learn_hook = par.mode == 2 && (learn_record || learned);
+    const int rc = stage(src, par.mode == 1 ? 0 : par.sc_itr_max, par.sc_conv, false);
+    learn_hook = false;
+    seed_pending = false;
+    BTRY(rc);
+    if (checked) BTRY(verify_and_fallback(src));
+    else first_its = sc_its;
+    if (learn_record) {
+      hipLaunchKernelGGL(k_learn_density, dim3(n_gc), dim3(64), 0, stream, d_chans, n_gc, view(true), d_hf_e, d_hf_x,
+                         d_hf_n, d_target, kLearnOut, 0);
+      HTRY(hipGetLastError());
+      learn_valid = true;
+    }
   } else {
     HTRY(hipEventRecord(ev[3], stream));
     HTRY(hipEventRecord(ev[4], stream));
@@ -975,6 +1030,103 @@ int Batch::solve() {
   (void)hipEventElapsedTime(&tot, ev[0], ev[5]);
   stage_ms[5] = tot;
   HTRY(hipMemcpy(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost));
+  return BCD_OK;
+}
+
+int Batch::learn_alloc() {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t g = (size_t)n_gc, n = (size_t)n_inst;
+  const size_t b_feat = al(sizeof(float) * g * kLearnIn), b_out = al(sizeof(float) * g * kLearnOut);
+  const size_t b_if = al(sizeof(float) * n * kLearnInstFeat), b_he = al(sizeof(double) * g * kMaxNI);
+  const size_t b_hx = al(sizeof(double) * g * kMaxNI * kMaxNI), b_hn = al(sizeof(int) * 2 * g);
+  const size_t b_n = al(sizeof(int) * n);
+  const size_t tot = b_feat + 2 * b_out + b_if + b_he + b_hx + b_hn + 2 * b_n;
+  const bool fresh = a_learn.cap < tot || !d_feat;
+  BTRY(a_learn.grow(tot));
+  char *q = (char *)a_learn.ptr;
+  d_feat = (float *)q; q += b_feat;
+  d_target = (float *)q; q += b_out;
+  d_pred = (float *)q; q += b_out;
+  d_inst_feat = (float *)q; q += b_if;
+  d_hf_e = (double *)q; q += b_he;
+  d_hf_x = (double *)q; q += b_hx;
+  d_hf_n = (int *)q; q += b_hn;
+  d_seed_on = (int *)q; q += b_n;
+  d_flag = (int *)q;
+  if (fresh) {
+    HTRY(hipMemcpyAsync(d_inst_feat, h_inst_feat.data(), sizeof(float) * h_inst_feat.size(), hipMemcpyHostToDevice,
+                        stream));
+    HTRY(hipMemsetAsync(d_target, 0, b_out, stream));   // outputs past the first kLearnTri stay 0
+  }
+  return BCD_OK;
+}
+
+// After the first sc0 sweep: its density D1 into the inputs and, with a
+// network, the prediction that the next sweep starts from.
+int Batch::after_first_sweep() {
+  hipLaunchKernelGGL(k_learn_density, dim3(n_gc), dim3(64), 0, stream, d_chans, n_gc, view(true), d_hf_e, d_hf_x,
+                     d_hf_n, d_feat, kLearnIn, kFeatD1);
+  HTRY(hipGetLastError());
+  if (mlp) {
+    BTRY(mlp->forward_dev(n_gc, d_feat, d_pred, stream));
+    seed_pending = true;
+  }
+  return BCD_OK;
+}
+
+int Batch::learn_features() {
+  BTRY(learn_alloc());
+  hipLaunchKernelGGL(k_learn_features, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, n_gc, view(true),
+                     d_inst_feat, d_cs, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_feat, d_hf_e, d_hf_x,
+                     d_hf_n);
+  HTRY(hipGetLastError());
+  return BCD_OK;
+}
+
+// After a seeded or density-mixed sc0 stage: one plain sweep from every
+// instance's result measures its fixed-point residual (the stage's
+// convergence measure of that sweep); instances whose residual exceeds
+// fallback_tol, or that did not converge, are solved again by the plain
+// iteration from their HF propagator.
+int Batch::verify_and_fallback(int src) {
+  const std::vector<int> its0 = sc_its, ok0 = sc_ok;
+  first_its = its0;
+  const int accel = par.accel;
+  par.accel = 0;
+  int rc = stage(src, 0, par.sc_conv, false);
+  par.accel = accel;
+  BTRY(rc);
+  std::vector<int> flag(n_inst, 0);
+  int nfb = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    sc_its[i] = its0[i] + (h_is[i].failed ? 0 : 1);
+    if (h_is[i].failed) { sc_ok[i] = 0; continue; }
+    resid[i] = h_is[i].diff;
+    const bool good = ok0[i] && resid[i] <= fallback_tol;
+    sc_ok[i] = good ? 1 : 0;
+    if (!good) { flag[i] = 1; ++nfb; }
+  }
+  if (!nfb) return BCD_OK;
+  HTRY(hipMemcpyAsync(d_flag, flag.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_learn_restore, dim3(n_gc), dim3(64), 0, stream, d_chans, n_gc, view(true), d_flag, d_hf_e,
+                     d_hf_x, d_hf_n);
+  HTRY(hipGetLastError());
+  const std::vector<int> its1 = sc_its, ok1 = sc_ok;
+  stage_mask.assign(flag.begin(), flag.end());
+  par.accel = 0;
+  rc = stage(src, par.sc_itr_max, par.sc_conv, false);
+  par.accel = accel;
+  stage_mask.clear();
+  BTRY(rc);
+  for (int i = 0; i < n_inst; ++i) {
+    if (flag[i]) {
+      fell_back[i] = 1;
+      sc_its[i] = its1[i] + h_is[i].its;
+    } else {
+      sc_its[i] = its1[i];
+      sc_ok[i] = ok1[i];
+    }
+  }
   return BCD_OK;
 }
 

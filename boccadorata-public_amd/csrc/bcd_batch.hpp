@@ -11,6 +11,8 @@
 
 namespace bcd {
 
+struct Mlp;
+
 constexpr int kLdsBytes = 160 * 1024 - 1024;   // dynamic LDS ceiling used by the Dyson kernels
 constexpr int kMaxSweeps = 2000;
 constexpr int kSweepChunk = 4;                 // sweeps queued between host reads of the convergence flags
@@ -145,6 +147,25 @@ struct Batch {
   PoleCfg pole_cfg() const;
   void launch_poles(const PoleCfg &c, const int *order, int njobs, hipStream_t s);
   void general_poles(const PoleCfg &c, hipStream_t s);
+  // learned start and the fallback to the plain iteration
+  Mlp *mlp = nullptr;                 // network seeding the sc0 iteration (not owned; nullptr: off)
+  bool learn_record = false;          // compute the network's inputs and targets during solve()
+  bool learn_valid = false;           // d_feat / d_target hold the last solve's
+  bool seed_pending = false;          // the next sweep applies the predicted densities
+  double fallback_tol = 1e-6;         // residual above which a seeded or mixed solve is redone plainly
+  DevArena a_learn;
+  float *d_feat = nullptr, *d_target = nullptr, *d_pred = nullptr, *d_inst_feat = nullptr;
+  double *d_hf_e = nullptr, *d_hf_x = nullptr;
+  int *d_hf_n = nullptr, *d_seed_on = nullptr, *d_flag = nullptr;
+  std::vector<float> h_inst_feat;     // per instance: hbar*omega estimate, rms of V, Nmax, 0
+  std::vector<int> seeded, fell_back, first_its;
+  std::vector<double> resid;          // plain-sweep residual after a seeded or mixed stage (-1: not checked)
+  std::vector<char> stage_mask;       // non-empty: the instances the next stage iterates
+  bool learn_hook = false;            // the stage runs after_first_sweep() after its first sweep
+  int after_first_sweep();
+  int learn_alloc();
+  int learn_features();
+  int verify_and_fallback(int src);
   int solve();
   int result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep, double *zp, double *fqp,
              double *eh, double *zh, double *fqh);

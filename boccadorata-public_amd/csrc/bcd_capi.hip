@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "bcd_batch.hpp"
+#include "bcd_mlp.hpp"
 #include "bcd_ctx.hpp"
 #include "bcdor_amd.h"
 
@@ -799,6 +800,51 @@ int bcd_batch_result(bcd_batch *b, int inst, int nspb, int *np, int *nh, int *n_
                      double *zp, double *fqp, double *eh, double *zh, double *fqh) {
   if (!b) return BCD_ERR_ARG;
   return b->b.result(inst, nspb, np, nh, n_qp, n_qh, ep, zp, fqp, eh, zh, fqh);
+}
+
+int bcd_batch_set_learned_start(bcd_batch *b, bcd_mlp *m) {
+  if (!b || !b->ready) return BCD_ERR_ARG;
+  if (m && (m->net.n_in != BCD_LEARN_IN || m->net.n_out != BCD_LEARN_OUT || m->net.device != b->device))
+    return BCD_ERR_ARG;
+  b->b.mlp = m ? &m->net : nullptr;
+  return BCD_OK;
+}
+
+int bcd_batch_set_fallback_tol(bcd_batch *b, double tol) {
+  if (!b || !(tol == tol)) return BCD_ERR_ARG;
+  b->b.fallback_tol = tol;
+  return BCD_OK;
+}
+
+int bcd_batch_record_learning(bcd_batch *b, int on) {
+  if (!b) return BCD_ERR_ARG;
+  b->b.learn_record = on != 0;
+  return BCD_OK;
+}
+
+int bcd_batch_num_channels(const bcd_batch *b) { return (b && b->ready) ? b->b.n_gc : BCD_ERR_ARG; }
+
+int bcd_batch_learning_data(bcd_batch *b, float *feat, float *target, int *chan_inst, int n_chan) {
+  if (!b || !b->ready || n_chan != b->b.n_gc || !b->b.learn_valid) return BCD_ERR_ARG;
+  Batch &B = b->b;
+  HTRY(hipSetDevice(b->device));
+  HTRY(hipStreamSynchronize(B.stream));
+  if (feat) HTRY(hipMemcpy(feat, B.d_feat, sizeof(float) * (size_t)n_chan * BCD_LEARN_IN, hipMemcpyDeviceToHost));
+  if (target)
+    HTRY(hipMemcpy(target, B.d_target, sizeof(float) * (size_t)n_chan * BCD_LEARN_OUT, hipMemcpyDeviceToHost));
+  if (chan_inst)
+    for (int gc = 0; gc < n_chan; ++gc) chan_inst[gc] = B.h_chans[gc].inst;
+  return BCD_OK;
+}
+
+int bcd_batch_start_info(bcd_batch *b, int inst, int *seeded, int *fell_back, int *first_its, double *residual) {
+  if (!b || inst < 0 || inst >= b->b.n_inst || b->b.seeded.empty()) return BCD_ERR_ARG;
+  Batch &B = b->b;
+  if (seeded) *seeded = B.seeded[inst];
+  if (fell_back) *fell_back = B.fell_back[inst];
+  if (first_its) *first_its = B.first_its[inst];
+  if (residual) *residual = B.resid[inst];
+  return BCD_OK;
 }
 
 int bcd_batch_kernel_times(bcd_batch *b, int k, double *avg_ms, long *launches) {

@@ -149,12 +149,12 @@ __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_b
 // over the differences dX_i, dF_i of the last m iterates; the fixed point
 // rho_out(rho) = rho is the same.  One block per instance; hist holds the
 // iterates X and residuals F in rings of m + 1 slots, each slot laid out like
-// rho (rho_tot doubles).  Mixing starts once |f| <= kMixStart: the random
+// rho (rho_tot doubles).  Mixing starts once |f| <= mix_start: the random
 // interactions here give several self-consistent solutions, and the plain
 // iteration decides which one a solve converges to; until then the steps are
 // plain.  A residual that grows tenfold restarts the history.
 __global__ __launch_bounds__(256) void k_mix(const InstDev *insts, InstState *is, double *rho, double *hist,
-                                             long rho_tot, int m) {
+                                             long rho_tot, int m, double mix_start) {
   InstState *S = is + blockIdx.x;
   if (!S->active) return;
   const InstDev I = insts[blockIdx.x];
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_mix(const InstDev *insts, InstState *is
     const double fn = sqrt(G[kNd - 1]);
     // plain steps until the residual is small (the plain iteration picks the
     // fixed point, mixing then converges onto it); restart on growth
-    if (fn > kMixStart || (n > 0 && fn > 10.0 * S->mix_fn)) { s = k - 1; n = 0; }
+    if (fn > mix_start || (n > 0 && fn > 10.0 * S->mix_fn)) { s = k - 1; n = 0; }
     // normal equations (dF^T dF) gamma = dF^T f, Gaussian elimination with
     // partial pivoting; a singular or non-finite system falls back to the
     // plain step
@@ -1885,7 +1885,11 @@ __global__ __launch_bounds__(256) void k_finalize(const InstDev *insts, const Ch
     __syncthreads();
   }
   if (init) {
-    if (tid == 0) { S.its = 0; S.active = S.failed ? 0 : 1; S.diff = 0.0; }
+    // the host has set which instances the stage iterates
+    if (tid == 0) {
+      if (S.failed) S.active = 0;
+      if (S.active) { S.its = 0; S.diff = 0.0; }
+    }
     return;
   }
   // storage-index difference between buffers cur (old) and nxt (new)

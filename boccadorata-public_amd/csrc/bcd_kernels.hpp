@@ -28,14 +28,15 @@ constexpr int lanczos_stash_bytes(int nt, int per) { return 8 * nt * per <= 128 
 // element pairs are 16-byte aligned.
 __host__ __device__ inline int lanczos_stride(int M) { return M + (M & 1); }
 constexpr int kMixMax = 8;        // iterates of the Anderson density mixing
-constexpr double kMixStart = 1e-3;  // density residual norm below which the mixing starts
+constexpr double kMixStart = 1e-3;  // default density residual norm below which the mixing starts
 
 __global__ void k_wbuild(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end, double *W);
 __global__ void k_density(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
                           PropView P, double *rho);
 __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_begin, int gc_end,
                             const InstState *is, const double *W, const double *rho, double *h);
-__global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *hist, long rho_tot, int m);
+__global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *hist, long rho_tot, int m,
+                      double mix_start);
 __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
                               int *blk_bk_off, Sig2Work *work_list, int *work_count, int only_ish);

@@ -39,6 +39,20 @@ int HostSystem::build(const bcd_modspace *ms, const bcd_vpp *vpp) {
   orb_sh.resize(norb);
   for (int i = 0; i < nsubsh; ++i)
     for (int k = 0; k < no[i]; ++k) orb_sh[orb0[i] + k] = i;
+  {
+    double sen = 0.0, snn = 0.0;
+    nmax_est = 0;
+    for (int a = 0; a < norb; ++a) {
+      const int N = 2 * orb_n[a] + l[orb_sh[a]];
+      if (N > nmax_est) nmax_est = N;
+      sen += orb_e[a] * (N + 1.5);
+      snn += (N + 1.5) * (N + 1.5);
+    }
+    hw_est = snn > 0.0 ? sen / snn : 0.0;
+    double sv = 0.0;
+    for (long k = 0; k < (vpp ? vpp->n_me : 0); ++k) sv += vpp->v[k] * vpp->v[k];
+    v_rms = (vpp && vpp->n_me > 0) ? std::sqrt(sv / (double)vpp->n_me) : 0.0;
+  }
   qoff.resize(nsubsh);
   nq = 0;
   for (int i = 0; i < nsubsh; ++i) { qoff[i] = nq; nq += no[i] * no[i]; }

@@ -29,6 +29,12 @@ struct HostSystem {
   int max_vt = 0;                              // largest nI * pairs * n_g of a Sigma(2) block
   int max_dg = 0;                              // largest nI * n_g of a Sigma(2) block
   long dropped = 0;                            // interaction keys outside the pair tables
+  // instance descriptors of the learned start, estimated from the inputs:
+  // hbar*omega as the least-squares slope of the spe column against the
+  // oscillator quantum 2n + l + 3/2, the largest 2n + l, the rms of the
+  // stored interaction elements
+  double hw_est = 0.0, v_rms = 0.0;
+  int nmax_est = 0;
 
   int build(const bcd_modspace *ms, const bcd_vpp *vpp);
   // one-body part U1 of subshell ish: (hw * T_HO) * t_fac, then + diag(spe)

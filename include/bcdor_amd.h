@@ -173,12 +173,17 @@ typedef struct bcd_solve_params {
   int accel;         /* 0: the reference's iteration (the hole density of each
                         sweep's output feeds the next Sigma^inf); m = 1..8:
                         Anderson mixing of that density over the last m
-                        iterates.  Same fixed point and stopping rule, fewer
-                        sweeps; results differ from the plain iteration
-                        within what the convergence tolerance leaves open. */
+                        iterates in the sc0 stage (the HF stage stays
+                        plain).  Same stopping rule, fewer sweeps; where
+                        the interaction admits several self-consistent
+                        solutions the mixed iteration can settle on another
+                        one than the plain iteration does. */
   double hw;         /* htom (hbar*omega) of the kinetic term: U1 = (hw * T_HO) *
                         t_fac, the order of Get_Tab_kin_ho(...) *= U1body_fac
                         (Utilities.cpp:106-124, Main.cpp:266-270)              */
+  double mix_start;  /* accel > 0: the density residual norm below which the
+                        mixing starts (plain sweeps until then, so that the
+                        plain iteration picks the fixed point); <= 0: 1e-3 */
   int piv_round;     /* 1: the Lanczos pivots derived from the HF propagator are
                         rounded as the pivots file carries them (DysPivots
                         prints %18.6le, Make_LancPivots reads it back;
@@ -217,7 +222,7 @@ int bcd_batch_end(bcd_batch *b);
  * terminating 0 to buf; returns the full length. */
 int bcd_batch_error_detail(const bcd_batch *b, char *buf, int n);
 /* Change the iteration settings of a built batch for its later solves:
- * precision, accel, hf/sc ItrMax, conv_kind and the tolerances.  The other
+ * precision, accel, mix_start, hf/sc ItrMax, conv_kind and the tolerances.  The other
  * fields must equal the batch's (they size its device state); otherwise
  * BCD_ERR_ARG. */
 int bcd_batch_set_params(bcd_batch *b, const bcd_solve_params *par);
@@ -292,6 +297,75 @@ int bcd_batch_autotune(bcd_batch *b);
  * timed ms of every (bucket, shape) pair as tune_ms[bucket * 17 + shape]
  * (0 where not timed). */
 int bcd_batch_lanczos_shapes(bcd_batch *b, int *shapes, float *tune_ms, int n);
+
+/* ------------------------------------------------ learned start (network)
+ * No counterpart in the reference, whose ExtSE iteration always starts from
+ * the propagator file it is given (BcDor-Main.cpp:80, SpProp_t propV(...),
+ * Main.cpp:216-451).  A 3-layer perceptron n_in -> n_hidden -> n_hidden ->
+ * n_out (ReLU between layers) runs on the bf16 matrix cores: weights held
+ * as FP32 masters with BF16 copies, activations rounded to BF16 where they
+ * enter a product, FP32 accumulation.  Training (mean-squared loss, Adam,
+ * beta = 0.9 / 0.999, eps = 1e-8) runs on the device with the library's own
+ * forward and backward kernels.  Parameters are laid out as W1 [n_hidden x
+ * n_in], b1, W2 [n_hidden x n_hidden], b2, W3 [n_out x n_hidden], b3.
+ * Calls on one network are not reentrant. */
+typedef struct bcd_mlp bcd_mlp;
+int bcd_mlp_create(int device, int n_in, int n_hidden, int n_out, unsigned long long seed, bcd_mlp **out);
+void bcd_mlp_destroy(bcd_mlp *m);
+long bcd_mlp_num_params(const bcd_mlp *m);
+int bcd_mlp_get_params(bcd_mlp *m, float *p);
+int bcd_mlp_set_params(bcd_mlp *m, const float *p);
+/* y[n x n_out] = net(x[n x n_in]); host arrays. */
+int bcd_mlp_forward(bcd_mlp *m, int n, const float *x, float *y);
+/* loss = mean over n rows and n_out outputs of (net(x) - t)^2 and its
+ * gradient with respect to every parameter (grad[num_params], may be NULL). */
+int bcd_mlp_loss_grad(bcd_mlp *m, int n, const float *x, const float *t, float *loss, float *grad);
+/* epochs passes of Adam over the n samples in shuffled minibatches of batch
+ * rows (seeded), the whole set resident on the device; epoch_loss[epochs]
+ * (may be NULL) receives the mean minibatch loss of every epoch. */
+int bcd_mlp_fit(bcd_mlp *m, int n, const float *x, const float *t, int epochs, int batch, float lr,
+                unsigned long long seed, float *epoch_loss);
+
+/* The network as the start of a batch's sc0 iteration.  The sc0 sweep reads
+ * its input propagator only through the quasihole density rho of every
+ * channel (Sigma^inf, Sp_Propagator_a.cpp:39-130), so the start is a
+ * predicted converged rho.  The first sc0 sweep runs plainly from the HF
+ * propagator; then per channel (subshell of an instance, at most
+ * BCD_LEARN_NI orbits) the network maps BCD_LEARN_IN inputs -- HF energies,
+ * their distance to the Fermi energy, occupations, the second-order estimate
+ * D2 of the density correction from the Sigma(2) rows, the first sweep's
+ * correction D1, estimates of hbar*omega, the interaction scale and Nmax,
+ * and the channel's l, j, charge -- to the remaining step D - D1 of the
+ * converged correction D (rho in the channel's HF orbital basis minus the HF
+ * occupations, upper triangle, times 10, in the first 21 of BCD_LEARN_OUT
+ * outputs), and the second sweep starts from rho = HF occupations + D1 +
+ * that step.  m must have n_in = BCD_LEARN_IN, n_out = BCD_LEARN_OUT and
+ * live on the batch's device; NULL switches the learned start off.  The
+ * batch keeps the pointer. */
+#define BCD_LEARN_IN 80
+#define BCD_LEARN_OUT 32
+#define BCD_LEARN_NI 6
+int bcd_batch_set_learned_start(bcd_batch *b, bcd_mlp *m);
+/* Fallback to the plain iteration after a seeded (learned start) or
+ * density-mixed (accel > 0) sc0 stage: one plain sweep from every result
+ * measures its fixed-point residual (the stage's conv_kind measure); an
+ * instance whose residual exceeds tol, or that did not converge, is solved
+ * again by the plain iteration from its HF propagator.  Default 1e-6; tol <= 0
+ * switches the check off. */
+int bcd_batch_set_fallback_tol(bcd_batch *b, double tol);
+/* Record the network's per-channel inputs and targets (D of the converged
+ * propagator; the network is trained on D - D1) in the following solves. */
+int bcd_batch_record_learning(bcd_batch *b, int on);
+int bcd_batch_num_channels(const bcd_batch *b);
+/* Inputs [n_chan x BCD_LEARN_IN], targets [n_chan x BCD_LEARN_OUT] and the
+ * instance of every channel (any may be NULL) of the last recorded solve;
+ * n_chan must equal bcd_batch_num_channels.  Channels with more than
+ * BCD_LEARN_NI orbits have all-zero rows. */
+int bcd_batch_learning_data(bcd_batch *b, float *feat, float *target, int *chan_inst, int n_chan);
+/* How the last solve of instance i started and ended: seeded by the network,
+ * redone by the plain iteration, sweeps of the seeded or mixed stage before
+ * the residual check, and that residual (-1 when not checked). */
+int bcd_batch_start_info(bcd_batch *b, int inst, int *seeded, int *fell_back, int *first_its, double *residual);
 
 /* ------------------------------------------ two-body propagator matrices
  * The pp/hh ladder of gII_prop_t (gII_b.cpp:62-563) and the ph ring of
