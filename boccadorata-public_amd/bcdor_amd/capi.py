@@ -60,7 +60,7 @@ EXPORTS = ["bcd_version", "bcd_error_string", "bcd_device_count", "bcd_ctx_creat
            "bcd_t2_kernel_times", "bcd_t2_work", "bcd_mlp_create", "bcd_mlp_destroy", "bcd_mlp_num_params",
            "bcd_mlp_get_params", "bcd_mlp_set_params", "bcd_mlp_forward", "bcd_mlp_loss_grad", "bcd_mlp_fit",
            "bcd_batch_set_learned_start", "bcd_batch_set_fallback_tol", "bcd_batch_record_learning",
-           "bcd_batch_num_channels", "bcd_batch_learning_data", "bcd_batch_start_info"]
+           "bcd_batch_num_channels", "bcd_batch_learning_data", "bcd_batch_start_info", "bcd_batch_set_graphs"]
 
 LEARN_IN, LEARN_OUT, LEARN_NI = 80, 32, 6
 LEARN_D1, LEARN_D2, LEARN_TRI = 53, 24, 21
@@ -159,6 +159,7 @@ def lib():
         L.bcd_batch_num_channels.argtypes = [vp]
         L.bcd_batch_learning_data.argtypes = [vp, FP, FP, IP, C.c_int]
         L.bcd_batch_start_info.argtypes = [vp, C.c_int, IP, IP, IP, DP]
+        L.bcd_batch_set_graphs.argtypes = [vp, C.c_int]
         L.bcd_t2_create.argtypes = [vp, PP, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
         L.bcd_t2_destroy.argtypes = [vp]
         L.bcd_t2_destroy.restype = None
@@ -620,6 +621,10 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+    def set_graphs(self, on: bool):
+        """Replay full sweep chunks from a captured HIP graph (default) or launch every sweep."""
+        check(lib().bcd_batch_set_graphs(self.h, int(on)), "bcd_batch_set_graphs")
 
     def set_profile(self, on: bool):
         check(lib().bcd_batch_set_profile(self.h, int(on)), "bcd_batch_set_profile")

@@ -461,13 +461,27 @@ int Batch::sweep(int src) {
     } else {
       // one launch per (block size, accumulator bound) class: each channel
       // runs in the launch of its own class, so its reductions and its
-      // unrolled code do not depend on the other channels of the batch
-      for (const auto &cls : arrow_classes) {
+      // unrolled code do not depend on the other channels of the batch.  The
+      // classes touch disjoint channels and run concurrently, spread over
+      // the batch's stream and the auxiliary ones.
+      const int ncls = (int)arrow_classes.size();
+      const int naux = std::min(kLancAux, ncls - 1);
+      if (naux > 0) {
+        HTRY(hipEventRecord(lfork, stream));
+        for (int k = 0; k < naux; ++k) HTRY(hipStreamWaitEvent(lstream[k], lfork, 0));
+      }
+      for (int c = 0; c < ncls; ++c) {
+        const auto &cls = arrow_classes[c];
         a.nt_class = cls.first;
         a.nb_class = cls.second;
         const int cmax = cls.first == kArrowMaxThreads ? (int)capmax : std::min<int>(cls.first, (int)capmax);
-        launch_arrow_sweep(a, cls.second, !aws_len, cls.first, aws_len ? 0 : arrow_ws_bytes(cmax, std::min(cls.second, nImax)),
-                           stream);
+        const hipStream_t sc = (naux > 0 && c % (naux + 1)) ? lstream[c % (naux + 1) - 1] : stream;
+        launch_arrow_sweep(a, cls.second, !aws_len, cls.first,
+                           aws_len ? 0 : arrow_ws_bytes(cmax, std::min(cls.second, nImax)), sc);
+      }
+      for (int k = 0; k < naux; ++k) {
+        HTRY(hipEventRecord(ljoin[k], lstream[k]));
+        HTRY(hipStreamWaitEvent(stream, ljoin[k], 0));
       }
     }
     kt_end(KT_ARROW);
@@ -534,19 +548,49 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
   HTRY(hipGetLastError());
   const int hard_cap = itr_max >= 0 ? itr_max + 1 : kMaxSweeps;
   int n_active = n_inst, done = 0;
+  hipGraphExec_t gexec = nullptr;
   while (done < hard_cap && n_active > 0) {
     // sweeps are queued kSweepChunk at a time between reads of the flags,
     // twice as many in the HF stage of the arrowhead path, whose sweeps cost
     // about as much as one read
     const int chunk = (hf && par.dyson_solver == 0) ? 2 * kSweepChunk : kSweepChunk;
     const int k = std::min(chunk, hard_cap - done);
-    for (int q = 0; q < k; ++q) {
-      BTRY(sweep(src));
-      if (learn_hook && done + q == 0) BTRY(after_first_sweep());
+    const bool hook = learn_hook && done == 0;
+    if (use_graphs && !profile && !hook && !seed_pending && k == chunk && par.dyson_solver == 0) {
+      // a full chunk: captured once per stage into a graph, replayed after
+      if (!gexec) {
+        hipGraph_t g = nullptr;
+        HTRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = BCD_OK;
+        for (int q = 0; q < k && rc == BCD_OK; ++q) rc = sweep(src);
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc == BCD_OK && e == hipSuccess) rc = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) == hipSuccess
+                                                      ? BCD_OK : BCD_ERR_DEVICE;
+        else if (rc == BCD_OK) rc = BCD_ERR_DEVICE;
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != BCD_OK) {
+          if (gexec) (void)hipGraphExecDestroy(gexec);
+          return rc;
+        }
+      }
+      if (hipGraphLaunch(gexec, stream) != hipSuccess) {
+        (void)hipGraphExecDestroy(gexec);
+        return BCD_ERR_DEVICE;
+      }
+    } else {
+      for (int q = 0; q < k; ++q) {
+        BTRY(sweep(src));
+        if (hook && q == 0) BTRY(after_first_sweep());
+      }
     }
     done += k;
-    BTRY(check(n_active));
+    const int rc = check(n_active);
+    if (rc != BCD_OK) {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      return rc;
+    }
   }
+  if (gexec) HTRY(hipGraphExecDestroy(gexec));
   for (int i = 0; i < n_inst; ++i) {
     (hf ? hf_its : sc_its)[i] = h_is[i].its;
     (hf ? hf_ok : sc_ok)[i] = h_is[i].diff < conv && !h_is[i].failed;

@@ -108,6 +108,7 @@ struct Batch {
   hipEvent_t ev[6] = {};
   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
   bool profile = false;
+  bool use_graphs = true;            // full sweep chunks replayed from a captured graph (off while profiling)
   std::pair<hipEvent_t, hipEvent_t> kev[KT_N] = {};
   double ktime[KT_N] = {};
   long kcount[KT_N] = {};

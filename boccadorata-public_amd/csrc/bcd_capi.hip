@@ -772,6 +772,12 @@ int bcd_batch_set_profile(bcd_batch *b, int on) {
   return BCD_OK;
 }
 
+int bcd_batch_set_graphs(bcd_batch *b, int on) {
+  if (!b) return BCD_ERR_ARG;
+  b->b.use_graphs = on != 0;
+  return BCD_OK;
+}
+
 int bcd_batch_timings(bcd_batch *b, double *ms, int n) {
   if (!b || !ms || n < 6) return BCD_ERR_ARG;
   for (int k = 0; k < 6; ++k) ms[k] = b->b.stage_ms[k];

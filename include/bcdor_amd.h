@@ -244,6 +244,12 @@ int bcd_batch_solve(bcd_batch *b);
 /* 1: time every kernel family with hipEvents during bcd_batch_solve (adds a
  * synchronisation per launch); 0 (default): stage events only. */
 int bcd_batch_set_profile(bcd_batch *b, int on);
+/* The sweeps of a stage run in chunks between reads of the convergence
+ * flags; a full chunk is captured once per stage into a HIP graph and
+ * replayed (on by default; off while profiling, and for the chunk that
+ * applies a learned start).  on = 0 launches every sweep directly.  Results
+ * are the same either way. */
+int bcd_batch_set_graphs(bcd_batch *b, int on);
 /* Per-stage device time of the last solve in ms (hipEvents on the batch's
  * stream): [0] W/Sigma^inf set-up, [1] HF loop, [2] Sigma(2) build,
  * [3] Lanczos, [4] sc0 loop, [5] total.  n >= 6. */
