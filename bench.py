@@ -475,8 +475,9 @@ def main_nmax7(args):
                                  "solves_per_s": r["rate"], "ms_per_step": 1e3 * r["dt"] / args.steps,
                                  "max_error_vs_fp64": r["max_err"], "instances_within_bound": int(r["ok"].sum()),
                                  "mean_sweeps": r["sweeps"]} for k, r in runs.items()},
-                   "learned_start": dict(learn_info, mean_sc0_sweeps_saved_vs_plain=float(
-                       np.mean(runs["fp64"]["sc_sweeps"] - runs["fp64_learned"]["sc_sweeps"])),
+                   "learned_start": None if net is None else dict(
+                       learn_info, mean_sc0_sweeps_saved_vs_plain=float(
+                           np.mean(runs["fp64"]["sc_sweeps"] - runs["fp64_learned"]["sc_sweeps"])),
                        note="sweeps include the residual-check sweep; saved < 0 means the seeded "
                             "iteration took more sweeps than the plain one"),
                    "plain_fp64_single_instance_solves_per_s": single_rate,
@@ -514,7 +515,8 @@ def screened_batch(n, threads):
     within the caps.  Random interactions of these scales can blur the shell
     gap so that HF or sc0 oscillates at the Fermi surface (the reference
     algorithm does the same on them); such an instance gets a new interaction
-    seed, keeping its hw, sigma, Nmax and Lanczos length.  The candidates are
+    seed, keeping its hw, sigma, Nmax and Lanczos length, and after 6 such
+    rounds also a new hw and sigma drawn from the workload's ranges.  The candidates are
     screened in one untimed solve of the whole batch, replacements in small
     batches of their own, then the final batch is built."""
     import dataclasses
@@ -529,9 +531,17 @@ def screened_batch(n, threads):
     print(f"# screening {time.perf_counter() - t0:.1f} s: {len(bad)} of {n} not converged ({why})",
           file=sys.stderr, flush=True)
     reseeded, attempt = 0, 0
-    while bad and attempt < 12:
+    while bad and attempt < 24:
         attempt += 1
-        trial = [dataclasses.replace(specs[i], seed=specs[i].seed + 1000003 * attempt) for i in bad]
+        # a new interaction seed; after 6 rounds also a new draw of hw and
+        # sigma from the workload's ranges (Nmax and Lanczos length kept)
+        trial = []
+        for i in bad:
+            sp = dataclasses.replace(specs[i], seed=specs[i].seed + 1000003 * attempt)
+            if attempt > 6:
+                r = np.random.default_rng(7919 * specs[i].seed + attempt)
+                sp = dataclasses.replace(sp, hw=float(r.uniform(10.0, 30.0)), sigma=float(r.uniform(1.0, 3.0)))
+            trial.append(sp)
         t = build_batched(trial, params(), threads)
         t.solve()
         still = []
@@ -627,12 +637,19 @@ def train_learned_start(n, threads):
 
 def main_batched(args):
     n = args.batch
-    specs, plain, reseeded = screened_batch(n, args.threads)
+    if args.no_screen:
+        specs = synth.batched_specs(n, BATCH_SEED)
+        plain, reseeded = build_batched(specs, params(), args.threads), 0
+    else:
+        specs, plain, reseeded = screened_batch(n, args.threads)
     by_nmax = {k: sum(1 for s in specs if s.nmax == k) for k in sorted({s.nmax for s in specs})}
-    net, learn_info = train_learned_start(LEARN_TRAIN, args.threads)
+    wanted = {"fp64"} | set(args.modes.split(","))
+    net, learn_info = (train_learned_start(LEARN_TRAIN, args.threads) if "fp64_learned" in wanted
+                       else (None, None))
     # mode: (parameters, learned start)
     modes = {"fp64": (params(0, 0), False), "fp64_anderson": (params(0, 5), False),
              "fp64_learned": (params(0, 0), True), "mixed_anderson": (params(1, 5), False)}
+    modes = {k: v for k, v in modes.items() if k in wanted}
     runs, results = {}, {}
     for label, (par, seeded) in modes.items():
         plain.set_params(par)
@@ -688,6 +705,8 @@ def main_batched(args):
     sample = []
     for k in by_nmax:
         sample += [i for i in range(n) if specs[i].nmax == k and runs["fp64"]["conv"][i]][:SINGLE_PER_NMAX]
+    if args.no_single:
+        sample = []
     t_single, err_single = {}, 0.0
     for i in sample:
         b = capi.Batch([specs[i].make()], params(0, 0), lanc_n=[specs[i].lanc_n])
@@ -698,7 +717,7 @@ def main_batched(args):
         err_single = max(err_single, spectral_error(b.result(0), ref[i], zs[i]))
         b.close()
     frac = {k: v / n for k, v in by_nmax.items()}
-    single_rate = 1.0 / sum(frac[k] * float(np.mean(t_single[k])) for k in frac)
+    single_rate = 1.0 / sum(frac[k] * float(np.mean(t_single[k])) for k in frac) if sample else float("nan")
     print(f"# single-instance fp64: {json.dumps({k: float(np.mean(v)) for k, v in t_single.items()})} s per solve "
           f"by Nmax; {single_rate:.2f} solves/s for the mix; max error vs batched {err_single:.2e}",
           file=sys.stderr, flush=True)
@@ -737,8 +756,9 @@ def main_batched(args):
                                  "converged_instances": int(np.sum(r["conv"] == args.steps)),
                                  "mean_sweeps": r["sweeps"],
                                  "stages_ms": r["stages"]} for k, r in runs.items()},
-                   "learned_start": dict(learn_info, mean_sc0_sweeps_saved_vs_plain=float(
-                       np.mean(runs["fp64"]["sc_sweeps"] - runs["fp64_learned"]["sc_sweeps"])),
+                   "learned_start": None if net is None else dict(
+                       learn_info, mean_sc0_sweeps_saved_vs_plain=float(
+                           np.mean(runs["fp64"]["sc_sweeps"] - runs["fp64_learned"]["sc_sweeps"])),
                        note="sweeps include the residual-check sweep; saved < 0 means the seeded "
                             "iteration took more sweeps than the plain one"),
                    "plain_fp64_single_instance_solves_per_s": single_rate,
@@ -772,6 +792,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=16, help="host threads generating the instances")
+    ap.add_argument("--modes", default="fp64,fp64_anderson,fp64_learned,mixed_anderson",
+                    help="batched: comma-separated modes to time (fp64 is always timed: it is the reference)")
+    ap.add_argument("--no-screen", action="store_true",
+                    help="batched: keep the generator's instances as drawn (short profiling runs)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="batched: skip the one-instance-per-solve sample")
     ap.add_argument("--workload", choices=("batched", "nmax7", "twobody"), default="batched",
                     help="batched: the heterogeneous converged-solve benchmark (default); nmax7: one model "
                          "space; twobody: ladder + ring matrices")

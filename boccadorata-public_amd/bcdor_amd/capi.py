@@ -486,7 +486,8 @@ def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1
                  hf_conv=1e-8, sc_conv=1e-8, t_fac=0.0, ext_u1=1, dyson_solver=0, precision=0,
                  accel=0, hw=0.0, piv_round=0, mix_start=0.0) -> BcdSolveParams:
     """dyson_solver 0: arrowhead updates (default); 1: dense Householder + QL per sweep.
-    precision 0: FP64 throughout; 1: mixed (FP32 Sigma(2) coupling contraction).
+    precision 0: FP64 throughout; mixed Sigma(2) coupling contraction: 1 matrix cores, FP32 operands;
+    2 matrix cores, BF16 operands (both FP32 accumulation); 3 vector units, FP32 operands, FP64 sums.
     accel 0: the reference's iteration; m = 1..8: Anderson mixing of the density over m iterates,
     starting once the density residual norm is below mix_start (<= 0: 1e-3).
     t_fac = U1body_fac and hw = htom of the kinetic term (hw * T_HO) * t_fac.

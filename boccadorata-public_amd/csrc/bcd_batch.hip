@@ -354,7 +354,7 @@ int Batch::set_params(const bcd_solve_params &p) {
       p.dyson_solver != par.dyson_solver || p.piv_round != par.piv_round || p.t_fac != par.t_fac ||
       p.hw != par.hw || p.ext_u1 != par.ext_u1)
     return BCD_ERR_ARG;
-  if (p.precision < 0 || p.precision > 1 || p.accel < 0 || p.accel > kMixMax) return BCD_ERR_ARG;
+  if (p.precision < 0 || p.precision > 3 || p.accel < 0 || p.accel > kMixMax) return BCD_ERR_ARG;
   HTRY(hipSetDevice(device));
   if (p.accel > 0) {
     BTRY(a_mix.grow(sizeof(double) * 2 * (size_t)(p.accel + 1) * std::max<long>(rho_len, 1) + 64));
@@ -710,9 +710,15 @@ int Batch::build_sigma2(int only_ish) {
     vt_cap = std::max(vt_cap, (S->max_vt + 1) & ~1);
     c_cap = std::max(c_cap, (S->max_dg + 1) & ~1);
   }
-  // precision 1: FP32 operands and products, FP64 sums
-  auto kern = par.precision == 1 ? k_sigma2<float, double> : k_sigma2<double, double>;
-  hipLaunchKernelGGL(kern, dim3(kSigGrid), dim3(64), sigma2_lds(vt_cap, c_cap), stream, d_insts, d_chans, d_blocks, P,
+  // precision 1 / 2: the contraction on the matrix cores with FP32 / BF16
+  // operands and FP32 accumulation; 3: FP32 operands and products on the
+  // vector units, FP64 sums
+  auto kern = par.precision == 1 ? k_sigma2<float, double, 1>
+              : par.precision == 2 ? k_sigma2<float, double, 2>
+              : par.precision == 3 ? k_sigma2<float, double, 0> : k_sigma2<double, double, 0>;
+  const size_t lds = sigma2_lds(vt_cap, c_cap, par.precision == 1 || par.precision == 2);
+  if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(kern, dim3(kSigGrid), dim3(64), lds, stream, d_insts, d_chans, d_blocks, P,
                      d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base, d_row_bk_base, d_rows_fw,
                      d_rows_bk, vt_cap, c_cap);
   kt_end(KT_SIGMA2);

@@ -674,7 +674,7 @@ int bcd_batch_begin(int device, int n_inst, const bcd_solve_params *par, bcd_bat
   if (!out || n_inst < 1 || !par) return BCD_ERR_ARG;
   *out = nullptr;
   if (par->mode < 0 || par->mode > 2 || par->dyson_solver < 0 || par->dyson_solver > 1 || par->precision < 0 ||
-      par->precision > 1 || par->accel < 0 || par->accel > kMixMax || par->hw < 0.0 ||
+      par->precision > 3 || par->accel < 0 || par->accel > kMixMax || par->hw < 0.0 ||
       (par->t_fac != 0.0 && par->hw <= 0.0))
     return BCD_ERR_ARG;
   BTRY(device_ok(device));

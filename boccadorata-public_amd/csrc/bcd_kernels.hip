@@ -473,6 +473,109 @@ __device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *
   }
 }
 
+typedef short sig_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float sig_f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ short sig_bf16(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// Pass 1 of one block side on the matrix cores (mixed precision): the
+// coupling contraction m(p, d) = x4 sum_{(alpha beta), gamma} Vt(d; alpha beta,
+// gamma) X2(alpha beta; na nb) y_g(ng; gamma) of Self_energy_ext_Op_c.cpp:
+// 454-468 as the GEMM T[(d, gamma), (na, nb)] = Vt x X2 over the (alpha, beta)
+// pairs -- operands FP32 (MF 1, v_mfma_f32_16x16x4f32) or BF16 (MF 2,
+// v_mfma_f32_16x16x16bf16_1k), FP32 accumulation -- followed by the gamma sum
+// with y_g in FP32.  X2 is the pair amplitude x2 of the reference (with the
+// exchange term and the 1/sqrt2 of alpha == beta when a == b).  Operands come
+// from LDS: the V table, the staged amplitudes, and two scratch tiles for X2
+// and T (pitch kSigMf).  One wave.
+template <int MF>
+__device__ void sigma2_pass1_mfma(const float *xa, const float *xb, const float *xg, const float *vt, int noa,
+                                  int nob, int nog, int nI, int npole, int a_eq_b, int nJab, int cA, int cB, int cG,
+                                  float *x2s, float *ts, double *rows, int w, double *rb) {
+  const int lane = threadIdx.x;
+  const float xJab = (float)(1 - 2 * nJab), kS2 = (float)kSqrt2;
+  const int npair = a_eq_b ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
+  const int R = npole / cG, Mr = nI * nog;
+  auto frag = [&](int r, int &na, int &nb) {
+    if (a_eq_b) {
+      na = 0;
+      int len = cA - nJab;
+      while (r >= len) { r -= len; ++na; --len; }
+      nb = na + nJab + r;
+    } else {
+      na = r / cB; nb = r % cB;
+    }
+  };
+  for (int e = lane; e < npair * R; e += 64) {
+    const int pi = e / R, r = e % R;
+    int al, be, na, nb;
+    if (a_eq_b) {
+      int q = pi, len = noa - nJab;
+      al = 0;
+      while (q >= len) { q -= len; ++al; --len; }
+      be = al + nJab + q;
+    } else {
+      al = pi / nob; be = pi % nob;
+    }
+    frag(r, na, nb);
+    const float *ya = xa + na * noa, *yb = xb + nb * nob;
+    float x2 = ya[al] * yb[be];
+    if (a_eq_b) {
+      x2 += xJab * ya[be] * yb[al];
+      if (al == be) x2 /= kS2;
+    }
+    x2s[pi * kSigMf + r] = x2;
+  }
+  __syncthreads();
+  const int l16 = lane & 15, lq = lane >> 4;
+  for (int m0 = 0; m0 < Mr; m0 += 16)
+    for (int n0 = 0; n0 < R; n0 += 16) {
+      sig_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      const int m = m0 + l16, n = n0 + l16;
+      const int d = m / nog, g = m % nog;
+      if constexpr (MF == 2) {
+        for (int k0 = 0; k0 < npair; k0 += 16) {
+          sig_bf16x4 a, b;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int k = k0 + 4 * lq + i;
+            a[i] = sig_bf16((k < npair && m < Mr) ? vt[(d * npair + k) * nog + g] : 0.f);
+            b[i] = sig_bf16((k < npair && n < R) ? x2s[k * kSigMf + n] : 0.f);
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+      } else {
+        for (int k0 = 0; k0 < npair; k0 += 4) {
+          const int k = k0 + lq;
+          const float a = (k < npair && m < Mr) ? vt[(d * npair + k) * nog + g] : 0.f;
+          const float b = (k < npair && n < R) ? x2s[k * kSigMf + n] : 0.f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int mr = m0 + 4 * lq + i;
+        if (mr < Mr && n < R) ts[mr * kSigMf + n] = acc[i];
+      }
+    }
+  __syncthreads();
+  for (int q = lane; q < npole * nI; q += 64) {
+    const int p = q / nI, d = q % nI, r = p / cG, ng = p % cG;
+    int na, nb;
+    frag(r, na, nb);
+    float s = 0.f;
+    for (int g = 0; g < nog; ++g) s += ts[(d * nog + g) * kSigMf + r] * xg[ng * nog + g];
+    if (a_eq_b && na == nb) s /= kS2;
+    if (rb) rb[q] = (double)s;
+    else rows[(long)p * w + 2 + d] = (double)s;
+  }
+}
+
 // The coupling build of Build_Dynamic_SlfEn_2ndOrder (Self_energy_ext_Op_c.cpp:
 // 380-539), one wave per block, grid-stride over the compact list of
 // non-empty blocks.  Per block the interaction part Vt(d; alpha beta, gamma)
@@ -481,8 +584,11 @@ __device__ __forceinline__ void sigma2_pass1(const S *xa, const S *xb, const S *
 // (pole, d)) and pass 2 the energies and sums of squares (one lane per pole,
 // over d in order).  Rows match the reference to the last bit.
 // T: arithmetic of the contraction -- double (the reference's) or float
-// (FP32 operands and products); A: type of the sums.  Rows are stored in FP64.
-template <class T, class A>
+// (FP32 operands and products); A: type of the sums; MF: 0 on the vector
+// units, 1 / 2 on the matrix cores with FP32 / BF16 operands (T = float;
+// sides whose sizes exceed kSigMf, or whose amplitudes are not staged, take
+// the vector path).  Rows are stored in FP64.
+template <class T, class A, int MF>
 __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanDev *chans,
                                                const BlockDev *blocks, PropView P,
                                                const int *blk_fw_off, const int *blk_bk_off,
@@ -498,6 +604,8 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
   T *stg = (T *)(sig_lds + vt_cap);
   double *rbuf = sig_lds + vt_cap + kSigStage;
   int *rowA = (int *)(rbuf + kSigRowBuf), *rowC = rowA + kMaxPairs, *sgnC = rowC + c_cap;
+  // matrix-core scratch (MF > 0): X2 and T tiles, 16-byte aligned
+  float *mfx = (float *)(((unsigned long)(sgnC + c_cap) + 15) & ~15ul), *mft = mfx + kSigMf * kSigMf;
   const int lane = threadIdx.x;
   const int total = *work_count;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -594,7 +702,12 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
       __syncthreads();
       // pass-1 results stay in LDS for pass 2 when they fit
       double *rb = npole * nI <= kSigRowBuf ? rbuf : nullptr;
-      if (staged)
+      const int npair_s = (a == b) ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
+      if (MF > 0 && staged && npair_s <= kSigMf && npole / cG <= kSigMf && nI * nog <= kSigMf)
+        sigma2_pass1_mfma<MF>((const float *)stg, (const float *)((a == b) ? stg : stg + nsa),
+                              (const float *)(stg + nsa + nsb), (const float *)vt, noa, nob, nog, nI, npole, a == b,
+                              nJab, cA, cB, cG, mfx, mft, rows, w, rb);
+      else if (staged)
         sigma2_pass1<T, A, T, true>(stg, (a == b) ? stg : stg + nsa, stg + nsa + nsb, vt, noa, nob, nog, nI, npole,
                               a == b, nJab, nJgd, cA, cB, cG, og0, C.orb0, rows, w, rb);
       else
@@ -632,12 +745,14 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
   }
 }
 
-#define BCD_SIGMA2_INST(T, A)                                                                               \
-  template __global__ void k_sigma2<T, A>(const InstDev *, const ChanDev *, const BlockDev *, PropView, const int *, \
+#define BCD_SIGMA2_INST(T, A, MF)                                                                           \
+  template __global__ void k_sigma2<T, A, MF>(const InstDev *, const ChanDev *, const BlockDev *, PropView, const int *, \
                                           const int *, const Sig2Work *, const int *, const long *, const long *,  \
                                           double *, double *, int, int);
-BCD_SIGMA2_INST(double, double)
-BCD_SIGMA2_INST(float, double)
+BCD_SIGMA2_INST(double, double, 0)
+BCD_SIGMA2_INST(float, double, 0)
+BCD_SIGMA2_INST(float, double, 1)
+BCD_SIGMA2_INST(float, double, 2)
 #undef BCD_SIGMA2_INST
 
 // ------------------------------------------------------------- Lanczos

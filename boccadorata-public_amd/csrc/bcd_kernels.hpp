@@ -40,7 +40,9 @@ __global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *
 __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
                               int *blk_bk_off, Sig2Work *work_list, int *work_count, int only_ish);
-template <class T, class A>
+constexpr int kSigMf = 48;        // largest (alpha, beta) pairs, (na, nb) fragment pairs and nI * n_g of a
+                                  // block side contracted on the matrix cores
+template <class T, class A, int MF>
 __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const BlockDev *blocks, PropView P,
                          const int *blk_fw_off, const int *blk_bk_off, const Sig2Work *work_list,
                          const int *work_count, const long *row_fw_base, const long *row_bk_base,
@@ -48,8 +50,9 @@ __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const Block
 // dynamic LDS bytes of k_sigma2 for a V table of vt_cap doubles
 // dynamic LDS bytes of k_sigma2: V table of vt_cap entries, the staged
 // amplitudes and couplings, and the pair rows (c_cap = the largest nI * n_g)
-inline size_t sigma2_lds(int vt_cap, int c_cap) {
-  return sizeof(double) * ((size_t)vt_cap + kSigStage + kSigRowBuf) + sizeof(int) * (kMaxPairs + 2 * (size_t)c_cap);
+inline size_t sigma2_lds(int vt_cap, int c_cap, bool mfma = false) {
+  return sizeof(double) * ((size_t)vt_cap + kSigStage + kSigRowBuf) + sizeof(int) * (kMaxPairs + 2 * (size_t)c_cap) +
+         (mfma ? 16 + sizeof(float) * 2 * kSigMf * kSigMf : 0);
 }
 constexpr int kMergeMaxNI = 4;    // J-degenerate pole groups are merged in channels up to this nI
 constexpr int kMergeMaxJ = 16;    // J blocks of one (a, b, g) triple (J <= j_a + j_b)

@@ -165,11 +165,16 @@ typedef struct bcd_solve_params {
                         1: dense Householder + QL of the whole Dyson matrix per
                         sweep.  Both give DysonMatrix's eigenpairs.              */
   int precision;     /* 0: FP64 throughout (Sigma(2) rows bit-identical to the
-                        reference); 1: mixed -- the Sigma(2) coupling
-                        contraction with FP32 operands and products (sums
-                        in FP64); the Lanczos reduction, eigensolves and sc
-                        loop in FP64.  At Nmax = 7 its energies differ from
-                        the FP64 solve by up to ~1e-5 relative.            */
+                        reference).  Mixed: the Sigma(2) coupling
+                        contraction (Self_energy_ext_Op_c.cpp:454-468) on
+                        the matrix cores with FP32 accumulation and 1: FP32
+                        operands, 2: BF16 operands; 3: on the vector units
+                        with FP32 operands and products, FP64 sums.  The
+                        Lanczos reduction, eigensolves and sc loop stay
+                        FP64.  The rows' errors propagate to the energies
+                        (cancellation in the contraction amplifies them);
+                        each mode's error against FP64 is measured, not
+                        bounded.                                          */
   int accel;         /* 0: the reference's iteration (the hole density of each
                         sweep's output feeds the next Sigma^inf); m = 1..8:
                         Anderson mixing of that density over the last m
