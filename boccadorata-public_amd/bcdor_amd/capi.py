@@ -474,7 +474,8 @@ class Context(_TwoBodyMixin):
 # k_lanczos tile shapes (threads per job, elements per thread) and the upper
 # pole counts of the buckets the autotuner picks them for
 LANC_SHAPES = [(64, 4), (64, 8), (128, 8), (192, 8), (256, 8), (128, 16), (192, 16), (256, 16), (384, 8), (192, 24),
-               (256, 24), (384, 16), (384, 24), (768, 24), (1024, 24), (1024, 32), (1024, 40)]
+               (256, 24), (384, 16), (384, 24), (768, 24), (1024, 24), (1024, 32), (1024, 40), (512, 16), (768, 16),
+               (1024, 16), (512, 24), (768, 32)]
 LANC_BUCKET_MAX = [256, 512, 1024, 2048, 4096, 8192, 16384, 24576, 32768, 40960]
 LANC_BUCKETS = len(LANC_BUCKET_MAX)
 # a second valid shape per bucket, different from the default one (its block

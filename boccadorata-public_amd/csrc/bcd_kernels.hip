@@ -1100,8 +1100,11 @@ __device__ __forceinline__ void buf_st2(double a, double b, __amdgpu_buffer_rsrc
 // pair q = tid + NT*t being elements 2q and 2q+1, so that every access moves
 // 16 bytes per lane; vectors are stored with an even stride Ms, and the
 // padding element of an odd M is kept at zero.
-// E stays in registers for the whole job (read once from the input rows);
-// behind the nLanc vectors the workspace holds the NB coupling columns of the
+// E stays in registers for the whole job (read once from the input rows)
+// where the tile leaves room for it (lanczos_ev_in_regs); otherwise it is
+// kept as one more contiguous workspace vector, in the last E * v_last slot
+// (the last pivot chain never fills it), and read where it is used.
+// Behind the nLanc vectors the workspace holds the NB coupling columns of the
 // input rows as contiguous vectors, and E * v_last of every finished pivot
 // chain, so the per-step sums read contiguous vectors only.  With a stash
 // (NT * PER doubles of LDS), each thread also keeps its own elements of the
@@ -1122,22 +1125,41 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
   const __amdgpu_buffer_rsrc_t rV = buf_rsrc(Jb.V, 8L * nvec * Ms), rM = buf_rsrc(Min, 8L * M * Tin);
   const int vo = 16 * tid, mo = 16 * tid * Tin;
   const int co = 8 * nLanc * Ms, wo = co + 8 * NB * Ms;   // couplings, E * v_last
-  double ev[PER];
+  constexpr bool kEvReg = lanczos_ev_in_regs(NT, PER);
+  const int eo = wo + 8 * (kMaxPiv - 1) * Ms;               // E when not in registers
+  double ev[kEvReg ? PER : 2];
 #pragma unroll
   for (int t = 0; t < PP; ++t) {
     const int p = 2 * (tid + NT * t), ms = 16 * NT * t * Tin;
-    ev[2 * t] = ev[2 * t + 1] = 0.0;
+    double e0 = 0.0, e1 = 0.0;
     if (p < M) {
       const bool h1 = p + 1 < M;
-      if (h1) ev[2 * t + 1] = buf_ld(rM, mo, ms + 8 * Tin);
-      ev[2 * t] = buf_ld(rM, mo, ms);
+      if (h1) e1 = buf_ld(rM, mo, ms + 8 * Tin);
+      e0 = buf_ld(rM, mo, ms);
+      if constexpr (!kEvReg) buf_st2(e0, e1, rV, vo, eo + 16 * NT * t);
       for (int l = 0; l < NB; ++l) {
         double c1 = 0.0;
         if (h1) c1 = buf_ld(rM, mo, ms + 8 * (Tin + NA + l));
         buf_st2(buf_ld(rM, mo, ms + 8 * (NA + l)), c1, rV, vo, co + 8 * l * Ms + 16 * NT * t);
       }
     }
+    if constexpr (kEvReg) {
+      ev[2 * t] = e0;
+      ev[2 * t + 1] = e1;
+    }
   }
+  // E at pair t (0 past the end)
+  auto evp = [&](int t) -> double2 {
+    double2 r;
+    if constexpr (kEvReg) {
+      r.x = ev[2 * t];
+      r.y = ev[2 * t + 1];
+    } else {
+      r.x = r.y = 0.0;
+      if (2 * (tid + NT * t) < M) r = buf_ld2(rV, vo, eo + 16 * NT * t);
+    }
+    return r;
+  };
   double v[PER], vi[PER];
   int i_pvt = -1, n_pvt = -1, j_next = 0, ph = 0;
   int status = 0;
@@ -1172,7 +1194,11 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
       }
     } else {
 #pragma unroll
-      for (int t = 0; t < PER; ++t) v[t] *= ev[t];
+      for (int t = 0; t < PP; ++t) {
+        const double2 e = evp(t);
+        v[2 * t] *= e.x;
+        v[2 * t + 1] *= e.y;
+      }
     }
   // Gram-Schmidt against every earlier vector, latest first (Lanczos.cpp:164-173)
     for (int i = j - 1; i >= 0; --i) {
@@ -1207,7 +1233,11 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
       double acc = 0.0;
       if (k == 0) {
 #pragma unroll
-        for (int t = 0; t < PER; ++t) acc += (ev[t] * v[t]) * v[t];
+        for (int t = 0; t < PP; ++t) {
+          const double2 e = evp(t);
+          acc += (e.x * v[2 * t]) * v[2 * t];
+          acc += (e.y * v[2 * t + 1]) * v[2 * t + 1];
+        }
       } else if (k == 1) {
 #pragma unroll
         for (int t = 0; t < PER; ++t) acc += v[t] * v[t];
@@ -1259,7 +1289,10 @@ __device__ __forceinline__ void lanczos_job(const LancJob &Jb, double *red, doub
       }
       if (p < M) {
         buf_st2(v[2 * t], v[2 * t + 1], rV, vo, 8 * j * Ms + 16 * NT * t);
-        if (chain_end) buf_st2(v[2 * t] * ev[2 * t], v[2 * t + 1] * ev[2 * t + 1], rV, vo, wo + 8 * n_pvt * Ms + 16 * NT * t);
+        if (chain_end) {
+          const double2 e = evp(t);
+          buf_st2(v[2 * t] * e.x, v[2 * t + 1] * e.y, rV, vo, wo + 8 * n_pvt * Ms + 16 * NT * t);
+        }
       }
     }
   }
@@ -1322,10 +1355,15 @@ void launch_lanczos(int shape, int njobs, const LancJob *jobs, const int *order,
     case 13: launch_shape<13>(njobs, jobs, order, stream); break;
     case 14: launch_shape<14>(njobs, jobs, order, stream); break;
     case 15: launch_shape<15>(njobs, jobs, order, stream); break;
-    default: launch_shape<16>(njobs, jobs, order, stream); break;
+    case 16: launch_shape<16>(njobs, jobs, order, stream); break;
+    case 17: launch_shape<17>(njobs, jobs, order, stream); break;
+    case 18: launch_shape<18>(njobs, jobs, order, stream); break;
+    case 19: launch_shape<19>(njobs, jobs, order, stream); break;
+    case 20: launch_shape<20>(njobs, jobs, order, stream); break;
+    default: launch_shape<21>(njobs, jobs, order, stream); break;
   }
 }
-static_assert(kNLancShapes == 17, "launch_lanczos covers every shape");
+static_assert(kNLancShapes == 22, "launch_lanczos covers every shape");
 
 // ------------------------------------------------------------- Dyson
 

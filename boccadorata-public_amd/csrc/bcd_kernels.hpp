@@ -77,8 +77,16 @@ struct LancShape { int nt, per; };
 constexpr LancShape kLancShapes[] = {{64, 4},   {64, 8},   {128, 8},   {192, 8},   {256, 8},
                                      {128, 16}, {192, 16}, {256, 16},  {384, 8},   {192, 24},
                                      {256, 24}, {384, 16}, {384, 24},  {768, 24},  {1024, 24},
-                                     {1024, 32}, {1024, 40}};
+                                     {1024, 32}, {1024, 40}, {512, 16}, {768, 16}, {1024, 16}, {512, 24},
+                                     {768, 32}};
 constexpr int kNLancShapes = sizeof(kLancShapes) / sizeof(kLancShapes[0]);
+// Whether a tile keeps the pole energies in registers: three PER-element
+// vectors (current, projected-out, energies) plus working registers within
+// the VGPRs a thread gets at NT threads per block on one CU; otherwise the
+// energies are re-read from the workspace where used.
+constexpr bool lanczos_ev_in_regs(int nt, int per) {
+  return 6 * per + 64 <= (nt <= 256 ? 256 : 512 / ((nt + 255) / 256));
+}
 // The last two buckets take channels whose HF occupations moved (more
 // quasiholes than the bare propagator gives), found at Nmax 11.
 constexpr int kLancBucket[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 24576, 32768, 40960};

@@ -305,7 +305,7 @@ int bcd_batch_set_lanczos_shapes(bcd_batch *b, const int *shapes, int n);
  * results agree to rounding. */
 int bcd_batch_autotune(bcd_batch *b);
 /* Shape index chosen per bucket (n >= 10) and, if tune_ms is not NULL, the
- * timed ms of every (bucket, shape) pair as tune_ms[bucket * 17 + shape]
+ * timed ms of every (bucket, shape) pair as tune_ms[bucket * 22 + shape]
  * (0 where not timed). */
 int bcd_batch_lanczos_shapes(bcd_batch *b, int *shapes, float *tune_ms, int n);
 
