@@ -149,7 +149,8 @@ def moment_error(g, ref):
 # HBM bytes per launch of the dominant kernel family, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes of this workload (calibrated against a 1 GiB
 # copy): (kernel family, instances per step) -> (bytes, summary file)
-MEASURED_TRAFFIC = {("nmax7", "lanczos", 256): (225.7e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md")}
+MEASURED_TRAFFIC = {("nmax7", "lanczos", 256): (225.7e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md"),
+                    ("batched", "lanczos", 256): (863.7e9, "profiles/scgf_batched256_lanczos_traffic.md")}
 
 
 def roofline(batch, workload):
