@@ -471,8 +471,6 @@ def main_nmax7(args):
                                     "instance plainly above 1e-6",
                    "modes": {k: {"precision": "FP32 Sigma(2) coupling operands" if MODES[k][0] else "FP64",
                                  "density_mixing": f"Anderson, {MODES[k][1]} iterates" if MODES[k][1] else "none",
-                                 "learned_start": k == "fp64_learned",
-                                 "fallbacks_to_plain": r["fallbacks"],
                                  "solves_per_s": r["rate"], "ms_per_step": 1e3 * r["dt"] / args.steps,
                                  "max_error_vs_fp64": r["max_err"], "instances_within_bound": int(r["ok"].sum()),
                                  "mean_sweeps": r["sweeps"]} for k, r in runs.items()},
