@@ -4,6 +4,7 @@
 // MakeSelfEn 2nd [Lanczos] -> ExtSE=-1: every stage is a short sequence of
 // kernel launches over all channels of all instances on one stream; the host
 // only checks the per-instance convergence flags between sweeps.
+#include "bcd_hipsync.hpp"
 #include "bcd_batch.hpp"
 #include "bcd_learn.hpp"
 #include "bcd_mlp.hpp"
@@ -15,20 +16,25 @@
 
 namespace bcd {
 
+std::mutex &hip_sync_lock() {
+  static std::mutex m;
+  return m;
+}
+
 static long pad8(long x) { return (x + 7) & ~7L; }
 
 int DevArena::grow(size_t bytes) {
   if (bytes <= cap) return BCD_OK;
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) (void)bcd::sync_free(ptr);
   ptr = nullptr;
   cap = 0;
-  if (hipMalloc(&ptr, bytes) != hipSuccess) { ptr = nullptr; return BCD_ERR_NOMEM; }
+  if (bcd::sync_malloc(&ptr, bytes) != hipSuccess) { ptr = nullptr; return BCD_ERR_NOMEM; }
   cap = bytes;
   return BCD_OK;
 }
 
 void DevArena::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) (void)bcd::sync_free(ptr);
   ptr = nullptr;
   cap = 0;
 }
@@ -53,10 +59,21 @@ std::vector<DevArena *> Batch::arenas() {
           &a_lws, &a_mat, &a_jobs, &a_aws, &a_prof, &a_warm, &a_mix, &a_learn};
 }
 
+// The last HIP error a batch call of this thread met, with its source line;
+// bcd_batch_solve appends it to the batch's error detail.
+static thread_local std::string t_hip_error;
+
+static void note_hip(hipError_t e, int line) {
+  t_hip_error = std::string(hipGetErrorName(e)) + " at bcd_batch.hip:" + std::to_string(line);
+}
+
 #define HTRY(x)                                   \
   do {                                            \
     hipError_t e_ = (x);                          \
-    if (e_ != hipSuccess) return BCD_ERR_DEVICE;  \
+    if (e_ != hipSuccess) {                       \
+      note_hip(e_, __LINE__);                     \
+      return BCD_ERR_DEVICE;                      \
+    }                                             \
   } while (0)
 
 #define BTRY(x)                 \
@@ -80,11 +97,15 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   }
   learn_valid = false;
   HTRY(hipSetDevice(device));
-  HTRY(hipStreamCreate(&stream));
+  // non-blocking streams: synchronous copies other host threads issue on the
+  // null stream neither wait for this batch nor break its graph captures
+  // (every synchronous copy of this batch follows a synchronisation of its
+  // stream)
+  HTRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (auto &e : ev) HTRY(hipEventCreate(&e));
   HTRY(hipEventCreateWithFlags(&lfork, hipEventDisableTiming));
   for (int k = 0; k < kLancAux; ++k) {
-    HTRY(hipStreamCreate(&lstream[k]));
+    HTRY(hipStreamCreateWithFlags(&lstream[k], hipStreamNonBlocking));
     HTRY(hipEventCreateWithFlags(&ljoin[k], hipEventDisableTiming));
   }
 
@@ -111,9 +132,9 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   BTRY(a_tab_i.grow(sizeof(int) * std::max<size_t>(ti.size(), 1)));
   BTRY(a_tab_l.grow(sizeof(long) * std::max<size_t>(tl.size(), 1)));
   BTRY(a_tab_d.grow(sizeof(double) * std::max<size_t>(td.size(), 1)));
-  HTRY(hipMemcpy(a_tab_i.ptr, ti.data(), sizeof(int) * ti.size(), hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(a_tab_l.ptr, tl.data(), sizeof(long) * tl.size(), hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(a_tab_d.ptr, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(a_tab_i.ptr, ti.data(), sizeof(int) * ti.size(), hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(a_tab_l.ptr, tl.data(), sizeof(long) * tl.size(), hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(a_tab_d.ptr, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice));
   const int *Ti = (const int *)a_tab_i.ptr;
   const long *Tl = (const long *)a_tab_l.ptr;
   const double *Td = (const double *)a_tab_d.ptr;
@@ -218,7 +239,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   BTRY(a_inst.grow(pad8(sizeof(InstDev) * n_inst) + pad8(sizeof(InstState) * n_inst) + 64));
   d_insts = (InstDev *)a_inst.ptr;
   d_is = (InstState *)((char *)a_inst.ptr + pad8(sizeof(InstDev) * n_inst));
-  HTRY(hipMemcpy(d_insts, h_insts.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(d_insts, h_insts.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice));
 
   size_t chan_bytes = pad8(sizeof(ChanDev) * n_gc) + pad8(sizeof(ChanState) * n_gc) + 2 * pad8(sizeof(long) * n_gc) +
                       2 * pad8(sizeof(long) * n_gc) + pad8(sizeof(int) * n_gc) + 64;
@@ -231,12 +252,12 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_e_off = (long *)cp8; cp8 += pad8(sizeof(long) * n_gc);
   d_x_off = (long *)cp8; cp8 += pad8(sizeof(long) * n_gc);
   d_cap = (int *)cp8;
-  HTRY(hipMemcpy(d_chans, h_chans.data(), sizeof(ChanDev) * n_gc, hipMemcpyHostToDevice));
-  HTRY(hipMemset(d_cs, 0, sizeof(ChanState) * n_gc));
-  HTRY(hipMemset(d_row_fw_base, 0, sizeof(long) * n_gc * 2));
-  HTRY(hipMemcpy(d_e_off, e_off.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(d_x_off, x_off.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(d_cap, cap.data(), sizeof(int) * n_gc, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(d_chans, h_chans.data(), sizeof(ChanDev) * n_gc, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memset(d_cs, 0, sizeof(ChanState) * n_gc));
+  HTRY(bcd::sync_memset(d_row_fw_base, 0, sizeof(long) * n_gc * 2));
+  HTRY(bcd::sync_memcpy(d_e_off, e_off.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(d_x_off, x_off.data(), sizeof(long) * n_gc, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(d_cap, cap.data(), sizeof(int) * n_gc, hipMemcpyHostToDevice));
 
   const long nts = std::max<long>((long)h_trip_start.size(), 1), ntb = std::max<long>((long)h_trip_blk.size(), 1);
   BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 2 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) +
@@ -249,11 +270,11 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_trip_start = (int *)cp8; cp8 += pad8(sizeof(int) * nts);
   d_trip_blk = (int *)cp8; cp8 += pad8(sizeof(int) * ntb);
   d_work_list = (Sig2Work *)cp8;
-  if (n_blk) HTRY(hipMemcpy(d_blocks, h_blocks.data(), sizeof(BlockDev) * n_blk, hipMemcpyHostToDevice));
+  if (n_blk) HTRY(bcd::sync_memcpy(d_blocks, h_blocks.data(), sizeof(BlockDev) * n_blk, hipMemcpyHostToDevice));
   if (!h_trip_start.empty())
-    HTRY(hipMemcpy(d_trip_start, h_trip_start.data(), sizeof(int) * h_trip_start.size(), hipMemcpyHostToDevice));
+    HTRY(bcd::sync_memcpy(d_trip_start, h_trip_start.data(), sizeof(int) * h_trip_start.size(), hipMemcpyHostToDevice));
   if (!h_trip_blk.empty())
-    HTRY(hipMemcpy(d_trip_blk, h_trip_blk.data(), sizeof(int) * h_trip_blk.size(), hipMemcpyHostToDevice));
+    HTRY(bcd::sync_memcpy(d_trip_blk, h_trip_blk.data(), sizeof(int) * h_trip_blk.size(), hipMemcpyHostToDevice));
 
   // work doubles: u1, h, W, rho, tri, piv, lrows, kolt ; ints: status, work_count
   l_u1 = 0; l_h = l_u1 + pad8(o_u1); l_w = l_h + pad8(o_h); l_rho = l_w + pad8(o_w);
@@ -273,8 +294,8 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   }
   BTRY(a_prof.grow(sizeof(unsigned long long) * 8 * (size_t)n_gc));
   d_prof = (unsigned long long *)a_prof.ptr;
-  HTRY(hipMemset(a_work.ptr, 0, sizeof(double) * l_end));
-  HTRY(hipMemcpy(d_u1, u1.data(), sizeof(double) * o_u1, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memset(a_work.ptr, 0, sizeof(double) * l_end));
+  HTRY(bcd::sync_memcpy(d_u1, u1.data(), sizeof(double) * o_u1, hipMemcpyHostToDevice));
 
   // propagators: 3 buffers (initial, ping, pong), counts
   const long pe = pad8(o_e), px = pad8(o_x), pn = pad8(n_gc);
@@ -284,7 +305,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   int *C0 = (int *)(P0 + 3 * (pe + px));
   for (int b = 0; b < 3; ++b) { pnqp_[b] = C0 + 2 * b * pn; pnqh_[b] = pnqp_[b] + pn; }
   prop_e_len = pe; prop_x_len = px; prop_n_len = pn;
-  HTRY(hipMemset(a_prop.ptr, 0, a_prop.cap));
+  HTRY(bcd::sync_memset(a_prop.ptr, 0, a_prop.cap));
 
   // initial propagator
   std::vector<double> he(pe, 0.0), hx(px, 0.0);
@@ -306,10 +327,10 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
       }
     }
   }
-  HTRY(hipMemcpy(pe_[2], he.data(), sizeof(double) * pe, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(px_[2], hx.data(), sizeof(double) * px, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(pnqp_[2], hqp.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(pnqh_[2], hqh.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(pe_[2], he.data(), sizeof(double) * pe, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(px_[2], hx.data(), sizeof(double) * px, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(pnqp_[2], hqp.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(pnqh_[2], hqh.data(), sizeof(int) * pn, hipMemcpyHostToDevice));
 
   // jobs
   BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) +
@@ -320,7 +341,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   d_lorder = (int *)((char *)d_ljobs + pad8(sizeof(LancJob) * 2 * n_gc));
   d_cspack = (int *)((char *)d_lorder + pad8(sizeof(int) * 2 * n_gc));
   h_cspack.resize((size_t)kCsPack * n_gc);
-  HTRY(hipMemset(a_jobs.ptr, 0, a_jobs.cap));
+  HTRY(bcd::sync_memset(a_jobs.ptr, 0, a_jobs.cap));
 
   // dense matrices in HBM when the largest one does not fit in LDS: the
   // whole Dyson matrix (dense solver) or one reduced side (arrowhead solver)
@@ -374,7 +395,7 @@ int Batch::set_efermi(int inst, int ish, double ef) {
   C.ef_set = 1;
   C.ef_force = ef;
   HTRY(hipSetDevice(device));
-  HTRY(hipMemcpy(d_chans + h_insts[inst].gc0 + ish, &C, sizeof(ChanDev), hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(d_chans + h_insts[inst].gc0 + ish, &C, sizeof(ChanDev), hipMemcpyHostToDevice));
   return BCD_OK;
 }
 
@@ -525,7 +546,7 @@ int Batch::check(int &n_active) {
     n_active += s.active;
   }
   int st = 0;
-  HTRY(hipMemcpy(&st, d_status, sizeof(int), hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(&st, d_status, sizeof(int), hipMemcpyDeviceToHost));
   if (st) return BCD_ERR_EIGEN;
   return BCD_OK;
 }
@@ -560,20 +581,31 @@ int Batch::stage(int src, int itr_max, double conv, bool hf) {
       // a full chunk: captured once per stage into a graph, replayed after
       if (!gexec) {
         hipGraph_t g = nullptr;
-        HTRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        // relaxed: the region holds only this batch's launches and its own
+        // event records and waits; other host threads may keep issuing
+        // synchronous calls meanwhile
+        std::unique_lock<std::mutex> cap_lock(hip_sync_lock());
+        HTRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         int rc = BCD_OK;
         for (int q = 0; q < k && rc == BCD_OK; ++q) rc = sweep(src);
         const hipError_t e = hipStreamEndCapture(stream, &g);
-        if (rc == BCD_OK && e == hipSuccess) rc = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) == hipSuccess
-                                                      ? BCD_OK : BCD_ERR_DEVICE;
-        else if (rc == BCD_OK) rc = BCD_ERR_DEVICE;
+        cap_lock.unlock();
+        if (rc == BCD_OK && e == hipSuccess) {
+          const hipError_t ei = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+          if (ei != hipSuccess) { note_hip(ei, __LINE__); rc = BCD_ERR_DEVICE; }
+        } else if (rc == BCD_OK) {
+          note_hip(e, __LINE__);
+          rc = BCD_ERR_DEVICE;
+        }
         if (g) (void)hipGraphDestroy(g);
         if (rc != BCD_OK) {
           if (gexec) (void)hipGraphExecDestroy(gexec);
           return rc;
         }
       }
-      if (hipGraphLaunch(gexec, stream) != hipSuccess) {
+      const hipError_t el = hipGraphLaunch(gexec, stream);
+      if (el != hipSuccess) {
+        note_hip(el, __LINE__);
         (void)hipGraphExecDestroy(gexec);
         return BCD_ERR_DEVICE;
       }
@@ -675,10 +707,10 @@ int Batch::build_sigma2(int only_ish) {
     // 2 * poles * K * n per block, K = (alpha,beta) pairs * gamma, from the
     // fragment counts the build used (reference formulation, Op_c.cpp:454-468)
     std::vector<int> cq(pnqp_[0] ? n_gc : 0), chh(n_gc), cq1(n_gc), ch1(n_gc);
-    HTRY(hipMemcpy(cq.data(), pnqp_[0], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
-    HTRY(hipMemcpy(chh.data(), pnqh_[0], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
-    HTRY(hipMemcpy(cq1.data(), pnqp_[1], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
-    HTRY(hipMemcpy(ch1.data(), pnqh_[1], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(cq.data(), pnqp_[0], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(chh.data(), pnqh_[0], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(cq1.data(), pnqp_[1], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(ch1.data(), pnqh_[1], sizeof(int) * n_gc, hipMemcpyDeviceToHost));
     double f = 0.0;
     for (const BlockDev &B : h_blocks) {
       const ChanDev &C = h_chans[B.gc];
@@ -923,7 +955,7 @@ int Batch::work(double *w, int n) {
   if (n >= 15) {
     std::vector<unsigned long long> t(8 * (size_t)n_gc);
     int khz = 0;
-    HTRY(hipMemcpy(t.data(), d_prof, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(t.data(), d_prof, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
     HTRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
     for (int k = 0; k < 8; ++k) {
       double tk = 0.0;
@@ -932,7 +964,7 @@ int Batch::work(double *w, int n) {
     }
   }
   w[0] = coupling_flops;
-  if (d_work) HTRY(hipMemcpy(&w[5], d_work, sizeof(double), hipMemcpyDeviceToHost));
+  if (d_work) HTRY(bcd::sync_memcpy(&w[5], d_work, sizeof(double), hipMemcpyDeviceToHost));
   for (int gc = 0; gc < (int)h_cs.size() && par.lanczos && par.dyson_solver == 0; ++gc) {
     const double lf = std::max(h_cs[gc].lfw, 0), lb = std::max(h_cs[gc].lbk, 0);
     w[6] += 9.0 * (lf * lf * lf + lb * lb * lb);
@@ -1004,6 +1036,13 @@ void Batch::note_error(const std::string &what) {
 }
 
 int Batch::solve() {
+  t_hip_error.clear();
+  const int rc = solve_impl();
+  if (rc == BCD_ERR_DEVICE && !t_hip_error.empty()) note_error(t_hip_error);
+  return rc;
+}
+
+int Batch::solve_impl() {
   HTRY(hipSetDevice(device));
   err_detail.clear();
   for (int k = 0; k < KT_N; ++k) { ktime[k] = 0.0; kcount[k] = 0; }
@@ -1079,7 +1118,7 @@ int Batch::solve() {
   float tot = 0.f;
   (void)hipEventElapsedTime(&tot, ev[0], ev[5]);
   stage_ms[5] = tot;
-  HTRY(hipMemcpy(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(h_is.data(), d_is, sizeof(InstState) * n_inst, hipMemcpyDeviceToHost));
   return BCD_OK;
 }
 
@@ -1187,8 +1226,8 @@ int Batch::result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, do
   const HostSystem &S = *sys[inst];
   const int cur = h_is[inst].cur;
   std::vector<int> cq(S.nsubsh), ch_(S.nsubsh);
-  HTRY(hipMemcpy(cq.data(), pnqp_[cur] + I.gc0, sizeof(int) * S.nsubsh, hipMemcpyDeviceToHost));
-  HTRY(hipMemcpy(ch_.data(), pnqh_[cur] + I.gc0, sizeof(int) * S.nsubsh, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(cq.data(), pnqp_[cur] + I.gc0, sizeof(int) * S.nsubsh, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(ch_.data(), pnqh_[cur] + I.gc0, sizeof(int) * S.nsubsh, hipMemcpyDeviceToHost));
   int tq = 0, th = 0;
   for (int s = 0; s < S.nsubsh; ++s) { tq += cq[s]; th += ch_[s]; }
   if (n_qp) *n_qp = tq;
@@ -1196,15 +1235,15 @@ int Batch::result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, do
   if (!ep) return BCD_OK;
   if (nspb < S.nspb) return BCD_ERR_ARG;
   std::vector<long> eo(n_gc), xo(n_gc);
-  HTRY(hipMemcpy(eo.data(), d_e_off, sizeof(long) * n_gc, hipMemcpyDeviceToHost));
-  HTRY(hipMemcpy(xo.data(), d_x_off, sizeof(long) * n_gc, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(eo.data(), d_e_off, sizeof(long) * n_gc, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(xo.data(), d_x_off, sizeof(long) * n_gc, hipMemcpyDeviceToHost));
   int kp = 0, kh = 0;
   for (int s = 0; s < S.nsubsh; ++s) {
     const int gc = I.gc0 + s, nI = S.no[s], tot = cq[s] + ch_[s];
     std::vector<double> e(std::max(tot, 1)), x(std::max(tot * nI, 1));
     if (tot) {
-      HTRY(hipMemcpy(e.data(), pe_[cur] + eo[gc], sizeof(double) * tot, hipMemcpyDeviceToHost));
-      HTRY(hipMemcpy(x.data(), px_[cur] + xo[gc], sizeof(double) * tot * nI, hipMemcpyDeviceToHost));
+      HTRY(bcd::sync_memcpy(e.data(), pe_[cur] + eo[gc], sizeof(double) * tot, hipMemcpyDeviceToHost));
+      HTRY(bcd::sync_memcpy(x.data(), px_[cur] + xo[gc], sizeof(double) * tot * nI, hipMemcpyDeviceToHost));
     }
     if (np) np[s] = cq[s];
     if (nh) nh[s] = ch_[s];

@@ -168,6 +168,7 @@ struct Batch {
   int learn_features();
   int verify_and_fallback(int src);
   int solve();
+  int solve_impl();
   int result(int inst, int nspb, int *np, int *nh, int *n_qp, int *n_qh, double *ep, double *zp, double *fqp,
              double *eh, double *zh, double *fqh);
 };

@@ -13,6 +13,7 @@
 #include <new>
 #include <vector>
 
+#include "bcd_hipsync.hpp"
 #include "bcd_batch.hpp"
 #include "bcd_mlp.hpp"
 #include "bcd_ctx.hpp"
@@ -120,8 +121,8 @@ int stage_batch(bcd_ctx *ctx, const bcd_prop *g, int mode, Batch &b) {
 
 struct DevBuf {
   void *p = nullptr;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  int alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 8) == hipSuccess ? BCD_OK : BCD_ERR_NOMEM; }
+  ~DevBuf() { if (p) (void)bcd::sync_free(p); }
+  int alloc(size_t bytes) { return bcd::sync_malloc(&p, bytes > 0 ? bytes : 8) == hipSuccess ? BCD_OK : BCD_ERR_NOMEM; }
 };
 
 // Argument checks of Reduce_Mtx_Lncz (Lanczos.cpp:50-82), returning its
@@ -194,10 +195,10 @@ int lanczos_in_region(bcd_ctx *ctx, char *base, const LancLayout &L, int nLanc, 
   HTRY(hipMemcpyAsync(&nout, base + L.n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HTRY(hipStreamSynchronize(ctx->stream));
   if (nout < 0) return nout;
-  HTRY(hipMemcpy(Mout, base + L.out, sizeof(double) * (size_t)Tout * nLanc, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(Mout, base + L.out, sizeof(double) * (size_t)Tout * nLanc, hipMemcpyDeviceToHost));
   int npv = 0;
-  HTRY(hipMemcpy(&npv, base + L.ipv, sizeof(int), hipMemcpyDeviceToHost));
-  HTRY(hipMemcpy(i_piv_out, base + L.ipv, sizeof(int) * (npv + 2), hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(&npv, base + L.ipv, sizeof(int), hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(i_piv_out, base + L.ipv, sizeof(int) * (npv + 2), hipMemcpyDeviceToHost));
   // the reference normalises every pivot it starts from in place
   int ip = -1;
   for (int k = 0; k < npv; ++k) {
@@ -328,10 +329,10 @@ int bcd_sigma2_build(bcd_ctx *ctx, int ish, const bcd_prop *g, double *rows_fw, 
   if (cs.mfw > cap_fw || cs.mbk > cap_bk) return BCD_ERR_CAPACITY;
   const int w = 2 + ctx->sys.no[ish];
   long bf = 0, bb = 0;
-  HTRY(hipMemcpy(&bf, b.d_row_fw_base + ish, sizeof(long), hipMemcpyDeviceToHost));
-  HTRY(hipMemcpy(&bb, b.d_row_bk_base + ish, sizeof(long), hipMemcpyDeviceToHost));
-  if (cs.mfw && rows_fw) HTRY(hipMemcpy(rows_fw, b.d_rows_fw + bf, sizeof(double) * cs.mfw * w, hipMemcpyDeviceToHost));
-  if (cs.mbk && rows_bk) HTRY(hipMemcpy(rows_bk, b.d_rows_bk + bb, sizeof(double) * cs.mbk * w, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(&bf, b.d_row_fw_base + ish, sizeof(long), hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(&bb, b.d_row_bk_base + ish, sizeof(long), hipMemcpyDeviceToHost));
+  if (cs.mfw && rows_fw) HTRY(bcd::sync_memcpy(rows_fw, b.d_rows_fw + bf, sizeof(double) * cs.mfw * w, hipMemcpyDeviceToHost));
+  if (cs.mbk && rows_bk) HTRY(bcd::sync_memcpy(rows_bk, b.d_rows_bk + bb, sizeof(double) * cs.mbk * w, hipMemcpyDeviceToHost));
   return BCD_OK;
 }
 
@@ -363,7 +364,7 @@ int bcd_reduce_mtx_lncz(bcd_ctx *ctx, int NA_out, int NB_out, int NPLS_out, doub
   HTRY(hipSetDevice(ctx->device));
   DevBuf dmin;
   BTRY(dmin.alloc(sizeof(double) * (size_t)NPLS_in * (NA_in + NB_in)));
-  HTRY(hipMemcpy(dmin.p, Min, sizeof(double) * (size_t)NPLS_in * (NA_in + NB_in), hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(dmin.p, Min, sizeof(double) * (size_t)NPLS_in * (NA_in + NB_in), hipMemcpyHostToDevice));
   return run_lanczos(ctx, NA_out, NB_out, NPLS_out, Mout, i_piv_out, NA_in, NB_in, NPLS_in,
                      (const double *)dmin.p, nt_pvts, pivots, n_itr);
 }
@@ -418,17 +419,17 @@ int bcd_reduce_selfen_lncz(bcd_ctx *ctx, int n_orb, int nfw, double *rows_fw, in
                        (double *)out.p);
   };
   if (nfw) {
-    HTRY(hipMemcpy(dfw_in.p, rows_fw, sizeof(double) * (size_t)nfw * T, hipMemcpyHostToDevice));
+    HTRY(bcd::sync_memcpy(dfw_in.p, rows_fw, sizeof(double) * (size_t)nfw * T, hipMemcpyHostToDevice));
     heap(dfw_in, nfw, 0, dfw);
   }
   if (nbk) {
-    HTRY(hipMemcpy(dbk_in.p, rows_bk, sizeof(double) * (size_t)nbk * T, hipMemcpyHostToDevice));
+    HTRY(bcd::sync_memcpy(dbk_in.p, rows_bk, sizeof(double) * (size_t)nbk * T, hipMemcpyHostToDevice));
     heap(dbk_in, nbk, 1, dbk);
   }
   HTRY(hipGetLastError());
   HTRY(hipStreamSynchronize(ctx->stream));
-  if (nfw) HTRY(hipMemcpy(rows_fw, dfw.p, sizeof(double) * (size_t)nfw * T, hipMemcpyDeviceToHost));
-  if (nbk) HTRY(hipMemcpy(rows_bk, dbk.p, sizeof(double) * (size_t)nbk * T, hipMemcpyDeviceToHost));
+  if (nfw) HTRY(bcd::sync_memcpy(rows_fw, dfw.p, sizeof(double) * (size_t)nfw * T, hipMemcpyDeviceToHost));
+  if (nbk) HTRY(bcd::sync_memcpy(rows_bk, dbk.p, sizeof(double) * (size_t)nbk * T, hipMemcpyDeviceToHost));
   int tot[2] = {0, 0};
   for (int k = 0; k < nt_pvts; ++k) {
     if (n_itr_fw[k] > 0) tot[0] += n_itr_fw[k];
@@ -491,17 +492,17 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   BTRY(dj.alloc(sizeof(DysonJob) * 2));
   BTRY(dja.alloc(sizeof(ArrowJob)));
   BTRY(dst.alloc(sizeof(int)));
-  HTRY(hipMemcpy(dh.p, h, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(du.p, u1, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
-  if (nfw) HTRY(hipMemcpy(dfw.p, rows_fw, sizeof(double) * (size_t)nfw * Tf, hipMemcpyHostToDevice));
-  if (nbk) HTRY(hipMemcpy(dbk.p, rows_bk, sizeof(double) * (size_t)nbk * Tb, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(dh.p, h, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(du.p, u1, sizeof(double) * n_orb * n_orb, hipMemcpyHostToDevice));
+  if (nfw) HTRY(bcd::sync_memcpy(dfw.p, rows_fw, sizeof(double) * (size_t)nfw * Tf, hipMemcpyHostToDevice));
+  if (nbk) HTRY(bcd::sync_memcpy(dbk.p, rows_bk, sizeof(double) * (size_t)nbk * Tb, hipMemcpyHostToDevice));
   int pf[kMaxPiv + 2], pb[kMaxPiv + 2];
   for (int k = 0; k < kMaxPiv + 2; ++k) { pf[k] = -100; pb[k] = -100; }
   if (npiv_fw > 0) std::copy(i_piv_fw, i_piv_fw + npiv_fw + 2, pf);
   if (npiv_bk > 0) std::copy(i_piv_bk, i_piv_bk + npiv_bk + 2, pb);
-  HTRY(hipMemcpy(dpf.p, pf, sizeof pf, hipMemcpyHostToDevice));
-  HTRY(hipMemcpy(dpb.p, pb, sizeof pb, hipMemcpyHostToDevice));
-  HTRY(hipMemset(dst.p, 0, sizeof(int)));
+  HTRY(bcd::sync_memcpy(dpf.p, pf, sizeof pf, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(dpb.p, pb, sizeof pb, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memset(dst.p, 0, sizeof(int)));
   const double *rows[2] = {(const double *)dfw.p, (const double *)dbk.p};
   const int *ipv[2] = {(const int *)dpf.p, (const int *)dpb.p};
   double *pole = (double *)dpole.p;
@@ -538,7 +539,7 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   HTRY(hipFuncSetAttribute((const void *)k_ql, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
   if (arrow_set_lds_limit(kLdsBytes)) return BCD_ERR_DEVICE;
   if (njob) {
-    HTRY(hipMemcpy(dj.p, P, sizeof(DysonJob) * njob, hipMemcpyHostToDevice));
+    HTRY(bcd::sync_memcpy(dj.p, P, sizeof(DysonJob) * njob, hipMemcpyHostToDevice));
     const DysonJob *jp = (const DysonJob *)dj.p;
     if (t_lds)
       hipLaunchKernelGGL(k_tridiag<true>, dim3(njob), dim3(256), t_aux + (size_t)lside * lside * sizeof(double),
@@ -554,13 +555,13 @@ int bcd_dyson(bcd_ctx *ctx, int n_orb, int j2, const double *h, const double *u1
   HTRY(hipGetLastError());
   HTRY(hipStreamSynchronize(ctx->stream));
   int st = 0, cnt[2] = {0, 0};
-  HTRY(hipMemcpy(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost));
   if (st) return BCD_ERR_EIGEN;
-  HTRY(hipMemcpy(cnt, dc.p, sizeof cnt, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(cnt, dc.p, sizeof cnt, hipMemcpyDeviceToHost));
   std::vector<double> e(n), x((size_t)n * n_orb);
-  HTRY(hipMemcpy(e.data(), de.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-  HTRY(hipMemcpy(x.data(), dx.p, sizeof(double) * (size_t)n * n_orb, hipMemcpyDeviceToHost));
-  if (kolt3) HTRY(hipMemcpy(kolt3, dk.p, sizeof(double) * 3, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(e.data(), de.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(x.data(), dx.p, sizeof(double) * (size_t)n * n_orb, hipMemcpyDeviceToHost));
+  if (kolt3) HTRY(bcd::sync_memcpy(kolt3, dk.p, sizeof(double) * 3, hipMemcpyDeviceToHost));
   const int w = n_orb + 2;
   for (int k = 0; k < n; ++k) {
     double *rec = k < cnt[0] ? sol_qp + (size_t)k * w : sol_qh + (size_t)(k - cnt[0]) * w;
@@ -597,9 +598,9 @@ int bcd_syevd_batched(bcd_ctx *ctx, int nmat, const int *n, const long *off, dou
     BTRY(dk.alloc(sizeof(double) * 3));
     BTRY(dj.alloc(sizeof(DysonJob)));
     BTRY(dst.alloc(sizeof(int)));
-    HTRY(hipMemcpy(dh.p, a + off[k], sizeof(double) * m * m, hipMemcpyHostToDevice));
-    HTRY(hipMemset(du.p, 0, sizeof(double) * m * m));
-    HTRY(hipMemset(dst.p, 0, sizeof(int)));
+    HTRY(bcd::sync_memcpy(dh.p, a + off[k], sizeof(double) * m * m, hipMemcpyHostToDevice));
+    HTRY(bcd::sync_memset(du.p, 0, sizeof(double) * m * m));
+    HTRY(bcd::sync_memset(dst.p, 0, sizeof(int)));
     DysonJob J;
     std::memset(&J, 0, sizeof J);
     J.nI = m; J.n = m; J.inst = -1; J.den_fw = J.den_bk = 2; J.efermi = HUGE_VAL;
@@ -607,7 +608,7 @@ int bcd_syevd_batched(bcd_ctx *ctx, int nmat, const int *n, const long *off, dou
     J.tri = (double *)dtri.p; J.mat = (double *)dmat.p;
     J.out_e = (double *)de.p; J.out_x = (double *)dx.p;
     J.out_nqp = (int *)dc.p; J.out_nqh = (int *)dc.p + 1; J.kolt = (double *)dk.p;
-    HTRY(hipMemcpy(dj.p, &J, sizeof J, hipMemcpyHostToDevice));
+    HTRY(bcd::sync_memcpy(dj.p, &J, sizeof J, hipMemcpyHostToDevice));
     const DysonJob *jp = (const DysonJob *)dj.p;
     HTRY(hipFuncSetAttribute((const void *)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
     HTRY(hipFuncSetAttribute((const void *)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
@@ -621,11 +622,11 @@ int bcd_syevd_batched(bcd_ctx *ctx, int nmat, const int *n, const long *off, dou
     HTRY(hipGetLastError());
     HTRY(hipStreamSynchronize(ctx->stream));
     int st = 0;
-    HTRY(hipMemcpy(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost));
     if (st) return BCD_ERR_EIGEN;
     std::vector<double> e(m), x((size_t)m * m);
-    HTRY(hipMemcpy(e.data(), de.p, sizeof(double) * m, hipMemcpyDeviceToHost));
-    HTRY(hipMemcpy(x.data(), dx.p, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(e.data(), de.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(x.data(), dx.p, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost));
     // all values sit below the infinite split point, so the records come out
     // by descending value: reverse to LAPACK's ascending order
     for (int i = 0; i < m; ++i) {
@@ -702,7 +703,7 @@ int bcd_batch_add(bcd_batch *b, int inst, const bcd_modspace *ms, const bcd_vpp 
   HTRY(hipSetDevice(b->device));
   DevArena &A = b->vbufs[inst];
   BTRY(A.grow(sizeof(double) * (size_t)std::max<long>(S->v_len, 1)));
-  if (S->v_len) HTRY(hipMemcpy(A.ptr, S->V.data(), sizeof(double) * S->v_len, hipMemcpyHostToDevice));
+  if (S->v_len) HTRY(bcd::sync_memcpy(A.ptr, S->V.data(), sizeof(double) * S->v_len, hipMemcpyHostToDevice));
   S->dV = (const double *)A.ptr;
   std::vector<double>().swap(S->V);
   b->g0[inst] = std::move(G);
@@ -753,11 +754,11 @@ int bcd_pivot_round(int device, const double *x, double *out, long n) {
   HTRY(hipSetDevice(device));
   DevBuf d;
   BTRY(d.alloc(sizeof(double) * (size_t)n));
-  HTRY(hipMemcpy(d.p, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+  HTRY(bcd::sync_memcpy(d.p, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_pivot_round, dim3((unsigned)std::min<long>((n + 255) / 256, 4096)), dim3(256), 0, nullptr,
                      (const double *)d.p, (double *)d.p, n);
   HTRY(hipGetLastError());
-  HTRY(hipMemcpy(out, d.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  HTRY(bcd::sync_memcpy(out, d.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
   return BCD_OK;
 }
 
@@ -835,9 +836,9 @@ int bcd_batch_learning_data(bcd_batch *b, float *feat, float *target, int *chan_
   Batch &B = b->b;
   HTRY(hipSetDevice(b->device));
   HTRY(hipStreamSynchronize(B.stream));
-  if (feat) HTRY(hipMemcpy(feat, B.d_feat, sizeof(float) * (size_t)n_chan * BCD_LEARN_IN, hipMemcpyDeviceToHost));
+  if (feat) HTRY(bcd::sync_memcpy(feat, B.d_feat, sizeof(float) * (size_t)n_chan * BCD_LEARN_IN, hipMemcpyDeviceToHost));
   if (target)
-    HTRY(hipMemcpy(target, B.d_target, sizeof(float) * (size_t)n_chan * BCD_LEARN_OUT, hipMemcpyDeviceToHost));
+    HTRY(bcd::sync_memcpy(target, B.d_target, sizeof(float) * (size_t)n_chan * BCD_LEARN_OUT, hipMemcpyDeviceToHost));
   if (chan_inst)
     for (int gc = 0; gc < n_chan; ++gc) chan_inst[gc] = B.h_chans[gc].inst;
   return BCD_OK;

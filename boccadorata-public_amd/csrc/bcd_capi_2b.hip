@@ -12,6 +12,7 @@
 #include <new>
 #include <vector>
 
+#include "bcd_hipsync.hpp"
 #include "bcd_ctx.hpp"
 #include "bcdor_amd.h"
 
@@ -28,7 +29,7 @@ struct bcd_t2 {
   double kt_ms[kT2Kernels] = {};
   long kt_n[kT2Kernels] = {};
   ~bcd_t2() {
-    for (void *p : bufs) (void)hipFree(p);
+    for (void *p : bufs) (void)bcd::sync_free(p);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -73,9 +74,9 @@ HostFrags host_frags(const HostSystem &S, const bcd_prop *g) {
 template <class T>
 int upload(std::vector<void *> &bufs, const T *src, size_t n, const T **dst) {
   void *p = nullptr;
-  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return BCD_ERR_NOMEM;
+  if (bcd::sync_malloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return BCD_ERR_NOMEM;
   bufs.push_back(p);
-  if (n && hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return BCD_ERR_DEVICE;
+  if (n && bcd::sync_memcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return BCD_ERR_DEVICE;
   *dst = (const T *)p;
   return BCD_OK;
 }
@@ -180,7 +181,7 @@ int fill_common(bcd_ctx *ctx, const bcd_prop *g, T2Job job, double *Sab, int ld,
   BTRY(t.plan.launch(*ctx->t2, t.fd, ctx->stream));
   HTRY(hipStreamSynchronize(ctx->stream));
   if (ndim > 0)
-    HTRY(hipMemcpy(Sab, t.plan.d_out + t.plan.jobs[0].out_off, sizeof(double) * ((size_t)(ndim - 1) * ld + ndim),
+    HTRY(bcd::sync_memcpy(Sab, t.plan.d_out + t.plan.jobs[0].out_off, sizeof(double) * ((size_t)(ndim - 1) * ld + ndim),
                    hipMemcpyDeviceToHost));
   if (free_poles)
     for (int k = 0; k < ndim; ++k)
@@ -335,7 +336,7 @@ int bcd_t2_matrix(const bcd_t2 *t, int k, double *Sab, double *free_poles) {
   const T2Job &j = t->plan.jobs[k];
   const int ndim = j.nf + j.nb;
   if (Sab && ndim > 0)
-    HTRY(hipMemcpy(Sab, t->plan.d_out + j.out_off, sizeof(double) * ((size_t)(ndim - 1) * j.ld + ndim),
+    HTRY(bcd::sync_memcpy(Sab, t->plan.d_out + j.out_off, sizeof(double) * ((size_t)(ndim - 1) * j.ld + ndim),
                    hipMemcpyDeviceToHost));
   if (free_poles)
     for (int i = 0; i < ndim; ++i)

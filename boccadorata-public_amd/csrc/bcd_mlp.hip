@@ -12,6 +12,7 @@
 #include <random>
 #include <vector>
 
+#include "bcd_hipsync.hpp"
 #include "bcd_mlp.hpp"
 #include "bcdor_amd.h"
 
@@ -197,7 +198,7 @@ Mlp::~Mlp() {
   if (device >= 0) (void)hipSetDevice(device);
   for (void *q : {(void *)p, (void *)g, (void *)m1, (void *)m2, (void *)wb, (void *)x, (void *)h1, (void *)h2,
                   (void *)y, (void *)t, (void *)dy, (void *)dh2, (void *)dh1, (void *)part, (void *)perm})
-    if (q) (void)hipFree(q);
+    if (q) (void)bcd::sync_free(q);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -213,11 +214,11 @@ int Mlp::init(int dev, int nin, int nhid, int nout, unsigned long long seed) {
   n_par = o;
   MTRY(hipSetDevice(dev));
   MTRY(hipStreamCreate(&stream));
-  MTRY(hipMalloc(&p, sizeof(float) * n_par));
-  MTRY(hipMalloc(&g, sizeof(float) * n_par));
-  MTRY(hipMalloc(&m1, sizeof(float) * n_par));
-  MTRY(hipMalloc(&m2, sizeof(float) * n_par));
-  MTRY(hipMalloc(&wb, sizeof(unsigned short) * n_par));
+  MTRY(bcd::sync_malloc(&p, sizeof(float) * n_par));
+  MTRY(bcd::sync_malloc(&g, sizeof(float) * n_par));
+  MTRY(bcd::sync_malloc(&m1, sizeof(float) * n_par));
+  MTRY(bcd::sync_malloc(&m2, sizeof(float) * n_par));
+  MTRY(bcd::sync_malloc(&wb, sizeof(unsigned short) * n_par));
   // He initialisation of the weights, zero biases
   std::vector<float> h(n_par, 0.f);
   std::mt19937_64 rng(seed);
@@ -226,10 +227,10 @@ int Mlp::init(int dev, int nin, int nhid, int nout, unsigned long long seed) {
     std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (float)fan[l]) * (l == 2 ? 0.1f : 1.f));
     for (long i = 0; i < sz[2 * l]; ++i) h[off_w[l] + i] = nd(rng);
   }
-  MTRY(hipMemcpy(p, h.data(), sizeof(float) * n_par, hipMemcpyHostToDevice));
-  MTRY(hipMemset(m1, 0, sizeof(float) * n_par));
-  MTRY(hipMemset(m2, 0, sizeof(float) * n_par));
-  MTRY(hipMemset(g, 0, sizeof(float) * n_par));
+  MTRY(bcd::sync_memcpy(p, h.data(), sizeof(float) * n_par, hipMemcpyHostToDevice));
+  MTRY(bcd::sync_memset(m1, 0, sizeof(float) * n_par));
+  MTRY(bcd::sync_memset(m2, 0, sizeof(float) * n_par));
+  MTRY(bcd::sync_memset(g, 0, sizeof(float) * n_par));
   adam_t = 0;
   int r = refresh_bf16(stream);
   if (r) return r;
@@ -242,19 +243,19 @@ int Mlp::reserve(int rows) {
   MTRY(hipSetDevice(device));
   MTRY(hipStreamSynchronize(stream));
   for (float **q : {&x, &h1, &h2, &y, &t, &dy, &dh2, &dh1, &part})
-    if (*q) { MTRY(hipFree(*q)); *q = nullptr; }
-  if (perm) { MTRY(hipFree(perm)); perm = nullptr; }
+    if (*q) { MTRY(bcd::sync_free(*q)); *q = nullptr; }
+  if (perm) { MTRY(bcd::sync_free(perm)); perm = nullptr; }
   const long r = rows;
-  MTRY(hipMalloc(&x, sizeof(float) * r * n_in));
-  MTRY(hipMalloc(&h1, sizeof(float) * r * n_hid));
-  MTRY(hipMalloc(&h2, sizeof(float) * r * n_hid));
-  MTRY(hipMalloc(&y, sizeof(float) * r * n_out));
-  MTRY(hipMalloc(&t, sizeof(float) * r * n_out));
-  MTRY(hipMalloc(&dy, sizeof(float) * r * n_out));
-  MTRY(hipMalloc(&dh2, sizeof(float) * r * n_hid));
-  MTRY(hipMalloc(&dh1, sizeof(float) * r * n_hid));
-  MTRY(hipMalloc(&part, sizeof(float) * 1056));   // loss partials [0, 1024), the loss at 1024
-  MTRY(hipMalloc(&perm, sizeof(int) * r));
+  MTRY(bcd::sync_malloc(&x, sizeof(float) * r * n_in));
+  MTRY(bcd::sync_malloc(&h1, sizeof(float) * r * n_hid));
+  MTRY(bcd::sync_malloc(&h2, sizeof(float) * r * n_hid));
+  MTRY(bcd::sync_malloc(&y, sizeof(float) * r * n_out));
+  MTRY(bcd::sync_malloc(&t, sizeof(float) * r * n_out));
+  MTRY(bcd::sync_malloc(&dy, sizeof(float) * r * n_out));
+  MTRY(bcd::sync_malloc(&dh2, sizeof(float) * r * n_hid));
+  MTRY(bcd::sync_malloc(&dh1, sizeof(float) * r * n_hid));
+  MTRY(bcd::sync_malloc(&part, sizeof(float) * 1056));   // loss partials [0, 1024), the loss at 1024
+  MTRY(bcd::sync_malloc(&perm, sizeof(int) * r));
   cap = rows;
   return BCD_OK;
 }
@@ -355,7 +356,7 @@ int bcd_mlp_get_params(bcd_mlp *m, float *p) {
   if (!m || !p) return BCD_ERR_ARG;
   CTRY(hipSetDevice(m->net.device));
   CTRY(hipStreamSynchronize(m->net.stream));
-  CTRY(hipMemcpy(p, m->net.p, sizeof(float) * m->net.n_par, hipMemcpyDeviceToHost));
+  CTRY(bcd::sync_memcpy(p, m->net.p, sizeof(float) * m->net.n_par, hipMemcpyDeviceToHost));
   return BCD_OK;
 }
 
@@ -363,7 +364,7 @@ int bcd_mlp_set_params(bcd_mlp *m, const float *p) {
   if (!m || !p) return BCD_ERR_ARG;
   CTRY(hipSetDevice(m->net.device));
   CTRY(hipStreamSynchronize(m->net.stream));
-  CTRY(hipMemcpy(m->net.p, p, sizeof(float) * m->net.n_par, hipMemcpyHostToDevice));
+  CTRY(bcd::sync_memcpy(m->net.p, p, sizeof(float) * m->net.n_par, hipMemcpyHostToDevice));
   RTRY(m->net.refresh_bf16(m->net.stream));
   CTRY(hipStreamSynchronize(m->net.stream));
   return BCD_OK;
@@ -410,15 +411,15 @@ int bcd_mlp_fit(bcd_mlp *m, int n, const float *x, const float *t, int epochs, i
   int *P = nullptr;
   const int nbat = (n + batch - 1) / batch;
   auto cleanup = [&]() {
-    if (X) (void)hipFree(X);
-    if (T) (void)hipFree(T);
-    if (L) (void)hipFree(L);
-    if (P) (void)hipFree(P);
+    if (X) (void)bcd::sync_free(X);
+    if (T) (void)bcd::sync_free(T);
+    if (L) (void)bcd::sync_free(L);
+    if (P) (void)bcd::sync_free(P);
   };
-  if (hipMalloc(&X, sizeof(float) * (long)n * N.n_in) != hipSuccess ||
-      hipMalloc(&T, sizeof(float) * (long)n * N.n_out) != hipSuccess ||
-      hipMalloc(&L, sizeof(float) * std::max(1, epochs) * (long)nbat) != hipSuccess ||
-      hipMalloc(&P, sizeof(int) * (long)n) != hipSuccess) {
+  if (bcd::sync_malloc(&X, sizeof(float) * (long)n * N.n_in) != hipSuccess ||
+      bcd::sync_malloc(&T, sizeof(float) * (long)n * N.n_out) != hipSuccess ||
+      bcd::sync_malloc(&L, sizeof(float) * std::max(1, epochs) * (long)nbat) != hipSuccess ||
+      bcd::sync_malloc(&P, sizeof(int) * (long)n) != hipSuccess) {
     cleanup();
     return BCD_ERR_NOMEM;
   }
@@ -426,8 +427,8 @@ int bcd_mlp_fit(bcd_mlp *m, int n, const float *x, const float *t, int epochs, i
   std::vector<int> perm(n);
   std::mt19937_64 rng(seed);
   do {
-    if (hipMemcpy(X, x, sizeof(float) * (long)n * N.n_in, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(T, t, sizeof(float) * (long)n * N.n_out, hipMemcpyHostToDevice) != hipSuccess) {
+    if (bcd::sync_memcpy(X, x, sizeof(float) * (long)n * N.n_in, hipMemcpyHostToDevice) != hipSuccess ||
+        bcd::sync_memcpy(T, t, sizeof(float) * (long)n * N.n_out, hipMemcpyHostToDevice) != hipSuccess) {
       rc = BCD_ERR_DEVICE;
       break;
     }
@@ -436,7 +437,7 @@ int bcd_mlp_fit(bcd_mlp *m, int n, const float *x, const float *t, int epochs, i
       std::shuffle(perm.begin(), perm.end(), rng);
       // the previous epoch's gathers have finished reading P
       if (hipStreamSynchronize(N.stream) != hipSuccess ||
-          hipMemcpy(P, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+          bcd::sync_memcpy(P, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
         rc = BCD_ERR_DEVICE;
         break;
       }
@@ -453,7 +454,7 @@ int bcd_mlp_fit(bcd_mlp *m, int n, const float *x, const float *t, int epochs, i
     if (hipStreamSynchronize(N.stream) != hipSuccess) { rc = BCD_ERR_DEVICE; break; }
     if (epoch_loss && epochs > 0) {
       std::vector<float> h((size_t)epochs * nbat);
-      if (hipMemcpy(h.data(), L, sizeof(float) * h.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+      if (bcd::sync_memcpy(h.data(), L, sizeof(float) * h.size(), hipMemcpyDeviceToHost) != hipSuccess) {
         rc = BCD_ERR_DEVICE;
         break;
       }

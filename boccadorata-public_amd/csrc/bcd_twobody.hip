@@ -6,6 +6,7 @@
 //   k_t2_pandya Pandya-transformed interaction per ring channel (PolProp_b.cpp:569-614)
 //   k_t2_gemm   FP64 MFMA GEMMs G = T K and G T^T with the block epilogue
 //   k_t2_diag   free-pole energies on the diagonal (gII_b.cpp:552-556, PolProp_b.cpp:542-556)
+#include "bcd_hipsync.hpp"
 #include "bcd_twobody.hpp"
 
 #include <algorithm>
@@ -337,9 +338,9 @@ bool tri2(int a, int b, int c) {
 template <class T>
 int dev_copy(std::vector<void *> &bufs, const T *src, size_t n, T **dst) {
   void *p = nullptr;
-  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return BCD_ERR_NOMEM;
+  if (bcd::sync_malloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return BCD_ERR_NOMEM;
   bufs.push_back(p);
-  if (n && hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return BCD_ERR_DEVICE;
+  if (n && bcd::sync_memcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return BCD_ERR_DEVICE;
   *dst = (T *)p;
   return BCD_OK;
 }
@@ -347,7 +348,7 @@ int dev_copy(std::vector<void *> &bufs, const T *src, size_t n, T **dst) {
 }  // namespace
 
 void T2System::release() {
-  for (void *p : bufs) (void)hipFree(p);
+  for (void *p : bufs) (void)bcd::sync_free(p);
   bufs.clear();
   d_pairs = nullptr;
   d_jidx = nullptr;
@@ -656,7 +657,7 @@ struct Emit {
 }  // namespace
 
 void T2Plan::release() {
-  for (void *p : bufs) (void)hipFree(p);
+  for (void *p : bufs) (void)bcd::sync_free(p);
   bufs.clear();
   d_out = nullptr;
   d_sixj = nullptr;
@@ -780,7 +781,7 @@ int T2Plan::prepare(const T2System &S, const HostSystem &H, const HostFrags &F, 
   emit_all(unused);
   // allocate
   auto dmalloc = [&](size_t bytes, void **p) {
-    if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return BCD_ERR_NOMEM;
+    if (bcd::sync_malloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return BCD_ERR_NOMEM;
     bufs.push_back(*p);
     return BCD_OK;
   };

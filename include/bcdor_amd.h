@@ -253,7 +253,10 @@ int bcd_batch_set_profile(bcd_batch *b, int on);
  * flags; a full chunk is captured once per stage into a HIP graph and
  * replayed (on by default; off while profiling, and for the chunk that
  * applies a learned start).  on = 0 launches every sweep directly.  Results
- * are the same either way. */
+ * are the same either way.  The library makes its own device-synchronising
+ * HIP calls (null-stream copies, allocation) under a lock it shares with its
+ * captures; a program that makes such calls itself on other host threads
+ * while a batch solves should switch the batch's graphs off. */
 int bcd_batch_set_graphs(bcd_batch *b, int on);
 /* Per-stage device time of the last solve in ms (hipEvents on the batch's
  * stream): [0] W/Sigma^inf set-up, [1] HF loop, [2] Sigma(2) build,

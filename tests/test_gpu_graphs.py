@@ -35,3 +35,38 @@ def test_graph_replay_equals_direct_launches(accel):
             assert np.array_equal(g.ep, ref[i][0].ep) and np.array_equal(g.eh, ref[i][0].eh), (rep, i)
             assert np.array_equal(g.fqp, ref[i][0].fqp) and np.array_equal(g.fqh, ref[i][0].fqh), (rep, i)
     print("  sc0 ms with graphs", b.timings()["sc0"], flush=True)
+
+
+def test_batches_on_concurrent_host_threads():
+    """Three batches solved at once from three host threads (each capturing
+    its sweep chunks into graphs on its own streams while the others copy
+    synchronously) give what each gives alone."""
+    import threading
+    sp = [s for s in synth.batched_specs(64, 4) if s.nmax <= 7][:12]
+    parts = [sp[k::3] for k in range(3)]
+    batches = [capi.Batch.streamed(len(p), lambda i, p=p: p[i].make(), params(5), lanc_n=[s.lanc_n for s in p],
+                                   threads=4) for p in parts]
+    alone = []
+    for b in batches:
+        b.solve()
+        alone.append([b.result(i).ep.copy() for i in range(b.n)])
+    errors = []
+
+    def run(b):
+        try:
+            for _ in range(3):
+                b.solve()
+                for i in range(b.n):
+                    b.result(i)
+        except Exception as e:          # reported below, in the main thread
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=run, args=(b,)) for b in batches]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for b, ref in zip(batches, alone):
+        for i in range(b.n):
+            assert np.array_equal(b.result(i).ep, ref[i]), i
