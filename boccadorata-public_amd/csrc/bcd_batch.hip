@@ -748,11 +748,37 @@ int Batch::build_sigma2(int only_ish) {
   auto kern = par.precision == 1 ? k_sigma2<float, double, 1>
               : par.precision == 2 ? k_sigma2<float, double, 2>
               : par.precision == 3 ? k_sigma2<float, double, 0> : k_sigma2<double, double, 0>;
-  const size_t lds = sigma2_lds(vt_cap, c_cap, par.precision == 1 || par.precision == 2);
-  if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(kern, dim3(kSigGrid), dim3(64), lds, stream, d_insts, d_chans, d_blocks, P,
-                     d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base, d_row_bk_base, d_rows_fw,
-                     d_rows_bk, vt_cap, c_cap);
+  // one launch per V-table size class, each with the LDS of its largest
+  // table, the classes on concurrent streams (they write disjoint rows)
+  // (blocks without (alpha, beta) pairs have nvt = 0 and still write their
+  // rows: the first class starts below 0)
+  std::vector<std::pair<int, int>> classes;
+  int lo = -1;
+  for (int cap : kSigVtClass) {
+    if (lo >= vt_cap) break;
+    classes.push_back({lo, std::min(cap, vt_cap)});
+    lo = cap;
+  }
+  if (lo < vt_cap) classes.push_back({lo, vt_cap});
+  const int naux = std::min(kLancAux, (int)classes.size() - 1);
+  if (naux > 0) {
+    HTRY(hipEventRecord(lfork, stream));
+    for (int k = 0; k < naux; ++k) HTRY(hipStreamWaitEvent(lstream[k], lfork, 0));
+  }
+  for (int c = 0; c < (int)classes.size(); ++c) {
+    const int cap = (classes[c].second + 1) & ~1;
+    const size_t lds = sigma2_lds(cap, c_cap, par.precision == 1 || par.precision == 2);
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const hipStream_t sc = c ? lstream[(c - 1) % kLancAux] : stream;
+    hipLaunchKernelGGL(kern, dim3(kSigGrid), dim3(64), lds, sc, d_insts, d_chans, d_blocks, P, d_blk_fw_off,
+                       d_blk_bk_off, d_work_list, d_work_count, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk,
+                       classes[c].first, classes[c].second, c_cap);
+  }
+  for (int k = 0; k < naux; ++k) {
+    HTRY(hipEventRecord(ljoin[k], lstream[k]));
+    HTRY(hipStreamWaitEvent(stream, ljoin[k], 0));
+  }
   kt_end(KT_SIGMA2);
   HTRY(hipGetLastError());
   return BCD_OK;

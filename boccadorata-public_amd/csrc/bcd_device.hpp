@@ -91,6 +91,8 @@ struct Sig2Work {
   int bidx, gc, inst;
   int n01, n23, n45;             // counts 0..5 as (lo | hi << 16)
   int off_fw, off_bk;
+  int nvt;                       // entries of the block's V table, nI * pairs * n_g (its LDS size class)
+  int _pad;
 };
 
 // Per-channel dynamic state (fragment counts live in separate arrays).

@@ -364,6 +364,13 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
         W.bidx = (int)(C.blk_off + b); W.gc = gc; W.inst = C.inst;
         W.n01 = f6[0] | (f6[1] << 16); W.n23 = f6[2] | (f6[3] << 16); W.n45 = f6[4] | (f6[5] << 16);
         W.off_fw = ofw; W.off_bk = obk;
+        {
+          const BlockDev B = blocks[C.blk_off + b];
+          const int noa = I.no[B.a], nob = I.no[B.b];
+          const int npair = B.a == B.b ? noa * (noa + 1) / 2 - B.nJab * noa : noa * nob;
+          W.nvt = C.nI * npair * I.no[B.g];
+          W._pad = 0;
+        }
         work_list[base_nz + pz + __popcll(mask & ((1ull << lane) - 1ull))] = W;
       }
     }
@@ -594,7 +601,8 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
                                                const int *blk_fw_off, const int *blk_bk_off,
                                                const Sig2Work *work_list, const int *work_count,
                                                const long *row_fw_base, const long *row_bk_base,
-                                               double *rows_fw, double *rows_bk, int vt_cap, int c_cap) {
+                                               double *rows_fw, double *rows_bk, int vt_lo, int vt_cap,
+                                               int c_cap) {
 #pragma clang fp contract(off)
   // dynamic LDS: the block's V table (vt_cap entries, the batch's largest),
   // kSigStage staged amplitudes, kSigRowBuf pass-1 couplings, then the pair
@@ -610,6 +618,9 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
   const int total = *work_count;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     const Sig2Work Wk = work_list[t];
+    // this launch builds the blocks of one V-table size class (vt_lo, vt_cap]
+    // (vt_lo = -1 for the first class: blocks without pairs have nvt = 0)
+    if (Wk.nvt <= vt_lo || Wk.nvt > vt_cap) continue;
     const BlockDev B = blocks[Wk.bidx];
     const ChanDev C = chans[Wk.gc];
     const InstDev I = insts[Wk.inst];
@@ -748,7 +759,7 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
 #define BCD_SIGMA2_INST(T, A, MF)                                                                           \
   template __global__ void k_sigma2<T, A, MF>(const InstDev *, const ChanDev *, const BlockDev *, PropView, const int *, \
                                           const int *, const Sig2Work *, const int *, const long *, const long *,  \
-                                          double *, double *, int, int);
+                                          double *, double *, int, int, int);
 BCD_SIGMA2_INST(double, double, 0)
 BCD_SIGMA2_INST(float, double, 0)
 BCD_SIGMA2_INST(float, double, 1)

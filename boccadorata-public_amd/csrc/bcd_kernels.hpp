@@ -40,13 +40,16 @@ __global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *
 __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
                               int *blk_bk_off, Sig2Work *work_list, int *work_count, int only_ish);
+// V-table size classes of the Sigma(2) build: each class is launched with
+// LDS for its largest table, so the many small blocks run at high occupancy
+constexpr int kSigVtClass[] = {256, 768};
 constexpr int kSigMf = 48;        // largest (alpha, beta) pairs, (na, nb) fragment pairs and nI * n_g of a
                                   // block side contracted on the matrix cores
 template <class T, class A, int MF>
 __global__ void k_sigma2(const InstDev *insts, const ChanDev *chans, const BlockDev *blocks, PropView P,
                          const int *blk_fw_off, const int *blk_bk_off, const Sig2Work *work_list,
                          const int *work_count, const long *row_fw_base, const long *row_bk_base,
-                         double *rows_fw, double *rows_bk, int vt_cap, int c_cap);
+                         double *rows_fw, double *rows_bk, int vt_lo, int vt_cap, int c_cap);
 // dynamic LDS bytes of k_sigma2 for a V table of vt_cap doubles
 // dynamic LDS bytes of k_sigma2: V table of vt_cap entries, the staged
 // amplitudes and couplings, and the pair rows (c_cap = the largest nI * n_g)
