@@ -481,6 +481,10 @@ LANC_BUCKETS = len(LANC_BUCKET_MAX)
 # a second valid shape per bucket, different from the default one (its block
 # sums are split differently): the rounding probe of a solve
 LANC_ALT_SHAPES = [1, 2, 3, 7, 11, 13, 14, 15, 16, 16]
+# the autotuner keeps the shape of the next larger occupied bucket for a
+# bucket where that shape is within this fraction of the fastest (both
+# buckets then run as one launch)
+LANC_MERGE_TOL = 0.05
 
 
 def solve_params(mode=2, hf_itr_max=-1, sc_itr_max=-1, lanczos=True, lanc_kind=1, lanc_n=50, conv_kind=1,

@@ -787,7 +787,7 @@ int Batch::build_sigma2(int only_ish) {
 // Time every tile shape that holds the bucket's largest job (and is at most
 // four times wider than the bucket) on the bucket's own jobs, keep the
 // fastest.  The jobs are idempotent: each run rebuilds the same outputs.
-int Batch::tune_bucket(int bk, int mmax, int nj, const int *ord) {
+int Batch::tune_bucket(int bk, int mmax, int nj, const int *ord, int prev) {
   hipEvent_t e0, e1;
   HTRY(hipEventCreate(&e0));
   HTRY(hipEventCreate(&e1));
@@ -805,6 +805,11 @@ int Batch::tune_bucket(int bk, int mmax, int nj, const int *ord) {
     lanc_tune_ms[bk][sh] = ms;
     if (best_ms == 0.f || ms < best_ms) { best_ms = ms; best = sh; }
   }
+  // the shape of the bucket of the next larger jobs, when it is nearly as
+  // fast here: the two buckets then run as one launch (one queue of jobs,
+  // largest first), which fills the device better than two launches
+  if (prev >= 0 && lanc_tune_ms[bk][prev] > 0.f && lanc_tune_ms[bk][prev] <= (1.f + kLancMergeTol) * best_ms)
+    best = prev;
   lanc_shape[bk] = best;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -860,42 +865,86 @@ int Batch::lanczos() {
   for (size_t k = 0; k < jm.size(); ++k) order[k] = jm[k].second;
   HTRY(hipMemcpyAsync(d_lorder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
   kt_begin(KT_LANCZOS);
-  // The bucket of the largest jobs runs on the batch's stream, the others
-  // round-robin on the extra streams, all after the fork and joined back
-  // before the stage ends.  Tuning runs the buckets one at a time so that
-  // each shape is timed alone.
+  // the buckets: [k0, k0 + nj) of the job order, largest jobs first
+  struct LBucket { int bk, k0, nj; double est; };
+  std::vector<LBucket> lb;
+  for (size_t k0 = 0; k0 < jm.size();) {
+    const int bk = jm[k0].first > 0 ? lanczos_bucket(jm[k0].first) : 0;
+    size_t k1 = k0;
+    while (k1 < jm.size() && (jm[k1].first > 0 ? lanczos_bucket(jm[k1].first) : 0) == bk) ++k1;
+    lb.push_back({bk, (int)k0, (int)(k1 - k0), 0.0});
+    k0 = k1;
+  }
+  // Tuning runs the buckets one at a time, largest first, so that each
+  // shape is timed alone.  Otherwise the buckets run concurrently on
+  // kLancStreams streams (the batch's and the first auxiliary ones), joined
+  // back before the stage ends.  The buckets are still launched largest
+  // jobs first: the longest single jobs set the stage's critical path, and
+  // launched after a bucket of thousands of blocks they would wait for free
+  // CUs.  Each goes to the stream with the least estimated work queued (list
+  // scheduling), so that no stream runs several large buckets back to back
+  // after the others have drained.  A bucket's estimate is its autotuned
+  // time alone on the device where there is one, else its bytes at the
+  // device's streaming rate or its largest job's bytes at one CU's rate,
+  // whichever is longer; plus its pole blocks.
   const bool fork = !tune_lanczos;
-  bool used[kLancAux] = {};
+  if (tune_lanczos)
+    for (size_t k = 0; k < lb.size(); ++k)
+      BTRY(tune_bucket(lb[k].bk, std::max(jm[lb[k].k0].first, 1), lb[k].nj, d_lorder + lb[k].k0,
+                       k ? lanc_shape[lb[k - 1].bk] : -1));
+  if (fork) {
+    // neighbouring buckets with the same tile shape run as one launch
+    std::vector<LBucket> merged;
+    for (const LBucket &b : lb)
+      if (!merged.empty() && lanc_shape[merged.back().bk] == lanc_shape[b.bk]) merged.back().nj += b.nj;
+      else merged.push_back(b);
+    lb.swap(merged);
+    for (LBucket &b : lb) {
+      const float tuned = lanc_tune_ms[b.bk][lanc_shape[b.bk]];
+      if (tuned > 0.f) {
+        b.est = tuned;
+      } else {
+        double bytes = 0.0, big = 0.0;
+        for (int k = b.k0; k < b.k0 + b.nj; ++k) {
+          const int gc = jm[k].second / 2, side = jm[k].second % 2;
+          const double M = jm[k].first, nI = h_chans[gc].nI;
+          const double L = std::min<double>(side ? h_chans[gc].lbk_cap : h_chans[gc].lfw_cap, M);
+          const double jb = 8.0 * M * (0.5 * L * L + L * (2.0 + nI));
+          bytes += jb;
+          big = std::max(big, jb);
+        }
+        b.est = std::max(bytes / kLancChipBytesPerMs, big / kLancCuBytesPerMs);
+      }
+      b.est += kLancPoleMs;
+    }
+    if (kLancOrderByEst)
+      std::stable_sort(lb.begin(), lb.end(), [](const LBucket &a, const LBucket &b) { return a.est > b.est; });
+  }
+  hipStream_t ls[kLancStreams] = {stream};
+  for (int a = 1; a < kLancStreams; ++a) ls[a] = lstream[a - 1];
+  double load[kLancStreams] = {};
+  bool used[kLancStreams] = {};
   if (fork) HTRY(hipEventRecord(lfork, stream));
   // outside tuning and profiling, each bucket's pole blocks are diagonalised
   // on its stream right after its Lanczos jobs (they need nothing else), so
   // that they overlap the other buckets' reductions
   const PoleCfg pc = pole_cfg();
   poles_done = fork && !profile && par.dyson_solver == 0;
-  size_t k0 = 0;
-  int q = 0;
-  while (k0 < jm.size()) {
-    const int bk = jm[k0].first > 0 ? lanczos_bucket(jm[k0].first) : 0;
-    size_t k1 = k0;
-    while (k1 < jm.size() && (jm[k1].first > 0 ? lanczos_bucket(jm[k1].first) : 0) == bk) ++k1;
-    const int nj = (int)(k1 - k0), mmax = std::max(jm[k0].first, 1);
-    if (tune_lanczos) BTRY(tune_bucket(bk, mmax, nj, d_lorder + k0));
-    hipStream_t s = stream;
-    if (fork && q > 0) {
-      const int a = (q - 1) % kLancAux;
-      if (!used[a]) HTRY(hipStreamWaitEvent(lstream[a], lfork, 0));
-      used[a] = true;
-      s = lstream[a];
-    }
-    launch_lanczos(lanc_shape[bk], nj, d_ljobs, d_lorder + k0, s);
-    if (poles_done) launch_poles(pc, d_lorder + k0, nj, s);
-    ++q;
-    k0 = k1;
+  for (const LBucket &b : lb) {
+    int a = 0;
+    if (fork)
+      for (int k = 1; k < kLancStreams; ++k)
+        if (load[k] < load[a]) a = k;
+    load[a] += b.est;
+    if (a > 0 && !used[a]) HTRY(hipStreamWaitEvent(ls[a], lfork, 0));
+    used[a] = true;
+    launch_lanczos(lanc_shape[b.bk], b.nj, d_ljobs, d_lorder + b.k0, ls[a]);
+    if (poles_done) launch_poles(pc, d_lorder + b.k0, b.nj, ls[a]);
   }
-  for (int a = 0; a < kLancAux; ++a)
+  for (int a = 1; a < kLancStreams; ++a)
     if (used[a]) {
-      HTRY(hipEventRecord(ljoin[a], lstream[a]));
-      HTRY(hipStreamWaitEvent(stream, ljoin[a], 0));
+      HTRY(hipEventRecord(ljoin[a - 1], ls[a]));
+      HTRY(hipStreamWaitEvent(stream, ljoin[a - 1], 0));
     }
   tune_lanczos = false;
   kt_end(KT_LANCZOS);
@@ -924,7 +973,7 @@ Batch::PoleCfg Batch::pole_cfg() const {
       else c.any_general = true;
     }
   c.ldw = c.lw | 1;
-  c.wlds = sizeof(double) * ((size_t)c.ldw * c.ldw + 128);
+  c.wlds = sizeof(double) * ((size_t)c.ldw * c.ldw + 2 * kPoleWaveN);
   c.in_lds = tridiag_lds(c.lmax, true) <= (size_t)kLdsBytes;
   return c;
 }
@@ -936,6 +985,11 @@ void Batch::launch_poles(const PoleCfg &c, const int *order, int njobs, hipStrea
   hipLaunchKernelGGL(k_poles_prep, dim3((njobs + 127) / 128), dim3(128), 0, s, d_chans, n_gc, d_cs, d_lrows, d_tri,
                      c.in_lds ? nullptr : (double *)a_mat.ptr, d_poles, d_pjobs, order, njobs);
   if (!c.any_wave) return;
+  // above 64 KiB the kernels need the larger dynamic-LDS limit
+  if (c.wlds > 65536) {
+    (void)hipFuncSetAttribute((const void *)k_tridiag_poles<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.wlds);
+    (void)hipFuncSetAttribute((const void *)k_tridiag_poles<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.wlds);
+  }
   hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
   if (nImax > 4)
     hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);

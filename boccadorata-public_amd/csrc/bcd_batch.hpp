@@ -38,7 +38,26 @@ enum KernelTimer {
 
 // Extra streams the Lanczos buckets run on beside the batch's own stream, so
 // that the tail of one bucket's launch overlaps the others.
-constexpr int kLancAux = 3;
+constexpr int kLancAux = 2;
+// Streams the Lanczos buckets are spread over (the batch's own and the first
+// kLancStreams - 1 auxiliary ones).  The device runs four hardware queues per
+// process by default (GPU_MAX_HW_QUEUES); with more streams than the queues
+// left beside the copy traffic, two streams share a queue and their buckets
+// run one after the other.
+constexpr int kLancStreams = 3;
+// Launch order of the Lanczos buckets: by estimated time (true) or by job
+// size, largest first (false).
+constexpr bool kLancOrderByEst = true;
+// The autotuner gives a bucket the shape of the next larger bucket when that
+// shape is within this fraction of the bucket's fastest.
+constexpr float kLancMergeTol = 0.05f;
+static_assert(kLancStreams >= 1 && kLancStreams <= kLancAux + 1, "Lanczos streams exist");
+// Estimates for spreading the Lanczos buckets over the streams when a bucket
+// has no autotuned time: the device's streaming rate and one CU's (bytes per
+// ms), and the pole blocks that follow each bucket (ms).
+constexpr double kLancChipBytesPerMs = 5.0e9;
+constexpr double kLancCuBytesPerMs = 1.1e7;
+constexpr double kLancPoleMs = 5.0;
 
 struct Batch {
   struct PoleCfg {
@@ -128,7 +147,7 @@ struct Batch {
   int check(int &n_active);
   int stage(int src, int itr_max, double conv, bool hf);
   int build_sigma2(int only_ish);
-  int tune_bucket(int bk, int mmax, int nj, const int *ord);
+  int tune_bucket(int bk, int mmax, int nj, const int *ord, int prev);
   // Lanczos tile shape per pole-count bucket; tune_lanczos: pick them by
   // timing during the next solve
   int lanc_shape[kNLancBuckets] = {kLancDefault[0], kLancDefault[1], kLancDefault[2], kLancDefault[3],

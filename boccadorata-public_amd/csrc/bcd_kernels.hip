@@ -1544,13 +1544,15 @@ __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs
   for (long i = tid; i < (long)nI * n; i += blockDim.x) tq[i] = Qh[i];
 }
 
-// k_tridiag for one Lanczos-reduced side in poles mode with n <= 64: one wave
-// per matrix, lane j owns column j.  The symmetric block sits in LDS with an
-// odd row stride and is only ever read along rows, so lanes touch consecutive
-// words; the tracked rows (the couplings) stay in registers; every reduction
-// is a wave shuffle, so no step waits on a block barrier.  Same reflections
-// as k_tridiag (Golub & Van Loan, Alg. 8.3.1).  ld: LDS row stride (odd,
-// >= n); dynamic LDS: ld * ld + 128 doubles.
+// k_tridiag for one Lanczos-reduced side in poles mode with n <= 128: one
+// wave per matrix, lane j owning columns j and j + 64.  The symmetric block
+// sits in LDS with an odd row stride and is only ever read along rows, so
+// lanes touch consecutive words; the tracked rows (the couplings) stay in
+// registers; every reduction is a wave shuffle over the lane's two columns,
+// so no step waits on a block barrier.  A column past n holds zeros, and
+// adding them leaves every sum of an n <= 64 side as with one column per
+// lane.  Same reflections as k_tridiag (Golub & Van Loan, Alg. 8.3.1).  ld:
+// LDS row stride (odd, >= n); dynamic LDS: ld * ld + 2 * kPoleWaveN doubles.
 template <int NIMAX>
 __global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, const int *order, int njobs, int ld) {
   extern __shared__ double sm[];
@@ -1559,48 +1561,61 @@ __global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, cons
   const int n = J.n, nI = J.nI, j = threadIdx.x;
   // each job in the launch of its own accumulator bound (4 or 8), so that
   // the unrolled code that runs it does not depend on the batch
-  if (n <= 0 || n > 64 || n > ld || nI > NIMAX || (NIMAX > 4 && nI <= 4) || !J.poles || J.path != 0) return;
-  double *A = sm, *v = A + (long)ld * ld, *wv = v + 64;
-  double q[NIMAX];
+  if (n <= 0 || n > kPoleWaveN || n > ld || nI > NIMAX || (NIMAX > 4 && nI <= 4) || !J.poles || J.path != 0) return;
+  double *A = sm, *v = A + (long)ld * ld, *wv = v + kPoleWaveN;
+  const int col[2] = {j, j + 64};
+  double q[2][NIMAX];
 #pragma unroll
-  for (int r = 0; r < NIMAX; ++r) q[r] = 0.0;
-  if (j < n)
-    for (int i = 0; i < n; ++i) A[(long)i * ld + j] = 0.0;
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int r = 0; r < NIMAX; ++r) q[c][r] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+    if (col[c] < n)
+      for (int i = 0; i < n; ++i) A[(long)i * ld + col[c]] = 0.0;
   __syncthreads();
   // the fishbone block and its couplings (DysonMatrix.cpp:303-333)
   const int nd = J.nfw, den = J.den_fw, T = den + nI, npiv = J.npiv_fw;
   const double *rows = J.rfw;
-  if (j < nd) {
-    const double *row = rows + (long)j * T;
-    A[(long)j * ld + j] = row[0];
 #pragma unroll
-    for (int r = 0; r < NIMAX; ++r)
-      if (r < nI) q[r] = row[den + r];
-  }
+  for (int c = 0; c < 2; ++c)
+    if (col[c] < nd) {
+      const double *row = rows + (long)col[c] * T;
+      A[(long)col[c] * ld + col[c]] = row[0];
+#pragma unroll
+      for (int r = 0; r < NIMAX; ++r)
+        if (r < nI) q[c][r] = row[den + r];
+    }
   __syncthreads();
   if (npiv > 0) {
-    if (j >= 1 && j < nd) {
-      const double b = rows[(long)j * T + 1];
-      A[(long)j * ld + j - 1] = b;
-      A[(long)(j - 1) * ld + j] = b;
-    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (col[c] >= 1 && col[c] < nd) {
+        const double b = rows[(long)col[c] * T + 1];
+        A[(long)col[c] * ld + col[c] - 1] = b;
+        A[(long)(col[c] - 1) * ld + col[c]] = b;
+      }
     __syncthreads();
     for (int nc = 1; nc <= npiv; ++nc) {
       const int st = J.ipiv_fw[nc + 1];
       if (st < 1 || st >= nd) break;
-      if (j >= st && j < nd) {
-        const double gv = rows[(long)j * T + 1 + nc];
-        A[(long)j * ld + st - 1] = gv;
-        A[(long)(st - 1) * ld + j] = gv;
-      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        if (col[c] >= st && col[c] < nd) {
+          const double gv = rows[(long)col[c] * T + 1 + nc];
+          A[(long)col[c] * ld + st - 1] = gv;
+          A[(long)(st - 1) * ld + col[c]] = gv;
+        }
       __syncthreads();
     }
   }
   double *td = J.tri, *te = td + n, *tq = te + n;
   for (int k = 0; k + 2 < n; ++k) {
     const int m0 = k + 1;
-    const double akj = j > k && j < n ? A[(long)k * ld + j] : 0.0;
-    const double s2 = wave_sum(akj * akj);
+    double akj[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) akj[c] = col[c] > k && col[c] < n ? A[(long)k * ld + col[c]] : 0.0;
+    const double s2 = wave_sum(akj[0] * akj[0] + akj[1] * akj[1]);
     const double x0 = A[(long)k * ld + m0], tail = s2 - x0 * x0;
     if (tail <= 0.0) {
       if (j == 0) te[k] = x0;
@@ -1608,35 +1623,54 @@ __global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, cons
     }
     const double nrm = sqrt(s2), alpha = x0 > 0.0 ? -nrm : nrm, v0 = x0 - alpha;
     const double beta = 2.0 / (tail + v0 * v0);
-    const double vj = j == m0 ? v0 : akj;
-    v[j] = vj;
+    double vj[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      vj[c] = col[c] == m0 ? v0 : akj[c];
+      if (col[c] < kPoleWaveN) v[col[c]] = vj[c];
+    }
     if (j == 0) te[k] = alpha;
     __syncthreads();
     // p = beta A v over rows > k (A symmetric: row i read along row j of A^T)
-    double pj = 0.0;
-    if (j > k && j < n)
-      for (int i = m0; i < n; ++i) pj += A[(long)i * ld + j] * v[i];
-    pj *= beta;
+    double pj[2] = {0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (col[c] > k && col[c] < n)
+        for (int i = m0; i < n; ++i) pj[c] += A[(long)i * ld + col[c]] * v[i];
+      pj[c] *= beta;
+    }
     double qv[NIMAX];
 #pragma unroll
-    for (int r = 0; r < NIMAX; ++r) qv[r] = r < nI ? beta * wave_sum(q[r] * vj) : 0.0;
-    const double K = 0.5 * beta * wave_sum(pj * vj);
-    const double wj = pj - K * vj;
-    wv[j] = wj;
-    __syncthreads();
-    if (j > k && j < n)
-      for (int i = m0; i < n; ++i) A[(long)i * ld + j] -= v[i] * wj + wv[i] * vj;
+    for (int r = 0; r < NIMAX; ++r) qv[r] = r < nI ? beta * wave_sum(q[0][r] * vj[0] + q[1][r] * vj[1]) : 0.0;
+    const double K = 0.5 * beta * wave_sum(pj[0] * vj[0] + pj[1] * vj[1]);
+    double wj[2];
 #pragma unroll
-    for (int r = 0; r < NIMAX; ++r) q[r] -= qv[r] * vj;
+    for (int c = 0; c < 2; ++c) {
+      wj[c] = pj[c] - K * vj[c];
+      if (col[c] < kPoleWaveN) wv[col[c]] = wj[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (col[c] > k && col[c] < n)
+        for (int i = m0; i < n; ++i) A[(long)i * ld + col[c]] -= v[i] * wj[c] + wv[i] * vj[c];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < NIMAX; ++r) q[c][r] -= qv[r] * vj[c];
     __syncthreads();
   }
-  if (j < n) {
-    td[j] = A[(long)j * ld + j];
-    if (j == n - 2) te[j] = A[(long)j * ld + j + 1];
-    if (j == n - 1) te[j] = 0.0;
 #pragma unroll
-    for (int r = 0; r < NIMAX; ++r)
-      if (r < nI) tq[(long)r * n + j] = q[r];
+  for (int c = 0; c < 2; ++c) {
+    const int jc = col[c];
+    if (jc < n) {
+      td[jc] = A[(long)jc * ld + jc];
+      if (jc == n - 2) te[jc] = A[(long)jc * ld + jc + 1];
+      if (jc == n - 1) te[jc] = 0.0;
+#pragma unroll
+      for (int r = 0; r < NIMAX; ++r)
+        if (r < nI) tq[(long)r * n + jc] = q[c][r];
+    }
   }
 }
 

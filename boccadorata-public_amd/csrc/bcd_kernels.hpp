@@ -150,11 +150,14 @@ struct ArrowSweep {
 // The block size sets the order of the kernel's reductions, so it depends on
 // the channel alone: a channel gives the same bits in any batch.
 __host__ __device__ inline int arrow_threads(int cap) { return cap <= 64 ? 64 : (cap <= 128 ? 128 : kArrowMaxThreads); }
+// Largest reduced side of the one-wave pole-block kernels (two columns per
+// lane in k_tridiag_poles).
+constexpr int kPoleWaveN = 128;
 // Pole block of a reduced side with room for cap rows: the one-wave kernels
-// (k_tridiag_poles, k_ql_poles) when it has at most 64 rows and the channel
-// at most kQLGroupNI orbits, the general ones otherwise -- chosen per side,
-// not per batch, for the same reason.
-__host__ __device__ inline int pole_path(int cap, int nI) { return cap <= 64 && nI <= 8 ? 0 : 1; }
+// (k_tridiag_poles, k_ql_poles) when it has at most kPoleWaveN rows and the
+// channel at most kQLGroupNI orbits, the general ones otherwise -- chosen per
+// side, not per batch, for the same reason.
+__host__ __device__ inline int pole_path(int cap, int nI) { return cap <= kPoleWaveN && nI <= kQLGroupNI ? 0 : 1; }
 template <bool IN_LDS, int NIMAX>
 __global__ void k_arrow_sweep(ArrowSweep a);
 // A sweep without dynamic self-energy: one thread per channel with nI <=

@@ -102,17 +102,22 @@ def test_batch_equals_single():
         assert b.stats(i) == one.stats(0)
 
 
-@pytest.mark.parametrize("nmax,lanczos", [(3, False), (5, True), (7, True)])
-def test_arrowhead_equals_dense(nmax, lanczos):
+@pytest.mark.parametrize("nmax,lanczos,lanc_n", [(3, False, 50), (5, True, 50), (7, True, 50), (7, True, 70)])
+def test_arrowhead_equals_dense(nmax, lanczos, lanc_n):
     """The arrowhead Dyson solver (default) against the dense Householder + QL
     solver on the whole solve: same sweep counts and fragments to 1e-10.  With
     the full Sigma every fragment is compared; with Lanczos the HF stage's
     last-bit differences reach the Lanczos input and move the small-Z
-    fragments as rounding does in the reference, so the Z cut applies."""
+    fragments as rounding does in the reference, so the Z cut applies.  With
+    70 vectors per side the reduced sides exceed 64 rows: the one-wave pole
+    blocks then hold two columns per lane.  (Not 100: several 2h1p sides at
+    Nmax 7 hold fewer distinct pole energies than that, e.g. 61 among 96
+    poles with one coupling column, so the Krylov chain runs past breakdown
+    and the solutions it feeds are rounding-determined in either solver.)"""
     ms, vpp = synth.make_system(nmax, 0)
     out = []
     for solver in (0, 1):
-        par = capi.solve_params(mode=2, lanczos=lanczos, lanc_kind=1, lanc_n=50, conv_kind=1, hf_conv=1e-8,
+        par = capi.solve_params(mode=2, lanczos=lanczos, lanc_kind=1, lanc_n=lanc_n, conv_kind=1, hf_conv=1e-8,
                                 sc_conv=1e-8, sc_itr_max=40, dyson_solver=solver)
         b = capi.Batch([(ms, vpp)], par)
         b.solve()
@@ -196,9 +201,11 @@ def test_anderson_mixing_same_fixed_point(nmax):
 def test_lanczos_autotune():
     """The tile-shape autotuner times every shape that fits each pole-count
     bucket on the batch's own Lanczos jobs during one solve and keeps the
-    fastest.  Shapes only regroup the block sums, so the tuned solve agrees
-    with the default-shape solve to rounding (spectral parity at 1e-10), and
-    later solves with the chosen shapes repeat it bit for bit."""
+    fastest, or the shape of the next larger occupied bucket when that one is
+    within LANC_MERGE_TOL of the fastest.  Shapes only regroup the block
+    sums, so the tuned solve agrees with the default-shape solve to rounding
+    (spectral parity at 1e-10), and later solves with the chosen shapes
+    repeat it bit for bit."""
     systems = [synth.make_system(5, s) for s in (0, 1, 2)]
     par = capi.solve_params(mode=2, hf_itr_max=100, sc_itr_max=60, lanczos=True, lanc_kind=1, lanc_n=50,
                             conv_kind=1, hf_conv=1e-8, sc_conv=1e-8)
@@ -219,7 +226,11 @@ def test_lanczos_autotune():
     for bk in range(capi.LANC_BUCKETS):
         nt, per = capi.LANC_SHAPES[shapes[bk]]
         if bk in timed:
-            assert ms[bk, shapes[bk]] == ms[bk][ms[bk] > 0].min()
+            best = ms[bk][ms[bk] > 0].min()
+            larger = timed[timed > bk]
+            prev = shapes[larger.min()] if larger.size else -1
+            assert ms[bk, shapes[bk]] == best or (
+                shapes[bk] == prev and ms[bk, prev] <= np.float32(1.0 + capi.LANC_MERGE_TOL) * best), bk
         else:
             assert shapes[bk] == default[bk]
         assert nt * per >= (capi.LANC_BUCKET_MAX[bk - 1] + 1 if bk else 1)
