@@ -86,11 +86,11 @@ def screened_seeds(n):
     return [cand[k] for k in good], [systems[k] for k in good], good[-1] + 1 - n
 
 
-def timed(batch, n, steps, warmup, label):
-    """warmup untimed solves (the first one also picks the Lanczos tile shapes
-    with the batch autotuner), then steps timed ones; returns (seconds,
-    per-instance converged counts)."""
-    if warmup > 0:
+def timed(batch, n, steps, warmup, label, tune=True):
+    """warmup untimed solves (with tune, the first one also picks the Lanczos
+    tile shapes with the batch autotuner), then steps timed ones; returns
+    (seconds, per-instance converged counts)."""
+    if warmup > 0 and tune:
         batch.autotune()
     for w in range(warmup):
         batch.solve()
@@ -129,6 +129,15 @@ def spectral_error(g, ref, zsig=ZSIG):
             ma, mb = np.sum(a[1] * a[0] ** k), np.sum(b[1] * b[0] ** k)
             worst = max(worst, abs(ma - mb) / (np.sum(np.abs(b[1] * b[0] ** k)) + 1.0))
     return worst
+
+
+def strength_below(g, zsig):
+    """Share of the spectral strength (sum of Z over both sides and every
+    subshell) carried by solutions with Z below zsig: what the Z-floor
+    comparisons leave out."""
+    z = np.concatenate([g.zp, g.zh])
+    tot = float(np.sum(z))
+    return float(np.sum(z[z < zsig])) / tot if tot > 0.0 else 0.0
 
 
 def moment_error(g, ref):
@@ -202,6 +211,20 @@ def roofline(batch, workload):
         if meas:
             entry["traffic"] = meas[0]
             entry["traffic_note"] = f"HBM bytes per launch, rocprofv3 FETCH_SIZE/WRITE_SIZE, {meas[1]}"
+        # the other kernel families with an algorithmic count, each against
+        # its own roofline (their summed time over the solve)
+        others = {}
+        for k, (w, bound) in per_launch.items():
+            if k == dom or w <= 0.0 or k not in totals or totals[k] <= 0.0:
+                continue
+            rate = w / (totals[k] * 1e-3)
+            if bound == "hbm":
+                others[k] = {"achieved": rate / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": rate / 1e9 / HBM_PEAK_GBS, "ms": totals[k]}
+            else:
+                others[k] = {"achieved": rate / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": rate / 1e12 / FP64_PEAK_TFLOPS, "ms": totals[k]}
+        entry["other_kernels"] = others
     return entry, {k: {"avg_ms": v[0], "launches": v[1]} for k, v in kt.items()}, work
 
 
@@ -650,10 +673,14 @@ def main_batched(args):
              "fp64_learned": (params(0, 0), True), "mixed_anderson": (params(1, 5), False)}
     modes = {k: v for k, v in modes.items() if k in wanted}
     runs, results = {}, {}
-    for label, (par, seeded) in modes.items():
+    for q, (label, (par, seeded)) in enumerate(modes.items()):
         plain.set_params(par)
         plain.set_learned_start(net if seeded else None)
-        dt, conv = timed(plain, n, args.steps, args.warmup, label)
+        # the tile shapes are tuned once, in the first mode's warm-up: every
+        # mode then reduces the same rows in the same order, and the modes
+        # differ only where they are meant to (coupling precision, sc0 start
+        # and mixing)
+        dt, conv = timed(plain, n, args.steps, args.warmup, label, tune=q == 0)
         info = [plain.start_info(i) for i in range(n)]
         runs[label] = {"dt": dt, "conv": conv, "stages": plain.timings(),
                        "sweeps": float(np.mean([plain.stats(i)["hf_its"] + plain.stats(i)["sc_its"]
@@ -680,6 +707,7 @@ def main_batched(args):
     loose = floor > ERR_BOUND
     print(f"# rounding floor of fp64: {int(loose.sum())} of {n} instances above {ERR_BOUND} "
           f"(max {floor.max():.2e})", file=sys.stderr, flush=True)
+    below = np.array([strength_below(ref[i], zs[i]) for i in range(n)])
     for label, r in runs.items():
         errs = np.zeros(n) if label == "fp64" else \
             np.array([spectral_error(results[label][i], ref[i], zs[i]) for i in range(n)])
@@ -773,6 +801,9 @@ def main_batched(args):
                    "fp64_rounding_floor": {"instances_above_bound": int(loose.sum()),
                                            "max": float(floor.max()),
                                            "median": float(np.median(floor))},
+                   "strength_below_z_floor": {"note": "share of sum Z of the fp64 solve in solutions below the "
+                                                      "Nmax's Z floor (compared only through the moments)",
+                                              "max": float(below.max()), "median": float(np.median(below))},
                    "lanczos_tile_shapes": shapes,
                    "mean_hf_sweeps": float(np.mean([s["hf_its"] for s in its])),
                    "mean_sc0_sweeps": float(np.mean([s["sc_its"] for s in its]))},

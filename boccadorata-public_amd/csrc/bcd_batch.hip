@@ -295,6 +295,7 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   BTRY(a_prof.grow(sizeof(unsigned long long) * 8 * (size_t)n_gc));
   d_prof = (unsigned long long *)a_prof.ptr;
   HTRY(bcd::sync_memset(a_work.ptr, 0, sizeof(double) * l_end));
+  w_built = false;
   HTRY(bcd::sync_memcpy(d_u1, u1.data(), sizeof(double) * o_u1, hipMemcpyHostToDevice));
 
   // propagators: 3 buffers (initial, ping, pong), counts
@@ -818,11 +819,23 @@ int Batch::tune_bucket(int bk, int mmax, int nj, const int *ord, int prev) {
 
 int Batch::lanczos() {
   PropView P = view(true);
-  // merge the J-degenerate poles, then the jobs pick merged or plain rows
+  // merge the J-degenerate poles, then the jobs pick merged or plain rows;
+  // the (channel, side) blocks of the merge run largest first
+  {
+    std::vector<std::pair<int, int>> mo;
+    for (int gc = 0; gc < n_gc; ++gc)
+      for (int side = 0; side < 2; ++side)
+        mo.push_back({h_chans[gc].merge ? (side ? h_cs[gc].mbk : h_cs[gc].mfw) : 0, 2 * gc + side});
+    std::stable_sort(mo.begin(), mo.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first > b.first; });
+    std::vector<int> ord(mo.size());
+    for (size_t k = 0; k < mo.size(); ++k) ord[k] = mo[k].second;
+    HTRY(hipMemcpyAsync(d_lorder, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, stream));
+  }
   kt_begin(KT_MERGE);
   hipLaunchKernelGGL(k_merge_poles, dim3(2 * n_gc), dim3(256), 0, stream, d_chans, n_gc, d_cs, d_blocks, d_trip_start,
                      d_trip_blk, P, d_insts, d_blk_fw_off, d_blk_bk_off, d_row_fw_base, d_row_bk_base, d_rows_fw,
-                     d_rows_bk, d_crows_fw, d_crows_bk);
+                     d_rows_bk, d_crows_fw, d_crows_bk, d_lorder);
   kt_end(KT_MERGE);
   HTRY(hipGetLastError());
   hipLaunchKernelGGL(k_lanczos_prep, dim3((n_gc + 63) / 64), dim3(64), 0, stream, d_chans, 0, n_gc, P, d_cs, d_piv,
@@ -1146,10 +1159,15 @@ int Batch::solve_impl() {
   HTRY(hipMemcpyAsync(pnqp_[0], pnqp_[2], sizeof(int) * 2 * prop_n_len, hipMemcpyDeviceToDevice, stream));
   for (InstState &s : h_is) { std::memset(&s, 0, sizeof s); s.cur = 0; s.active = 1; }
   HTRY(hipMemcpyAsync(d_is, h_is.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice, stream));
+  // W depends on the interaction and the channel layout only, both fixed for
+  // the batch: it is built by the first solve and kept
   kt_begin(KT_WBUILD);
-  hipLaunchKernelGGL(k_wbuild, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_w);
+  if (!w_built) {
+    hipLaunchKernelGGL(k_wbuild, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, 0, n_gc, d_w);
+    HTRY(hipGetLastError());
+    w_built = true;
+  }
   kt_end(KT_WBUILD);
-  HTRY(hipGetLastError());
   HTRY(hipEventRecord(ev[1], stream));
   BTRY(stage(0, par.hf_itr_max, par.hf_conv, true));
   HTRY(hipEventRecord(ev[2], stream));

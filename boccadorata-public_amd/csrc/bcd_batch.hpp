@@ -157,6 +157,7 @@ struct Batch {
   float lanc_tune_ms[kNLancBuckets][kNLancShapes] = {};
   bool tune_lanczos = false;
   bool poles_done = false;       // lanczos() ran the pole-block kernels bucket by bucket
+  bool w_built = false;          // d_w holds the Sigma^inf tensor of this batch (k_wbuild)
   int prepare();
   int work(double *w, int n);
   double coupling_flops = -1.0;

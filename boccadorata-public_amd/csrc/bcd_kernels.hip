@@ -786,22 +786,24 @@ BCD_SIGMA2_INST(float, double, 2)
 // gives the same reduced rows in exact arithmetic; groups of k <= nI rows are
 // copied.  One block per (channel, side), one thread per triple: count the
 // triple's output rows, scan, write them.  Rows come out triple by triple,
-// fragments in order, each group's rows in J order.
+// fragments in order, each group's rows in J order.  Blocks take the (channel,
+// side) jobs 2 * gc + side in the given order (the host passes them largest
+// first, so that the longest blocks do not start last), or in index order.
 __global__ __launch_bounds__(256) void k_merge_poles(const ChanDev *chans, int n_gc, ChanState *cs,
                                                     const BlockDev *blocks, const int *trip_start,
                                                     const int *trip_blk, PropView P, const InstDev *insts,
                                                     const int *blk_fw_off, const int *blk_bk_off,
                                                     const long *row_fw_base, const long *row_bk_base,
                                                     const double *rows_fw, const double *rows_bk, double *crows_fw,
-                                                    double *crows_bk) {
+                                                    double *crows_bk, const int *order) {
   // the chunk's non-empty triples: J blocks, a == b, fragment counts, tuples
   // (inclusive prefix), diagonal-group size and output base, and per J block
   // its first row and nJab as (row << 1) | nJab
   __shared__ int q_nb[256], q_eq[256], q_cA[256], q_cB[256], q_cG[256], q_tp[256], q_n0[256], q_base[256];
   __shared__ int q_jb[kMergeMaxJ][256];
   __shared__ int wtot[3][4];
-  const int job = blockIdx.x;
-  if (job >= 2 * n_gc || !crows_fw) return;
+  if ((int)blockIdx.x >= 2 * n_gc || !crows_fw) return;
+  const int job = order ? order[blockIdx.x] : blockIdx.x;
   const int gc = job >> 1, side = job & 1, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const ChanDev C = chans[gc];
   if (!C.merge) {

@@ -63,7 +63,7 @@ __global__ void k_merge_poles(const ChanDev *chans, int n_gc, ChanState *cs, con
                               const int *trip_start, const int *trip_blk, PropView P, const InstDev *insts,
                               const int *blk_fw_off, const int *blk_bk_off, const long *row_fw_base,
                               const long *row_bk_base, const double *rows_fw, const double *rows_bk,
-                              double *crows_fw, double *crows_bk);
+                              double *crows_fw, double *crows_bk, const int *order);
 __global__ void k_lanczos_prep(const ChanDev *chans, int gc_begin, int gc_end, PropView P, ChanState *cs,
                                double *piv, int lanc_kind, int piv_round, const long *row_fw_base,
                                const long *row_bk_base, double *rows_fw, double *rows_bk, double *crows_fw,
