@@ -335,14 +335,23 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
 
   // jobs
   BTRY(a_jobs.grow(pad8(sizeof(DysonJob) * 3 * n_gc) + pad8(sizeof(LancJob) * 2 * n_gc) +
-                   pad8(sizeof(int) * 2 * n_gc) + pad8(sizeof(int) * kCsPack * n_gc) + 64));
+                   pad8(sizeof(int) * 2 * n_gc) + pad8(sizeof(int) * kCsPack * n_gc) + pad8(sizeof(int) * n_gc) + 64));
   d_djobs = (DysonJob *)a_jobs.ptr;
   d_pjobs = d_djobs + n_gc;
   d_ljobs = (LancJob *)((char *)a_jobs.ptr + pad8(sizeof(DysonJob) * 3 * n_gc));
   d_lorder = (int *)((char *)d_ljobs + pad8(sizeof(LancJob) * 2 * n_gc));
   d_cspack = (int *)((char *)d_lorder + pad8(sizeof(int) * 2 * n_gc));
+  d_porder = (int *)((char *)d_cspack + pad8(sizeof(int) * kCsPack * n_gc));
   h_cspack.resize((size_t)kCsPack * n_gc);
   HTRY(bcd::sync_memset(a_jobs.ptr, 0, a_jobs.cap));
+  {
+    // the Sigma(2) plan takes the channels with the most blocks first, so
+    // that the longest plan blocks do not start last
+    std::vector<int> po(n_gc);
+    for (int gc = 0; gc < n_gc; ++gc) po[gc] = gc;
+    std::stable_sort(po.begin(), po.end(), [&](int x, int y) { return h_chans[x].blk_cnt > h_chans[y].blk_cnt; });
+    HTRY(bcd::sync_memcpy(d_porder, po.data(), sizeof(int) * n_gc, hipMemcpyHostToDevice));
+  }
 
   // dense matrices in HBM when the largest one does not fit in LDS: the
   // whole Dyson matrix (dense solver) or one reduced side (arrowhead solver)
@@ -660,7 +669,7 @@ int Batch::build_sigma2(int only_ish) {
   HTRY(hipMemsetAsync(d_work_count, 0, sizeof(int), stream));
   kt_begin(KT_PLAN);
   hipLaunchKernelGGL(k_sigma2_plan, dim3(n_gc), dim3(256), 0, stream, d_chans, d_blocks, 0, n_gc, P, d_insts, d_cs,
-                     d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, only_ish);
+                     d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, only_ish, d_porder);
   kt_end(KT_PLAN);
   HTRY(hipGetLastError());
   BTRY(pull_counts());

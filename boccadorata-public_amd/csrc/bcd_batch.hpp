@@ -121,6 +121,7 @@ struct Batch {
   LancJob *d_ljobs = nullptr;
   int *d_lorder = nullptr;                           // Lanczos jobs, longest first
   int *d_cspack = nullptr;                           // kCsPack counts per channel (pull_counts)
+  int *d_porder = nullptr;                           // channels by Sigma(2) block count, most first
   std::vector<int> h_cspack;
   long aws_len = 0;                                  // global arrowhead workspace (doubles), 0: LDS
 

@@ -285,9 +285,10 @@ __global__ void k_cs_pack(const ChanState *cs, int n_gc, int *out) {
 __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin,
                                                     int gc_end, PropView P, const InstDev *insts, ChanState *cs,
                                                     int *blk_fw_off, int *blk_bk_off, Sig2Work *work_list,
-                                                    int *work_count, int only_ish) {
-  const int gc = gc_begin + blockIdx.x;
-  if (gc >= gc_end) return;
+                                                    int *work_count, int only_ish, const int *order) {
+  // channels gc_begin + order[k] (or in index order), k < gc_end - gc_begin
+  if ((int)blockIdx.x >= gc_end - gc_begin) return;
+  const int gc = gc_begin + (order ? order[blockIdx.x] : (int)blockIdx.x);
   const ChanDev C = chans[gc];
   if (only_ish >= 0 && C.ish != only_ish) {
     if (threadIdx.x == 0) { cs[gc].mfw = 0; cs[gc].mbk = 0; }

@@ -39,7 +39,8 @@ __global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *
                       double mix_start);
 __global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
-                              int *blk_bk_off, Sig2Work *work_list, int *work_count, int only_ish);
+                              int *blk_bk_off, Sig2Work *work_list, int *work_count, int only_ish,
+                              const int *order);
 // V-table size classes of the Sigma(2) build: each class is launched with
 // LDS for its largest table, so the many small blocks run at high occupancy
 constexpr int kSigVtClass[] = {256, 768};
