@@ -616,143 +616,152 @@ __global__ __launch_bounds__(64) void k_sigma2(const InstDev *insts, const ChanD
   // matrix-core scratch (MF > 0): X2 and T tiles, 16-byte aligned
   float *mfx = (float *)(((unsigned long)(sgnC + c_cap) + 15) & ~15ul), *mft = mfx + kSigMf * kSigMf;
   const int lane = threadIdx.x;
-  const int total = *work_count;
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    const Sig2Work Wk = work_list[t];
-    // this launch builds the blocks of one V-table size class (vt_lo, vt_cap]
-    // (vt_lo = -1 for the first class: blocks without pairs have nvt = 0)
-    if (Wk.nvt <= vt_lo || Wk.nvt > vt_cap) continue;
-    const BlockDev B = blocks[Wk.bidx];
-    const ChanDev C = chans[Wk.gc];
-    const InstDev I = insts[Wk.inst];
-    const int cur = prop_cur(P, Wk.inst);
-    // fragment counts: qp(a), qp(b), qh(g), qh(a), qh(b), qp(g)
-    const int f0 = Wk.n01 & 0xffff, f1 = Wk.n01 >> 16, f2 = Wk.n23 & 0xffff, f3 = Wk.n23 >> 16;
-    const int f4 = Wk.n45 & 0xffff, f5 = Wk.n45 >> 16;
-    const int nI = C.nI, a = B.a, b = B.b, g = B.g, J = B.J, nJab = B.nJab, nJgd = B.nJgd;
-    const int noa = I.no[a], nob = I.no[b], nog = I.no[g];
-    const int ga = I.gc0 + a, gb = I.gc0 + b, gg = I.gc0 + g;
-    const int oa0 = I.orb0[a], ob0 = I.orb0[b], og0 = I.orb0[g];
-    // (alpha, beta) pairs in the reference order (beta >= alpha + nJab when a == b)
-    const int npair = (a == b) ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
-    const int nvt = nI * npair * nog, nC = nI * nog;
-    const double x3 = sqrt((double)(J + J + 1) / (double)(1 + C.j2));
-    // V table Vt(d; alpha beta, gamma) = <alpha beta; J|V|d gamma; J> as VppInt_t::get
-    // returns it (Vpp_potential.cpp:557-607): the pair rows of (alpha, beta) and of
-    // (d, gamma) first, then one load per entry.  (alpha, beta) never needs the
-    // exchange swap; (d, gamma) does when d > gamma, with the phase
-    // (-1)^((j_d + j_gamma)/2 - J + 1).
-    const long pbase = (long)J * I.norb;
-    const bool jok = J <= I.jmax;
-    for (int pi = lane; pi < npair; pi += 64) {
-      int al, be;
-      if (a == b) {
-        int r = pi, len = noa - nJab;
-        al = 0;
-        while (r >= len) { r -= len; ++al; --len; }
-        be = al + nJab + r;
-      } else {
-        al = pi / nob; be = pi % nob;
-      }
-      rowA[pi] = jok ? I.pidx[(pbase + oa0 + al) * I.norb + ob0 + be] : -1;
-    }
-    const int phC = abs((C.j2 + I.j2[g]) / 2 - J + 1) & 1;
-    for (int q = lane; q < nC; q += 64) {
-      const int od = C.orb0 + q / nog, og = og0 + q % nog;
-      const int lo = od < og ? od : og, hi = od < og ? og : od;
-      rowC[q] = jok ? I.pidx[(pbase + lo) * I.norb + hi] : -1;
-      sgnC[q] = od > og ? phC : 0;
-    }
-    const long voff_b = B.c2b >= 0 ? I.voff[B.c2b] : 0;
-    const int np_b = B.c2b >= 0 ? I.npair[B.c2b] : 0;
-    __syncthreads();
-    for (int e = lane; e < nvt; e += 64) {
-      const int ga_ = e % nog, pi = (e / nog) % npair, d = e / (nog * npair);
-      const int od = C.orb0 + d, og = og0 + ga_;
-      double val = 0.0;
-      if (!(og == od && nJgd)) {
-        const int pa = rowA[pi], q = d * nog + ga_, pc = rowC[q];
-        double v = 0.0;
-        if (pa >= 0 && pc >= 0 && (pa >> 16) == (pc >> 16)) {
-          const int ca = pa >> 16;
-          const long vo = ca == B.c2b ? voff_b : I.voff[ca];
-          const int np = ca == B.c2b ? np_b : I.npair[ca];
-          v = I.V[vo + (long)(pa & 0xffff) * np + (pc & 0xffff)];
-          if (sgnC[q]) v = -v;
-        }
-        val = x3 * v;
-        if (og == od) val *= kSqrt2;
-      }
-      vt[e] = (T)val;
-    }
-    for (int side = 0; side < 2; ++side) {
-      // side 0: 2p1h from qp(a), qp(b), qh(g); side 1: 2h1p from qh(a), qh(b), qp(g)
-      const int cA = side ? f3 : f0, cB = side ? f4 : f1, cG = side ? f5 : f2;
-      // the quasiholes of a subshell are stored after its quasiparticles
-      const int oA = side ? f0 : 0, oB = side ? f1 : 0, oG = side ? 0 : f5;
-      int npole;
-      if (a == b) { int nb2 = cA + 1; if (nJab == 1) nb2 -= 2; npole = (cA * nb2 / 2) * cG; }
-      else npole = cA * cB * cG;
-      if (npole <= 0) continue;
-      const double *eA = P.e[cur] + P.e_off[ga] + oA, *eB = P.e[cur] + P.e_off[gb] + oB;
-      const double *eG = P.e[cur] + P.e_off[gg] + oG;
-      const double *xA = P.x[cur] + P.x_off[ga] + (long)oA * noa;
-      const double *xB = P.x[cur] + P.x_off[gb] + (long)oB * nob;
-      const double *xG = P.x[cur] + P.x_off[gg] + (long)oG * nog;
-      const int row_off = side ? Wk.off_bk : Wk.off_fw;
-      double *rows = (side ? rows_bk + row_bk_base[B.gc] : rows_fw + row_fw_base[B.gc]) + (long)row_off * (2 + nI);
-      const int w = 2 + nI;
-      // stage the amplitudes of the fragments this side uses (for a == b the
-      // two orbits share one fragment list)
-      const int nsa = cA * noa, nsb = (a == b) ? 0 : cB * nob, nsg = cG * nog;
-      const bool staged = nsa + nsb + nsg <= kSigStage;
-      if (staged) {
-        for (int k = lane; k < nsa; k += 64) stg[k] = (T)xA[k];
-        for (int k = lane; k < nsb; k += 64) stg[nsa + k] = (T)xB[k];
-        for (int k = lane; k < nsg; k += 64) stg[nsa + nsb + k] = (T)xG[k];
-      }
-      __syncthreads();
-      // pass-1 results stay in LDS for pass 2 when they fit
-      double *rb = npole * nI <= kSigRowBuf ? rbuf : nullptr;
-      const int npair_s = (a == b) ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
-      if (MF > 0 && staged && npair_s <= kSigMf && npole / cG <= kSigMf && nI * nog <= kSigMf)
-        sigma2_pass1_mfma<MF>((const float *)stg, (const float *)((a == b) ? stg : stg + nsa),
-                              (const float *)(stg + nsa + nsb), (const float *)vt, noa, nob, nog, nI, npole, a == b,
-                              nJab, cA, cB, cG, mfx, mft, rows, w, rb);
-      else if (staged)
-        sigma2_pass1<T, A, T, true>(stg, (a == b) ? stg : stg + nsa, stg + nsa + nsb, vt, noa, nob, nog, nI, npole,
-                              a == b, nJab, nJgd, cA, cB, cG, og0, C.orb0, rows, w, rb);
-      else
-        sigma2_pass1<T, A, double, false>(xA, xB, xG, vt, noa, nob, nog, nI, npole, a == b, nJab, nJgd, cA, cB, cG,
-                                   og0, C.orb0, rows, w, rb);
-      __syncthreads();
-      // pass 2: energy and sum of squares (over d in order) per pole
-      for (int p = lane; p < npole; p += 64) {
-        int na, nb, ng, r = p / cG;
-        ng = p % cG;
+  const int total = *work_count, G = gridDim.x;
+  // The wave's work items are t = blockIdx.x + k * G in order.  This launch
+  // builds the blocks of one V-table size class (vt_lo, vt_cap] (vt_lo = -1
+  // for the first class: blocks without pairs have nvt = 0): each lane reads
+  // the class of one of the next 64 items, and the wave runs the ones of its
+  // class, so items of other classes cost a 64th of a load each.
+  for (int t0 = blockIdx.x; t0 < total; t0 += 64 * G) {
+    const int tl = t0 + lane * G;
+    const int nvl = tl < total ? work_list[tl].nvt : -2;
+    unsigned long long pend = __ballot(tl < total && nvl > vt_lo && nvl <= vt_cap);
+    while (pend) {
+      const int t = t0 + (__ffsll((long long)pend) - 1) * G;
+      pend &= pend - 1;
+      const Sig2Work Wk = work_list[t];
+      const BlockDev B = blocks[Wk.bidx];
+      const ChanDev C = chans[Wk.gc];
+      const InstDev I = insts[Wk.inst];
+      const int cur = prop_cur(P, Wk.inst);
+      // fragment counts: qp(a), qp(b), qh(g), qh(a), qh(b), qp(g)
+      const int f0 = Wk.n01 & 0xffff, f1 = Wk.n01 >> 16, f2 = Wk.n23 & 0xffff, f3 = Wk.n23 >> 16;
+      const int f4 = Wk.n45 & 0xffff, f5 = Wk.n45 >> 16;
+      const int nI = C.nI, a = B.a, b = B.b, g = B.g, J = B.J, nJab = B.nJab, nJgd = B.nJgd;
+      const int noa = I.no[a], nob = I.no[b], nog = I.no[g];
+      const int ga = I.gc0 + a, gb = I.gc0 + b, gg = I.gc0 + g;
+      const int oa0 = I.orb0[a], ob0 = I.orb0[b], og0 = I.orb0[g];
+      // (alpha, beta) pairs in the reference order (beta >= alpha + nJab when a == b)
+      const int npair = (a == b) ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
+      const int nvt = nI * npair * nog, nC = nI * nog;
+      const double x3 = sqrt((double)(J + J + 1) / (double)(1 + C.j2));
+      // V table Vt(d; alpha beta, gamma) = <alpha beta; J|V|d gamma; J> as VppInt_t::get
+      // returns it (Vpp_potential.cpp:557-607): the pair rows of (alpha, beta) and of
+      // (d, gamma) first, then one load per entry.  (alpha, beta) never needs the
+      // exchange swap; (d, gamma) does when d > gamma, with the phase
+      // (-1)^((j_d + j_gamma)/2 - J + 1).
+      const long pbase = (long)J * I.norb;
+      const bool jok = J <= I.jmax;
+      for (int pi = lane; pi < npair; pi += 64) {
+        int al, be;
         if (a == b) {
-          na = 0;
-          int len = cA - nJab;
-          while (r >= len) { r -= len; ++na; --len; }
-          nb = na + nJab + r;
+          int r = pi, len = noa - nJab;
+          al = 0;
+          while (r >= len) { r -= len; ++al; --len; }
+          be = al + nJab + r;
         } else {
-          na = r / cB; nb = r % cB;
+          al = pi / nob; be = pi % nob;
         }
-        double *row = rows + (long)p * w;
-        row[0] = eA[na] + eB[nb] - eG[ng];
-        double sum2 = 0.0;
-        if (rb) {
-          for (int d = 0; d < nI; ++d) {
-            const double m = rb[p * nI + d];
-            row[2 + d] = m;
-            sum2 += m * m;
-          }
-        } else {
-          for (int d = 0; d < nI; ++d) sum2 += row[2 + d] * row[2 + d];
-        }
-        row[1] = sum2;
+        rowA[pi] = jok ? I.pidx[(pbase + oa0 + al) * I.norb + ob0 + be] : -1;
       }
+      const int phC = abs((C.j2 + I.j2[g]) / 2 - J + 1) & 1;
+      for (int q = lane; q < nC; q += 64) {
+        const int od = C.orb0 + q / nog, og = og0 + q % nog;
+        const int lo = od < og ? od : og, hi = od < og ? og : od;
+        rowC[q] = jok ? I.pidx[(pbase + lo) * I.norb + hi] : -1;
+        sgnC[q] = od > og ? phC : 0;
+      }
+      const long voff_b = B.c2b >= 0 ? I.voff[B.c2b] : 0;
+      const int np_b = B.c2b >= 0 ? I.npair[B.c2b] : 0;
       __syncthreads();
+      for (int e = lane; e < nvt; e += 64) {
+        const int ga_ = e % nog, pi = (e / nog) % npair, d = e / (nog * npair);
+        const int od = C.orb0 + d, og = og0 + ga_;
+        double val = 0.0;
+        if (!(og == od && nJgd)) {
+          const int pa = rowA[pi], q = d * nog + ga_, pc = rowC[q];
+          double v = 0.0;
+          if (pa >= 0 && pc >= 0 && (pa >> 16) == (pc >> 16)) {
+            const int ca = pa >> 16;
+            const long vo = ca == B.c2b ? voff_b : I.voff[ca];
+            const int np = ca == B.c2b ? np_b : I.npair[ca];
+            v = I.V[vo + (long)(pa & 0xffff) * np + (pc & 0xffff)];
+            if (sgnC[q]) v = -v;
+          }
+          val = x3 * v;
+          if (og == od) val *= kSqrt2;
+        }
+        vt[e] = (T)val;
+      }
+      for (int side = 0; side < 2; ++side) {
+        // side 0: 2p1h from qp(a), qp(b), qh(g); side 1: 2h1p from qh(a), qh(b), qp(g)
+        const int cA = side ? f3 : f0, cB = side ? f4 : f1, cG = side ? f5 : f2;
+        // the quasiholes of a subshell are stored after its quasiparticles
+        const int oA = side ? f0 : 0, oB = side ? f1 : 0, oG = side ? 0 : f5;
+        int npole;
+        if (a == b) { int nb2 = cA + 1; if (nJab == 1) nb2 -= 2; npole = (cA * nb2 / 2) * cG; }
+        else npole = cA * cB * cG;
+        if (npole <= 0) continue;
+        const double *eA = P.e[cur] + P.e_off[ga] + oA, *eB = P.e[cur] + P.e_off[gb] + oB;
+        const double *eG = P.e[cur] + P.e_off[gg] + oG;
+        const double *xA = P.x[cur] + P.x_off[ga] + (long)oA * noa;
+        const double *xB = P.x[cur] + P.x_off[gb] + (long)oB * nob;
+        const double *xG = P.x[cur] + P.x_off[gg] + (long)oG * nog;
+        const int row_off = side ? Wk.off_bk : Wk.off_fw;
+        double *rows = (side ? rows_bk + row_bk_base[B.gc] : rows_fw + row_fw_base[B.gc]) + (long)row_off * (2 + nI);
+        const int w = 2 + nI;
+        // stage the amplitudes of the fragments this side uses (for a == b the
+        // two orbits share one fragment list)
+        const int nsa = cA * noa, nsb = (a == b) ? 0 : cB * nob, nsg = cG * nog;
+        const bool staged = nsa + nsb + nsg <= kSigStage;
+        if (staged) {
+          for (int k = lane; k < nsa; k += 64) stg[k] = (T)xA[k];
+          for (int k = lane; k < nsb; k += 64) stg[nsa + k] = (T)xB[k];
+          for (int k = lane; k < nsg; k += 64) stg[nsa + nsb + k] = (T)xG[k];
+        }
+        __syncthreads();
+        // pass-1 results stay in LDS for pass 2 when they fit
+        double *rb = npole * nI <= kSigRowBuf ? rbuf : nullptr;
+        const int npair_s = (a == b) ? noa * (noa + 1) / 2 - nJab * noa : noa * nob;
+        if (MF > 0 && staged && npair_s <= kSigMf && npole / cG <= kSigMf && nI * nog <= kSigMf)
+          sigma2_pass1_mfma<MF>((const float *)stg, (const float *)((a == b) ? stg : stg + nsa),
+                                (const float *)(stg + nsa + nsb), (const float *)vt, noa, nob, nog, nI, npole, a == b,
+                                nJab, cA, cB, cG, mfx, mft, rows, w, rb);
+        else if (staged)
+          sigma2_pass1<T, A, T, true>(stg, (a == b) ? stg : stg + nsa, stg + nsa + nsb, vt, noa, nob, nog, nI, npole,
+                                a == b, nJab, nJgd, cA, cB, cG, og0, C.orb0, rows, w, rb);
+        else
+          sigma2_pass1<T, A, double, false>(xA, xB, xG, vt, noa, nob, nog, nI, npole, a == b, nJab, nJgd, cA, cB, cG,
+                                     og0, C.orb0, rows, w, rb);
+        __syncthreads();
+        // pass 2: energy and sum of squares (over d in order) per pole
+        for (int p = lane; p < npole; p += 64) {
+          int na, nb, ng, r = p / cG;
+          ng = p % cG;
+          if (a == b) {
+            na = 0;
+            int len = cA - nJab;
+            while (r >= len) { r -= len; ++na; --len; }
+            nb = na + nJab + r;
+          } else {
+            na = r / cB; nb = r % cB;
+          }
+          double *row = rows + (long)p * w;
+          row[0] = eA[na] + eB[nb] - eG[ng];
+          double sum2 = 0.0;
+          if (rb) {
+            for (int d = 0; d < nI; ++d) {
+              const double m = rb[p * nI + d];
+              row[2 + d] = m;
+              sum2 += m * m;
+            }
+          } else {
+            for (int d = 0; d < nI; ++d) sum2 += row[2 + d] * row[2 + d];
+          }
+          row[1] = sum2;
+        }
+        __syncthreads();
+      }
     }
   }
 }
