@@ -260,17 +260,31 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(bcd::sync_memcpy(d_cap, cap.data(), sizeof(int) * n_gc, hipMemcpyHostToDevice));
 
   const long nts = std::max<long>((long)h_trip_start.size(), 1), ntb = std::max<long>((long)h_trip_blk.size(), 1);
-  BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 2 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) +
+  BTRY(a_blocks.grow(pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1)) + 3 * pad8(sizeof(int) * std::max<long>(n_blk, 1)) +
                      pad8(sizeof(Sig2Work) * std::max<long>(n_blk, 1)) + pad8(sizeof(int) * nts) +
                      pad8(sizeof(int) * ntb) + 64));
   cp8 = (char *)a_blocks.ptr;
   d_blocks = (BlockDev *)cp8; cp8 += pad8(sizeof(BlockDev) * std::max<long>(n_blk, 1));
+  d_bpack = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
   d_blk_fw_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
   d_blk_bk_off = (int *)cp8; cp8 += pad8(sizeof(int) * std::max<long>(n_blk, 1));
   d_trip_start = (int *)cp8; cp8 += pad8(sizeof(int) * nts);
   d_trip_blk = (int *)cp8; cp8 += pad8(sizeof(int) * ntb);
   d_work_list = (Sig2Work *)cp8;
   if (n_blk) HTRY(bcd::sync_memcpy(d_blocks, h_blocks.data(), sizeof(BlockDev) * n_blk, hipMemcpyHostToDevice));
+  if (n_blk) {
+    // the fields of each block the Sigma(2) plan reads, one int per block
+    std::vector<int> bp((size_t)n_blk);
+    for (long k = 0; k < n_blk; ++k) {
+      const BlockDev &B = h_blocks[k];
+      if (B.a > 255 || B.b > 255 || B.g > 255) {
+        note_error("Sigma(2) plan: a model space with more than 256 subshells");
+        return BCD_ERR_CAPACITY;
+      }
+      bp[k] = block_pack(B);
+    }
+    HTRY(bcd::sync_memcpy(d_bpack, bp.data(), sizeof(int) * n_blk, hipMemcpyHostToDevice));
+  }
   if (!h_trip_start.empty())
     HTRY(bcd::sync_memcpy(d_trip_start, h_trip_start.data(), sizeof(int) * h_trip_start.size(), hipMemcpyHostToDevice));
   if (!h_trip_blk.empty())
@@ -668,7 +682,7 @@ int Batch::build_sigma2(int only_ish) {
   PropView P = view(true);
   HTRY(hipMemsetAsync(d_work_count, 0, sizeof(int), stream));
   kt_begin(KT_PLAN);
-  hipLaunchKernelGGL(k_sigma2_plan, dim3(n_gc), dim3(256), 0, stream, d_chans, d_blocks, 0, n_gc, P, d_insts, d_cs,
+  hipLaunchKernelGGL(k_sigma2_plan, dim3(n_gc), dim3(256), 0, stream, d_chans, d_bpack, 0, n_gc, P, d_insts, d_cs,
                      d_blk_fw_off, d_blk_bk_off, d_work_list, d_work_count, only_ish, d_porder);
   kt_end(KT_PLAN);
   HTRY(hipGetLastError());

@@ -104,6 +104,7 @@ struct Batch {
   long *d_row_fw_base = nullptr, *d_row_bk_base = nullptr, *d_e_off = nullptr, *d_x_off = nullptr;
   int *d_cap = nullptr;
   BlockDev *d_blocks = nullptr;
+  int *d_bpack = nullptr;                            // block_pack() of every block (Sigma(2) plan)
   int *d_blk_fw_off = nullptr, *d_blk_bk_off = nullptr, *d_work_count = nullptr;
   Sig2Work *d_work_list = nullptr;
   int *d_trip_start = nullptr, *d_trip_blk = nullptr;

@@ -305,7 +305,7 @@ int bcd_count_poles(bcd_ctx *ctx, int ish, const bcd_prop *g, int *nfw, int *nbk
   BTRY(stage_batch(ctx, g, 1, b));
   PropView P = b.view(true);
   HTRY(hipMemsetAsync(b.d_work_count, 0, sizeof(int), b.stream));
-  hipLaunchKernelGGL(k_sigma2_plan, dim3(b.n_gc), dim3(256), 0, b.stream, b.d_chans, b.d_blocks, 0, b.n_gc, P,
+  hipLaunchKernelGGL(k_sigma2_plan, dim3(b.n_gc), dim3(256), 0, b.stream, b.d_chans, b.d_bpack, 0, b.n_gc, P,
                      b.d_insts, b.d_cs, b.d_blk_fw_off, b.d_blk_bk_off, b.d_work_list, b.d_work_count, ish, nullptr);
   HTRY(hipGetLastError());
   ChanState cs;

@@ -82,6 +82,12 @@ struct BlockDev {
   int c2b;                       // two-body channel of the block's (alpha beta; J) pairs, -1: none
 };
 
+// The subshells a, b, g (below 256) and nJab of a block in one int, for the
+// passes of the Sigma(2) plan, which read nothing else of it.
+__host__ __device__ inline int block_pack(const BlockDev &B) {
+  return B.a | (B.b << 8) | (B.g << 16) | ((B.nJab & 1) << 24);
+}
+
 // One non-empty Sigma(2) block in the compact work list, with everything the
 // build needs before its first dependent load: the block, its channel and
 // instance, the fragment counts of both sides (2p1h: qp(a), qp(b), qh(g);

@@ -282,7 +282,7 @@ __global__ void k_cs_pack(const ChanState *cs, int n_gc, int *out) {
 // Pole counts of each (J,a,b,g) block from the fragment counts of the current
 // propagator (Count_2p1h_2h1p_poles, Self_energy_ext_Op_c.cpp:76-90), row
 // offsets in build order, and a compact list of the non-empty blocks.
-__global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin,
+__global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const int *bpack, int gc_begin,
                                                     int gc_end, PropView P, const InstDev *insts, ChanState *cs,
                                                     int *blk_fw_off, int *blk_bk_off, Sig2Work *work_list,
                                                     int *work_count, int only_ish, const int *order) {
@@ -304,16 +304,17 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
   auto counts = [&](int b, int &cfw, int &cbk, int *f6) {
     cfw = 0; cbk = 0;
     if (b >= C.blk_cnt) return;
-    const BlockDev B = blocks[C.blk_off + b];
-    const int ga = I.gc0 + B.a, gb = I.gc0 + B.b, gg = I.gc0 + B.g;
+    const int pk = bpack[C.blk_off + b];
+    const int ba = pk & 0xff, bb = (pk >> 8) & 0xff, bg = (pk >> 16) & 0xff, bJab = (pk >> 24) & 1;
+    const int ga = I.gc0 + ba, gb = I.gc0 + bb, gg = I.gc0 + bg;
     int na = P.nqp[cur][ga], nb = P.nqp[cur][gb], ng = P.nqh[cur][gg];
     if (f6) { f6[0] = na; f6[1] = nb; f6[2] = ng; }
     cfw = na * nb * ng;
-    if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cfw = (na * nb / 2) * ng; }
+    if (ba == bb) { nb = na + 1; if (bJab == 1) nb -= 2; cfw = (na * nb / 2) * ng; }
     na = P.nqh[cur][ga]; nb = P.nqh[cur][gb]; ng = P.nqp[cur][gg];
     if (f6) { f6[3] = na; f6[4] = nb; f6[5] = ng; }
     cbk = na * nb * ng;
-    if (B.a == B.b) { nb = na + 1; if (B.nJab == 1) nb -= 2; cbk = (na * nb / 2) * ng; }
+    if (ba == bb) { nb = na + 1; if (bJab == 1) nb -= 2; cbk = (na * nb / 2) * ng; }
   };
   // first pass: the channel's non-empty blocks, then one atomic for its
   // slice of the work list
@@ -366,10 +367,11 @@ __global__ __launch_bounds__(256) void k_sigma2_plan(const ChanDev *chans, const
         W.n01 = f6[0] | (f6[1] << 16); W.n23 = f6[2] | (f6[3] << 16); W.n45 = f6[4] | (f6[5] << 16);
         W.off_fw = ofw; W.off_bk = obk;
         {
-          const BlockDev B = blocks[C.blk_off + b];
-          const int noa = I.no[B.a], nob = I.no[B.b];
-          const int npair = B.a == B.b ? noa * (noa + 1) / 2 - B.nJab * noa : noa * nob;
-          W.nvt = C.nI * npair * I.no[B.g];
+          const int pk = bpack[C.blk_off + b];
+          const int ba = pk & 0xff, bb = (pk >> 8) & 0xff, bg = (pk >> 16) & 0xff, bJab = (pk >> 24) & 1;
+          const int noa = I.no[ba], nob = I.no[bb];
+          const int npair = ba == bb ? noa * (noa + 1) / 2 - bJab * noa : noa * nob;
+          W.nvt = C.nI * npair * I.no[bg];
           W._pad = 0;
         }
         work_list[base_nz + pz + __popcll(mask & ((1ull << lane) - 1ull))] = W;

@@ -37,7 +37,7 @@ __global__ void k_sigma_inf(const InstDev *insts, const ChanDev *chans, int gc_b
                             const InstState *is, const double *W, const double *rho, double *h);
 __global__ void k_mix(const InstDev *insts, InstState *is, double *rho, double *hist, long rho_tot, int m,
                       double mix_start);
-__global__ void k_sigma2_plan(const ChanDev *chans, const BlockDev *blocks, int gc_begin, int gc_end,
+__global__ void k_sigma2_plan(const ChanDev *chans, const int *bpack, int gc_begin, int gc_end,
                               PropView P, const InstDev *insts, ChanState *cs, int *blk_fw_off,
                               int *blk_bk_off, Sig2Work *work_list, int *work_count, int only_ish,
                               const int *order);
