@@ -5,25 +5,14 @@
 // k_arrow_sweep block per channel (DysonMatrix.cpp:123-532 for the matrix,
 // :436-484 for the split and ordering, :542-594 for Calc_En).
 #include "bcd_kernels.hpp"
+#include "bcd_wave.hpp"
 
 namespace bcd {
 
 namespace {
 
-__device__ __forceinline__ double shfl_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-
-__device__ __forceinline__ double shfl_max(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
-  return x;
-}
-
-// The block an arrowhead body runs on: reductions go through wave shuffles
-// and one LDS slot per wave.
+// The block an arrowhead body runs on: reductions go through wave-level DPP
+// sums (or ballots) and one LDS slot per wave.
 struct DevBlock {
   int *status;
   double *work;                  // FLOP counter (may be null)
@@ -49,25 +38,23 @@ struct DevBlock {
     r = fma(r, fma(-x, r, 1.0), r);
     return r;
   }
-  // exclusive prefix sum of v over the block (thread order); total in tot
-  __device__ int scan(int *tmp, int v, int &tot) const {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) tmp[w] = x;
+  // exclusive prefix sum of keep (0 or 1) over the block (thread order);
+  // total in tot.  Within a wave the prefix is the count of set ballot bits
+  // below the lane.
+  __device__ int scan01(int *tmp, int keep, int &tot) const {
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned long long b = __ballot(keep != 0);
+    const int below = __builtin_amdgcn_mbcnt_hi((unsigned int)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)b, 0));
+    if ((threadIdx.x & 63) == 0) tmp[w] = __popcll(b);
     __syncthreads();
     int off = 0, t = 0;
     for (int i = 0; i < nw; ++i) { const int c = tmp[i]; off += i < w ? c : 0; t += c; }
     __syncthreads();
     tot = t;
-    return off + x - v;
+    return off + below;
   }
   __device__ double sum(double *red, double v) const {
-    v = shfl_sum(v);
+    v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = 0.0;
@@ -75,8 +62,26 @@ struct DevBlock {
     __syncthreads();
     return s;
   }
+  // N sums at once, with one pair of barriers; red holds N slots per wave
+  template <int N>
+  __device__ void sums(double *red, double *v) const {
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int k = 0; k < N; ++k) red[w * N + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < nw; ++i) s += red[i * N + k];
+      v[k] = s;
+    }
+    __syncthreads();
+  }
   __device__ double max(double *red, double v) const {
-    v = shfl_max(v);
+    v = wave_max(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = red[0];

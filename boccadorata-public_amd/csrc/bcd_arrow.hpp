@@ -47,7 +47,7 @@ constexpr int kArrowNI = 16;       // largest nI (orbits per subshell)
 __host__ __device__ inline int arrow_ni_bound(int nI) { return nI <= 4 ? 4 : (nI <= 8 ? 8 : 16); }
 constexpr int kArrowMaxIt = 100;   // secular-equation iterations per root
 constexpr int kArrowMaxThreads = 256;
-constexpr int kArrowRed = 16;      // reduction scratch (doubles): one slot per wave
+constexpr int kArrowRed = 40;      // reduction scratch (doubles): ten slots per wave of a 256-thread block
 
 // One channel's Dyson problem in arrowhead form.  Poles come from up to two
 // sources: energy i of source s is E[s][i*es[s]], its coupling to orbit r is
@@ -243,6 +243,12 @@ __host__ __device__ inline double arrow_block_max(const Blk &B, double *red, dou
   return B.max(red, v);
 }
 
+// N block-wide sums at once (each in the order arrow_block_sum uses).
+template <int N, class Blk>
+__host__ __device__ inline void arrow_block_sums(const Blk &B, double *red, double *v) {
+  B.template sums<N>(red, v);
+}
+
 // The whole channel: poles, nI arrowhead updates, split at the Fermi energy,
 // descending sort per side and the Calc_En partial sums (DysonMatrix.cpp:
 // 436-484, 542-594).  ws holds arrow_ws_bytes(J.n, J.nI) bytes; NIMAX >= J.nI
@@ -308,7 +314,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
         const int i = b0 + tid;
         const int keep = (i < g && fabs(z[i]) > tol) ? 1 : 0;
         int tot = 0;
-        const int pre = B.scan((int *)red, keep, tot);
+        const int pre = B.scan01((int *)red, keep, tot);
         if (i < g) {
           if (keep) nd[kept + pre] = i;
           else dl[gone + (tid - pre)] = i;
@@ -489,8 +495,12 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       kb[3 * bk] += x2; kb[3 * bk + 1] += x1 * di; kb[3 * bk + 2] += x1;
     }
   }
-  for (int b = 0; b < 9; ++b) kb[b] = arrow_block_sum(B, red, kb[b]);
-  flops = arrow_block_sum(B, red, flops);
+  double tail[10];
+  for (int b = 0; b < 9; ++b) tail[b] = kb[b];
+  tail[9] = flops;
+  arrow_block_sums<10>(B, red, tail);
+  for (int b = 0; b < 9; ++b) kb[b] = tail[b];
+  flops = tail[9];
   if (tid == 0) {
     *J.out_nqp = nqp;
     *J.out_nqh = nqh;

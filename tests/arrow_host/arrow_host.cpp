@@ -47,7 +47,7 @@ struct HostBlock {
   void add_work(double f) const { work[t] += f; }
   double rcp(double x) const { return 1.0 / x; }
   unsigned long long lap(int, unsigned long long) const { return 0; }
-  int scan(int *, int v, int &tot) const {
+  int scan01(int *, int v, int &tot) const {
     acc[t] = v;
     sync();
     int pre = 0;
@@ -57,6 +57,10 @@ struct HostBlock {
     return pre;
   }
   double sum(double *, double v) const { return reduce(v, false); }
+  template <int N>
+  void sums(double *, double *v) const {
+    for (int k = 0; k < N; ++k) v[k] = reduce(v[k], false);
+  }
   double max(double *, double v) const { return reduce(v, true); }
   double reduce(double v, bool mx) const {
     acc[t] = v;
