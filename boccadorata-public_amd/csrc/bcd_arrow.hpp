@@ -71,8 +71,12 @@ struct ArrowJob {
   int wld, _wpad;
 };
 
+// Rows per column of the amplitude buffers kept for each of H and G: nI,
+// padded to an even count so that a column is read in 16-byte pairs.
+__host__ __device__ inline int arrow_h2(int nI) { return (nI + 1) & ~1; }
+
 __host__ __device__ inline size_t arrow_ws_doubles(int n, int nI) {
-  return (size_t)(6 + 4 * nI) * n + kArrowRed;
+  return (size_t)(6 + 4 * arrow_h2(nI)) * n + kArrowRed;
 }
 
 __host__ __device__ inline size_t arrow_ws_bytes(int n, int nI) {
@@ -257,10 +261,15 @@ template <int NIMAX, class Blk>
 __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws) {
   const int nI = J.nI, N = J.n, tid = B.tid(), nt = B.nt();
   const int m = J.np[0] + J.np[1];
-  const long LN = N;
+  // amplitude buffers A and B, column-major: column c holds its nI H rows
+  // (the eigenvector amplitudes on the orbits), padded to H2, then its nI G
+  // rows (the couplings still to be added), padded likewise -- S contiguous
+  // doubles, so that the eigenvector sums read a column in 16-byte pairs
+  const int H2 = arrow_h2(nI);
+  const long S = 2 * H2;
   double *dA = ws, *dB = dA + N;
-  double *HA = dB + N, *HB = HA + nI * LN, *GA = HB + nI * LN, *GB = GA + nI * LN;
-  double *dn = GB + nI * LN, *zn = dn + N, *tau = zn + N, *zh = tau + N, *red = zh + N;
+  double *HGA = dB + N, *HGB = HGA + S * N;
+  double *dn = HGB + S * N, *zn = dn + N, *tau = zn + N, *zh = tau + N, *red = zh + N;
   int *org = (int *)(red + kArrowRed), *nd = org + N, *dl = nd + N, *cnt = dl + N;
   const double eps = 2.2204460492503131e-16;
 
@@ -276,7 +285,9 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
   for (int i = tid; i < m; i += nt) {
     const int s = i < J.np[0] ? 0 : 1, ii = s ? i - J.np[0] : i;
     dB[i] = J.E[s][(long)ii * J.es[s]];
-    for (int r = 0; r < nI; ++r) GB[r * LN + i] = J.C[s][(long)ii * J.ci[s] + (long)r * J.cr[s]];
+    double *cb = HGB + i * S;
+    for (int r = 0; r < S; ++r) cb[r] = 0.0;
+    for (int r = 0; r < nI; ++r) cb[H2 + r] = J.C[s][(long)ii * J.ci[s] + (long)r * J.cr[s]];
   }
   B.sync();
   // ascending by energy, ties by position, into the A buffers
@@ -286,7 +297,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
 #pragma unroll 4
     for (int k = 0; k < m; ++k) { const double w = dB[k]; rk += (w < v) || (w == v && k < i); }
     dA[rk] = v;
-    for (int r = 0; r < nI; ++r) { GA[r * LN + rk] = GB[r * LN + i]; HA[r * LN + rk] = 0.0; }
+    for (int r = 0; r < S; ++r) HGA[rk * S + r] = HGB[i * S + r];
   }
   B.sync();
 
@@ -294,9 +305,10 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
   tk = B.lap(0, tk);
   for (int k = 0; k < nI; ++k) {
     const double alpha = J.u1[k * nI + k] + J.h[k * nI + k];
-    double *z = GA + k * LN;
+    // z[i * S]: coupling k of column i
+    double *z = HGA + H2 + k;
     double s2 = 0.0, dmx = 0.0;
-    for (int i = tid; i < g; i += nt) { s2 += z[i] * z[i]; dmx = fmax(dmx, fabs(dA[i])); }
+    for (int i = tid; i < g; i += nt) { s2 += z[i * S] * z[i * S]; dmx = fmax(dmx, fabs(dA[i])); }
     const double znorm = sqrt(arrow_block_sum(B, red, s2));
     const double dmax = arrow_block_max(B, red, dmx);
     const double tol = 8.0 * eps * fmax(fmax(fabs(alpha), dmax), znorm);
@@ -312,7 +324,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       int kept = 0, gone = 0;
       for (int b0 = 0; b0 < g; b0 += nt) {
         const int i = b0 + tid;
-        const int keep = (i < g && fabs(z[i]) > tol) ? 1 : 0;
+        const int keep = (i < g && fabs(z[i * S]) > tol) ? 1 : 0;
         int tot = 0;
         const int pre = B.scan01((int *)red, keep, tot);
         if (i < g) {
@@ -326,7 +338,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
       B.sync();
       for (int a = tid; a + 1 < kept; a += nt) {
         const int i0 = nd[a], i1 = nd[a + 1];
-        const double za = z[i0], zb = z[i1], t = hypot(za, zb);
+        const double za = z[i0 * S], zb = z[i1 * S], t = hypot(za, zb);
         if (fabs((dA[i1] - dA[i0]) * (za / t) * (zb / t)) <= tol) cnt[3] = 1;
       }
       B.sync();
@@ -335,24 +347,25 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
         for (int a = 0; a < kept; ++a) {
           const int i = nd[a], prev = p > 0 ? nd[p - 1] : -1;
           if (prev >= 0) {
-            const double za = z[prev], zb = z[i], t = hypot(za, zb), c = za / t, sn = zb / t;
+            const double za = z[prev * S], zb = z[i * S], t = hypot(za, zb), c = za / t, sn = zb / t;
             if (fabs((dA[i] - dA[prev]) * c * sn) <= tol) {
               // rotate (prev, i): i carries the whole coupling, prev leaves
+              double *cp = HGA + prev * S, *cq = HGA + i * S;
               for (int r = 0; r < nI; ++r) {
-                const double x = HA[r * LN + prev], y = HA[r * LN + i];
-                HA[r * LN + prev] = sn * x - c * y;
-                HA[r * LN + i] = c * x + sn * y;
+                const double x = cp[r], y = cq[r];
+                cp[r] = sn * x - c * y;
+                cq[r] = c * x + sn * y;
               }
               for (int r = k + 1; r < nI; ++r) {
-                const double x = GA[r * LN + prev], y = GA[r * LN + i];
-                GA[r * LN + prev] = sn * x - c * y;
-                GA[r * LN + i] = c * x + sn * y;
+                const double x = cp[H2 + r], y = cq[H2 + r];
+                cp[H2 + r] = sn * x - c * y;
+                cq[H2 + r] = c * x + sn * y;
               }
               const double da = dA[prev], db = dA[i];
               dA[prev] = sn * sn * da + c * c * db;
               dA[i] = c * c * da + sn * sn * db;
-              z[prev] = 0.0;
-              z[i] = t;
+              z[prev * S] = 0.0;
+              z[i * S] = t;
               --p;
               dl[q++] = prev;
             }
@@ -367,7 +380,7 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     const int p = cnt[0], q = cnt[1];
     // zh holds the squared couplings for the roots until the Loewner step
     // overwrites it
-    for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i]]; zh[i] = zn[i] * zn[i]; }
+    for (int i = tid; i < p; i += nt) { dn[i] = dA[nd[i]]; zn[i] = z[nd[i] * S]; zh[i] = zn[i] * zn[i]; }
     B.sync();
     tk = B.lap(1, tk);
     // roots, started from the previous sweep's where the problem kept its size
@@ -430,28 +443,40 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
         for (int i = 0; i < p; ++i) {
           const double w = zh[i] * B.rcp(t - (dn[i] - dofs));
           nu2 += w * w;
-          const long c = nd[i];
+          const double *col = HGA + nd[i] * S;
+          // pairs of rows; a pad row holds 0 and rows of G up to k are not
+          // used, so the pair's other entry may be added freely
 #pragma unroll
-          for (int r = 0; r < NIMAX; ++r) {
-            if (r < nI) ah[r] += w * HA[r * LN + c];
-            if (r > k && r < nI) ag[r] += w * GA[r * LN + c];
+          for (int q = 0; q < NIMAX; q += 2) {
+            if (q < nI) {
+              const double h0 = col[q], h1 = col[q + 1];
+              ah[q] += w * h0;
+              ah[q + 1] += w * h1;
+              if (q + 1 > k) {
+                const double g0 = col[H2 + q], g1 = col[H2 + q + 1];
+                ag[q] += w * g0;
+                ag[q + 1] += w * g1;
+              }
+            }
           }
         }
       }
       const double inv = 1.0 / sqrt(nu2);
       flops += (5.0 + 2.0 * (2 * nI - k - 1)) * p;
       dB[j] = lam;
+      double *cb = HGB + j * S;
 #pragma unroll
       for (int r = 0; r < NIMAX; ++r) {
-        if (r < nI) HB[r * LN + j] = ah[r] * inv;
-        if (r > k && r < nI) GB[r * LN + j] = ag[r] * inv;
+        if (r < H2) {
+          cb[r] = r < nI ? ah[r] * inv : 0.0;
+          cb[H2 + r] = (r > k && r < nI) ? ag[r] * inv : 0.0;
+        }
       }
     }
     for (int t = tid; t < q; t += nt) {
       const long c = dl[t], pos = p + 1 + t;
       dB[pos] = dA[c];
-      for (int r = 0; r < nI; ++r) HB[r * LN + pos] = HA[r * LN + c];
-      for (int r = k + 1; r < nI; ++r) GB[r * LN + pos] = GA[r * LN + c];
+      for (int r = 0; r < S; ++r) HGB[pos * S + r] = HGA[c * S + r];
     }
     B.sync();
     tk = B.lap(4, tk);
@@ -462,15 +487,15 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
 #pragma unroll 4
       for (int l = 0; l < g; ++l) { const double w = dB[l]; rk += (w < v) || (w == v && l < i); }
       dA[rk] = v;
-      for (int r = 0; r < nI; ++r) HA[r * LN + rk] = HB[r * LN + i];
-      for (int r = k + 1; r < nI; ++r) GA[r * LN + rk] = GB[r * LN + i];
+      for (int r = 0; r < S; ++r) HGA[rk * S + r] = HGB[i * S + r];
     }
     B.sync();
     tk = B.lap(5, tk);
   }
 
   tk = B.lap(6, tk);
-  // dA ascending, amplitudes in HA: split at E_F, both sides descending
+  // dA ascending, amplitudes in the H rows of HGA: split at E_F, both sides
+  // descending
   const double ef = J.efermi;
   int c_le = 0;
   for (int i = tid; i < N; i += nt) c_le += (dA[i] <= ef);
@@ -483,14 +508,14 @@ __host__ __device__ void arrow_body(const Blk &B, const ArrowJob &J, double *ws)
     J.out_e[slot] = di;
     double x1 = 0.0;
     for (int r = 0; r < nI; ++r) {
-      const double xr = HA[r * LN + i];
+      const double xr = HGA[i * S + r];
       J.out_x[slot * nI + r] = xr;
       x1 += xr * xr;
     }
     if (di <= ef) {
       double x2 = 0.0;
       for (int r = 0; r < nI; ++r)
-        for (int c = 0; c < nI; ++c) x2 += J.u1[r * nI + c] * HA[r * LN + i] * HA[c * LN + i];
+        for (int c = 0; c < nI; ++c) x2 += J.u1[r * nI + c] * HGA[i * S + r] * HGA[i * S + c];
       const int bk = x1 > 0.01 ? 0 : (x1 > 1.e-5 ? 1 : 2);
       kb[3 * bk] += x2; kb[3 * bk + 1] += x1 * di; kb[3 * bk + 2] += x1;
     }
