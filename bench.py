@@ -159,7 +159,7 @@ def moment_error(g, ref):
 # FETCH_SIZE / WRITE_SIZE passes of this workload (calibrated against a 1 GiB
 # copy): (kernel family, instances per step) -> (bytes, summary file)
 MEASURED_TRAFFIC = {("nmax7", "lanczos", 256): (225.7e9, "profiles/scgf_config2_nmax7_b256_lanczos_traffic.md"),
-                    ("batched", "lanczos", 256): (869.1e9, "profiles/scgf_batched256_lanczos_traffic.md")}
+                    ("batched", "lanczos", 256): (864.2e9, "profiles/scgf_batched256_lanczos_traffic.md")}
 
 
 def roofline(batch, workload):
