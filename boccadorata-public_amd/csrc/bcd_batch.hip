@@ -47,6 +47,8 @@ Batch::~Batch() {
     if (kv.second) (void)hipEventDestroy(kv.second);
   }
   if (lfork) (void)hipEventDestroy(lfork);
+  if (lt0) (void)hipEventDestroy(lt0);
+  for (auto &e : lbt) { if (e) (void)hipEventDestroy(e); }
   for (int k = 0; k < kLancAux; ++k) {
     if (ljoin[k]) (void)hipEventDestroy(ljoin[k]);
     if (lstream[k]) (void)hipStreamDestroy(lstream[k]);
@@ -104,6 +106,8 @@ int Batch::setup(int dev, const std::vector<const HostSystem *> &systems, const 
   HTRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (auto &e : ev) HTRY(hipEventCreate(&e));
   HTRY(hipEventCreateWithFlags(&lfork, hipEventDisableTiming));
+  HTRY(hipEventCreate(&lt0));
+  for (auto &e : lbt) HTRY(hipEventCreate(&e));
   for (int k = 0; k < kLancAux; ++k) {
     HTRY(hipStreamCreateWithFlags(&lstream[k], hipStreamNonBlocking));
     HTRY(hipEventCreateWithFlags(&ljoin[k], hipEventDisableTiming));
@@ -958,7 +962,83 @@ int Batch::lanczos() {
   }
   hipStream_t ls[kLancStreams] = {stream};
   for (int a = 1; a < kLancStreams; ++a) ls[a] = lstream[a - 1];
-  double load[kLancStreams] = {};
+  // the stream of each launch: list scheduling by estimate
+  std::vector<int> place(lb.size(), 0);
+  {
+    double load[kLancStreams] = {};
+    for (size_t i = 0; i < lb.size(); ++i) {
+      int a = 0;
+      if (fork)
+        for (int k = 1; k < kLancStreams; ++k)
+          if (load[k] < load[a]) a = k;
+      load[a] += lb[i].est;
+      place[i] = a;
+    }
+  }
+  // The estimates are times alone on the device; run together, the launches
+  // share it unevenly (a launch of a few long jobs holds its stream for twice
+  // its time alone while taking little of the device), so the streams can
+  // end far apart.  The launches' ends measured in the last forked stage
+  // correct that: the last launch of the stream that ended last moves to
+  // the stream that ended first when that ends the stage earlier by the
+  // measured times.  The placement changes no result.
+  if (fork) {
+    std::vector<int> key(lb.size());
+    for (size_t i = 0; i < lb.size(); ++i) key[i] = lb[i].bk;
+    if (key != lp_key) {
+      lp_key = key;
+      lp_used.clear();
+      lp_best.clear();
+      lp_timed = lp_fixed = lp_frozen = false;
+      lp_moves = 0;
+    }
+    if (lp_fixed) place = lp_used;
+    if (lp_timed && !lp_frozen && lp_used.size() == lb.size() && lb.size() <= (size_t)kNLancBuckets) {
+      float t[kNLancBuckets] = {};
+      bool ok = hipEventQuery(lt0) == hipSuccess;
+      for (size_t i = 0; ok && i < lb.size(); ++i)
+        ok = hipEventQuery(lbt[i]) == hipSuccess && hipEventElapsedTime(&t[i], lt0, lbt[i]) == hipSuccess;
+      bool moved = false;
+      if (ok) {
+        float send[kLancStreams] = {};
+        int last[kLancStreams], cnt[kLancStreams] = {};
+        for (int s = 0; s < kLancStreams; ++s) last[s] = -1;
+        for (size_t i = 0; i < lb.size(); ++i) {
+          const int s = lp_used[i];
+          ++cnt[s];
+          if (last[s] < 0 || t[i] >= send[s]) { send[s] = t[i]; last[s] = (int)i; }
+        }
+        int L = 0, F = 0;
+        for (int s = 1; s < kLancStreams; ++s) {
+          if (send[s] > send[L]) L = s;
+          if (send[s] < send[F]) F = s;
+        }
+        if (lp_best.empty() || send[L] < lp_best_end) { lp_best = lp_used; lp_best_end = send[L]; }
+        if (lp_moves < kLancMoves && L != F && cnt[L] >= 2) {
+          float before = 0.f, other = 0.f;   // end of L's launch before its last; the third stream's end
+          for (size_t i = 0; i < lb.size(); ++i)
+            if (lp_used[i] == L && (int)i != last[L]) before = std::max(before, t[i]);
+          for (int s = 0; s < kLancStreams; ++s)
+            if (s != L && s != F) other = std::max(other, send[s]);
+          const float to_f = send[F] + (send[L] - before);
+          if (std::max(std::max(before, to_f), other) + kLancMoveGainMs < send[L]) {
+            place = lp_used;
+            place[last[L]] = F;
+            ++lp_moves;
+            moved = true;
+          }
+        }
+      }
+      if (!moved) {
+        // keep the placement that ended earliest of those measured
+        if (!lp_best.empty()) place = lp_best;
+        lp_frozen = true;
+      }
+      lp_fixed = true;
+    }
+  } else {
+    lp_key.clear();
+  }
   bool used[kLancStreams] = {};
   if (fork) HTRY(hipEventRecord(lfork, stream));
   // outside tuning and profiling, each bucket's pole blocks are diagonalised
@@ -966,16 +1046,20 @@ int Batch::lanczos() {
   // that they overlap the other buckets' reductions
   const PoleCfg pc = pole_cfg();
   poles_done = fork && !profile && par.dyson_solver == 0;
-  for (const LBucket &b : lb) {
-    int a = 0;
-    if (fork)
-      for (int k = 1; k < kLancStreams; ++k)
-        if (load[k] < load[a]) a = k;
-    load[a] += b.est;
+  const bool timed = poles_done && lb.size() <= (size_t)kNLancBuckets;
+  if (timed) HTRY(hipEventRecord(lt0, stream));
+  for (size_t i = 0; i < lb.size(); ++i) {
+    const LBucket &b = lb[i];
+    const int a = place[i];
     if (a > 0 && !used[a]) HTRY(hipStreamWaitEvent(ls[a], lfork, 0));
     used[a] = true;
     launch_lanczos(lanc_shape[b.bk], b.nj, d_ljobs, d_lorder + b.k0, ls[a]);
     if (poles_done) launch_poles(pc, d_lorder + b.k0, b.nj, ls[a]);
+    if (timed) HTRY(hipEventRecord(lbt[i], ls[a]));
+  }
+  if (fork) {
+    lp_used = place;
+    lp_timed = timed;
   }
   for (int a = 1; a < kLancStreams; ++a)
     if (used[a]) {

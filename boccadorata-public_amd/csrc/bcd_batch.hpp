@@ -58,6 +58,12 @@ static_assert(kLancStreams >= 1 && kLancStreams <= kLancAux + 1, "Lanczos stream
 constexpr double kLancChipBytesPerMs = 5.0e9;
 constexpr double kLancCuBytesPerMs = 1.1e7;
 constexpr double kLancPoleMs = 5.0;
+// Measured placement: after a forked Lanczos stage, the last launch of the
+// stream that ended last moves to the stream that ended first when the
+// measured times predict the stage to end at least kLancMoveGainMs earlier;
+// at most kLancMoves such moves for one set of launches, then it is kept.
+constexpr int kLancMoves = 2;
+constexpr float kLancMoveGainMs = 1.0f;
 
 struct Batch {
   struct PoleCfg {
@@ -73,6 +79,17 @@ struct Batch {
   hipStream_t stream = nullptr;
   hipStream_t lstream[kLancAux] = {};
   hipEvent_t lfork = nullptr, ljoin[kLancAux] = {};
+  // Placement of the Lanczos launches over the streams, measured: lt0 at the
+  // fork and lbt[i] after launch i's pole blocks on its stream give each
+  // launch's end in the last forked stage (lp_timed).  lp_key: the launches'
+  // first buckets in launch order; lp_used: the streams they ran on; lp_fixed:
+  // the placement comes from a measurement; lp_frozen: no more measuring;
+  // lp_moves: moves made for this set of launches (lanczos()).
+  hipEvent_t lt0 = nullptr, lbt[kNLancBuckets] = {};
+  std::vector<int> lp_key, lp_used, lp_best;   // lp_best: the placement that ended earliest
+  float lp_best_end = 0.f;
+  bool lp_timed = false, lp_fixed = false, lp_frozen = false;
+  int lp_moves = 0;
   bcd_solve_params par{};
   std::vector<const HostSystem *> sys;
   std::vector<int> inst_lanc_n;      // Lanczos length per instance (empty: par.lanc_n for all); set before setup
