@@ -1113,8 +1113,8 @@ void Batch::launch_poles(const PoleCfg &c, const int *order, int njobs, hipStrea
   hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
   if (nImax > 4)
     hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
-  hipLaunchKernelGGL(k_ql_poles, dim3((njobs + 7) / 8), dim3(64),
-                     (size_t)8 * (2 + std::min(nImax, kQLGroupNI)) * c.lw * sizeof(double), s, d_pjobs, order, njobs,
+  hipLaunchKernelGGL(k_ql_poles, dim3((njobs + 7) / 8), dim3(64), (size_t)8 * 2 * c.lw * sizeof(double), s,
+                     d_pjobs, order, njobs,
                      d_status, c.lw, std::min(nImax, kQLGroupNI));
 }
 

@@ -1861,53 +1861,97 @@ __global__ __launch_bounds__(64) void k_ql(const DysonJob *jobs, int njobs, int 
   }
 }
 
-// The pole-block diagonalisation (poles mode) of k_ql for n <= kPoleWaveN
-// and nI <= 8, eight matrices per wave: each group of eight lanes runs one
-// matrix's implicit QL (the same iteration as k_ql) with d, e and the
-// tracked rows in its own LDS slice; lane r < nI of a group rotates tracked
-// row r.  Every lane of a group computes the same scalar chain, so a wave
-// does the work of eight matrices in the instructions k_ql spends on one.
-// ld: LDS doubles per d or e array (the batch's largest side); qrows: tracked
-// rows per slice (the batch's largest nI).
+// A group's negligibility mask: bit p (p < 128) in word p / 64, kept in two
+// scalars so that no step indexes a local array.
+struct NegMask {
+  unsigned long long w0 = 0ull, w1 = 0ull;
+  __device__ bool get(int p) const { return ((p < 64 ? w0 >> p : w1 >> (p - 64)) & 1ull) != 0ull; }
+  __device__ void put(int p, bool v) {
+    if (p < 64) w0 = v ? (w0 | (1ull << p)) : (w0 & ~(1ull << p));
+    else w1 = v ? (w1 | (1ull << (p - 64))) : (w1 & ~(1ull << (p - 64)));
+  }
+  // first set (want) or clear bit at or after p < 128; 128 when there is none
+  __device__ int find(int p, bool want) const {
+    unsigned long long x0 = want ? w0 : ~w0, x1 = want ? w1 : ~w1;
+    if (p < 64) {
+      x0 &= ~0ull << p;
+      if (x0) return __ffsll((long long)x0) - 1;
+    } else {
+      x1 &= ~0ull << (p - 64);
+    }
+    return x1 ? 64 + __ffsll((long long)x1) - 1 : 128;
+  }
+};
+
+// The pole-block diagonalisation (poles mode) of k_ql for n <= kPoleWaveN and
+// nI <= 8, eight matrices per wave: each group of eight lanes runs one
+// matrix's implicit QL (the same iteration as k_ql) with d and e in its own
+// LDS slice; lane r < nI of a group rotates tracked row r where the
+// tridiagonalisation left it, in the job's buffer (tri + 2n), and finally
+// copies it out.  Every lane of a group computes the same scalar chain, so a
+// wave does the work of eight matrices in the instructions k_ql spends on
+// one; with only d and e in LDS, sixteen waves fit a CU, which hides the
+// chain's latency (the tracked rows in LDS allowed four).
+// The block end m of k_ql's search (the first p >= lo with |e_p| <=
+// eps (|d_p| + |d_p+1|), else n - 1) comes from a mask of that test per p,
+// kept current as the rotations change d and e (each changed position is
+// tested again once its three values are final), so no step scans e.  The
+// next step's d, e and tracked-row entries are loaded while the current one
+// computes (the row entry a step ahead hides its load), and the tracked
+// row's intermediate entry stays in a register.  ld: LDS doubles per d or e
+// array (the batch's largest side); qrows: the batch's largest nI.
 __global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, const int *order, int njobs, int *status, int ld,
                                                   int qrows) {
   extern __shared__ double sm[];
   const int lane = threadIdx.x, grp = lane >> 3, gl = lane & 7;
   const int job = blockIdx.x * 8 + grp;
-  double *d = sm + (long)grp * (2 + qrows) * ld, *e = d + ld, *Q = e + ld;
+  double *d = sm + (long)grp * 2 * ld, *e = d + ld;
   const double eps = 2.2204460492503131e-16;
   int n = 0, nI = 0;
   const DysonJob *J = job < njobs ? jobs + (order ? order[job] : job) : nullptr;
   if (J) { n = J->path == 0 ? J->n : 0; nI = J->nI; }
-  if (n > ld || nI > qrows || nI > kQLGroupNI) {
+  if (n > ld || n > kPoleWaveN || nI > qrows || nI > kQLGroupNI) {
     // does not fit this kernel's slices (the host never launches such jobs here)
     if (gl == 0 && n > 0) atomicAdd(status, 1);
     n = 0;
   }
   for (int i = gl; i < n; i += 8) { d[i] = J->tri[i]; e[i] = J->tri[n + i]; }
-  for (int i = gl; i < nI * n; i += 8) Q[i] = J->tri[2L * n + i];
   __syncthreads();
+  auto negl = [&](double ep, double dp, double dq) { return fabs(ep) <= eps * (fabs(dp) + fabs(dq)); };
+  // the mask; position n - 1 stands for k_ql's default block end
+  NegMask nb;
+  for (int b0 = 0; b0 < ld && b0 < kPoleWaveN; b0 += 8) {
+    const int p = b0 + gl;
+    const bool neg = p < n - 1 && negl(e[p], d[p], d[p + 1]);
+    const unsigned long long bits = (__ballot(neg) >> (8 * grp)) & 0xffull;
+    if (b0 < 64) nb.w0 |= bits << b0;
+    else nb.w1 |= bits << (b0 - 64);
+  }
+  if (n > 0) nb.put(n - 1, true);
+  const bool track = n > 0 && gl < nI;
+  double *row = track ? J->tri + 2L * n + (long)gl * n : nullptr;
   // phase 0: find the block end m for lo; 1: one rotation of the chain; 2: done
   int lo = 0, it = 0, m = 0, i = 0, phase = n > 0 ? 0 : 2, fail = 0;
   double s = 1.0, c = 1.0, g = 0.0, pp = 0.0, di1 = 0.0;
+  // this step's e_i, d_i and tracked entry, the tracked row's carried entry
+  // (position i + 1) and d_{i+2} as the previous step left it
+  double ce = 0.0, cd = 0.0, cq = 0.0, qc = 0.0, dab = 0.0;
   while (__ballot(phase != 2)) {
     if (phase == 0) {
-      int mm = n - 1;
-      for (int b0 = lo; b0 < n - 1; b0 += 8) {
-        const int p = b0 + gl;
-        const bool neg = p < n - 1 && fabs(e[p]) <= eps * (fabs(d[p]) + fabs(d[p + 1]));
-        const unsigned bits = (unsigned)((__ballot(neg) >> (8 * grp)) & 0xffull);
-        if (bits) { mm = b0 + __ffs(bits) - 1; break; }
-      }
-      if (mm == lo) {
-        ++lo;
-        it = 0;
-        if (lo >= n) phase = 2;
+      // positions whose e is negligible end a block of one: lo moves past
+      // them (k_ql's m == lo case, one lo at a time)
+      const int nlo = nb.get(lo) ? nb.find(lo, false) : lo;
+      if (nlo != lo) { lo = nlo; it = 0; }
+      if (lo >= n) {
+        phase = 2;
       } else if (it >= 60) {
         fail = 1;
         phase = 2;
+      } else if ((m = nb.find(lo, true)) >= n) {
+        // cannot happen (bit n - 1 stays set); kept so that no index leaves the slice
+        fail = 1;
+        phase = 2;
       } else {
-        m = mm;
         const double dlo = d[lo], elo = e[lo];
         g = (d[lo + 1] - dlo) / (2.0 * elo);
         const double r = ql_hyp(g, 1.0);
@@ -1915,16 +1959,27 @@ __global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, const int
         g = di1 - dlo + elo / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
         s = 1.0; c = 1.0; pp = 0.0;
         i = m - 1;
+        ce = e[i]; cd = d[i];
+        if (track) { cq = row[i]; qc = row[m]; }
         phase = 1;
       }
     } else if (phase == 1) {
-      const double ei = e[i], di = d[i];
+      const double ei = ce, di = cd, qi = cq;
+      if (i - 1 >= lo) {
+        ce = e[i - 1]; cd = d[i - 1];
+        if (track) cq = row[i - 1];
+      }
       const double f = s * ei, b = c * ei;
-      double r = ql_hyp(f, g);
-      e[i + 1] = r;
+      const double eh = ql_hyp(f, g);
+      double r = eh;
+      e[i + 1] = eh;
       if (r == 0.0) {
-        d[i + 1] = di1 - pp;
+        const double dn = di1 - pp;
+        d[i + 1] = dn;
         e[m] = 0.0;
+        if (track) row[i + 1] = qc;
+        if (i + 1 < m) nb.put(i + 1, negl(eh, dn, dab));
+        nb.put(i, negl(ei, di, dn));
         ++it;
         phase = 0;
       } else {
@@ -1932,19 +1987,23 @@ __global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, const int
         g = di1 - pp;
         r = (di - g) * s + 2.0 * c * b;
         pp = s * r;
-        d[i + 1] = g + pp;
+        const double dn = g + pp;
+        d[i + 1] = dn;
         g = c * r - b;
-        if (gl < nI) {
-          double *row = Q + (long)gl * n;
-          const double t = row[i + 1];
-          row[i + 1] = s * row[i] + c * t;
-          row[i] = c * row[i] - s * t;
+        if (track) {
+          row[i + 1] = s * qi + c * qc;
+          qc = c * qi - s * qc;
         }
+        if (i + 1 < m) nb.put(i + 1, negl(eh, dn, dab));
+        dab = dn;
         di1 = di;
         if (--i < lo) {
-          d[lo] -= pp;
+          const double dl = d[lo] - pp;
+          d[lo] = dl;
           e[lo] = g;
           e[m] = 0.0;
+          if (track) row[lo] = qc;
+          nb.put(lo, negl(g, dl, dab));
           ++it;
           phase = 0;
         }
@@ -1954,10 +2013,9 @@ __global__ __launch_bounds__(64) void k_ql_poles(const DysonJob *jobs, const int
   __syncthreads();
   if (!J || n == 0) return;
   if (fail) { if (gl == 0) atomicAdd(status, 1); return; }
-  for (int k = gl; k < n; k += 8) {
-    J->out_lam[k] = d[k];
-    for (int r = 0; r < nI; ++r) J->out_ct[(long)r * J->ct_ld + k] = Q[(long)r * n + k];
-  }
+  for (int k = gl; k < n; k += 8) J->out_lam[k] = d[k];
+  if (track)
+    for (int k = 0; k < n; ++k) J->out_ct[(long)gl * J->ct_ld + k] = row[k];
 }
 
 // The reference's heapsort of pole rows by column col (Sort_u_double2dim /
