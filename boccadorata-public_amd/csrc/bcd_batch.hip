@@ -1107,12 +1107,9 @@ void Batch::launch_poles(const PoleCfg &c, const int *order, int njobs, hipStrea
   if (!c.any_wave) return;
   // above 64 KiB the kernels need the larger dynamic-LDS limit
   if (c.wlds > 65536) {
-    (void)hipFuncSetAttribute((const void *)k_tridiag_poles<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.wlds);
-    (void)hipFuncSetAttribute((const void *)k_tridiag_poles<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.wlds);
+    (void)hipFuncSetAttribute((const void *)k_tridiag_poles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.wlds);
   }
-  hipLaunchKernelGGL(k_tridiag_poles<4>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
-  if (nImax > 4)
-    hipLaunchKernelGGL(k_tridiag_poles<8>, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
+  hipLaunchKernelGGL(k_tridiag_poles, dim3(njobs), dim3(64), c.wlds, s, d_pjobs, order, njobs, c.ldw);
   hipLaunchKernelGGL(k_ql_poles, dim3((njobs + 7) / 8), dim3(64), (size_t)8 * 2 * c.lw * sizeof(double), s,
                      d_pjobs, order, njobs,
                      d_status, c.lw, std::min(nImax, kQLGroupNI));

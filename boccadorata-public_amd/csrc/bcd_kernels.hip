@@ -1556,15 +1556,10 @@ __global__ __launch_bounds__(256) void k_tridiag(const DysonJob *jobs, int njobs
 // adding them leaves every sum of an n <= 64 side as with one column per
 // lane.  Same reflections as k_tridiag (Golub & Van Loan, Alg. 8.3.1).  ld:
 // LDS row stride (odd, >= n); dynamic LDS: ld * ld + 2 * kPoleWaveN doubles.
+// NIMAX bounds the tracked rows held per lane (nI <= NIMAX).
 template <int NIMAX>
-__global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, const int *order, int njobs, int ld) {
-  extern __shared__ double sm[];
-  if ((int)blockIdx.x >= njobs) return;
-  const DysonJob J = jobs[order ? order[blockIdx.x] : blockIdx.x];
+__device__ __forceinline__ void tridiag_poles_body(const DysonJob &J, int ld, double *sm) {
   const int n = J.n, nI = J.nI, j = threadIdx.x;
-  // each job in the launch of its own accumulator bound (4 or 8), so that
-  // the unrolled code that runs it does not depend on the batch
-  if (n <= 0 || n > kPoleWaveN || n > ld || nI > NIMAX || (NIMAX > 4 && nI <= 4) || !J.poles || J.path != 0) return;
   double *A = sm, *v = A + (long)ld * ld, *wv = v + kPoleWaveN;
   const int col[2] = {j, j + 64};
   double q[2][NIMAX];
@@ -1677,9 +1672,17 @@ __global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, cons
   }
 }
 
-template __global__ void k_tridiag_poles<4>(const DysonJob *, const int *, int, int);
-template __global__ void k_tridiag_poles<8>(const DysonJob *, const int *, int, int);
-template __global__ void k_tridiag_poles<16>(const DysonJob *, const int *, int, int);
+// One launch for every one-wave job: each runs the body of its own
+// accumulator bound (4 or 8), so that the unrolled code that runs a job does
+// not depend on the batch.
+__global__ __launch_bounds__(64) void k_tridiag_poles(const DysonJob *jobs, const int *order, int njobs, int ld) {
+  extern __shared__ double sm[];
+  if ((int)blockIdx.x >= njobs) return;
+  const DysonJob J = jobs[order ? order[blockIdx.x] : blockIdx.x];
+  if (J.n <= 0 || J.n > kPoleWaveN || J.n > ld || J.nI > kQLGroupNI || !J.poles || J.path != 0) return;
+  if (J.nI <= 4) tridiag_poles_body<4>(J, ld, sm);
+  else tridiag_poles_body<8>(J, ld, sm);
+}
 
 template __global__ void k_tridiag<true>(const DysonJob *, int);
 template __global__ void k_tridiag<false>(const DysonJob *, int);

@@ -116,7 +116,6 @@ template <bool IN_LDS>
 __global__ void k_tridiag(const DysonJob *jobs, int njobs);
 // The pole-block kernels take njobs jobs, job k being jobs[order[k]] (or
 // jobs[k] when order is null).
-template <int NIMAX>
 __global__ void k_tridiag_poles(const DysonJob *jobs, const int *order, int njobs, int ld);
 __global__ void k_ql(const DysonJob *jobs, int njobs, int *status);
 constexpr int kQLGroupNI = 8;     // tracked rows of k_ql_poles (one lane of an 8-lane group each)
