@@ -1282,7 +1282,7 @@ int Batch::solve_impl() {
     // are in place
     const bool checked = par.mode == 2 && (mlp || par.accel > 0) && fallback_tol > 0.0;
     const bool learned = mlp && par.mode == 2;
-    if (learn_record || checked || learned) BTRY(learn_features());
+    if (learn_record || checked || learned) BTRY(learn_features(learn_record || learned));
     if (learned) {
       BTRY(mlp->reserve(n_gc));
       for (int i = 0; i < n_inst; ++i) seeded[i] = h_is[i].failed ? 0 : 1;
@@ -1365,11 +1365,11 @@ int Batch::after_first_sweep() {
   return BCD_OK;
 }
 
-int Batch::learn_features() {
+int Batch::learn_features(bool features) {
   BTRY(learn_alloc());
   hipLaunchKernelGGL(k_learn_features, dim3(n_gc), dim3(256), 0, stream, d_insts, d_chans, n_gc, view(true),
                      d_inst_feat, d_cs, d_row_fw_base, d_row_bk_base, d_rows_fw, d_rows_bk, d_feat, d_hf_e, d_hf_x,
-                     d_hf_n);
+                     d_hf_n, features ? 1 : 0);
   HTRY(hipGetLastError());
   return BCD_OK;
 }

@@ -204,7 +204,7 @@ struct Batch {
   bool learn_hook = false;            // the stage runs after_first_sweep() after its first sweep
   int after_first_sweep();
   int learn_alloc();
-  int learn_features();
+  int learn_features(bool features);   // false: the HF snapshot only
   int verify_and_fallback(int src);
   int solve();
   int solve_impl();

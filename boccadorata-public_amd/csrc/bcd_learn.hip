@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_learn_features(const InstDev *insts, co
                                                         PropView P, const float *inst_feat, const ChanState *cs,
                                                         const long *row_fw_base, const long *row_bk_base,
                                                         const double *rows_fw, const double *rows_bk, float *feat,
-                                                        double *hf_e, double *hf_x, int *hf_n) {
+                                                        double *hf_e, double *hf_x, int *hf_n, int features) {
   __shared__ LearnBasis B;
   __shared__ int ok;
   __shared__ double red[4][kLearnTri];
@@ -70,6 +70,7 @@ __global__ __launch_bounds__(256) void k_learn_features(const InstDev *insts, co
   for (int i = tid; i < ns; i += 256) he[i] = e[i];
   for (int i = tid; i < ns * nI; i += 256) hx[i] = x[i];
   if (tid == 0) { hf_n[2 * gc] = nqp; hf_n[2 * gc + 1] = nqh; }
+  if (!features) return;
   float *f = feat + (long)gc * kLearnIn;
   for (int i = tid; i < kLearnIn; i += 256) f[i] = 0.f;
   __syncthreads();

@@ -39,11 +39,12 @@ __host__ __device__ inline int learn_tri(int k, int kp) { return k * kLearnNI - 
 
 // inputs (after the Sigma(2) build, the propagator being the HF one) and a
 // snapshot of the HF propagator: energies [n_gc x kMaxNI], amplitudes
-// [n_gc x kMaxNI^2], counts [n_gc x 2]
+// [n_gc x kMaxNI^2], counts [n_gc x 2].  features = 0: the snapshot only
+// (what the residual check's fallback restarts from).
 __global__ void k_learn_features(const InstDev *insts, const ChanDev *chans, int n_gc, PropView P,
                                  const float *inst_feat, const ChanState *cs, const long *row_fw_base,
                                  const long *row_bk_base, const double *rows_fw, const double *rows_bk, float *feat,
-                                 double *hf_e, double *hf_x, int *hf_n);
+                                 double *hf_e, double *hf_x, int *hf_n, int features);
 // D of the current propagator in the snapshot's basis, times kLearnScale,
 // into out[gc * stride + off + t]: the first sc0 sweep's D1 (an input) and
 // the converged D (the target)
